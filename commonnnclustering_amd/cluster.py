@@ -1,0 +1,212 @@
+"""User API: ``Clustering`` (cnnclustering cluster.pyx:102-392).
+
+``Clustering(data, recipe=..., **recipe_kwargs)`` builds the component graph
+through the recipe builder; ``fit(radius_cutoff, cnn_cutoff, member_cutoff,
+...)`` keeps the reference's contract: cnn_offset subtraction and the
+``cnn_cutoff >= 0`` assertion, fresh or frozen label initialisation,
+``make_parameters``, timed fitter call, label meta information,
+``sort_by_size`` and the ``Record`` appended to the summary.
+"""
+
+from __future__ import annotations
+
+import weakref
+from collections import Counter
+
+import numpy as np
+
+from ._primitive_types import P_AINDEX
+from ._types import InputData, Labels
+from ._fit import Fitter, FitterGPU
+from .report import Record, Summary, timed
+from . import recipes
+
+
+class Bundle:
+    """Holder of input data, labels and summary (the container role of _bundle.pyx:20-272)."""
+
+    def __init__(self, input_data=None, labels=None, *, summary=None, meta=None):
+        self._input_data = input_data
+        self._labels = labels
+        self._children = None
+        self._summary = Summary() if summary is None else summary
+        self.meta = {} if meta is None else meta
+
+    @property
+    def input_data(self):
+        return None if self._input_data is None else self._input_data.data
+
+    @property
+    def labels(self):
+        return None if self._labels is None else self._labels.labels
+
+    @labels.setter
+    def labels(self, value):
+        if isinstance(value, Labels):
+            self._labels = value
+        else:
+            self._labels = Labels.from_sequence(value)
+
+    @property
+    def summary(self):
+        return self._summary
+
+
+class Clustering:
+    """Organises a clustering (cluster.pyx:56-147).
+
+    Args:
+        data: raw input (2-D sequence, array or CUDA tensor of point
+            coordinates; neighbour lists for the neighbourhood recipes), an
+            ``InputData`` instance or a ``Bundle``.
+        fitter: a ``Fitter`` instance overriding the recipe.
+        recipe: a registered recipe name (default ``"coordinates"``) or a
+            mapping of component keys to component types.
+        **recipe_kwargs: recipe overrides, e.g.
+            ``fitter__ngetter__precision=...`` via a tuple spec.
+    """
+
+    def __init__(self, data=None, *, fitter=None, hierarchical_fitter=None, predictor=None,
+                 bundle_kwargs=None, recipe=None, **recipe_kwargs):
+        builder = recipes.Builder(recipe, **recipe_kwargs)
+        if bundle_kwargs is None:
+            bundle_kwargs = {}
+        if data is None:
+            self._bundle = None
+        elif isinstance(data, Bundle):
+            self._bundle = data
+        elif isinstance(data, InputData):
+            self._bundle = Bundle(data, **bundle_kwargs)
+        else:
+            self._bundle = Bundle(builder.make_input_data(data), **bundle_kwargs)
+
+        if fitter is not None:
+            if not isinstance(fitter, Fitter):
+                raise TypeError(f"Can't use object of type {type(fitter).__name__} as fitter. "
+                                f"Expected type {Fitter.__name__}.")
+            self._fitter = fitter
+        else:
+            f = builder.make_component("fitter")
+            self._fitter = None if f is object else f
+        if hierarchical_fitter is not None or "hierarchical_fitter" in builder.recipe:
+            raise NotImplementedError("hierarchical fitters are not part of this engine; "
+                                      "use commonnnclustering_amd.batch.scan for (r, c) scans")
+        self._hierarchical_fitter = None
+        self._predictor = predictor
+
+    @property
+    def root(self):
+        return self._bundle
+
+    @property
+    def labels(self):
+        if self._bundle is None or self._bundle._labels is None:
+            return None
+        return self._bundle._labels.labels
+
+    @labels.setter
+    def labels(self, value):
+        if self._bundle is None:
+            raise ValueError("Can't set labels because there is no root bundle")
+        self._bundle.labels = value
+
+    @property
+    def input_data(self):
+        if self._bundle is None or self._bundle._input_data is None:
+            return None
+        return self._bundle._input_data.data
+
+    @property
+    def fitter(self):
+        return self._fitter
+
+    @fitter.setter
+    def fitter(self, value):
+        if value is not None and not isinstance(value, Fitter):
+            raise TypeError(f"Can't use object of type {type(value).__name__} as fitter. "
+                            f"Expected type {Fitter.__name__}.")
+        self._fitter = value
+
+    @property
+    def summary(self):
+        return self._bundle._summary
+
+    def __str__(self):
+        return (f"{type(self).__name__}(input_data={None if self._bundle is None else self._bundle._input_data}, "
+                f"fitter={self._fitter}, hfitter={self._hierarchical_fitter}, predictor={self._predictor})")
+
+    def _fit(self, cluster_params, bundle=None):
+        """Low-level fit without parameter preparation (cluster.pyx:230-254)."""
+        if bundle is None:
+            bundle = self._bundle
+        self._fitter.fit(bundle._input_data, bundle._labels, cluster_params)
+
+    def fit(self, radius_cutoff: float, cnn_cutoff: int, member_cutoff: int = None, max_clusters: int = None,
+            cnn_offset: int = None, sort_by_size: bool = True, info: bool = True, record: bool = True,
+            record_time: bool = True, v: bool = True, purge: bool = False, bundle=None) -> None:
+        """Execute the clustering procedure (cluster.pyx:256-392)."""
+        if bundle is None:
+            bundle = self._bundle
+        if cnn_offset is None:
+            cnn_offset = 0
+        cnn_cutoff -= cnn_offset
+        assert cnn_cutoff >= 0
+
+        n = bundle._input_data.n_points
+        fresh = (bundle._labels is None) or purge or (not bundle._labels.meta.get("frozen", False))
+        if fresh:
+            old_label_set = set()
+            current_start = 1
+        else:
+            old_label_set = bundle._labels.to_set()
+            current_start = max(old_label_set) + 1
+
+        cluster_params = self._fitter.make_parameters(radius_cutoff, cnn_cutoff, current_start)
+
+        fused = fresh and sort_by_size and isinstance(self._fitter, FitterGPU)
+        if fused:
+            # Device-resident fit + sort_by_size in one call; the same labels as
+            # fit() on fresh Labels followed by Labels.sort_by_size.
+            def run():
+                return self._fitter.fit_fresh_sorted(bundle._input_data, cluster_params, member_cutoff,
+                                                     max_clusters)
+
+            if record_time:
+                new_labels, execution_time = timed(run)()
+            else:
+                new_labels, execution_time = run(), None
+            unsorted_params = None
+            bundle._labels = Labels(np.ascontiguousarray(new_labels, dtype=P_AINDEX),
+                                    consider=np.zeros(n, dtype=np.uint8))
+        else:
+            if fresh:
+                bundle._labels = Labels(np.zeros(n, order="C", dtype=P_AINDEX))
+            if record_time:
+                _, execution_time = timed(self._fitter.fit)(bundle._input_data, bundle._labels, cluster_params)
+            else:
+                self._fitter.fit(bundle._input_data, bundle._labels, cluster_params)
+                execution_time = None
+            unsorted_params = True
+
+        if info:
+            new_label_set = bundle._labels.to_set()
+            params = {k: (radius_cutoff, cnn_cutoff) for k in new_label_set - old_label_set if k != 0}
+            meta = {"params": params, "reference": weakref.proxy(bundle), "origin": "fit"}
+            old_params = bundle._labels.meta.get("params", {})
+            old_params.update(meta["params"])
+            meta["params"] = old_params
+            bundle._labels.meta.update(meta)
+
+        if sort_by_size and unsorted_params:
+            bundle._labels.sort_by_size(member_cutoff, max_clusters)
+
+        if record:
+            frequencies = Counter(bundle._labels.labels.tolist())
+            n_noise = frequencies.pop(0, 0)
+            n_largest = frequencies.most_common(1)[0][1] if frequencies else 0
+            rec = Record(n, radius_cutoff, cnn_cutoff, member_cutoff, max_clusters,
+                         len(bundle._labels.to_set() - {0}), n_largest / n if n else 0.0,
+                         n_noise / n if n else 0.0, execution_time)
+            if v:
+                print(rec)
+            bundle._summary.append(rec)

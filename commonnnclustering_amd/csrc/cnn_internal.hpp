@@ -1,0 +1,237 @@
+// Internal declarations shared by the HIP translation units of libcnn_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cnn_gpu.h"
+
+namespace cnn {
+
+constexpr int kTile = 64;        // points per tile: one wave64 lane per row
+constexpr int kWave = 64;
+constexpr int kSpatialMaxD = 8;  // d <= 8: spatial (Morton) order + tile-box culling
+constexpr int kMfmaMinD = 16;    // d >= 16: FP32 Gram form on MFMA in CNN_F32_BAND
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// Device-memory arena with best-fit reuse.  Blocks stay allocated until trim();
+// all users enqueue on the context stream, so reuse is stream-ordered.
+class Arena {
+public:
+    explicit Arena(int device) : device_(device) {}
+    ~Arena() { trim(); }
+    void *get(size_t bytes);
+    void put(void *p);
+    void trim();
+    int64_t bytes() const { return total_; }
+
+private:
+    struct Block { void *p; size_t size; bool used; };
+    int device_;
+    std::vector<Block> blocks_;
+    int64_t total_ = 0;
+};
+
+}  // namespace cnn
+
+struct cnn_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    cnn::Arena *arena = nullptr;
+    std::string last_error;
+    hipEvent_t ev[8] = {};
+    int cu_count = 256;
+};
+
+struct cnn_graph {
+    int device = 0;
+    int64_t n = 0;
+    int32_t d = 0;
+    int32_t dtype = CNN_F32;
+    int32_t precision = CNN_PLAIN_F64;
+    double radius = 0.0;
+    double r2 = 0.0;
+    int32_t layout = 0;          // 0 identity, 1 spatial order
+    int64_t n_tiles = 0;
+    int64_t nnz = 0;
+    // device buffers owned by the graph
+    int32_t *perm = nullptr;     // working -> caller index (nullptr when identity)
+    double *xw64 = nullptr;      // working-order coordinates, FP64, n_tiles*64 rows
+    float *xw32 = nullptr;       // working-order coordinates, FP32 (band precision)
+    int64_t *row_ptr = nullptr;  // working CSR
+    int32_t *col = nullptr;
+    int64_t bytes = 0;
+    cnn_graph_stats stats{};
+};
+
+namespace cnn {
+
+// RAII scratch buffer from the context arena.
+template <typename T>
+struct Scratch {
+    cnn_ctx *ctx;
+    T *p = nullptr;
+    Scratch(cnn_ctx *c, size_t count) : ctx(c) {
+        if (count == 0) count = 1;
+        p = static_cast<T *>(ctx->arena->get(count * sizeof(T)));
+    }
+    ~Scratch() { if (p) ctx->arena->put(p); }
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+};
+
+struct HipFailure {
+    hipError_t err;
+    const char *what;
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw HipFailure{e, what};
+}
+
+#define CNN_HIP(call) ::cnn::hip_check((call), #call)
+
+struct InvalidArg { std::string msg; };
+struct OutOfMemory { std::string msg; };
+struct Capacity { std::string msg; };
+
+// Convert exceptions raised inside an entry point into a status code.
+template <typename F>
+int guarded(cnn_ctx *ctx, F &&f) {
+    try {
+        return f();
+    } catch (const HipFailure &h) {
+        if (ctx) {
+            ctx->last_error = std::string(h.what) + ": " + hipGetErrorString(h.err);
+            (void)hipGetLastError();
+        }
+        return h.err == hipErrorOutOfMemory ? CNN_ERR_OOM : CNN_ERR_HIP;
+    } catch (const InvalidArg &e) {
+        if (ctx) ctx->last_error = e.msg;
+        return CNN_ERR_INVALID;
+    } catch (const OutOfMemory &e) {
+        if (ctx) ctx->last_error = e.msg;
+        return CNN_ERR_OOM;
+    } catch (const Capacity &e) {
+        if (ctx) ctx->last_error = e.msg;
+        return CNN_ERR_CAPACITY;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        return CNN_ERR_HIP;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown failure";
+        return CNN_ERR_HIP;
+    }
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// --------------------------------------------------------------- host helpers
+void exclusive_scan_i32_to_i64(cnn_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+void exclusive_scan_i64(cnn_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
+void inclusive_scan_u8_to_i32(cnn_ctx *ctx, const uint8_t *in, int32_t *out, int64_t n);
+void sort_pairs_u64_i32(cnn_ctx *ctx, uint64_t *keys, int32_t *vals, int64_t n, int end_bit);
+void sort_pairs_i64_i32(cnn_ctx *ctx, int64_t *keys, int32_t *vals, int64_t n);
+void segmented_sort_i32(cnn_ctx *ctx, int32_t *keys, int64_t nnz, const int64_t *offsets,
+                        int64_t n_segments, int end_bit);
+void fill_i32(cnn_ctx *ctx, int32_t *p, int32_t v, int64_t n);
+void fill_u8(cnn_ctx *ctx, uint8_t *p, uint8_t v, int64_t n);
+void fill_i64(cnn_ctx *ctx, int64_t *p, int64_t v, int64_t n);
+
+template <typename T>
+T read_scalar(cnn_ctx *ctx, const T *dev) {
+    T h{};
+    CNN_HIP(hipMemcpyAsync(&h, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    CNN_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+// --------------------------------------------------------------- graph / labels
+cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+                       double r2, int32_t precision);
+void graph_free(cnn_graph *g);
+void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col);
+cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);
+
+struct LabelRequest {
+    const int64_t *row_ptr;
+    const int32_t *col;
+    int64_t n;
+    const int32_t *perm;   // working -> caller index, nullptr for identity
+    int32_t c_sim;
+    int32_t m_core;
+    uint8_t *consider;     // caller order, nullable
+    int64_t current_start;
+    int32_t sim_mode;
+    bool sort;             // sort_by_size on the fresh labels
+    bool write_all;        // write every label (0 for noise) instead of members only
+    int64_t member_cutoff;
+    int64_t max_clusters;
+    int64_t *labels;       // caller order
+};
+void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st);
+int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,
+                             int64_t max_clusters);
+
+}  // namespace cnn
+
+// --------------------------------------------------------------- device helpers
+#if defined(__HIPCC__)
+namespace cnn {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int atomic_load_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace cnn
+#endif

@@ -1,0 +1,601 @@
+// Phases S and U: common-neighbour similarity, union-find labelling, sort_by_size.
+//
+// Replaces FitterExtBFS._fit (_fit.pyx:408-491) with its connected-components
+// equivalent: core(i) = consider(i) && |N(i)| > m_core (_types.pyx:1561-1564);
+// edge(i, j) = core(i) && core(j) && j in N(i) && |N(i) & N(j)| >= c_sim
+// (SimilarityCheckerExtScreensorted._check, _types.pyx:2654-2706, early exit at
+// c_sim).  The BFS opens clusters in ascending init-point order and never
+// reaches an earlier cluster through a valid edge, so its labels are the
+// components of (core points, edges) numbered by smallest point index; a
+// union-find whose links always point from the larger to the smaller root
+// yields the same partition regardless of scheduling.
+//
+// Work plan on the GPU (all on the context stream):
+//   k_init      core flags, parent[i] = i
+//   k_round_a   each core row tests its nearest core neighbours in list order
+//               (sampling; links most of every dense region cheaply)
+//   k_compress  parent[i] = root(i)
+//   k_largest   most populated root L; inL[i] snapshot
+//   k_round_b   every core row outside L tests each candidate edge whose
+//               endpoints are still in different components (edges inside L
+//               are already joined; edges between L and the rest are seen from
+//               the non-L endpoint; others from the smaller endpoint)
+//   k_compress, component statistics, label assignment / sort_by_size
+#include <algorithm>
+#include <climits>
+#include <memory>
+
+#include "cnn_internal.hpp"
+
+namespace cnn {
+
+// ------------------------------------------------------------------ union-find
+
+__device__ __forceinline__ int uf_find(int *parent, int x) {
+    int p = atomic_load_relaxed(&parent[x]);
+    while (p != x) {
+        int gp = atomic_load_relaxed(&parent[p]);
+        if (gp == p) return p;
+        __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // path halving
+        x = gp;
+        p = atomic_load_relaxed(&parent[x]);
+    }
+    return x;
+}
+
+// Link the larger root below the smaller one (parent[x] <= x always holds).
+__device__ __forceinline__ void uf_union(int *parent, int a, int b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }
+        int old = atomicCAS(&parent[a], a, b);
+        if (old == a) return;
+    }
+}
+
+// ------------------------------------------------------------------ similarity
+
+// |A & B| >= c for sorted, duplicate-free lists, evaluated by the whole wave:
+// 64-wide windows of both lists advance like the two-pointer merge; each lane
+// looks its B element up in the A window by a shuffle binary search.  Every
+// common element <= min(last A, last B) lies in both windows, so each is
+// counted once.  Wave-uniform result.
+__device__ bool wave_sim(const int32_t *__restrict__ A, int64_t na, const int32_t *__restrict__ B, int64_t nb,
+                         int c) {
+    if (c <= 0) return true;
+    if (na < c || nb < c) return false;
+    const int lane = __lane_id();
+    int64_t ia = 0, ib = 0;
+    int count = 0;
+    while (ia < na && ib < nb) {
+        const int va = (int)min<int64_t>(64, na - ia);
+        const int vb = (int)min<int64_t>(64, nb - ib);
+        const int a = lane < va ? A[ia + lane] : INT_MAX;
+        const int b = lane < vb ? B[ib + lane] : INT_MAX;
+        const int alast = __shfl(a, va - 1, 64);
+        const int blast = __shfl(b, vb - 1, 64);
+        const int m = min(alast, blast);
+        int lo = 0, hi = va;
+#pragma unroll
+        for (int step = 0; step < 7; ++step) {
+            const int mid = (lo + hi) >> 1;
+            const int am = __shfl(a, min(mid, 63), 64);
+            if (lo < hi) {
+                if (am < b) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        const int found = __shfl(a, min(lo, 63), 64);
+        const bool hit = (lane < vb) && (b <= m) && (lo < va) && (found == b);
+        count += __popcll(__ballot(hit));
+        if (count >= c) return true;
+        ia += __popcll(__ballot(lane < va && a <= m));
+        ib += __popcll(__ballot(lane < vb && b <= m));
+        if ((int64_t)count + min(na - ia, nb - ib) < (int64_t)c) return false;
+    }
+    return false;
+}
+
+struct LabelArgs {
+    const int64_t *row_ptr;
+    const int32_t *col;
+    int64_t n;
+    int c_sim;
+    int m_core;
+    uint8_t *core;
+    uint8_t *inL;
+    int *parent;
+    unsigned long long *stats;  // [0] tests, [1] passed
+};
+
+__global__ void k_init(LabelArgs a, const uint8_t *__restrict__ consider, const int32_t *__restrict__ perm) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.n) return;
+    const int64_t deg = a.row_ptr[w + 1] - a.row_ptr[w];
+    bool ok = deg > (int64_t)a.m_core;
+    if (consider) ok = ok && consider[perm ? perm[w] : w] != 0;
+    a.core[w] = ok ? 1 : 0;
+    a.parent[w] = (int)w;
+}
+
+__device__ __forceinline__ void test_and_link(const LabelArgs &a, int w, int j, int64_t bw, int64_t ew,
+                                              unsigned &tests, unsigned &passed) {
+    const int rw = uf_find(a.parent, w);
+    const int rj = uf_find(a.parent, j);
+    if (rw == rj) return;
+    const int64_t bj = a.row_ptr[j], ej = a.row_ptr[j + 1];
+    const bool pass = wave_sim(a.col + bw, ew - bw, a.col + bj, ej - bj, a.c_sim);
+    ++tests;
+    if (pass) {
+        ++passed;
+        if (__lane_id() == 0) uf_union(a.parent, w, j);
+    }
+}
+
+// Sampling round: test the first core neighbour after and before the row's own
+// position in its sorted list (spatially closest in the working order).
+__global__ __launch_bounds__(256) void k_round_a(LabelArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= a.n || !a.core[w]) return;
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    int64_t lo = b, hi = e;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a.col[mid] < (int32_t)w) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t p = lo;
+    unsigned tests = 0, passed = 0;
+    // forward
+    for (int64_t t0 = p; t0 < e && t0 < p + 256; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const int j = t < e ? a.col[t] : -1;
+        const bool cand = (t < e) && j != (int)w && a.core[j];
+        const uint64_t bits = __ballot(cand);
+        if (bits) {
+            const int jj = __shfl(j, __ffsll((unsigned long long)bits) - 1, 64);
+            test_and_link(a, (int)w, jj, b, e, tests, passed);
+            break;
+        }
+    }
+    // backward
+    for (int64_t t1 = p; t1 > b && t1 > p - 256; t1 -= 64) {
+        const int64_t t = t1 - 1 - lane;
+        const int j = t >= b ? a.col[t] : -1;
+        const bool cand = (t >= b) && j != (int)w && a.core[j];
+        const uint64_t bits = __ballot(cand);
+        if (bits) {
+            const int jj = __shfl(j, __ffsll((unsigned long long)bits) - 1, 64);
+            test_and_link(a, (int)w, jj, b, e, tests, passed);
+            break;
+        }
+    }
+    if (lane == 0 && tests) {
+        atomicAdd(&a.stats[0], (unsigned long long)tests);
+        atomicAdd(&a.stats[1], (unsigned long long)passed);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= a.n || !a.core[w] || a.inL[w]) return;
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    unsigned tests = 0, passed = 0;
+    for (int64_t t0 = b; t0 < e; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const int j = t < e ? a.col[t] : -1;
+        bool cand = (t < e) && j != (int)w && a.core[j] && (j > (int)w || a.inL[j]);
+        if (cand) cand = uf_find(a.parent, (int)w) != uf_find(a.parent, j);
+        uint64_t bits = __ballot(cand);
+        while (bits) {
+            const int src = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int jj = __shfl(j, src, 64);
+            test_and_link(a, (int)w, jj, b, e, tests, passed);
+        }
+    }
+    if (lane == 0 && tests) {
+        atomicAdd(&a.stats[0], (unsigned long long)tests);
+        atomicAdd(&a.stats[1], (unsigned long long)passed);
+    }
+}
+
+__global__ void k_compress(int *parent, int64_t n) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    parent[w] = uf_find(parent, (int)w);
+}
+
+__global__ void k_root_count(const uint8_t *core, const int *parent, int64_t n, int32_t *cnt) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n || !core[w]) return;
+    atomicAdd(&cnt[parent[w]], 1);
+}
+
+__global__ void k_largest(const int32_t *cnt, int64_t n, unsigned long long *best) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (w < n && cnt[w] > 0) v = ((unsigned long long)(uint32_t)cnt[w] << 32) | (uint32_t)(~(uint32_t)w);
+    // wave max then one atomic per wave
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(best, v);
+}
+
+__global__ void k_mark_largest(const uint8_t *core, const int *parent, int64_t n, const unsigned long long *best,
+                               uint8_t *inL) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const unsigned long long b = *best;
+    const int L = b ? (int)(~(uint32_t)(b & 0xffffffffull)) : -1;
+    inL[w] = (core[w] && parent[w] == L) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ labels
+
+__global__ void k_comp_stats(const uint8_t *core, const int *parent, const int32_t *perm, int64_t n, int32_t *cmin,
+                             int32_t *csize, unsigned long long *counters) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool is_core = false, is_root = false;
+    if (w < n && core[w]) {
+        is_core = true;
+        const int r = parent[w];
+        is_root = r == (int)w;
+        atomicMin(&cmin[r], perm ? perm[w] : (int32_t)w);
+        atomicAdd(&csize[r], 1);
+    }
+    const uint64_t bc = __ballot(is_core), br = __ballot(is_root);
+    if ((threadIdx.x & 63) == 0) {
+        if (bc) atomicAdd(&counters[0], (unsigned long long)__popcll(bc));
+        if (br) atomicAdd(&counters[1], (unsigned long long)__popcll(br));
+    }
+}
+
+__global__ void k_collect_clusters(const uint8_t *core, const int *parent, const int32_t *cmin, const int32_t *csize,
+                                   int64_t n, int64_t member_cutoff, unsigned long long *K, uint64_t *keys,
+                                   int32_t *vals) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n || !core[w] || parent[w] != (int)w) return;
+    const int32_t sz = csize[w];
+    if ((int64_t)sz < member_cutoff) return;
+    const unsigned long long idx = atomicAdd(K, 1ull);
+    keys[idx] = ((uint64_t)(uint32_t)(0x7fffffff - sz) << 32) | (uint32_t)cmin[w];
+    vals[idx] = (int32_t)w;
+}
+
+__global__ void k_rank_clusters(const int32_t *vals, int64_t K, int64_t max_clusters, int32_t *newlab) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K) return;
+    newlab[vals[t]] = (max_clusters < 0 || t < max_clusters) ? (int32_t)(t + 1) : 0;
+}
+
+__global__ void k_write_sorted(const uint8_t *core, const int *parent, const int32_t *newlab, const int32_t *perm,
+                               int64_t n, int64_t *labels) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int64_t o = perm ? perm[w] : w;
+    labels[o] = core[w] ? (int64_t)newlab[parent[w]] : 0;
+}
+
+__global__ void k_mark_first(const uint8_t *core, const int *parent, const int32_t *cmin, int64_t n,
+                             uint8_t *isfirst) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n || !core[w] || parent[w] != (int)w) return;
+    isfirst[cmin[w]] = 1;
+}
+
+__global__ void k_write_unsorted(const uint8_t *core, const int *parent, const int32_t *cmin, const int32_t *rank_incl,
+                                 const int32_t *perm, int64_t n, int64_t current_start, bool write_all,
+                                 int64_t *labels) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int64_t o = perm ? perm[w] : w;
+    if (core[w]) labels[o] = current_start + (int64_t)rank_incl[cmin[parent[w]]] - 1;
+    else if (write_all) labels[o] = 0;
+}
+
+static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, ceil_div(n, per)); }
+
+void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = req.n;
+    if (req.c_sim < 0 || req.m_core < 0) throw InvalidArg{"c_sim and m_core must be >= 0"};
+    if (req.sim_mode == CNN_SIM_BITSET) throw InvalidArg{"CNN_SIM_BITSET is not available; use CNN_SIM_AUTO or CNN_SIM_MERGE"};
+    if (req.sim_mode != CNN_SIM_AUTO && req.sim_mode != CNN_SIM_MERGE) throw InvalidArg{"unknown sim_mode"};
+    cnn_label_stats local{};
+    if (n == 0) {
+        if (st) *st = local;
+        return;
+    }
+    Scratch<uint8_t> core(ctx, n), inL(ctx, n);
+    Scratch<int> parent(ctx, n);
+    Scratch<unsigned long long> counters(ctx, 8);
+    CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
+    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, inL.p, parent.p, counters.p};
+
+    CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
+    CNN_HIP(hipGetLastError());
+    k_round_a<<<blocks_for(n, 4), 256, 0, s>>>(a);
+    CNN_HIP(hipGetLastError());
+    k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    {
+        Scratch<int32_t> cnt(ctx, n);
+        fill_i32(ctx, cnt.p, 0, n);
+        k_root_count<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, n, cnt.p);
+        k_largest<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters.p + 4);
+        k_mark_largest<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, n, counters.p + 4, inL.p);
+        CNN_HIP(hipGetLastError());
+    }
+    k_round_b<<<blocks_for(n, 4), 256, 0, s>>>(a);
+    CNN_HIP(hipGetLastError());
+    k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    CNN_HIP(hipEventRecord(ctx->ev[1], s));
+
+    Scratch<int32_t> cmin(ctx, n), csize(ctx, n);
+    fill_i32(ctx, cmin.p, INT_MAX, n);
+    fill_i32(ctx, csize.p, 0, n);
+    k_comp_stats<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, req.perm, n, cmin.p, csize.p, counters.p + 2);
+    CNN_HIP(hipGetLastError());
+    if (req.sort) {
+        Scratch<uint64_t> keys(ctx, n);
+        Scratch<int32_t> vals(ctx, n), newlab(ctx, n);
+        fill_i32(ctx, newlab.p, 0, n);
+        k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n, req.member_cutoff,
+                                                         counters.p + 5, keys.p, vals.p);
+        CNN_HIP(hipGetLastError());
+        const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
+        sort_pairs_u64_i32(ctx, keys.p, vals.p, K, 63);
+        if (K > 0) k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(vals.p, K, req.max_clusters, newlab.p);
+        k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
+        CNN_HIP(hipGetLastError());
+        local.n_clusters = (req.max_clusters >= 0) ? std::min<int64_t>(K, req.max_clusters) : K;
+    } else {
+        Scratch<uint8_t> isfirst(ctx, n);
+        Scratch<int32_t> rank(ctx, n);
+        fill_u8(ctx, isfirst.p, 0, n);
+        k_mark_first<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, n, isfirst.p);
+        CNN_HIP(hipGetLastError());
+        inclusive_scan_u8_to_i32(ctx, isfirst.p, rank.p, n);
+        k_write_unsorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, rank.p, req.perm, n, req.current_start,
+                                                       req.write_all, req.labels);
+        CNN_HIP(hipGetLastError());
+    }
+    if (req.consider) fill_u8(ctx, req.consider, 0, n);
+    CNN_HIP(hipEventRecord(ctx->ev[2], s));
+    unsigned long long h[8];
+    CNN_HIP(hipMemcpyAsync(h, counters.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipStreamSynchronize(s));
+    float t1 = 0, t2 = 0;
+    CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
+    CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
+    local.tests = (int64_t)h[0];
+    local.tests_passed = (int64_t)h[1];
+    local.n_core = (int64_t)h[2];
+    local.n_components = (int64_t)h[3];
+    if (!req.sort) local.n_clusters = local.n_components;
+    local.ms_similarity = t1;
+    local.ms_labels = t2;
+    if (st) *st = local;
+}
+
+// ------------------------------------------------------------------ caller CSR validation
+
+// Flags rows that are not strictly ascending or hold indices outside [0, n),
+// and offsets that decrease.  Precondition (caller contract): col holds at
+// least row_ptr[n] entries and row_ptr is n+1 long.
+__global__ void k_validate_csr(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t n,
+                               int64_t nnz_cap, unsigned long long *bad) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    if (b < 0 || e < b || e > nnz_cap) { atomicOr(bad, 1ull); return; }
+    int32_t prev = -1;
+    for (int64_t t = b; t < e; ++t) {
+        const int32_t j = col[t];
+        if (j <= prev || (int64_t)j >= n) { atomicOr(bad, 2ull); return; }
+        prev = j;
+    }
+}
+
+static void validate_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n) {
+    if (n == 0) return;
+    hipStream_t s = ctx->stream;
+    const int64_t first = read_scalar(ctx, row_ptr);
+    const int64_t nnz = read_scalar(ctx, row_ptr + n);
+    if (first != 0 || nnz < 0) throw InvalidArg{"row_ptr must start at 0 and end at nnz >= 0"};
+    Scratch<unsigned long long> bad(ctx, 1);
+    CNN_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
+    k_validate_csr<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(row_ptr, col, n, nnz, bad.p);
+    CNN_HIP(hipGetLastError());
+    const unsigned long long flags = read_scalar(ctx, bad.p);
+    if (flags & 1ull) throw InvalidArg{"row_ptr is not non-decreasing within [0, nnz]"};
+    if (flags & 2ull) throw InvalidArg{"neighbour rows must be strictly ascending with indices in [0, n)"};
+}
+
+// ------------------------------------------------------------------ generic sort_by_size
+
+__global__ void k_iota(int32_t *v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (int32_t)i;
+}
+
+__global__ void k_heads(const int64_t *keys, int64_t n, uint8_t *head) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    head[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1 : 0;
+}
+
+__global__ void k_segments(const int64_t *keys, const int32_t *vals, const int32_t *segid, int64_t n,
+                           int32_t *seg_start, int32_t *seg_end) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int sid = segid[t] - 1;
+    if (t == 0 || keys[t] != keys[t - 1]) seg_start[sid] = (int32_t)t;
+    if (t == n - 1 || keys[t + 1] != keys[t]) seg_end[sid] = (int32_t)t;
+}
+
+__global__ void k_seg_collect(const int64_t *keys, const int32_t *vals, const int32_t *seg_start,
+                              const int32_t *seg_end, int64_t n_seg, unsigned long long *K, uint64_t *okeys,
+                              int32_t *ovals) {
+    int64_t sid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid >= n_seg) return;
+    const int32_t b = seg_start[sid];
+    if (keys[b] == 0) return;
+    const int32_t sz = seg_end[sid] - b + 1;
+    const unsigned long long idx = atomicAdd(K, 1ull);
+    okeys[idx] = ((uint64_t)(uint32_t)(0x7fffffff - sz) << 32) | (uint32_t)vals[b];
+    ovals[idx] = (int32_t)sid;
+}
+
+__global__ void k_seg_rank(const uint64_t *okeys, const int32_t *ovals, int64_t K, int64_t member_cutoff,
+                           int64_t max_clusters, int32_t *seglab) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K) return;
+    const int64_t sz = 0x7fffffff - (int64_t)(okeys[t] >> 32);
+    const bool keep = (max_clusters < 0 || t < max_clusters) && sz >= member_cutoff;
+    seglab[ovals[t]] = keep ? (int32_t)(t + 1) : 0;
+}
+
+__global__ void k_seg_write(const int32_t *vals, const int32_t *segid, const int32_t *seglab, int64_t n,
+                            int64_t *labels) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    labels[vals[t]] = seglab[segid[t] - 1];
+}
+
+int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff, int64_t max_clusters) {
+    hipStream_t s = ctx->stream;
+    if (n == 0) return 0;
+    if (n >= (int64_t)INT32_MAX) throw InvalidArg{"n exceeds the int32 index range"};
+    Scratch<int64_t> keys(ctx, n);
+    Scratch<int32_t> vals(ctx, n), segid(ctx, n);
+    Scratch<uint8_t> head(ctx, n);
+    CNN_HIP(hipMemcpyAsync(keys.p, labels, n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    k_iota<<<blocks_for(n), 256, 0, s>>>(vals.p, n);
+    CNN_HIP(hipGetLastError());
+    sort_pairs_i64_i32(ctx, keys.p, vals.p, n);  // stable: equal labels keep ascending index
+    k_heads<<<blocks_for(n), 256, 0, s>>>(keys.p, n, head.p);
+    inclusive_scan_u8_to_i32(ctx, head.p, segid.p, n);
+    const int64_t n_seg = read_scalar(ctx, segid.p + (n - 1));
+    Scratch<int32_t> seg_start(ctx, n_seg), seg_end(ctx, n_seg), seglab(ctx, n_seg);
+    k_segments<<<blocks_for(n), 256, 0, s>>>(keys.p, vals.p, segid.p, n, seg_start.p, seg_end.p);
+    fill_i32(ctx, seglab.p, 0, n_seg);
+    Scratch<unsigned long long> K(ctx, 1);
+    CNN_HIP(hipMemsetAsync(K.p, 0, sizeof(unsigned long long), s));
+    Scratch<uint64_t> okeys(ctx, n_seg);
+    Scratch<int32_t> ovals(ctx, n_seg);
+    k_seg_collect<<<blocks_for(n_seg), 256, 0, s>>>(keys.p, vals.p, seg_start.p, seg_end.p, n_seg, K.p, okeys.p,
+                                                    ovals.p);
+    CNN_HIP(hipGetLastError());
+    const int64_t k = (int64_t)read_scalar(ctx, K.p);
+    sort_pairs_u64_i32(ctx, okeys.p, ovals.p, k, 63);
+    if (k > 0) k_seg_rank<<<blocks_for(k), 256, 0, s>>>(okeys.p, ovals.p, k, member_cutoff, max_clusters, seglab.p);
+    k_seg_write<<<blocks_for(n), 256, 0, s>>>(vals.p, segid.p, seglab.p, n, labels);
+    CNN_HIP(hipGetLastError());
+    // number of survivors
+    int64_t kept = 0;
+    if (k > 0) {
+        std::vector<uint64_t> hk(k);
+        CNN_HIP(hipMemcpyAsync(hk.data(), okeys.p, k * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+        for (int64_t t = 0; t < k; ++t) {
+            const int64_t sz = 0x7fffffff - (int64_t)(hk[t] >> 32);
+            if ((max_clusters < 0 || t < max_clusters) && sz >= member_cutoff) ++kept;
+        }
+    } else {
+        CNN_HIP(hipStreamSynchronize(s));
+    }
+    return kept;
+}
+
+}  // namespace cnn
+
+// ================================================================== C-ABI: phase S+U
+using namespace cnn;
+
+extern "C" {
+
+int cnn_labels_from_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n, int32_t c_sim,
+                        int32_t m_core, uint8_t *consider, int64_t current_start, int32_t sim_mode, int64_t *labels,
+                        cnn_label_stats *stats_out) {
+    if (!ctx) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (n < 0) throw InvalidArg{"negative n"};
+        if (n > 0 && (!row_ptr || !col || !labels)) throw InvalidArg{"null buffer"};
+        if (n >= (int64_t)INT32_MAX) throw InvalidArg{"n exceeds the int32 index range"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        validate_csr(ctx, row_ptr, col, n);
+        LabelRequest r{row_ptr, col, n, nullptr, c_sim, m_core, consider, current_start, sim_mode,
+                       false, false, 2, -1, labels};
+        labels_run(ctx, r, stats_out);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_core, uint8_t *consider,
+                     int64_t current_start, int32_t sim_mode, int64_t *labels, cnn_label_stats *stats_out) {
+    if (!ctx || !g) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, consider, current_start, sim_mode,
+                       false, false, 2, -1, labels};
+        labels_run(ctx, r, stats_out);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_sort_by_size(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff, int64_t max_clusters,
+                     int64_t *n_clusters_out) {
+    if (!ctx) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (n < 0) throw InvalidArg{"negative n"};
+        if (n > 0 && !labels) throw InvalidArg{"null labels"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        int64_t k = sort_by_size_generic(ctx, labels, n, member_cutoff, max_clusters);
+        if (n_clusters_out) *n_clusters_out = k;
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_core, int32_t sort,
+                  int64_t member_cutoff, int64_t max_clusters, int64_t *labels, cnn_label_stats *lstats) {
+    if (!ctx || !g) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
+                       sort != 0, true, member_cutoff, max_clusters, labels};
+        labels_run(ctx, r, lstats);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2, int32_t c_sim,
+            int32_t m_core, int32_t precision, int32_t sort, int64_t member_cutoff, int64_t max_clusters,
+            int64_t *labels, cnn_graph **graph_out, cnn_graph_stats *gstats, cnn_label_stats *lstats) {
+    if (!ctx) return CNN_ERR_INVALID;
+    if (graph_out) *graph_out = nullptr;
+    return guarded(ctx, [&]() {
+        if (n > 0 && !labels) throw InvalidArg{"null labels"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision);
+        std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        if (gstats) *gstats = g->stats;
+        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
+                       sort != 0, true, member_cutoff, max_clusters, labels};
+        labels_run(ctx, r, lstats);
+        if (graph_out) *graph_out = hold.release();
+        return (int)CNN_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1049 @@
+// Phase N: radius neighbour graph.
+//
+// Replaces NeighboursGetterExtBruteForce._get (_types.pyx:1692-1710) with the
+// reduced Euclidean metric (_types.pyx:2253-2267, r2 = r*r from :2286-2287).
+//
+// Layout in HBM (working order = spatial Morton order for d <= 8, identity
+// otherwise):
+//   xw64[n_pad][d]   FP64 coordinates in working order (n_pad = 64 * n_tiles)
+//   xw32[n_pad][d]   FP32 coordinates (CNN_F32_BAND only)
+//   tp_ptr[R], tp_col candidate column tiles of row tile R (box-culled; implicit
+//                     "all tiles" for the dense layout)
+//   mask[k][64]      one 64-bit word per (tile pair k, row): bit b set when
+//                     column C*64+b is a neighbour of the row
+//   row_ptr[n+1] (int64), col[nnz] (int32): CSR in working order, rows ascending
+//
+// The distance decision is exact: plain FP64 kernels evaluate
+// t = a_k - b_k; s = s + t*t in ascending k with separately rounded operations
+// (this file is compiled with -ffp-contract=off); the FP32 kernels decide pairs
+// outside a guard band around r2 and recompute the band in that FP64 order.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <memory>
+#include <utility>
+
+#include "cnn_internal.hpp"
+
+namespace cnn {
+
+// ------------------------------------------------------------------ prep kernels
+
+template <typename Tin>
+__global__ void k_bbox_partial(const Tin *__restrict__ X, int64_t n, int d, double *__restrict__ part,
+                               double *__restrict__ part_norm) {
+    // part[block][k][0..1] = min/max of finite values of dim k; part_norm[block] = max ||x||^2
+    __shared__ double smin[256], smax[256];
+    __shared__ double snorm[256];
+    const int tid = threadIdx.x;
+    double nmax = 0.0;
+    for (int k = 0; k < d; ++k) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            double v = (double)X[i * d + k];
+            if (isfinite(v)) { lo = fmin(lo, v); hi = fmax(hi, v); }
+        }
+        smin[tid] = lo; smax[tid] = hi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) { smin[tid] = fmin(smin[tid], smin[tid + s]); smax[tid] = fmax(smax[tid], smax[tid + s]); }
+            __syncthreads();
+        }
+        if (tid == 0) { part[((int64_t)blockIdx.x * d + k) * 2] = smin[0]; part[((int64_t)blockIdx.x * d + k) * 2 + 1] = smax[0]; }
+        __syncthreads();
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < d; ++k) { double v = (double)X[i * d + k]; s += v * v; }
+        if (isfinite(s)) nmax = fmax(nmax, s);
+    }
+    snorm[tid] = nmax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) snorm[tid] = fmax(snorm[tid], snorm[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) part_norm[blockIdx.x] = snorm[0];
+}
+
+__device__ __forceinline__ uint64_t spread2(uint64_t x) {  // 32 bits -> even bits
+    x &= 0xffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+struct KeySpec {
+    int nk;         // dims in the key (1..3)
+    int dim[3];
+    double lo[3];
+    double scale[3];
+    double qmax;
+};
+
+template <typename Tin>
+__global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks, uint64_t *__restrict__ keys,
+                         int32_t *__restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t q[3] = {0, 0, 0};
+    for (int t = 0; t < ks.nk; ++t) {
+        double v = (double)X[i * d + ks.dim[t]];
+        double f = (v - ks.lo[t]) * ks.scale[t];
+        if (!(f >= 0.0)) f = 0.0;  // NaN / negative
+        if (f > ks.qmax) f = ks.qmax;
+        q[t] = (uint64_t)f;
+    }
+    uint64_t key;
+    if (ks.nk == 1) key = q[0];
+    else if (ks.nk == 2) key = spread2(q[0]) | (spread2(q[1]) << 1);
+    else key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    keys[i] = key;
+    vals[i] = (int32_t)i;
+}
+
+template <typename Tin>
+__global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n, int64_t n_pad,
+                         int d, double *__restrict__ xw64, float *__restrict__ xw32) {
+    const int64_t total = n_pad * d;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t w = idx / d;
+        int k = (int)(idx - w * d);
+        if (w < n) {
+            int64_t src = perm ? (int64_t)perm[w] : w;
+            Tin v = X[src * d + k];
+            xw64[idx] = (double)v;
+            if (xw32) xw32[idx] = (float)v;
+        } else {
+            xw64[idx] = 0.0;
+            if (xw32) xw32[idx] = 0.0f;
+        }
+    }
+}
+
+// Per-tile bounding boxes over all d dims (finite coordinates of valid rows).
+__global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, int64_t n_tiles,
+                             double *__restrict__ blo, double *__restrict__ bhi) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= n_tiles) return;
+    const int64_t i = t * 64 + lane;
+    for (int k = 0; k < d; ++k) {
+        double v = (i < n) ? xw64[i * d + k] : NAN;
+        double lo = isfinite(v) ? v : INFINITY;
+        double hi = isfinite(v) ? v : -INFINITY;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, off, 64));
+            hi = fmax(hi, __shfl_xor(hi, off, 64));
+        }
+        if (lane == 0) { blo[t * d + k] = lo; bhi[t * d + k] = hi; }
+    }
+}
+
+__device__ __forceinline__ bool tiles_close(const double *__restrict__ blo, const double *__restrict__ bhi, int d,
+                                            int64_t R, int64_t C, double thr) {
+    double s = 0.0;
+    for (int k = 0; k < d; ++k) {
+        double g = fmax(blo[C * d + k] - bhi[R * d + k], blo[R * d + k] - bhi[C * d + k]);
+        g = fmax(g, 0.0);
+        s += g * g;
+    }
+    return s <= thr;
+}
+
+// Count (fill == false) or list (fill == true) the candidate column tiles of each row tile.
+template <bool FILL>
+__global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi, int d, int64_t n_tiles,
+                             double thr, int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
+                             int32_t *__restrict__ tp_col) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    int64_t out = FILL ? tp_ptr[R] : 0;
+    for (int64_t C0 = 0; C0 < n_tiles; C0 += 64) {
+        const int64_t C = C0 + lane;
+        bool ok = (C < n_tiles) && tiles_close(blo, bhi, d, R, C, thr);
+        uint64_t b = __ballot(ok);
+        if (FILL && ok) {
+            int before = __popcll(b & ((1ull << lane) - 1ull));
+            tp_col[out + before] = (int32_t)C;
+        }
+        out += __popcll(b);
+    }
+    if (!FILL && lane == 0) cnt[R] = (int32_t)out;
+}
+
+// ------------------------------------------------------------------ distance kernels
+
+struct BandParams {
+    double r2;
+    float lo;    // FP32 d2 < lo: neighbour for certain
+    float hi;    // FP32 d2 > hi: not a neighbour for certain; otherwise recompute in FP64
+};
+
+struct DistArgs {
+    const double *xw64;
+    const float *xw32;
+    int64_t n;
+    int d;
+    int64_t R0;              // first row tile of this batch
+    int64_t n_tiles;
+    const int64_t *tp_ptr;   // explicit lists (nullptr for dense)
+    const int32_t *tp_col;
+    int64_t k_base;          // mask index of the batch's first tile pair
+    uint64_t *mask;
+    int32_t *deg;
+    BandParams bp;
+    unsigned long long *band_stats;  // [0] hits, [1] flips, [2] max err bits (double as u64)
+};
+
+__device__ __forceinline__ void tile_range(const DistArgs &a, int64_t R, int64_t &kb, int64_t &ke) {
+    if (a.tp_ptr) {
+        kb = a.tp_ptr[R];
+        ke = a.tp_ptr[R + 1];
+    } else {
+        kb = R * a.n_tiles;
+        ke = kb + a.n_tiles;
+    }
+}
+
+__device__ __forceinline__ int64_t tile_col(const DistArgs &a, int64_t k, int64_t kb) {
+    return a.tp_ptr ? (int64_t)a.tp_col[k] : (k - kb);
+}
+
+__device__ __forceinline__ uint64_t valid_cols(int64_t n, int64_t C) {
+    int64_t v = n - C * 64;
+    if (v >= 64) return ~0ull;
+    if (v <= 0) return 0ull;
+    return (1ull << v) - 1ull;
+}
+
+// Exact FP64 reduced distance in reference order (rows of the working layout).
+__device__ __forceinline__ double exact_d2(const double *__restrict__ xw64, int d, int64_t i, int64_t j) {
+    const double *a = xw64 + i * d;
+    const double *b = xw64 + j * d;
+    double s = 0.0;
+    for (int k = 0; k < d; ++k) {
+        double t = a[k] - b[k];
+        s = s + t * t;
+    }
+    return s;
+}
+
+__device__ __forceinline__ void atomic_max_double(unsigned long long *p, double v) {
+    // v >= 0: the IEEE bit pattern orders like the value
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Finish one wave's tile pair: apply validity, store the mask word, count.
+__device__ __forceinline__ int store_mask(const DistArgs &a, int64_t k, int64_t i, int64_t C, uint64_t m) {
+    m &= valid_cols(a.n, C);
+    if (i >= a.n) m = 0;
+    a.mask[(k - a.k_base) * 64 + (threadIdx.x & 63)] = m;
+    return __popcll(m);
+}
+
+__device__ __forceinline__ void finish_deg(const DistArgs &a, int64_t i, int mydeg) {
+    __shared__ int sdeg[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    sdeg[wave][lane] = mydeg;
+    __syncthreads();
+    if (wave == 0 && i < a.n) a.deg[i] = sdeg[0][lane] + sdeg[1][lane] + sdeg[2][lane] + sdeg[3][lane];
+}
+
+// Plain FP64, compile-time d (small d): row point in VGPRs, column points from
+// wave-uniform (scalar) loads.
+template <int D>
+__global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    double xi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = a.xw64[i * D + k];
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    int mydeg = 0;
+    for (int64_t k = kb + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const double *__restrict__ xc = a.xw64 + C * 64 * D;
+        uint64_t m = 0;
+#pragma unroll 16
+        for (int jj = 0; jj < 64; ++jj) {
+            double s = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 < D; ++k2) {
+                double t = xi[k2] - xc[jj * D + k2];
+                s = s + t * t;
+            }
+            m |= (uint64_t)(s <= a.bp.r2) << jj;
+        }
+        mydeg += store_mask(a, k, i, C, m);
+    }
+    finish_deg(a, i, mydeg);
+}
+
+// Plain FP64, runtime d: 64 per-column accumulators in registers, dims in
+// chunks of 8 (the per-pair summation stays in ascending k).
+__global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    const int d = a.d;
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    int mydeg = 0;
+    for (int64_t k = kb + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const double *__restrict__ xc = a.xw64 + C * 64 * d;
+        double acc[64];
+#pragma unroll
+        for (int jj = 0; jj < 64; ++jj) acc[jj] = 0.0;
+        for (int k0 = 0; k0 < d; k0 += 8) {
+            double xi[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) xi[kk] = (k0 + kk < d) ? a.xw64[i * d + k0 + kk] : 0.0;
+            const int kn = min(8, d - k0);
+#pragma unroll
+            for (int jj = 0; jj < 64; ++jj) {
+                const double *__restrict__ xj = xc + (int64_t)jj * d + k0;
+                double s = acc[jj];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    if (kk < kn) {
+                        double t = xi[kk] - xj[kk];
+                        s = s + t * t;
+                    }
+                }
+                acc[jj] = s;
+            }
+        }
+        uint64_t m = 0;
+#pragma unroll
+        for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= a.bp.r2) << jj;
+        mydeg += store_mask(a, k, i, C, m);
+    }
+    finish_deg(a, i, mydeg);
+}
+
+// Resolve FP32 decisions: bits certain below lo; band pairs recomputed in FP64.
+__device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, int64_t i, int64_t C, const float *acc,
+                                                 unsigned &hits, unsigned &flips, double &maxerr) {
+    uint64_t m = 0;
+#pragma unroll
+    for (int jj = 0; jj < 64; ++jj) {
+        float s = acc[jj];
+        bool acc_ok = s < a.bp.lo;
+        bool rej = s > a.bp.hi;
+        m |= (uint64_t)acc_ok << jj;
+        if (!acc_ok && !rej) {
+            int64_t j = C * 64 + jj;
+            if (i < a.n && j < a.n) {
+                double e = exact_d2(a.xw64, a.d, i, j);
+                bool in = e <= a.bp.r2;
+                m |= (uint64_t)in << jj;
+                ++hits;
+                flips += (in != ((double)s <= a.bp.r2));
+                double err = fabs((double)s - e);
+                if (err == err) maxerr = fmax(maxerr, err);
+            }
+        }
+    }
+    return m;
+}
+
+__device__ __forceinline__ void flush_band(const DistArgs &a, unsigned hits, unsigned flips, double maxerr) {
+    unsigned long long h = (unsigned long long)wave_sum((int)hits);
+    unsigned long long f = (unsigned long long)wave_sum((int)flips);
+    double mx = maxerr;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0 && h) {
+        atomicAdd(&a.band_stats[0], h);
+        atomicAdd(&a.band_stats[1], f);
+        atomic_max_double(&a.band_stats[2], mx);
+    }
+}
+
+// FP32 direct form, compile-time d: t = a - b; acc = fma(t, t, acc).
+template <int D>
+__global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    float xi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = a.xw32[i * D + k];
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    int mydeg = 0;
+    unsigned hits = 0, flips = 0;
+    double maxerr = 0.0;
+    for (int64_t k = kb + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const float *__restrict__ xc = a.xw32 + C * 64 * D;
+        float acc[64];
+#pragma unroll
+        for (int jj = 0; jj < 64; ++jj) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k2 = 0; k2 < D; ++k2) {
+                float t = xi[k2] - xc[jj * D + k2];
+                s = __builtin_fmaf(t, t, s);
+            }
+            acc[jj] = s;
+        }
+        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr);
+        mydeg += store_mask(a, k, i, C, m);
+    }
+    flush_band(a, hits, flips, maxerr);
+    finish_deg(a, i, mydeg);
+}
+
+// FP32 direct form, runtime d.
+__global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    const int d = a.d;
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    int mydeg = 0;
+    unsigned hits = 0, flips = 0;
+    double maxerr = 0.0;
+    for (int64_t k = kb + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const float *__restrict__ xc = a.xw32 + C * 64 * d;
+        float acc[64];
+#pragma unroll
+        for (int jj = 0; jj < 64; ++jj) acc[jj] = 0.0f;
+        for (int k0 = 0; k0 < d; k0 += 8) {
+            float xi[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) xi[kk] = (k0 + kk < d) ? a.xw32[i * d + k0 + kk] : 0.0f;
+            const int kn = min(8, d - k0);
+#pragma unroll
+            for (int jj = 0; jj < 64; ++jj) {
+                const float *__restrict__ xj = xc + (int64_t)jj * d + k0;
+                float s = acc[jj];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    if (kk < kn) {
+                        float t = xi[kk] - xj[kk];
+                        s = __builtin_fmaf(t, t, s);
+                    }
+                }
+                acc[jj] = s;
+            }
+        }
+        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr);
+        mydeg += store_mask(a, k, i, C, m);
+    }
+    flush_band(a, hits, flips, maxerr);
+    finish_deg(a, i, mydeg);
+}
+
+// ------------------------------------------------------------------ CSR emit
+
+// One wave per working row: lanes take consecutive tile pairs of the row tile,
+// a wave prefix sum places each lane's set bits.
+__global__ __launch_bounds__(256) void k_emit(DistArgs a, int64_t row_lo, int64_t row_hi,
+                                             const int64_t *__restrict__ row_ptr, int64_t col_base,
+                                             int32_t *__restrict__ col) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = row_lo + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= row_hi) return;
+    const int64_t R = w >> 6;
+    const int r = (int)(w & 63);
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    int64_t out = row_ptr[w] - col_base;
+    for (int64_t k0 = kb; k0 < ke; k0 += 64) {
+        const int64_t k = k0 + lane;
+        uint64_t m = (k < ke) ? a.mask[(k - a.k_base) * 64 + r] : 0ull;
+        const int c = __popcll(m);
+        const int incl = wave_incl_scan(c);
+        int64_t pos = out + incl - c;
+        const int64_t C = (k < ke) ? tile_col(a, k, kb) : 0;
+        while (m) {
+            int b = __ffsll((unsigned long long)m) - 1;
+            col[pos++] = (int32_t)(C * 64 + b);
+            m &= m - 1;
+        }
+        out += __shfl(incl, 63, 64);
+    }
+}
+
+__global__ void k_deg_from_rowptr_perm(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ perm,
+                                       int64_t n, int64_t *__restrict__ deg_o) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    deg_o[perm[w]] = row_ptr[w + 1] - row_ptr[w];
+}
+
+// Scatter working rows into caller order with caller column indices.
+__global__ __launch_bounds__(256) void k_scatter_rows(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                                                      const int32_t *__restrict__ perm, int64_t n,
+                                                      const int64_t *__restrict__ row_ptr_o, int32_t *__restrict__ col_o) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    const int64_t o = row_ptr_o[perm[w]];
+    for (int64_t t = b + lane; t < e; t += 64) col_o[o + (t - b)] = perm[col[t]];
+}
+
+__global__ void k_set_last(int64_t *p, int64_t idx, const int64_t *prev_excl, const int64_t *prev_val) {
+    p[idx] = prev_excl[idx - 1] + prev_val[idx - 1];
+}
+
+// ------------------------------------------------------------------ host side
+
+static double gamma_m(int m, double u) { return m * u / (1.0 - m * u); }
+
+// Guard band for the FP32 direct form.  |d2_fp32 - s64| <= E where, near the
+// threshold (magnitudes D <= 2 r2):
+//   FP32 accumulation of d rounded squares of rounded differences: gamma_{d+2}(u32) * D
+//   the reference's own FP64 rounding:                           gamma_{2d+2}(u64) * D
+//   rounding f64 coordinates to f32 (f64 input only):            2 u32 M sqrt(D) + u32^2 M^2
+// with M the largest point norm times two.  A safety factor 2 is applied.
+static BandParams make_band_direct(double r2, int d, bool rounded_coords, double max_norm) {
+    const double u32 = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
+    const double D = 2.0 * r2;
+    double E = (gamma_m(d + 2, u32) + gamma_m(2 * d + 2, u64)) * D;
+    if (rounded_coords) {
+        const double M = 2.0 * max_norm;
+        E += 2.0 * u32 * M * std::sqrt(D) + u32 * u32 * M * M;
+    }
+    E = 2.0 * E + 1e-300;
+    BandParams bp;
+    bp.r2 = r2;
+    double lo = r2 - E, hi = r2 + E;
+    float flo = (float)lo;
+    if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
+    float fhi = (hi >= (double)FLT_MAX) ? INFINITY : (float)hi;
+    if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
+    if (!(lo > 0.0)) flo = -1.0f;  // nothing certain: every pair at or below hi is rechecked
+    if (!(r2 == r2)) { flo = -1.0f; fhi = -2.0f; }  // NaN radius: no neighbours at all
+    bp.lo = flo;
+    bp.hi = fhi;
+    return bp;
+}
+
+static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band) {
+    dim3 grid((unsigned)n_rt), block(256);
+    hipStream_t s = ctx->stream;
+    if (!band) {
+        switch (a.d) {
+            case 1: k_dist_f64_small<1><<<grid, block, 0, s>>>(a); break;
+            case 2: k_dist_f64_small<2><<<grid, block, 0, s>>>(a); break;
+            case 3: k_dist_f64_small<3><<<grid, block, 0, s>>>(a); break;
+            case 4: k_dist_f64_small<4><<<grid, block, 0, s>>>(a); break;
+            default: k_dist_f64_generic<<<grid, block, 0, s>>>(a); break;
+        }
+    } else {
+        switch (a.d) {
+            case 1: k_dist_f32_small<1><<<grid, block, 0, s>>>(a); break;
+            case 2: k_dist_f32_small<2><<<grid, block, 0, s>>>(a); break;
+            case 3: k_dist_f32_small<3><<<grid, block, 0, s>>>(a); break;
+            case 4: k_dist_f32_small<4><<<grid, block, 0, s>>>(a); break;
+            default: k_dist_f32_generic<<<grid, block, 0, s>>>(a); break;
+        }
+    }
+    CNN_HIP(hipGetLastError());
+}
+
+template <typename T>
+static T *dev_alloc(cnn_graph *g, int64_t count) {
+    void *p = nullptr;
+    size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw OutOfMemory{"graph allocation of " + std::to_string(bytes) + " bytes failed"};
+    }
+    g->bytes += (int64_t)bytes;
+    return (T *)p;
+}
+
+void graph_free(cnn_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->perm) (void)hipFree(g->perm);
+    if (g->xw64) (void)hipFree(g->xw64);
+    if (g->xw32) (void)hipFree(g->xw32);
+    if (g->row_ptr) (void)hipFree(g->row_ptr);
+    if (g->col) (void)hipFree(g->col);
+    delete g;
+}
+
+struct GraphGuard {
+    cnn_graph *g;
+    ~GraphGuard() { if (g) graph_free(g); }
+};
+
+cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
+                       int32_t precision) {
+    if (n < 0 || d < 0) throw InvalidArg{"negative size"};
+    if (n > 0 && d > 0 && !X) throw InvalidArg{"X is null"};
+    if (n >= (int64_t)INT32_MAX - 64) throw InvalidArg{"n exceeds the int32 index range"};
+    if (dtype != CNN_F32 && dtype != CNN_F64) throw InvalidArg{"dtype must be CNN_F32 or CNN_F64"};
+    if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
+    CNN_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    GraphGuard guard{new cnn_graph()};
+    cnn_graph *g = guard.g;
+    g->device = ctx->device;
+    g->n = n;
+    g->d = d;
+    g->dtype = dtype;
+    g->precision = precision;
+    g->r2 = r2;
+    g->radius = std::sqrt(r2);
+    g->n_tiles = ceil_div(n, 64);
+    const int64_t n_pad = g->n_tiles * 64;
+    const bool band = precision == CNN_F32_BAND;
+
+    CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    g->row_ptr = dev_alloc<int64_t>(g, n + 1);
+    if (n == 0) {
+        fill_i64(ctx, g->row_ptr, 0, 1);
+        g->col = dev_alloc<int32_t>(g, 1);
+        CNN_HIP(hipStreamSynchronize(s));
+        g->stats = cnn_graph_stats{};
+        return std::exchange(guard.g, nullptr);
+    }
+
+    // bounding box and largest norm (host needs them for ordering and the band)
+    const int nb = (int)std::min<int64_t>(256, ceil_div(n, 256));
+    std::vector<double> part((size_t)nb * std::max(d, 1) * 2), pnorm(nb);
+    {
+        Scratch<double> dpart(ctx, part.size()), dnorm(ctx, nb);
+        if (dtype == CNN_F32)
+            k_bbox_partial<float><<<nb, 256, 0, s>>>((const float *)X, n, d, dpart.p, dnorm.p);
+        else
+            k_bbox_partial<double><<<nb, 256, 0, s>>>((const double *)X, n, d, dpart.p, dnorm.p);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipMemcpyAsync(pnorm.data(), dnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<double> lo(d, INFINITY), hi(d, -INFINITY);
+    double max_norm2 = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        for (int k = 0; k < d; ++k) {
+            lo[k] = std::fmin(lo[k], part[((size_t)b * d + k) * 2]);
+            hi[k] = std::fmax(hi[k], part[((size_t)b * d + k) * 2 + 1]);
+        }
+        max_norm2 = std::fmax(max_norm2, pnorm[b]);
+    }
+
+    // ordering: Morton order over the (up to) three widest dims for small d
+    g->layout = (d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
+    if (g->layout == 1) {
+        KeySpec ks{};
+        std::vector<int> dims(d);
+        for (int k = 0; k < d; ++k) dims[k] = k;
+        auto ext = [&](int k) { return (hi[k] >= lo[k]) ? hi[k] - lo[k] : 0.0; };
+        std::stable_sort(dims.begin(), dims.end(), [&](int x, int y) { return ext(x) > ext(y); });
+        ks.nk = std::min(d, 3);
+        const int bits = ks.nk == 1 ? 62 : (ks.nk == 2 ? 32 : 21);
+        ks.qmax = std::ldexp(1.0, bits) - 1.0;
+        for (int t = 0; t < ks.nk; ++t) {
+            int k = dims[t];
+            ks.dim[t] = k;
+            double e = ext(k);
+            ks.lo[t] = std::isfinite(lo[k]) ? lo[k] : 0.0;
+            ks.scale[t] = (e > 0.0 && std::isfinite(e)) ? ks.qmax / e : 0.0;
+        }
+        g->perm = dev_alloc<int32_t>(g, n);
+        Scratch<uint64_t> keys(ctx, n);
+        const unsigned blocks = (unsigned)ceil_div(n, 256);
+        if (dtype == CNN_F32)
+            k_morton<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
+        else
+            k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
+        CNN_HIP(hipGetLastError());
+        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
+    }
+    g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
+    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
+    {
+        const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * std::max(d, 1), 256), 16384);
+        if (dtype == CNN_F32)
+            k_gather<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
+        else
+            k_gather<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
+        CNN_HIP(hipGetLastError());
+    }
+
+    // candidate tile pairs
+    const int64_t T = g->n_tiles;
+    int64_t *tp_ptr = nullptr;
+    Scratch<int64_t> tp_ptr_s(ctx, g->layout == 1 ? T + 1 : 1);
+    int32_t *tp_col = nullptr;
+    std::unique_ptr<Scratch<int32_t>> tp_col_s;
+    int64_t total_pairs = T * T;
+    const double thr = g->r2 * (1.0 + 1e-6);
+    if (g->layout == 1) {
+        Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
+        k_tile_boxes<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(g->xw64, n, d, T, blo.p, bhi.p);
+        CNN_HIP(hipGetLastError());
+        Scratch<int32_t> cnt(ctx, T);
+        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, thr, cnt.p, nullptr, nullptr);
+        CNN_HIP(hipGetLastError());
+        tp_ptr = tp_ptr_s.p;
+        exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
+        // tp_ptr[T] = tp_ptr[T-1] + cnt[T-1]
+        int64_t last_excl = read_scalar(ctx, tp_ptr + (T - 1));
+        int32_t last_cnt = read_scalar(ctx, cnt.p + (T - 1));
+        total_pairs = last_excl + last_cnt;
+        CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        tp_col_s.reset(new Scratch<int32_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        tp_col = tp_col_s->p;
+        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, thr, nullptr, tp_ptr, tp_col);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<int64_t> h_tp_ptr;
+    if (g->layout == 1) {
+        h_tp_ptr.resize(T + 1);
+        CNN_HIP(hipMemcpyAsync(h_tp_ptr.data(), tp_ptr, (T + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+    }
+    auto pair_begin = [&](int64_t R) { return g->layout == 1 ? h_tp_ptr[R] : R * T; };
+    CNN_HIP(hipEventRecord(ctx->ev[1], s));
+
+    BandParams bp;
+    if (band) {
+        bp = make_band_direct(g->r2, d, dtype == CNN_F64, std::sqrt(max_norm2));
+    } else {
+        bp.r2 = g->r2;
+        bp.lo = 0.0f;
+        bp.hi = 0.0f;
+    }
+    Scratch<unsigned long long> bstats(ctx, 4);
+    CNN_HIP(hipMemsetAsync(bstats.p, 0, 4 * sizeof(unsigned long long), s));
+    Scratch<int32_t> deg(ctx, n);
+    Scratch<int64_t> row_ptr_tmp(ctx, 1);
+
+    // row-tile batches bounded by the mask budget (512 B per tile pair)
+    const int64_t mask_budget = (int64_t)8 << 30;
+    const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
+    std::vector<std::pair<int64_t, int64_t>> batches;
+    for (int64_t R = 0; R < T;) {
+        int64_t R1 = R + 1;
+        while (R1 < T && pair_begin(R1 + 1) - pair_begin(R) <= max_pairs_per_batch) ++R1;
+        batches.push_back({R, R1});
+        R = R1;
+    }
+    int64_t max_batch_pairs = 0;
+    for (auto &b : batches) max_batch_pairs = std::max(max_batch_pairs, pair_begin(b.second) - pair_begin(b.first));
+    Scratch<uint64_t> mask(ctx, std::max<int64_t>(max_batch_pairs, 1) * 64);
+
+    float ms_dist = 0.0f, ms_emit = 0.0f;
+    int64_t nnz_total = 0;
+    int64_t col_cap = 0;
+    int32_t *col = nullptr;
+    std::vector<int64_t> batch_nnz;
+    for (auto &b : batches) {
+        DistArgs a{};
+        a.xw64 = g->xw64;
+        a.xw32 = g->xw32;
+        a.n = n;
+        a.d = d;
+        a.R0 = b.first;
+        a.n_tiles = T;
+        a.tp_ptr = tp_ptr;
+        a.tp_col = tp_col;
+        a.k_base = pair_begin(b.first);
+        a.mask = mask.p;
+        a.deg = deg.p;
+        a.bp = bp;
+        a.band_stats = bstats.p;
+        CNN_HIP(hipEventRecord(ctx->ev[2], s));
+        launch_dist(ctx, a, b.second - b.first, band);
+        CNN_HIP(hipEventRecord(ctx->ev[3], s));
+        // row_ptr for this batch's rows: exclusive scan + running offset
+        const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
+        const int64_t rows = row_hi - row_lo;
+        exclusive_scan_i32_to_i64(ctx, deg.p + row_lo, g->row_ptr + row_lo, rows);
+        int64_t last_excl = read_scalar(ctx, g->row_ptr + row_hi - 1);
+        int32_t last_deg = read_scalar(ctx, deg.p + row_hi - 1);
+        const int64_t batch_total = last_excl + last_deg;
+        if (nnz_total) {
+            // shift by the entries of earlier batches
+            std::vector<int64_t> hbuf(rows);
+            CNN_HIP(hipMemcpyAsync(hbuf.data(), g->row_ptr + row_lo, rows * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            CNN_HIP(hipStreamSynchronize(s));
+            for (auto &v : hbuf) v += nnz_total;
+            CNN_HIP(hipMemcpyAsync(g->row_ptr + row_lo, hbuf.data(), rows * sizeof(int64_t), hipMemcpyHostToDevice, s));
+            CNN_HIP(hipStreamSynchronize(s));
+        }
+        const int64_t need = nnz_total + batch_total;
+        if (need > col_cap) {
+            int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
+            int32_t *nc = nullptr;
+            hipError_t e = hipMalloc(&nc, (size_t)std::max<int64_t>(new_cap, 1) * sizeof(int32_t));
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw OutOfMemory{"neighbour list allocation failed (" + std::to_string(need) + " entries)"};
+            }
+            if (col && nnz_total) CNN_HIP(hipMemcpyAsync(nc, col, nnz_total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+            if (col) {
+                CNN_HIP(hipStreamSynchronize(s));
+                (void)hipFree(col);
+                g->bytes -= col_cap * (int64_t)sizeof(int32_t);
+            }
+            col = nc;
+            col_cap = new_cap;
+            g->bytes += col_cap * (int64_t)sizeof(int32_t);
+            g->col = col;
+        }
+        CNN_HIP(hipEventRecord(ctx->ev[4], s));
+        if (rows > 0)
+            k_emit<<<(unsigned)ceil_div(rows, 4), 256, 0, s>>>(a, row_lo, row_hi, g->row_ptr, 0, col);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipEventRecord(ctx->ev[5], s));
+        CNN_HIP(hipEventSynchronize(ctx->ev[5]));
+        float t1 = 0, t2 = 0;
+        CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
+        CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[4], ctx->ev[5]));
+        ms_dist += t1;
+        ms_emit += t2;
+        nnz_total = need;
+    }
+    CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &nnz_total, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    unsigned long long hb[4];
+    CNN_HIP(hipMemcpyAsync(hb, bstats.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipStreamSynchronize(s));
+    g->nnz = nnz_total;
+    float ms_prep = 0.0f;
+    CNN_HIP(hipEventElapsedTime(&ms_prep, ctx->ev[0], ctx->ev[1]));
+    cnn_graph_stats &st = g->stats;
+    st.tile_pairs = total_pairs;
+    st.pairs_evaluated = total_pairs * 64 * 64;
+    st.band_hits = (int64_t)hb[0];
+    st.band_flips = (int64_t)hb[1];
+    st.max_abs_err_band = __builtin_bit_cast(double, (uint64_t)hb[2]);
+    st.nnz = nnz_total;
+    st.ms_prep = ms_prep;
+    st.ms_distance = ms_dist;
+    st.ms_emit = ms_emit;
+    st.distance_launches = (int32_t)batches.size();
+    st.layout = g->layout;
+    return std::exchange(guard.g, nullptr);
+}
+
+void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = g->n;
+    if (n == 0) {
+        fill_i64(ctx, row_ptr, 0, 1);
+        return;
+    }
+    if (!g->perm) {
+        CNN_HIP(hipMemcpyAsync(row_ptr, g->row_ptr, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        if (g->nnz)
+            CNN_HIP(hipMemcpyAsync(col, g->col, g->nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    Scratch<int64_t> deg_o(ctx, n);
+    k_deg_from_rowptr_perm<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(g->row_ptr, g->perm, n, deg_o.p);
+    CNN_HIP(hipGetLastError());
+    exclusive_scan_i64(ctx, deg_o.p, row_ptr, n);
+    k_set_last<<<1, 1, 0, s>>>(row_ptr, n, row_ptr, deg_o.p);
+    CNN_HIP(hipGetLastError());
+    k_scatter_rows<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(g->row_ptr, g->col, g->perm, n, row_ptr, col);
+    CNN_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < n) ++bits;
+    segmented_sort_i32(ctx, col, g->nnz, row_ptr, n, bits);
+}
+
+
+// ------------------------------------------------------------------ re-threshold
+
+// Keep the entries of a resident graph whose exact FP64 distance (reference
+// order) is within the new radius.  N_r'(i) is a subset of N_r(i) for r' <= r.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_rethreshold(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                                                     const double *__restrict__ xw64, int64_t n, int d, double r2,
+                                                     int32_t *__restrict__ deg, const int64_t *__restrict__ out_ptr,
+                                                     int32_t *__restrict__ out_col) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    int64_t out = FILL ? out_ptr[w] : 0;
+    for (int64_t t0 = b; t0 < e; t0 += 64) {
+        const int64_t t = t0 + lane;
+        bool keep = false;
+        int32_t j = 0;
+        if (t < e) {
+            j = col[t];
+            keep = exact_d2(xw64, d, w, j) <= r2;
+        }
+        const uint64_t bits = __ballot(keep);
+        if (FILL && keep) out_col[out + __popcll(bits & ((1ull << lane) - 1ull))] = j;
+        out += __popcll(bits);
+    }
+    if (!FILL && lane == 0) deg[w] = (int32_t)out;
+}
+
+cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int32_t precision) {
+    if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
+    if (!(r2 <= src->r2)) throw InvalidArg{"re-threshold radius must not exceed the graph radius"};
+    CNN_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    GraphGuard guard{new cnn_graph()};
+    cnn_graph *g = guard.g;
+    g->device = src->device;
+    g->n = src->n;
+    g->d = src->d;
+    g->dtype = src->dtype;
+    g->precision = precision;
+    g->r2 = r2;
+    g->radius = std::sqrt(r2);
+    g->layout = src->layout;
+    g->n_tiles = src->n_tiles;
+    const int64_t n = g->n, n_pad = g->n_tiles * 64;
+    const int dd = std::max(g->d, 1);
+    g->row_ptr = dev_alloc<int64_t>(g, n + 1);
+    if (src->perm) {
+        g->perm = dev_alloc<int32_t>(g, n);
+        CNN_HIP(hipMemcpyAsync(g->perm, src->perm, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    g->xw64 = dev_alloc<double>(g, n_pad * dd);
+    CNN_HIP(hipMemcpyAsync(g->xw64, src->xw64, n_pad * dd * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (src->xw32) {
+        g->xw32 = dev_alloc<float>(g, n_pad * dd);
+        CNN_HIP(hipMemcpyAsync(g->xw32, src->xw32, n_pad * dd * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    if (n == 0) {
+        fill_i64(ctx, g->row_ptr, 0, 1);
+        g->col = dev_alloc<int32_t>(g, 1);
+        CNN_HIP(hipStreamSynchronize(s));
+        return std::exchange(guard.g, nullptr);
+    }
+    CNN_HIP(hipEventRecord(ctx->ev[2], s));
+    Scratch<int32_t> deg(ctx, n);
+    k_rethreshold<false><<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(src->row_ptr, src->col, src->xw64, n, g->d, r2, deg.p,
+                                                                  nullptr, nullptr);
+    CNN_HIP(hipGetLastError());
+    exclusive_scan_i32_to_i64(ctx, deg.p, g->row_ptr, n);
+    const int64_t last_excl = read_scalar(ctx, g->row_ptr + (n - 1));
+    const int32_t last_deg = read_scalar(ctx, deg.p + (n - 1));
+    g->nnz = last_excl + last_deg;
+    CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &g->nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    g->col = dev_alloc<int32_t>(g, g->nnz);
+    k_rethreshold<true><<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(src->row_ptr, src->col, src->xw64, n, g->d, r2, nullptr,
+                                                                 g->row_ptr, g->col);
+    CNN_HIP(hipGetLastError());
+    CNN_HIP(hipEventRecord(ctx->ev[3], s));
+    CNN_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    CNN_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+    g->stats = cnn_graph_stats{};
+    g->stats.nnz = g->nnz;
+    g->stats.pairs_evaluated = src->nnz;
+    g->stats.ms_distance = ms;
+    g->stats.layout = g->layout;
+    return std::exchange(guard.g, nullptr);
+}
+
+}  // namespace cnn
+
+// ================================================================== C-ABI: phase N
+using namespace cnn;
+
+extern "C" {
+
+int cnn_graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
+                    int32_t precision, cnn_graph **out) {
+    if (!ctx || !out) return CNN_ERR_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&]() {
+        *out = graph_build(ctx, X, n, d, dtype, r2, precision);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_graph_destroy(cnn_graph *g) {
+    if (!g) return CNN_ERR_INVALID;
+    graph_free(g);
+    return CNN_OK;
+}
+
+int cnn_graph_nnz(const cnn_graph *g, int64_t *nnz) {
+    if (!g || !nnz) return CNN_ERR_INVALID;
+    *nnz = g->nnz;
+    return CNN_OK;
+}
+
+int cnn_graph_stats_get(const cnn_graph *g, cnn_graph_stats *out) {
+    if (!g || !out) return CNN_ERR_INVALID;
+    *out = g->stats;
+    return CNN_OK;
+}
+
+int cnn_graph_bytes(const cnn_graph *g, int64_t *bytes) {
+    if (!g || !bytes) return CNN_ERR_INVALID;
+    *bytes = g->bytes;
+    return CNN_OK;
+}
+
+int cnn_graph_export_csr(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col, int64_t col_cap) {
+    if (!ctx || !g || !row_ptr) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (col_cap < g->nnz) throw Capacity{"col capacity " + std::to_string(col_cap) + " < nnz " + std::to_string(g->nnz)};
+        if (g->nnz > 0 && !col) throw InvalidArg{"col is null"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        graph_export(ctx, g, row_ptr, col);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_radius_csr(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
+                   int32_t precision, int64_t *row_ptr, int32_t *col, int64_t col_cap, int64_t *nnz_out,
+                   cnn_graph_stats *stats_out) {
+    if (!ctx || !row_ptr || !nnz_out) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision);
+        GraphGuard guard{g};
+        *nnz_out = g->nnz;
+        if (stats_out) *stats_out = g->stats;
+        if (col_cap < g->nnz) throw Capacity{"col capacity " + std::to_string(col_cap) + " < nnz " + std::to_string(g->nnz)};
+        if (g->nnz > 0 && !col) throw InvalidArg{"col is null"};
+        graph_export(ctx, g, row_ptr, col);
+        CNN_HIP(hipStreamSynchronize(ctx->stream));
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision,
+                          cnn_graph **out) {
+    if (!ctx || !g || !out) return CNN_ERR_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&]() {
+        *out = graph_rethreshold(ctx, g, r2, precision);
+        return (int)CNN_OK;
+    });
+}
+
+}  // extern "C"

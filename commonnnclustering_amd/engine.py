@@ -1,0 +1,268 @@
+"""Device-level API: torch tensors on the GPU in, torch tensors out.
+
+Thin wrappers over the C-ABI in ``include/cnn_gpu.h``.  Radii are passed
+squared (``r2 = r * r`` in double), exactly the value the reference's
+``make_parameters`` stores in ``ClusterParameters.radius_cutoff``
+(_fit.pyx:136-141).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GraphStats, LabelStats, get_context, ptr
+
+__all__ = [
+    "NeighbourGraph",
+    "build_graph",
+    "radius_csr",
+    "labels_from_csr",
+    "sort_by_size",
+    "fit",
+    "fit_batch",
+    "to_device_points",
+]
+
+
+def to_device_points(X, device=None) -> torch.Tensor:
+    """Coordinates as a contiguous float32/float64 CUDA tensor (dtype kept)."""
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    if isinstance(X, torch.Tensor):
+        t = X
+        if t.dtype not in (torch.float32, torch.float64):
+            t = t.to(torch.float64)
+        return t.to(device).contiguous()
+    arr = np.asarray(X)
+    if arr.dtype not in (np.float32, np.float64):
+        arr = arr.astype(np.float64)
+    if arr.ndim == 1:
+        arr = arr.reshape(1, -1)
+    return torch.from_numpy(np.ascontiguousarray(arr)).to(device)
+
+
+def _check_points(X: torch.Tensor):
+    if not X.is_cuda:
+        raise ValueError("X must be a CUDA (HIP) tensor")
+    if X.dim() != 2:
+        raise ValueError(f"X must be 2-D (n, d), got shape {tuple(X.shape)}")
+    if not X.is_contiguous():
+        raise ValueError("X must be contiguous (row-major)")
+    return X.shape[0], X.shape[1], _lib.dtype_code(X)
+
+
+class NeighbourGraph:
+    """Resident radius graph on the device (cnn_graph handle).
+
+    Neighbour sets are those of NeighboursGetterExtBruteForce._get
+    (_types.pyx:1692-1710) with the reduced Euclidean metric at ``r2``.
+    """
+
+    def __init__(self, handle, ctx, X: torch.Tensor, r2: float, precision: int):
+        self._handle = handle
+        self._ctx = ctx
+        self.n, self.d = X.shape
+        self.device = X.device
+        self.r2 = float(r2)
+        self.precision = precision
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            raise RuntimeError("graph was released")
+        return self._handle
+
+    @property
+    def nnz(self) -> int:
+        v = ctypes.c_int64()
+        self._ctx.check(_lib.lib().cnn_graph_nnz(self.handle, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def stats(self) -> dict:
+        s = GraphStats()
+        self._ctx.check(_lib.lib().cnn_graph_stats_get(self.handle, ctypes.byref(s)))
+        return s.to_dict()
+
+    @property
+    def nbytes(self) -> int:
+        v = ctypes.c_int64()
+        self._ctx.check(_lib.lib().cnn_graph_bytes(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def to_csr(self):
+        """(row_ptr int64[n+1], col int32[nnz]) in caller point order, rows ascending."""
+        ctx = get_context(self.device)
+        nnz = self.nnz
+        row_ptr = torch.empty(self.n + 1, dtype=torch.int64, device=self.device)
+        col = torch.empty(max(nnz, 1), dtype=torch.int32, device=self.device)
+        ctx.check(_lib.lib().cnn_graph_export_csr(ctx.handle, self.handle, ptr(row_ptr), ptr(col), nnz),
+                  "cnn_graph_export_csr")
+        return row_ptr, col[:nnz]
+
+    def rethreshold(self, r2: float, precision=None) -> "NeighbourGraph":
+        ctx = get_context(self.device)
+        p = self.precision if precision is None else _lib.precision_code(precision)
+        h = ctypes.c_void_p()
+        ctx.check(_lib.lib().cnn_graph_rethreshold(ctx.handle, self.handle, float(r2), p, ctypes.byref(h)),
+                  "cnn_graph_rethreshold")
+        g = NeighbourGraph.__new__(NeighbourGraph)
+        g._handle, g._ctx = h, ctx
+        g.n, g.d, g.device, g.r2, g.precision = self.n, self.d, self.device, float(r2), p
+        return g
+
+    def labels(self, c_sim: int, m_core: int, consider=None, current_start: int = 1, labels=None,
+               sim_mode: int = _lib.CNN_SIM_AUTO):
+        """In-place labelling (FitterExtBFS._fit semantics); returns (labels, stats)."""
+        ctx = get_context(self.device)
+        if labels is None:
+            labels = torch.zeros(self.n, dtype=torch.int64, device=self.device)
+        _check_labels(labels, self.n, consider)
+        st = LabelStats()
+        ctx.check(_lib.lib().cnn_graph_labels(ctx.handle, self.handle, int(c_sim), int(m_core), ptr(consider),
+                                              int(current_start), int(sim_mode), ptr(labels), ctypes.byref(st)),
+                  "cnn_graph_labels")
+        return labels, st.to_dict()
+
+    def fit(self, c_sim: int, m_core: int, sort: bool = True, member_cutoff=None, max_clusters=None,
+            labels=None):
+        """Fresh labels for all points (optionally sort_by_size); returns (labels, stats)."""
+        ctx = get_context(self.device)
+        if labels is None:
+            labels = torch.empty(self.n, dtype=torch.int64, device=self.device)
+        _check_labels(labels, self.n, None)
+        st = LabelStats()
+        ctx.check(_lib.lib().cnn_graph_fit(ctx.handle, self.handle, int(c_sim), int(m_core), int(bool(sort)),
+                                           _member_cutoff(member_cutoff), _max_clusters(max_clusters),
+                                           ptr(labels), ctypes.byref(st)), "cnn_graph_fit")
+        return labels, st.to_dict()
+
+    def release(self) -> None:
+        if self._handle is not None:
+            _lib.lib().cnn_graph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def _member_cutoff(v) -> int:
+    return 2 if v is None else int(v)
+
+
+def _max_clusters(v) -> int:
+    return -1 if v is None else int(v)
+
+
+def _check_labels(labels, n, consider):
+    if labels.dtype != torch.int64 or not labels.is_cuda or labels.numel() != n or not labels.is_contiguous():
+        raise ValueError("labels must be a contiguous int64 CUDA tensor of length n")
+    if consider is not None:
+        if consider.dtype != torch.uint8 or not consider.is_cuda or consider.numel() != n:
+            raise ValueError("consider must be a uint8 CUDA tensor of length n")
+
+
+def build_graph(X: torch.Tensor, r2: float, precision="f64") -> NeighbourGraph:
+    n, d, dt = _check_points(X)
+    p = _lib.precision_code(precision)
+    ctx = get_context(X.device)
+    h = ctypes.c_void_p()
+    ctx.check(_lib.lib().cnn_graph_build(ctx.handle, ptr(X), n, d, dt, float(r2), p, ctypes.byref(h)),
+              "cnn_graph_build")
+    return NeighbourGraph(h, ctx, X, r2, p)
+
+
+def radius_csr(X: torch.Tensor, r2: float, precision="f64"):
+    """Neighbour lists as CSR in caller order: (row_ptr int64, col int32, stats)."""
+    g = build_graph(X, r2, precision)
+    try:
+        row_ptr, col = g.to_csr()
+        return row_ptr, col, g.stats
+    finally:
+        g.release()
+
+
+def labels_from_csr(row_ptr: torch.Tensor, col: torch.Tensor, c_sim: int, m_core: int, consider=None,
+                    current_start: int = 1, labels=None, sim_mode: int = _lib.CNN_SIM_AUTO):
+    """Labels from a caller CSR (rows sorted ascending, unique); in-place semantics."""
+    n = row_ptr.numel() - 1
+    device = row_ptr.device
+    ctx = get_context(device)
+    if row_ptr.dtype != torch.int64 or col.dtype != torch.int32:
+        raise ValueError("row_ptr must be int64 and col int32")
+    if not (row_ptr.is_cuda and col.is_cuda and row_ptr.is_contiguous() and col.is_contiguous()):
+        raise ValueError("row_ptr and col must be contiguous CUDA tensors")
+    if n > 0:
+        if int(row_ptr[0]) != 0 or bool((row_ptr[1:] < row_ptr[:-1]).any()):
+            raise ValueError("row_ptr must start at 0 and be non-decreasing")
+        if int(row_ptr[-1]) > col.numel():
+            raise ValueError(f"row_ptr[-1] = {int(row_ptr[-1])} exceeds len(col) = {col.numel()}")
+    if labels is None:
+        labels = torch.zeros(n, dtype=torch.int64, device=device)
+    _check_labels(labels, n, consider)
+    col_arg = col if col.numel() else torch.zeros(1, dtype=torch.int32, device=device)
+    st = LabelStats()
+    ctx.check(_lib.lib().cnn_labels_from_csr(ctx.handle, ptr(row_ptr), ptr(col_arg), n, int(c_sim), int(m_core),
+                                             ptr(consider), int(current_start), int(sim_mode), ptr(labels),
+                                             ctypes.byref(st)), "cnn_labels_from_csr")
+    return labels, st.to_dict()
+
+
+def sort_by_size(labels: torch.Tensor, member_cutoff=None, max_clusters=None) -> int:
+    """Labels.sort_by_size (_types.pyx:212-286) in place on a CUDA int64 tensor."""
+    if labels.dtype != torch.int64 or not labels.is_cuda or not labels.is_contiguous():
+        raise ValueError("labels must be a contiguous int64 CUDA tensor")
+    ctx = get_context(labels.device)
+    k = ctypes.c_int64()
+    ctx.check(_lib.lib().cnn_sort_by_size(ctx.handle, ptr(labels), labels.numel(), _member_cutoff(member_cutoff),
+                                          _max_clusters(max_clusters), ctypes.byref(k)), "cnn_sort_by_size")
+    return k.value
+
+
+def fit(X: torch.Tensor, r2: float, c_sim: int, m_core: int, precision="f64", sort: bool = True,
+        member_cutoff=None, max_clusters=None, labels=None, keep_graph: bool = False):
+    """Fused fresh fit: returns (labels int64[n], graph or None, graph stats, label stats)."""
+    n, d, dt = _check_points(X)
+    p = _lib.precision_code(precision)
+    ctx = get_context(X.device)
+    if labels is None:
+        labels = torch.empty(n, dtype=torch.int64, device=X.device)
+    _check_labels(labels, n, None)
+    gs, ls = GraphStats(), LabelStats()
+    h = ctypes.c_void_p()
+    ctx.check(_lib.lib().cnn_fit(ctx.handle, ptr(X), n, d, dt, float(r2), int(c_sim), int(m_core), p,
+                                 int(bool(sort)), _member_cutoff(member_cutoff), _max_clusters(max_clusters),
+                                 ptr(labels), ctypes.byref(h) if keep_graph else None,
+                                 ctypes.byref(gs), ctypes.byref(ls)), "cnn_fit")
+    graph = NeighbourGraph(h, ctx, X, r2, p) if keep_graph else None
+    return labels, graph, gs.to_dict(), ls.to_dict()
+
+
+def fit_batch(X: torch.Tensor, offsets, r2, c_sim, m_core, precision="f64", member_cutoff=None,
+              max_clusters=None, labels=None):
+    """Many datasets stacked row-wise in X; dataset b is rows offsets[b]:offsets[b+1]."""
+    n, d, dt = _check_points(X)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n_sets = len(off) - 1
+    if n_sets < 0 or off[-1] != n:
+        raise ValueError("offsets must start a partition of the rows of X")
+    r2a = np.ascontiguousarray(np.broadcast_to(np.asarray(r2, dtype=np.float64), (n_sets,)))
+    ca = np.ascontiguousarray(np.broadcast_to(np.asarray(c_sim, dtype=np.int32), (n_sets,)))
+    ma = np.ascontiguousarray(np.broadcast_to(np.asarray(m_core, dtype=np.int32), (n_sets,)))
+    p = _lib.precision_code(precision)
+    ctx = get_context(X.device)
+    if labels is None:
+        labels = torch.empty(n, dtype=torch.int64, device=X.device)
+    gs, ls = GraphStats(), LabelStats()
+    ctx.check(_lib.lib().cnn_fit_batch(ctx.handle, ptr(X), off.ctypes.data, n_sets, d, dt, r2a.ctypes.data,
+                                       ca.ctypes.data, ma.ctypes.data, p, _member_cutoff(member_cutoff),
+                                       _max_clusters(max_clusters), ptr(labels), ctypes.byref(gs),
+                                       ctypes.byref(ls)), "cnn_fit_batch")
+    return labels, gs.to_dict(), ls.to_dict()

@@ -1,0 +1,198 @@
+/*
+ * cnn_gpu.h -- C-ABI of the MI355X CommonNN clustering engine (libcnn_hip.so).
+ *
+ * Plain C types only: device pointers are `void *` / typed pointers to HBM,
+ * sizes are int64_t.  Every function returns 0 (CNN_OK) or a negative
+ * cnn_status; cnn_last_error() gives the message.  No C++ exception crosses
+ * this boundary.  All work is enqueued on the context's stream; functions
+ * that return a host-side size synchronise that stream.
+ *
+ * Reference interfaces replaced (cnnclustering 0.5.0, janjoswig/CommonNNClustering):
+ *   cnn_graph_build / cnn_radius_csr
+ *       NeighboursGetterExtBruteForce._get          _types.pyx:1692-1710
+ *       DistanceGetterExtMetric._get_single         _types.pyx:1979-1985
+ *       MetricExtEuclideanReduced._calc_distance    _types.pyx:2253-2267
+ *       MetricExtEuclideanReduced._adjust_radius    _types.pyx:2286-2287
+ *   cnn_labels_from_csr / cnn_graph_labels
+ *       FitterExtBFS._fit                           _fit.pyx:408-491
+ *       NeighboursExtVectorCPPUnorderedSet._enough  _types.pyx:1561-1564
+ *       SimilarityCheckerExtSwitchContains._check   _types.pyx:2580-2610
+ *       SimilarityCheckerExtScreensorted._check     _types.pyx:2654-2706
+ *   cnn_sort_by_size
+ *       Labels.sort_by_size                         _types.pyx:212-286
+ *   cnn_fit
+ *       Clustering.fit (fit + sort_by_size part)    cluster.pyx:256-364
+ *   cnn_graph_rethreshold
+ *       NeighboursGetterRecomputeLookup radius cache _types.pyx:1867-1873
+ *       (re-fit at a smaller radius from a resident graph)
+ *
+ * Radius argument: every entry point takes r2, the squared cutoff computed in
+ * double as r*r -- the value the reference's make_parameters stores in
+ * ClusterParameters.radius_cutoff (_fit.pyx:136-141) and the getter compares
+ * against (_types.pyx:1708).  Passing r2 avoids a lossy sqrt/square round trip.
+ *
+ * Semantics (bit-exact with the reference):
+ *   s_ij = sum_k (x_ik - x_jk)^2, k ascending, FP64, each op separately rounded
+ *   N(i) = { j : s_ij <= r2 }, i in N(i); core(i) = consider(i) && |N(i)| > m_core
+ *   edge(i,j) = core(i) && core(j) && j in N(i) && |N(i) & N(j)| >= c_sim
+ *   labels = connected components of (core points, edges), numbered by
+ *   ascending smallest point index from current_start.
+ *   With cnn_cutoff c, the reference's make_parameters gives c_sim = c + 2 and
+ *   m_core = c + 1 (_fit.pyx:143-159).
+ */
+#ifndef CNN_GPU_H
+#define CNN_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum {
+    CNN_OK = 0,
+    CNN_ERR_INVALID = -1,   /* bad argument (null pointer, negative size, bad enum) */
+    CNN_ERR_OOM = -2,       /* device allocation failed or the memory budget is exceeded */
+    CNN_ERR_HIP = -3,       /* a HIP runtime call failed */
+    CNN_ERR_CAPACITY = -4,  /* caller buffer too small; required size returned */
+    CNN_ERR_UNSUPPORTED = -5
+} cnn_status;
+
+/* dtype of a coordinate buffer */
+enum { CNN_F32 = 0, CNN_F64 = 1 };
+
+/* neighbour-search precision */
+enum {
+    CNN_PLAIN_F64 = 0,  /* every distance in FP64, reference order */
+    CNN_F32_BAND = 1    /* FP32 distances (VALU direct form for d < 16, MFMA Gram
+                           form for d >= 16); pairs within the guard band of r*r
+                           are recomputed in FP64 in reference order */
+};
+
+/* similarity kernel selection */
+enum { CNN_SIM_AUTO = 0, CNN_SIM_MERGE = 1, CNN_SIM_BITSET = 2 };
+
+typedef struct cnn_ctx cnn_ctx;
+typedef struct cnn_graph cnn_graph;
+
+typedef struct {
+    int64_t pairs_evaluated;   /* point pairs whose distance was computed (tile culling applied) */
+    int64_t band_hits;         /* FP32 pairs recomputed in FP64 */
+    int64_t band_flips;        /* band pairs whose FP64 decision differed from the FP32 one */
+    double max_abs_err_band;   /* max |d2_fp32 - d2_fp64| over band pairs */
+    int64_t tile_pairs;        /* 64x64 tile pairs evaluated */
+    int64_t nnz;               /* neighbour-list entries (self included) */
+    float ms_prep;             /* ordering, tiling, candidate tile pairs */
+    float ms_distance;         /* distance kernel */
+    float ms_emit;             /* degree scan + CSR emit */
+    int32_t distance_launches; /* launches of the distance kernel timed in ms_distance */
+    int32_t layout;            /* 0 identity order, 1 spatial (Morton) order */
+} cnn_graph_stats;
+
+typedef struct {
+    int64_t n_core;
+    int64_t n_components;      /* components among core points (before sort_by_size) */
+    int64_t n_clusters;        /* clusters after sort_by_size */
+    int64_t tests;             /* similarity tests executed */
+    int64_t tests_passed;
+    float ms_similarity;       /* similarity + union-find kernels */
+    float ms_labels;           /* compress, label assignment, sort_by_size */
+} cnn_label_stats;
+
+/* ------------------------------------------------------------------ context */
+/* Create a context on HIP device `device`.  The context owns a non-blocking
+ * stream (default priority) and a workspace arena. */
+int cnn_ctx_create(int device, cnn_ctx **out);
+int cnn_ctx_destroy(cnn_ctx *ctx);
+/* Message of the last failing call on this context (never NULL). */
+const char *cnn_last_error(const cnn_ctx *ctx);
+/* Enqueue subsequent work on `stream` (a hipStream_t; NULL is the default
+ * (null) stream).  Until this is called the context uses its own stream. */
+int cnn_ctx_set_stream(cnn_ctx *ctx, void *stream);
+void *cnn_ctx_get_stream(cnn_ctx *ctx);
+int cnn_ctx_synchronize(cnn_ctx *ctx);
+/* Device bytes currently held by the context's workspace arena. */
+int cnn_ctx_workspace_bytes(const cnn_ctx *ctx, int64_t *bytes);
+/* Release the workspace arena (kept between calls otherwise). */
+int cnn_ctx_trim(cnn_ctx *ctx);
+
+/* ------------------------------------------------------------------ phase N */
+/* Build the radius-r neighbour graph of X (device, row-major n x d, dtype
+ * CNN_F32 or CNN_F64) and keep it resident in device memory.  The graph is
+ * stored in an internal point order (spatial order for d <= 8); export with
+ * cnn_graph_export_csr. */
+int cnn_graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+                    double r2, int32_t precision, cnn_graph **out);
+int cnn_graph_destroy(cnn_graph *g);
+int cnn_graph_nnz(const cnn_graph *g, int64_t *nnz);
+int cnn_graph_stats_get(const cnn_graph *g, cnn_graph_stats *out);
+/* Device bytes held by the graph. */
+int cnn_graph_bytes(const cnn_graph *g, int64_t *bytes);
+/* Write the neighbour lists in the caller's point order, each row ascending:
+ * row_ptr (device int64[n+1]), col (device int32[nnz]).  col_cap < nnz returns
+ * CNN_ERR_CAPACITY. */
+int cnn_graph_export_csr(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col,
+                         int64_t col_cap);
+/* One-call neighbour search (build + export + destroy).  When col_cap is too
+ * small, *nnz_out holds the required size and CNN_ERR_CAPACITY is returned. */
+int cnn_radius_csr(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+                   double r2, int32_t precision, int64_t *row_ptr, int32_t *col,
+                   int64_t col_cap, int64_t *nnz_out, cnn_graph_stats *stats_out);
+/* Derive the graph at a smaller radius from a resident graph (neighbour sets
+ * at r2' <= r2 are subsets; distances are re-evaluated exactly in FP64). */
+int cnn_graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2,
+                          int32_t precision, cnn_graph **out);
+
+/* ------------------------------------------------------------------ phase S+U */
+/* Labels from a resident graph.  consider (device uint8[n], caller order) may
+ * be NULL (all considered); when given it is set to 0 for every point, as the
+ * reference's BFS leaves it.  labels (device int64[n], caller order) is written
+ * only for points that join a cluster (other entries are left untouched), with
+ * clusters numbered from current_start by ascending smallest point index. */
+int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_core,
+                     uint8_t *consider, int64_t current_start, int32_t sim_mode,
+                     int64_t *labels, cnn_label_stats *stats_out);
+/* The same from a caller CSR (device; rows sorted ascending, no duplicates). */
+int cnn_labels_from_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n,
+                        int32_t c_sim, int32_t m_core, uint8_t *consider, int64_t current_start,
+                        int32_t sim_mode, int64_t *labels, cnn_label_stats *stats_out);
+/* Labels.sort_by_size in place on device int64 labels: clusters ranked by size
+ * descending, ties by first occurrence; ranks beyond max_clusters (< 0: no
+ * limit) and clusters smaller than member_cutoff become 0; survivors are
+ * renumbered 1..K.  *n_clusters_out = K (may be NULL). */
+int cnn_sort_by_size(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,
+                     int64_t max_clusters, int64_t *n_clusters_out);
+
+/* ------------------------------------------------------------------ fused fit */
+/* Fresh fit: neighbour graph, similarity, union-find and (sort != 0)
+ * sort_by_size, writing all n labels (device int64, caller order).  When
+ * graph_out is not NULL the neighbour graph stays resident for re-fits. */
+int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+            double r2, int32_t c_sim, int32_t m_core, int32_t precision,
+            int32_t sort, int64_t member_cutoff, int64_t max_clusters, int64_t *labels,
+            cnn_graph **graph_out, cnn_graph_stats *gstats, cnn_label_stats *lstats);
+/* Fresh fit on a resident graph (labels for all n points). */
+int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_core,
+                  int32_t sort, int64_t member_cutoff, int64_t max_clusters, int64_t *labels,
+                  cnn_label_stats *lstats);
+
+/* ------------------------------------------------------------------ batched */
+/* Many independent datasets in packed launches.  Dataset b has offsets[b] ..
+ * offsets[b+1] rows of X (device, row-major, common d and dtype; offsets is a
+ * HOST int64[n_sets+1]) and its own r2 / c_sim / m_core (HOST arrays).
+ * labels (device int64[offsets[n_sets]]) receives every dataset's sorted labels
+ * (sort_by_size per dataset with member_cutoff / max_clusters). */
+int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d,
+                  int32_t dtype, const double *r2, const int32_t *c_sim,
+                  const int32_t *m_core, int32_t precision, int64_t member_cutoff,
+                  int64_t max_clusters, int64_t *labels, cnn_graph_stats *gstats,
+                  cnn_label_stats *lstats);
+
+/* ------------------------------------------------------------------ info */
+const char *cnn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CNN_GPU_H */

@@ -1,0 +1,406 @@
+"""CPU oracle for the CommonNN ``Clustering.fit`` hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` may import this module, and only as the
+checker.  The product path (``commonnnclustering_amd``) never imports it and
+fails loudly when its HIP library is missing.
+
+This is a restatement, in numpy and plain Python, of the arithmetic and
+control flow of cnnclustering 0.5.0 (janjoswig/CommonNNClustering) for the
+default ``"coordinates"`` recipe:
+
+  input      recipes.py:52-147 (``prepare_points_from_parts``; float64, C order, :144)
+  metric     _types.pyx:2253-2267 (``MetricExtEuclideanReduced._calc_distance``)
+             _types.pyx:2286-2287 (``_adjust_radius``: r**2)
+  getter     _types.pyx:1692-1710 (``NeighboursGetterExtBruteForce._get``: s <= r2,
+             ascending j, self included)
+  enough     _types.pyx:1561-1564 (``n_points > member_cutoff``)
+  checker    _types.pyx:2580-2610 (``SimilarityCheckerExtSwitchContains._check``)
+             _types.pyx:2654-2706 (``SimilarityCheckerExtScreensorted._check``)
+  params     _fit.pyx:129-161 (``make_parameters``: r -> r*r, c -> c+2, m = c+1)
+  fitter     _fit.pyx:408-491 (``FitterExtBFS._fit``)
+  sort       _types.pyx:212-286 (``Labels.sort_by_size``)
+  wrapper    cluster.pyx:256-392 (``Clustering.fit``), report.py:91-191 (``Record``)
+
+How each part is pinned (tests/test_oracle.py):
+  * metric: bit-identical to a literal scalar Python loop (IEEE double, the
+    same sub/mul/add sequence with no FMA); error against an exact rational
+    evaluation stays within the textbook bound gamma_{2d} * d^2.
+  * neighbourhoods: boundary pairs at exactly r and one ulp beyond; symmetry;
+    self-inclusion; README/notebook known answers.
+  * similarity: the known-answer cases of test/unit/test_check_similarity.py:39-48.
+  * labelling: the literal BFS restatement (``fit_bfs``) and the
+    connected-components restatement (``fit_components``) agree on random
+    inputs; README.md:71-93 and basic_usage.ipynb:356 known answers; the
+    scikit-learn tutorial records (scikit_learn_datasets.ipynb cell 10 output).
+  * sort_by_size: test/unit/test_types.py:98-117 known answers; tie order
+    checked against ``collections.Counter.most_common`` semantics directly.
+"""
+
+from __future__ import annotations
+
+from collections import Counter
+from itertools import count
+
+import numpy as np
+
+__all__ = [
+    "prepare_points",
+    "adjust_radius",
+    "make_parameters",
+    "reduced_distance_scalar",
+    "reduced_distances",
+    "neighbourhoods",
+    "neighbourhoods_csr",
+    "check_similarity",
+    "similarity_count",
+    "fit_bfs",
+    "fit_components",
+    "sort_by_size",
+    "make_record",
+    "fit",
+]
+
+
+# ---------------------------------------------------------------- input / params
+
+def prepare_points(data) -> np.ndarray:
+    """Coerce point data to a C-contiguous float64 (n, d) array.
+
+    Follows recipes.py:52-147 for the 2D-sequence case: the result is
+    ``np.asarray(np.vstack(data), order="C", dtype=float64)`` (:144), so float32
+    input is upcast exactly.  Sequences of parts are stacked.
+    """
+    if isinstance(data, np.ndarray) and data.ndim == 2:
+        return np.asarray(data, order="C", dtype=np.float64)
+    if len(data) == 0:
+        return np.zeros((1, 0), dtype=np.float64)  # recipes.py:106-109: np.array([[]])
+    first = data[0]
+    try:
+        len(first[0])
+        parts = [np.asarray(x) for x in data]   # sequence of 2D parts
+    except TypeError:
+        try:
+            len(first)
+            parts = [np.asarray(data)]          # 2D sequence
+        except TypeError:
+            parts = [np.array([data])]          # one point
+    return np.asarray(np.vstack(parts), order="C", dtype=np.float64)
+
+
+def adjust_radius(radius_cutoff: float) -> float:
+    """r -> r*r in float64 (_types.pyx:2286-2287; GCC folds pow(r, 2) to r*r)."""
+    r = float(radius_cutoff)
+    return r * r
+
+
+def make_parameters(radius_cutoff: float, cnn_cutoff: int, current_start: int = 1) -> dict:
+    """Restates ``FitterExtInterface.make_parameters`` (_fit.pyx:129-161).
+
+    The brute-force getter is self-counting, so the member cutoff becomes
+    ``c + 1`` and the similarity cutoff ``c + 2``.
+    """
+    c = int(cnn_cutoff)
+    return {
+        "radius_cutoff": adjust_radius(radius_cutoff),
+        "similarity_cutoff": c + 2,
+        "similarity_cutoff_continuous": float(c + 2),
+        "n_member_cutoff": c + 1,
+        "current_start": int(current_start),
+    }
+
+
+# ---------------------------------------------------------------- metric
+
+def reduced_distance_scalar(a, b) -> float:
+    """Literal scalar restatement of _types.pyx:2253-2267.
+
+    ``total = 0; for k: total = total + (a_k - b_k) * (a_k - b_k)`` in IEEE
+    double, every operation separately rounded.
+    """
+    total = 0.0
+    for x, y in zip(a, b):
+        t = float(x) - float(y)
+        total = total + t * t
+    return total
+
+
+def reduced_distances(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """All-pairs reduced Euclidean distances, bit-identical to the scalar loop.
+
+    numpy's elementwise subtract, multiply and add are each correctly rounded
+    and never contracted, so looping over the dimension in order reproduces the
+    reference's summation exactly (_types.pyx:2262-2265).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    total = np.zeros((A.shape[0], B.shape[0]), dtype=np.float64)
+    for k in range(A.shape[1]):
+        diff = A[:, k, None] - B[None, :, k]
+        total = total + diff * diff
+    return total
+
+
+# ---------------------------------------------------------------- neighbours
+
+def neighbourhoods(X: np.ndarray, radius_cutoff: float, chunk: int = 1024) -> list:
+    """N(i) = {j : s_ij <= r*r}, ascending j, i included (_types.pyx:1692-1710)."""
+    X = np.asarray(X, dtype=np.float64)
+    r2 = adjust_radius(radius_cutoff)
+    out = []
+    for s in range(0, X.shape[0], chunk):
+        D = reduced_distances(X[s:s + chunk], X)
+        for row in D:
+            out.append(np.nonzero(row <= r2)[0].astype(np.int64))
+    return out
+
+
+def neighbourhoods_csr(X: np.ndarray, radius_cutoff: float, chunk: int = 1024):
+    """The same neighbourhoods as CSR: (row_ptr int64[n+1], col int64[nnz])."""
+    lists = neighbourhoods(X, radius_cutoff, chunk)
+    deg = np.array([len(x) for x in lists], dtype=np.int64)
+    row_ptr = np.zeros(len(lists) + 1, dtype=np.int64)
+    np.cumsum(deg, out=row_ptr[1:])
+    col = np.concatenate(lists) if lists else np.zeros(0, dtype=np.int64)
+    return row_ptr, col
+
+
+# ---------------------------------------------------------------- similarity
+
+def check_similarity(members_a, members_b, c: int) -> bool:
+    """Restates ``SimilarityCheckerExtSwitchContains._check`` (_types.pyx:2580-2610).
+
+    ``c == 0`` passes; otherwise iterate the shorter container and count
+    members contained in the other one, passing as soon as the count reaches c.
+    """
+    if c == 0:
+        return True
+    a, b = list(members_a), list(members_b)
+    if len(b) < len(a):
+        a, b = b, a
+    lookup = set(b)
+    common = 0
+    for m in a:
+        if m in lookup:
+            common += 1
+            if common == c:
+                return True
+    return False
+
+
+def similarity_count(members_a, members_b) -> int:
+    """Full common-member count (``SimilarityCheckerExtSwitchContains._get``, :2612-2634)."""
+    a, b = list(members_a), list(members_b)
+    if len(b) < len(a):
+        a, b = b, a
+    lookup = set(b)
+    return sum(1 for m in a if m in lookup)
+
+
+# ---------------------------------------------------------------- fitters
+
+def fit_bfs(neigh: list, params: dict, labels: np.ndarray, consider: np.ndarray) -> None:
+    """Literal restatement of ``FitterExtBFS._fit`` (_fit.pyx:408-491).
+
+    ``neigh[i]`` is N(i) as returned by the brute-force getter; the getter is
+    deterministic, so recomputing it on every call (as the reference does) and
+    looking it up give identical lists.  ``labels`` (int64) and ``consider``
+    (uint8) are updated in place.  FIFO queue as ``QueueExtFIFOQueue``
+    (_types.pyx:2802-2819).
+    """
+    m_cut = params["n_member_cutoff"]
+    c_sim = params["similarity_cutoff"]
+    current = params["current_start"]
+    n = len(neigh)
+    queue = []
+    head = 0
+    for init_point in range(n):
+        if consider[init_point] == 0:
+            continue
+        consider[init_point] = 0
+        na = neigh[init_point]
+        if not len(na) > m_cut:
+            continue
+        labels[init_point] = current
+        while True:
+            for member in na:
+                if consider[member] == 0:
+                    continue
+                nb = neigh[member]
+                if not len(nb) > m_cut:
+                    consider[member] = 0
+                    continue
+                if check_similarity(na, nb, c_sim):
+                    consider[member] = 0
+                    labels[member] = current
+                    queue.append(member)
+            if head == len(queue):
+                break
+            point = queue[head]
+            head += 1
+            na = neigh[point]
+        current += 1
+
+
+def fit_components(row_ptr: np.ndarray, col: np.ndarray, params: dict,
+                   labels: np.ndarray, consider: np.ndarray) -> None:
+    """Connected-components restatement of the same fit.
+
+    Points that are considered and have more than ``n_member_cutoff``
+    neighbours are core; an edge joins core points i, j when j is in N(i) and
+    the similarity check passes.  Components are numbered from
+    ``current_start`` in ascending order of their smallest point index, which
+    is the order in which the BFS of _fit.pyx:421-491 opens clusters.  After
+    the fit every ``consider`` flag is 0, as in the BFS (:422).
+    """
+    n = len(row_ptr) - 1
+    m_cut = params["n_member_cutoff"]
+    c_sim = params["similarity_cutoff"]
+    deg = np.diff(row_ptr)
+    core = (deg > m_cut) & (np.asarray(consider) != 0)
+    parent = np.arange(n, dtype=np.int64)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for i in np.nonzero(core)[0]:
+        ni = col[row_ptr[i]:row_ptr[i + 1]]
+        for j in ni:
+            if j <= i or not core[j]:
+                continue
+            ri, rj = find(i), find(j)
+            if ri == rj:
+                continue
+            if check_similarity(ni, col[row_ptr[j]:row_ptr[j + 1]], c_sim):
+                lo, hi = (ri, rj) if ri < rj else (rj, ri)
+                parent[hi] = lo
+    current = params["current_start"]
+    label_of_root = {}
+    for i in np.nonzero(core)[0]:   # ascending i: first visit of a root = its min index
+        r = find(i)
+        if r not in label_of_root:
+            label_of_root[r] = current
+            current += 1
+        labels[i] = label_of_root[r]
+    consider[:] = 0
+
+
+# ---------------------------------------------------------------- sort_by_size
+
+def sort_by_size(labels: np.ndarray, member_cutoff=None, max_clusters=None) -> None:
+    """Restates ``Labels.sort_by_size`` (_types.pyx:212-286) in place.
+
+    ``Counter(labels).most_common()`` sorts by count descending and keeps
+    first-occurrence order among equal counts; label 0 is dropped first
+    (:239-241).  Ranks beyond ``max_clusters`` and clusters smaller than
+    ``member_cutoff`` (default 2, :235-237) become 0; survivors are
+    renumbered 1, 2, ... (:254-268).
+    """
+    if member_cutoff is None:
+        member_cutoff = 2
+    frequencies = Counter(labels.tolist())
+    frequencies.pop(0, None)
+    if not frequencies:
+        return
+    if max_clusters is None:
+        max_clusters = len(frequencies)
+    reassign = {0: 0}
+    new_labels = count(1)
+    for rank, (old, size) in enumerate(frequencies.most_common(), 1):
+        if rank > max_clusters:
+            reassign[old] = 0
+        elif size >= member_cutoff:
+            reassign[old] = next(new_labels)
+        else:
+            reassign[old] = 0
+    labels[:] = np.array([reassign[x] for x in labels.tolist()], dtype=labels.dtype)
+
+
+def sort_by_size_np(labels: np.ndarray, member_cutoff=None, max_clusters=None) -> None:
+    """Vectorised equivalent of :func:`sort_by_size` for large n.
+
+    Key per cluster: (size descending, first occurrence ascending), which is
+    exactly the order ``Counter.most_common`` yields (stable sort on a
+    first-occurrence-ordered mapping).
+    """
+    if member_cutoff is None:
+        member_cutoff = 2
+    lab = np.asarray(labels)
+    uniq, first, inv, cnt = np.unique(lab, return_index=True, return_inverse=True,
+                                      return_counts=True)
+    keep = uniq != 0
+    if not keep.any():
+        return
+    if max_clusters is None:
+        max_clusters = int(keep.sum())
+    idx = np.nonzero(keep)[0]
+    order = idx[np.lexsort((first[idx], -cnt[idx]))]
+    new = np.zeros(len(uniq), dtype=np.int64)
+    ranks = np.arange(1, len(order) + 1)
+    ok = (ranks <= max_clusters) & (cnt[order] >= member_cutoff)
+    new[order[ok]] = np.arange(1, int(ok.sum()) + 1)
+    labels[:] = new[inv].astype(labels.dtype)
+
+
+# ---------------------------------------------------------------- record / fit
+
+def make_record(labels: np.ndarray, n_points: int, radius_cutoff, cnn_cutoff,
+                member_cutoff, max_clusters, execution_time=None) -> dict:
+    """The ``Record`` fields computed at cluster.pyx:366-390 (report.py:91-191)."""
+    freq = Counter(np.asarray(labels).tolist())
+    n_noise = freq.pop(0, 0)
+    n_largest = freq.most_common(1)[0][1] if freq else 0
+    return {
+        "n_points": n_points,
+        "radius_cutoff": radius_cutoff,
+        "cnn_cutoff": cnn_cutoff,
+        "member_cutoff": member_cutoff,
+        "max_clusters": max_clusters,
+        "n_clusters": len(set(np.asarray(labels).tolist()) - {0}),
+        "ratio_largest": n_largest / n_points if n_points else 0.0,
+        "ratio_noise": n_noise / n_points if n_points else 0.0,
+        "execution_time": execution_time,
+    }
+
+
+def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
+        cnn_offset=None, sort=True, method="components"):
+    """End-to-end restatement of ``Clustering.fit`` (cluster.pyx:256-392).
+
+    Returns ``(labels int64[n], record dict)``.  ``method`` selects the
+    literal BFS (small n) or the connected-components form.
+    """
+    X = prepare_points(data)
+    if cnn_offset is None:
+        cnn_offset = 0
+    cnn_cutoff = cnn_cutoff - cnn_offset
+    assert cnn_cutoff >= 0
+    n = X.shape[0]
+    labels = np.zeros(n, dtype=np.int64)
+    consider = np.ones(n, dtype=np.uint8)
+    params = make_parameters(radius_cutoff, cnn_cutoff, 1)
+    if method == "bfs":
+        fit_bfs(neighbourhoods(X, radius_cutoff), params, labels, consider)
+    else:
+        row_ptr, col = neighbourhoods_csr(X, radius_cutoff)
+        fit_components(row_ptr, col, params, labels, consider)
+    if sort:
+        sort_by_size_np(labels, member_cutoff, max_clusters)
+    rec = make_record(labels, n, radius_cutoff, cnn_cutoff, member_cutoff, max_clusters)
+    return labels, rec
+
+
+def gamma(m: int, u: float) -> float:
+    """Higham's gamma_m = m u / (1 - m u)."""
+    return m * u / (1.0 - m * u)
+
+
+def exact_reduced_distance(a, b):
+    """Exact rational value of sum_k (a_k - b_k)^2 as a Python Fraction."""
+    from fractions import Fraction
+    return sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(a, b))
+
+

@@ -1,0 +1,153 @@
+"""Extract the reference's known-answer vectors into tests/golden/kats.json.
+
+Reads the reference's documentation and test files as TEXT (nothing is
+imported or executed from the reference).  The fixture holds data only:
+inputs, parameters and expected outputs, each with its source file:line.
+
+Usage: python tests/golden/make_golden.py [/path/to/reference]
+"""
+
+import ast
+import json
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kats.json")
+
+
+def read(rel):
+    with open(os.path.join(REF, rel)) as fh:
+        return fh.read()
+
+
+def line_of(text, needle):
+    return text[: text.index(needle)].count("\n") + 1
+
+
+def quickstart():
+    text = read("README.md")
+    block = text[text.index("data_points = ["):]
+    block = block[: block.index("]\n") + 1]
+    pts = ast.literal_eval(re.sub(r"#.*", "", block.split("=", 1)[1]).replace("...", ""))
+    m = re.search(r"fit\(radius_cutoff=([\d.]+), cnn_cutoff=(\d+)", text)
+    lab = re.search(r"array\((\[[\d, ]+\])\)", text)
+    return {
+        "source": f"README.md:{line_of(text, 'data_points = [')}-{line_of(text, lab.group(0))}",
+        "points": pts,
+        "radius_cutoff": float(m.group(1)),
+        "cnn_cutoff": int(m.group(2)),
+        "labels": ast.literal_eval(lab.group(1)),
+    }
+
+
+def basic_usage():
+    nb = json.loads(read("docsrc/tutorial/basic_usage.ipynb"))
+    raw = read("docsrc/tutorial/basic_usage.ipynb")
+    out = {}
+    for cell in nb["cells"]:
+        if cell["cell_type"] != "code":
+            continue
+        src = "".join(cell["source"])
+        if src.strip() == "clustering.labels":
+            for o in cell.get("outputs", []):
+                txt = "".join(o.get("data", {}).get("text/plain", []))
+                if txt.startswith("array"):
+                    out["labels_r2"] = ast.literal_eval(txt[len("array("):-1])
+        if "fit(radius_cutoff=2.0" in src or "fit(radius_cutoff=1.5" in src:
+            for o in cell.get("outputs", []):
+                for line in "".join(o.get("text", [])).splitlines():
+                    if line.startswith("12 "):
+                        f = line.split()
+                        out.setdefault("records", []).append({
+                            "n_points": int(f[0]), "radius_cutoff": float(f[1]), "cnn_cutoff": int(f[2]),
+                            "n_clusters": int(f[5]), "ratio_largest": float(f[6]), "ratio_noise": float(f[7])})
+    seen = []
+    for r in out.get("records", []):
+        if r not in seen:
+            seen.append(r)
+    out["records"] = seen
+    out["source"] = f"docsrc/tutorial/basic_usage.ipynb:{line_of(raw, 'array([1, 1, 1, 1, 1, 1, 1, 0, 0, 2, 2, 2])')}"
+    return out
+
+
+def sklearn_records():
+    raw = read("docsrc/tutorial/scikit_learn_datasets.ipynb")
+    nb = json.loads(raw)
+    recs = []
+    for cell in nb["cells"]:
+        src = "".join(cell.get("source", []))
+        if cell["cell_type"] == "code" and "clustering = cluster.Clustering(data)" in src and \
+                "clustering.fit(**params)" in src:
+            for o in cell.get("outputs", []):
+                for line in "".join(o.get("text", [])).splitlines():
+                    if line.startswith("2000 "):
+                        f = line.split()
+                        recs.append({"n_points": 2000, "radius_cutoff": float(f[1]), "cnn_cutoff": int(f[2]),
+                                     "member_cutoff": int(f[3]), "n_clusters": int(f[5]),
+                                     "ratio_largest": float(f[6]), "ratio_noise": float(f[7]),
+                                     "execution_time_s": f[8]})
+            break
+    names = ["circles", "moons", "varied", "aniso", "blobs", "None"]
+    for r, name in zip(recs, names):
+        r["dataset"] = name
+    return {"source": "docsrc/tutorial/scikit_learn_datasets.ipynb:384-414 (data generation :205-270, "
+                      "parameters :285-322)", "records": recs}
+
+
+def similarity_cases():
+    text = read("test/unit/test_check_similarity.py")
+    block = text[text.index('"members_a,members_b,c,is_sorted,expected"'):]
+    block = block[block.index("["): block.index("]\n    )") + 1]
+    cases = ast.literal_eval(block)
+    return {"source": f"test/unit/test_check_similarity.py:{line_of(text, 'members_a,members_b,c,is_sorted,expected')}",
+            "cases": [{"a": a, "b": b, "c": c, "sorted": s, "expected": e} for a, b, c, s, e in cases]}
+
+
+def sort_cases():
+    text = read("test/unit/test_types.py")
+    i = text.index("def test_sort_by_size")
+    head = text[:i]
+    j = head.rindex('"kwargs,expected"')
+    block = head[head.index("[", j): head.rindex("]") + 1]
+    cases = ast.literal_eval(block)
+    labels = ast.literal_eval(re.search(r"np\.array\((\[[\d, ]+\]), dtype=P_AINDEX", text[i:]).group(1))
+    return {"source": f"test/unit/test_types.py:{line_of(text, chr(34) + 'kwargs,expected' + chr(34))}-"
+                      f"{line_of(text, 'def test_sort_by_size') + 10}",
+            "labels": labels, "cases": [{"kwargs": k, "expected": e} for k, e in cases]}
+
+
+def params_repr():
+    text = read("test/unit/test_types/test_create_params_0_1_1_.txt")
+    return {"source": "test/unit/test_types/test_create_params_0_1_1_.txt:1",
+            "args": [0.1, 1, 1], "repr": text.strip().splitlines()[0]}
+
+
+def metric_points():
+    text = read("test/unit/test_metric.py")
+    i = text.index("InputDataExtComponentsMemoryview,\n                np.array([")
+    blk = text[i:]
+    arrs = re.findall(r"np\.array\(\[(.*?)\], order", blk, re.S)[:2]
+    data = [ast.literal_eval("[" + a + "]") for a in arrs]
+    return {"source": f"test/unit/test_metric.py:{text[:i].count(chr(10)) + 1}",
+            "data": data[0], "other": data[1], "significant": 12}
+
+
+def main():
+    kats = {
+        "quickstart": quickstart(),
+        "basic_usage": basic_usage(),
+        "sklearn_records": sklearn_records(),
+        "similarity": similarity_cases(),
+        "sort_by_size": sort_cases(),
+        "params_repr": params_repr(),
+        "metric": metric_points(),
+    }
+    with open(OUT, "w") as fh:
+        json.dump(kats, fh, indent=1)
+    print(f"wrote {OUT}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+"""Phases S+U and the fit path: GPU labels equal the oracle's bit-for-bit."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle as C
+from oracle import cnn_oracle as O
+from kat_data import equalise_labels, kats, sklearn_tutorial_datasets
+from gpu_util import gpu_fit, to_dev
+
+pytestmark = pytest.mark.gpu
+K = kats()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+def test_blobs_2d_labels(cuda, precision):
+    from commonnnclustering_amd.synthetic import blobs_2d
+
+    X, p = blobs_2d()
+    want = C.fit(X, p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"])
+    got, gs, ls = gpu_fit(X, p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"],
+                          precision=precision)
+    assert (got == want).all()
+
+
+def test_quickstart_via_clustering(cuda):
+    from commonnnclustering_amd import Clustering
+
+    q = K["quickstart"]
+    cl = Clustering(q["points"])
+    cl.fit(radius_cutoff=q["radius_cutoff"], cnn_cutoff=q["cnn_cutoff"], v=False)
+    assert cl.labels.tolist() == q["labels"]
+    cl.fit(radius_cutoff=2.0, cnn_cutoff=1, v=False)
+    assert cl.labels.tolist() == K["basic_usage"]["labels_r2"]
+    recs = K["basic_usage"]["records"]
+    got = [(r.n_clusters, round(r.ratio_largest, 3), round(r.ratio_noise, 3)) for r in cl.summary]
+    want = [(r["n_clusters"], r["ratio_largest"], r["ratio_noise"]) for r in recs]
+    assert sorted(got) == sorted(want)
+
+
+@pytest.mark.parametrize("recipe", ["coordinates", "coordinates_f32"])
+def test_sklearn_tutorial_records(cuda, recipe):
+    from commonnnclustering_amd import Clustering
+
+    data = sklearn_tutorial_datasets()
+    for rec in K["sklearn_records"]["records"]:
+        X = data[rec["dataset"]]
+        cl = Clustering(X, recipe=recipe)
+        cl.fit(rec["radius_cutoff"], rec["cnn_cutoff"], member_cutoff=rec["member_cutoff"], v=False)
+        want = C.fit(X, rec["radius_cutoff"], rec["cnn_cutoff"], member_cutoff=rec["member_cutoff"])
+        assert (cl.labels == want).all(), rec["dataset"]
+        r = cl.summary[-1]
+        assert r.n_clusters == rec["n_clusters"]
+        assert f"{r.ratio_largest:.3f}" == f"{rec['ratio_largest']:.3f}"
+        assert f"{r.ratio_noise:.3f}" == f"{rec['ratio_noise']:.3f}"
+
+
+@pytest.mark.parametrize("case", K["sort_by_size"]["cases"])
+def test_sort_by_size_kat_gpu(cuda, case):
+    from commonnnclustering_amd import Labels
+
+    lab = Labels(np.array(K["sort_by_size"]["labels"], dtype=np.int64))
+    lab.sort_by_size(**case["kwargs"])
+    assert lab.labels.tolist() == case["expected"]
+
+
+def test_sort_by_size_random_ties(cuda):
+    from commonnnclustering_amd import engine
+
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        lab = (rng.integers(0, 30, size=int(rng.integers(1, 5000))) * int(rng.integers(1, 10**6))).astype(np.int64)
+        mc = int(rng.integers(1, 5))
+        mx = None if trial % 3 else int(rng.integers(1, 10))
+        want = lab.copy()
+        O.sort_by_size(want, mc, mx)
+        dev = to_dev(lab)
+        k = engine.sort_by_size(dev, mc, mx)
+        assert (dev.cpu().numpy() == want).all()
+        assert k == len(set(want.tolist()) - {0})
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_labels_consider_current_start(cuda, seed):
+    """In-place semantics of FitterExtBFS._fit: consider honoured, current_start numbering,
+    untouched entries kept, consider zeroed."""
+    from commonnnclustering_amd import engine
+
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(100, 3000))
+    d = int(rng.integers(1, 4))
+    X = np.concatenate([rng.normal(0, 0.5, size=(n // 2, d)), rng.uniform(-3, 3, size=(n - n // 2, d))])
+    r = float(rng.uniform(0.15, 0.6))
+    c = int(rng.integers(0, 10))
+    start = int(rng.integers(1, 50))
+    consider = (rng.uniform(size=n) > 0.2).astype(np.uint8)
+    init = np.where(consider == 0, rng.integers(1, 5, size=n), 0).astype(np.int64)
+    rp, col = C.neighbours_csr(X, r)
+    want, _ = C.labels_from_csr(rp, col, c + 2, c + 1, consider=consider, current_start=start,
+                                labels=init.copy())
+    g = engine.build_graph(to_dev(X), r * r, "f64")
+    lab_d = to_dev(init.copy())
+    con_d = to_dev(consider.copy())
+    g.labels(c + 2, c + 1, consider=con_d, current_start=start, labels=lab_d)
+    assert (lab_d.cpu().numpy() == want).all()
+    assert not con_d.cpu().numpy().any()
+    # the same through a caller CSR
+    lab2 = to_dev(init.copy())
+    con2 = to_dev(consider.copy())
+    engine.labels_from_csr(to_dev(rp), to_dev(col), c + 2, c + 1, consider=con2, current_start=start,
+                           labels=lab2)
+    assert (lab2.cpu().numpy() == want).all()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_fits(cuda, seed):
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(200, 6000))
+    d = int(rng.choice([1, 2, 3, 5, 17, 40]))
+    k = int(rng.integers(1, 6))
+    centres = rng.uniform(-5, 5, size=(k, d))
+    X = (centres[rng.integers(0, k, size=n)] + rng.normal(size=(n, d)) * rng.uniform(0.3, 1.0)).astype(
+        np.float32 if seed % 2 else np.float64)
+    D = np.sqrt(O.reduced_distances(X[:300], X[:300]))
+    r = float(np.percentile(D[D > 0], rng.uniform(1, 10)))
+    c = int(rng.integers(0, 30))
+    mc = int(rng.integers(1, 20))
+    want = C.fit(X, r, c, member_cutoff=mc, use_grid=d <= 3)
+    for precision in ("f64", "f32_band"):
+        got, gs, ls = gpu_fit(X, r, c, member_cutoff=mc, precision=precision)
+        assert (got == want).all(), precision
+        # unsorted numbering by smallest index
+        got_u, _, _ = gpu_fit(X, r, c, precision=precision, sort=False)
+        want_u = C.fit(X, r, c, use_grid=d <= 3, sort=False)
+        assert (got_u == want_u).all()
+
+
+def test_deterministic_repeats(cuda):
+    from commonnnclustering_amd.synthetic import blobs_2d
+
+    X, p = blobs_2d()
+    first = None
+    for _ in range(5):
+        got, _, _ = gpu_fit(X, p["radius_cutoff"], 2, member_cutoff=2)
+        if first is None:
+            first = got
+        assert (got == first).all()
+
+
+def test_moons_against_generator_truth(cuda):
+    from sklearn import datasets
+    from sklearn.preprocessing import StandardScaler
+
+    X, truth = datasets.make_moons(n_samples=1500)
+    X = StandardScaler().fit_transform(X)
+    truth = truth + 1
+    got, _, _ = gpu_fit(X, 0.2, 5)
+    found = equalise_labels(got)
+    truth[found == 0] = 0
+    assert (found == equalise_labels(truth)).all()
+
+
+def test_neighbourhood_recipes(cuda):
+    """Fit from precomputed neighbour lists (recipes "neighbourhoods" / "sorted_neighbourhoods")."""
+    from commonnnclustering_amd import Clustering
+
+    data = sklearn_tutorial_datasets()
+    X = data["moons"]
+    r, c = 0.5, 20
+    lists = O.neighbourhoods(X, r)
+    want = C.fit(X, r, c, member_cutoff=2)
+    for recipe in ("sorted_neighbourhoods", "neighbourhoods"):
+        shuffled = [np.random.default_rng(i).permutation(l) for i, l in enumerate(lists)]
+        cl = Clustering(shuffled if recipe == "neighbourhoods" else lists, recipe=recipe)
+        cl.fit(r, c, v=False)
+        assert (cl.labels == want).all(), recipe

@@ -1,0 +1,96 @@
+"""Phase N parity: GPU neighbour lists equal the oracle's bit-for-bit."""
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle as C
+from oracle import cnn_oracle as O
+from gpu_util import gpu_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_csr(X, r):
+    X64 = np.asarray(X, dtype=np.float64)
+    rp, col = C.neighbours_csr(X64, r, use_grid=X64.shape[1] <= 3)
+    return rp, col.astype(np.int64)
+
+
+def assert_same(X, r, precision):
+    rp, col, st = gpu_csr(X, r, precision)
+    rp0, col0 = oracle_csr(X, r)
+    assert rp.shape == rp0.shape
+    assert (rp == rp0).all(), "degrees differ"
+    assert (col == col0).all(), "neighbour lists differ"
+    return st
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+def test_blobs_2d_cfg(cuda, precision):
+    from commonnnclustering_amd.synthetic import blobs_2d
+
+    X, p = blobs_2d()
+    assert_same(X, p["radius_cutoff"], precision)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+@pytest.mark.parametrize("n,d,dtype", [(1, 2, np.float64), (63, 3, np.float32), (64, 1, np.float64),
+                                       (65, 2, np.float32), (500, 3, np.float64), (1000, 5, np.float32),
+                                       (777, 8, np.float64), (600, 16, np.float32), (300, 64, np.float32),
+                                       (257, 70, np.float64), (2000, 2, np.float32)])
+def test_random_shapes(cuda, precision, n, d, dtype):
+    rng = np.random.default_rng(n * 100 + d)
+    X = np.concatenate([rng.normal(0, 1, size=(n // 2, d)), rng.uniform(-3, 3, size=(n - n // 2, d))])
+    X = X.astype(dtype)
+    # radius at a low percentile of pair distances
+    D = np.sqrt(O.reduced_distances(X[:200], X[:200]))
+    r = float(np.percentile(D[D > 0], 5)) if n > 1 else 1.0
+    assert_same(X, r, precision)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+def test_boundary_pairs_exact(cuda, precision):
+    """Pairs at exactly r and one ulp either side, in FP64 and FP32 inputs."""
+    r = 1.5  # r*r = 2.25 exactly
+    base = []
+    for k, dist in enumerate([1.5, np.nextafter(1.5, 2.0), np.nextafter(1.5, 1.0)]):
+        base += [[0.0, 10.0 * k], [dist, 10.0 * k]]
+    # diagonal pair with d2 = 0.81 + 1.44 = 2.25 in exact arithmetic
+    base += [[100.0, 100.0], [100.9, 101.2]]
+    X = np.array(base, dtype=np.float64)
+    st = assert_same(X, r, precision)
+    X32 = X.astype(np.float32)
+    assert_same(X32, r, precision)
+    if precision == "f32_band":
+        assert st["band_hits"] > 0
+
+
+def test_band_hits_forced(cuda):
+    """Many pairs placed inside the FP32 guard band must come out exact."""
+    rng = np.random.default_rng(5)
+    n = 4096
+    d = 3
+    A = rng.normal(size=(n // 2, d))
+    u = rng.normal(size=(n // 2, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 0.5
+    eps = rng.uniform(-1e-6, 1e-6, size=(n // 2, 1))
+    B = A + u * (r + eps)
+    X = np.concatenate([A, B]).astype(np.float64)
+    st = assert_same(X, r, "f32_band")
+    assert st["band_hits"] > 100
+
+
+def test_duplicates_and_nonfinite(cuda):
+    X = np.array([[0, 0], [0, 0], [np.nan, 0], [np.inf, 1], [0.1, 0.1], [-np.inf, 5], [0.2, -0.1]],
+                 dtype=np.float64)
+    for precision in ("f64", "f32_band"):
+        rp, col, _ = gpu_csr(X, 0.5, precision)
+        rp0, col0 = O.neighbourhoods_csr(X, 0.5)
+        assert (rp == rp0).all() and (col == col0).all()
+
+
+def test_empty(cuda):
+    X = np.zeros((0, 3), dtype=np.float32)
+    rp, col, st = gpu_csr(X, 1.0)
+    assert rp.tolist() == [0] and col.size == 0

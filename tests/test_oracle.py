@@ -1,0 +1,225 @@
+"""Pins the CPU oracle (oracle/) against the reference's known answers and
+against mathematics independent of the oracle's own formulas."""
+
+from collections import Counter
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as O
+from oracle import c_oracle as C
+from kat_data import equalise_labels, kats, sklearn_tutorial_datasets
+
+K = kats()
+
+
+# ------------------------------------------------------------------ metric
+
+@pytest.mark.parametrize("d", [1, 2, 3, 7, 16, 64])
+def test_metric_matches_scalar_loop(d):
+    rng = np.random.default_rng(d)
+    A = rng.normal(size=(40, d)) * rng.uniform(0.1, 100)
+    B = rng.normal(size=(30, d))
+    D = O.reduced_distances(A, B)
+    for i in range(0, 40, 7):
+        for j in range(0, 30, 5):
+            # literal IEEE-double loop: total = total + (a-b)*(a-b)
+            t = 0.0
+            for k in range(d):
+                diff = float(A[i, k]) - float(B[j, k])
+                t = t + diff * diff
+            assert D[i, j] == t  # bit-identical
+
+
+def test_metric_error_bound_against_exact():
+    rng = np.random.default_rng(7)
+    u = 2.0 ** -53
+    for d in (2, 3, 64):
+        A = rng.normal(size=(20, d))
+        D = O.reduced_distances(A, A)
+        for i in range(20):
+            for j in range(20):
+                exact = O.exact_reduced_distance(A[i], A[j])
+                err = abs(Fraction(D[i, j]) - exact)
+                assert err <= Fraction(O.gamma(2 * d, u)) * exact
+
+
+def test_metric_kat_points_vs_closed_form():
+    m = K["metric"]
+    X = np.array(m["data"], dtype=np.float64)
+    Y = np.array(m["other"], dtype=np.float64)
+    D = O.reduced_distances(Y, X)
+    for i in range(len(Y)):
+        for j in range(len(X)):
+            ref = sum((a - b) ** 2 for a, b in zip(Y[i], X[j]))
+            np.testing.assert_approx_equal(D[i, j], ref, significant=m["significant"])
+
+
+def test_adjust_radius_is_square():
+    for r in (0.1, 0.25, 1.5, 9.99, 1e-300, 1e200):
+        assert O.adjust_radius(r) == r * r
+
+
+# ------------------------------------------------------------------ neighbourhoods
+
+def test_boundary_inclusive_and_one_ulp_beyond():
+    r = 1.5  # r*r = 2.25 exactly
+    far = np.nextafter(1.5, 2.0)
+    X = np.array([[0.0, 0.0], [1.5, 0.0], [0.0, far], [-1.5, 0.0]])
+    N = O.neighbourhoods(X, r)
+    assert N[0].tolist() == [0, 1, 3]       # exactly at r: neighbour (<=)
+    assert 2 not in N[0].tolist()           # one ulp beyond: not
+    assert N[2].tolist() == [2]
+
+
+def test_neighbourhoods_symmetric_and_self():
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(300, 3))
+    N = O.neighbourhoods(X, 0.6)
+    for i, row in enumerate(N):
+        assert i in row.tolist()
+        assert np.all(np.diff(row) > 0)
+        for j in row:
+            assert i in N[j].tolist()
+
+
+# ------------------------------------------------------------------ similarity
+
+@pytest.mark.parametrize("case", K["similarity"]["cases"])
+def test_similarity_kat(case):
+    assert O.check_similarity(case["a"], case["b"], case["c"]) == case["expected"]
+    if case["sorted"]:
+        a = np.array(case["a"], dtype=np.int32)
+        b = np.array(case["b"], dtype=np.int32)
+        assert bool(C.lib().oracle_check_similarity(a, len(a), b, len(b), case["c"])) == case["expected"]
+
+
+# ------------------------------------------------------------------ sort_by_size
+
+@pytest.mark.parametrize("case", K["sort_by_size"]["cases"])
+def test_sort_by_size_kat(case):
+    for fn in (O.sort_by_size, O.sort_by_size_np):
+        lab = np.array(K["sort_by_size"]["labels"], dtype=np.int64)
+        fn(lab, **case["kwargs"])
+        assert lab.tolist() == case["expected"]
+    lab = np.array(K["sort_by_size"]["labels"], dtype=np.int64)
+    kw = case["kwargs"]
+    C.sort_by_size(lab, kw.get("member_cutoff"), kw.get("max_clusters"))
+    assert lab.tolist() == case["expected"]
+
+
+def test_sort_by_size_ties_follow_counter_most_common():
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        lab = rng.integers(0, 12, size=rng.integers(1, 80)).astype(np.int64) * rng.integers(1, 1000)
+        # Counter.most_common is the definition of the order (stable on first occurrence)
+        freq = Counter(lab.tolist())
+        freq.pop(0, None)
+        order = [k for k, _ in freq.most_common()]
+        expect = lab.copy()
+        remap = {0: 0}
+        nxt = 1
+        for k in order:
+            if freq[k] >= 2:
+                remap[k] = nxt
+                nxt += 1
+            else:
+                remap[k] = 0
+        expect = np.array([remap[x] for x in lab.tolist()], dtype=np.int64)
+        for fn in (O.sort_by_size, O.sort_by_size_np):
+            got = lab.copy()
+            fn(got)
+            assert got.tolist() == expect.tolist()
+        got = lab.copy()
+        C.sort_by_size(got)
+        assert got.tolist() == expect.tolist()
+
+
+# ------------------------------------------------------------------ end-to-end KATs
+
+def test_readme_quickstart():
+    q = K["quickstart"]
+    for method in ("bfs", "components"):
+        labels, _ = O.fit(q["points"], q["radius_cutoff"], q["cnn_cutoff"], method=method)
+        assert labels.tolist() == q["labels"]
+
+
+def test_basic_usage_records():
+    b = K["basic_usage"]
+    pts = K["quickstart"]["points"]
+    labels, _ = O.fit(pts, 2.0, 1)
+    assert labels.tolist() == b["labels_r2"]
+    for rec in b["records"]:
+        _, got = O.fit(pts, rec["radius_cutoff"], rec["cnn_cutoff"])
+        assert got["n_clusters"] == rec["n_clusters"]
+        assert round(got["ratio_largest"], 3) == rec["ratio_largest"]
+        assert round(got["ratio_noise"], 3) == rec["ratio_noise"]
+
+
+def test_sklearn_tutorial_records():
+    data = sklearn_tutorial_datasets()
+    for rec in K["sklearn_records"]["records"]:
+        X = data[rec["dataset"]]
+        labels = C.fit(X, rec["radius_cutoff"], rec["cnn_cutoff"], member_cutoff=rec["member_cutoff"])
+        got = O.make_record(labels, len(X), rec["radius_cutoff"], rec["cnn_cutoff"], rec["member_cutoff"], None)
+        assert got["n_clusters"] == rec["n_clusters"], rec["dataset"]
+        assert f"{got['ratio_largest']:.3f}" == f"{rec['ratio_largest']:.3f}", rec["dataset"]
+        assert f"{got['ratio_noise']:.3f}" == f"{rec['ratio_noise']:.3f}", rec["dataset"]
+
+
+def test_moons_against_generator_truth():
+    """test_procedures.py:127-174, moons case: max_diff 0 against the generator labels."""
+    from sklearn import datasets
+    from sklearn.preprocessing import StandardScaler
+
+    X, truth = datasets.make_moons(n_samples=1500)
+    X = StandardScaler().fit_transform(X)
+    truth = truth + 1
+    labels = C.fit(X, 0.2, 5)
+    found = equalise_labels(labels)
+    truth[found == 0] = 0
+    assert (found == equalise_labels(truth)).all()
+
+
+# ------------------------------------------------------------------ BFS == components
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bfs_equals_components(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(50, 400))
+    d = int(rng.integers(1, 5))
+    X = np.concatenate([rng.normal(rng.uniform(-3, 3, d), rng.uniform(0.2, 1.0), size=(n // 2, d)),
+                        rng.uniform(-4, 4, size=(n - n // 2, d))])
+    r = float(rng.uniform(0.2, 1.2))
+    c = int(rng.integers(0, 8))
+    params = O.make_parameters(r, c, current_start=int(rng.integers(1, 5)))
+    consider = (rng.uniform(size=n) > 0.1).astype(np.uint8)
+    neigh = O.neighbourhoods(X, r)
+    row_ptr, col = O.neighbourhoods_csr(X, r)
+    l_bfs = np.zeros(n, dtype=np.int64)
+    c_bfs = consider.copy()
+    O.fit_bfs(neigh, params, l_bfs, c_bfs)
+    l_cc = np.zeros(n, dtype=np.int64)
+    c_cc = consider.copy()
+    O.fit_components(row_ptr, col, params, l_cc, c_cc)
+    assert (l_bfs == l_cc).all()
+    assert (c_bfs == c_cc).all() and not c_cc.any()
+    # C restatements (literal BFS and parallel components) agree as well
+    rp32, col32 = row_ptr, col.astype(np.int32)
+    for method in ("bfs", "cc"):
+        lab, con = C.labels_from_csr(rp32, col32, params["similarity_cutoff"], params["n_member_cutoff"],
+                                     consider=consider, current_start=params["current_start"], method=method)
+        assert (lab == l_bfs).all(), method
+        assert not con.any()
+
+
+@pytest.mark.parametrize("d,use_grid", [(1, True), (2, True), (3, True), (3, False), (8, False)])
+def test_c_oracle_neighbourhoods_match_numpy(d, use_grid):
+    rng = np.random.default_rng(d)
+    X = rng.normal(size=(700, d))
+    X[5] = X[6]          # duplicate point
+    X[7, 0] = X[8, 0] + 0.5
+    rp, col = O.neighbourhoods_csr(X, 0.5)
+    rp2, col2 = C.neighbours_csr(X, 0.5, use_grid=use_grid)
+    assert (rp == rp2).all() and (col == col2).all()
