@@ -103,8 +103,11 @@ def main():
     X, p = workload(args.workload)
     n, d = X.shape
     Xd = torch.from_numpy(X).cuda()
+    print(f"[bench] workload {args.workload}: n={n} d={d}", file=sys.stderr, flush=True)
     t_band, lab_band, gs_band, ls_band = run_fits(Xd, p, "f32_band", args.steps, args.warmup)
+    print(f"[bench] f32_band: {t_band * 1e3:.2f} ms/fit", file=sys.stderr, flush=True)
     t_plain, lab_plain, gs_plain, ls_plain = run_fits(Xd, p, "f64", args.steps, max(1, args.warmup // 2))
+    print(f"[bench] plain f64: {t_plain * 1e3:.2f} ms/fit", file=sys.stderr, flush=True)
     identical = bool(np.array_equal(lab_band, lab_plain))
 
     # dominant kernel of the banded fit and its roofline
@@ -132,6 +135,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(X, p, args.cpu_sample)
+        print(f"[bench] cpu baseline: {cpu['sample']}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "fit points/s (FP32-banded, bit-exact labels)",
@@ -152,6 +156,7 @@ def main():
         "band_vs_plain": t_plain / t_band,
         "labels_identical": identical,
         "n_clusters": int(ls_band[-1]["n_clusters"]),
+        "label_stats": ls_band[-1],
         "nnz": int(gs_band[-1]["nnz"]),
         "band_hits": int(gs_band[-1]["band_hits"]),
         "band_flips": int(gs_band[-1]["band_flips"]),

@@ -85,6 +85,8 @@ class LabelStats(ctypes.Structure):
         ("n_clusters", ctypes.c_int64),
         ("tests", ctypes.c_int64),
         ("tests_passed", ctypes.c_int64),
+        ("sampled_components", ctypes.c_int64),
+        ("rows_scanned", ctypes.c_int64),
         ("ms_similarity", ctypes.c_float),
         ("ms_labels", ctypes.c_float),
     ]

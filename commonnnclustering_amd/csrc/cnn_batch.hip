@@ -32,7 +32,8 @@ int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n
             cnn_graph *g = graph_build(ctx, xb, n, d, dtype, r2[b], precision);
             std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
             LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
-                           true, true, member_cutoff, max_clusters, labels + offsets[b]};
+                           true, true, member_cutoff, max_clusters, labels + offsets[b],
+                           g->tp_ptr, g->tp_col, g->n_tiles};
             cnn_label_stats one{};
             labels_run(ctx, r, &one);
             gs.pairs_evaluated += g->stats.pairs_evaluated;

@@ -70,6 +70,9 @@ struct cnn_graph {
     float *xw32 = nullptr;       // working-order coordinates, FP32 (band precision)
     int64_t *row_ptr = nullptr;  // working CSR
     int32_t *col = nullptr;
+    int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout only)
+    int32_t *tp_col = nullptr;
+    int64_t n_tile_pairs = 0;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
 };
@@ -178,6 +181,10 @@ struct LabelRequest {
     int64_t member_cutoff;
     int64_t max_clusters;
     int64_t *labels;       // caller order
+    // optional tile structure of the working order (rows of tile t are 64t..64t+63)
+    const int64_t *tp_ptr = nullptr;  // candidate column tiles per row tile
+    const int32_t *tp_col = nullptr;
+    int64_t n_tiles = 0;              // 0: no tile structure (caller CSR)
 };
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st);
 int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,

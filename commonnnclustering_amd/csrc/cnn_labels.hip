@@ -12,14 +12,15 @@
 //
 // Work plan on the GPU (all on the context stream):
 //   k_init      core flags, parent[i] = i
-//   k_round_a   each core row tests its nearest core neighbours in list order
-//               (sampling; links most of every dense region cheaply)
-//   k_compress  parent[i] = root(i)
-//   k_largest   most populated root L; inL[i] snapshot
-//   k_round_b   every core row outside L tests each candidate edge whose
-//               endpoints are still in different components (edges inside L
-//               are already joined; edges between L and the rest are seen from
-//               the non-L endpoint; others from the smaller endpoint)
+//   k_hook      each core row tests its nearest smaller-index core neighbours
+//               and hooks its parent to the first that passes (a forest, no
+//               atomics); links most of every dense region cheaply
+//   k_jump      pointer jumping until every parent is a root
+//   k_hook_roots roots re-point at a smaller neighbouring root on a pass
+//   k_snapshot  rank[i] = (component size << 32 | root) of core points
+//   k_round_b   every edge between different snapshot components is tested by
+//               the endpoint of lower rank, unless the endpoints were joined
+//               meanwhile; rows that own no edge are skipped by a per-tile bound
 //   k_compress, component statistics, label assignment / sort_by_size
 #include <algorithm>
 #include <climits>
@@ -98,6 +99,8 @@ __device__ bool wave_sim(const int32_t *__restrict__ A, int64_t na, const int32_
     return false;
 }
 
+constexpr int kStatSlots = 1024;  // striped counters: no same-address atomic hot spot
+
 struct LabelArgs {
     const int64_t *row_ptr;
     const int32_t *col;
@@ -105,10 +108,22 @@ struct LabelArgs {
     int c_sim;
     int m_core;
     uint8_t *core;
-    uint8_t *inL;
     int *parent;
-    unsigned long long *stats;  // [0] tests, [1] passed
+    unsigned long long *stats;  // [slot][0] tests, [slot][1] passed
+    const uint64_t *rank;       // snapshot rank of core points: (component size << 32) | root; 0 otherwise
+    const uint64_t *cand_max;   // per row tile: max rank over its candidate tiles (nullable)
 };
+
+__device__ __forceinline__ void add_stats(const LabelArgs &a, unsigned tests, unsigned passed, unsigned rows = 0) {
+    if ((threadIdx.x & 63) == 0 && (tests || rows)) {
+        const unsigned slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStatSlots - 1);
+        if (tests) {
+            atomicAdd(&a.stats[4 * slot], (unsigned long long)tests);
+            atomicAdd(&a.stats[4 * slot + 1], (unsigned long long)passed);
+        }
+        if (rows) atomicAdd(&a.stats[4 * slot + 2], (unsigned long long)rows);
+    }
+}
 
 __global__ void k_init(LabelArgs a, const uint8_t *__restrict__ consider, const int32_t *__restrict__ perm) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,11 +135,13 @@ __global__ void k_init(LabelArgs a, const uint8_t *__restrict__ consider, const 
     a.parent[w] = (int)w;
 }
 
-__device__ __forceinline__ void test_and_link(const LabelArgs &a, int w, int j, int64_t bw, int64_t ew,
+// Test edge (w, j) unless the endpoints are already joined; link on a pass.
+// Returns true when w and j are in one component afterwards.
+__device__ __forceinline__ bool test_and_link(const LabelArgs &a, int w, int j, int64_t bw, int64_t ew,
                                               unsigned &tests, unsigned &passed) {
     const int rw = uf_find(a.parent, w);
     const int rj = uf_find(a.parent, j);
-    if (rw == rj) return;
+    if (rw == rj) return true;
     const int64_t bj = a.row_ptr[j], ej = a.row_ptr[j + 1];
     const bool pass = wave_sim(a.col + bw, ew - bw, a.col + bj, ej - bj, a.c_sim);
     ++tests;
@@ -132,11 +149,16 @@ __device__ __forceinline__ void test_and_link(const LabelArgs &a, int w, int j, 
         ++passed;
         if (__lane_id() == 0) uf_union(a.parent, w, j);
     }
+    return pass;
 }
 
-// Sampling round: test the first core neighbour after and before the row's own
-// position in its sorted list (spatially closest in the working order).
-__global__ __launch_bounds__(256) void k_round_a(LabelArgs a) {
+// Sampling round (hooking): each core row tests its nearest smaller-index core
+// neighbours in list order (the previous points of the working order, which
+// are spatially closest) and points its parent at the first one that passes.
+// Every row writes only its own parent entry and parents have smaller
+// indices, so the hooks form a forest whose trees are the components of the
+// passing edges tested here -- no atomics, no contention.
+__global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w]) return;
@@ -147,61 +169,163 @@ __global__ __launch_bounds__(256) void k_round_a(LabelArgs a) {
         if (a.col[mid] < (int32_t)w) lo = mid + 1;
         else hi = mid;
     }
-    const int64_t p = lo;
     unsigned tests = 0, passed = 0;
-    // forward
-    for (int64_t t0 = p; t0 < e && t0 < p + 256; t0 += 64) {
-        const int64_t t = t0 + lane;
-        const int j = t < e ? a.col[t] : -1;
-        const bool cand = (t < e) && j != (int)w && a.core[j];
-        const uint64_t bits = __ballot(cand);
-        if (bits) {
-            const int jj = __shfl(j, __ffsll((unsigned long long)bits) - 1, 64);
-            test_and_link(a, (int)w, jj, b, e, tests, passed);
-            break;
-        }
-    }
-    // backward
-    for (int64_t t1 = p; t1 > b && t1 > p - 256; t1 -= 64) {
+    int attempts = 0;
+    // entries [b, lo) are the smaller indices, nearest first when walked backwards
+    for (int64_t t1 = lo; t1 > b && attempts < max_tests; t1 -= 64) {
         const int64_t t = t1 - 1 - lane;
         const int j = t >= b ? a.col[t] : -1;
-        const bool cand = (t >= b) && j != (int)w && a.core[j];
-        const uint64_t bits = __ballot(cand);
-        if (bits) {
-            const int jj = __shfl(j, __ffsll((unsigned long long)bits) - 1, 64);
-            test_and_link(a, (int)w, jj, b, e, tests, passed);
-            break;
+        uint64_t bits = __ballot(t >= b && a.core[j]);
+        bool done = false;
+        while (bits && attempts < max_tests) {
+            const int src = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int jj = __shfl(j, src, 64);
+            const int64_t bj = a.row_ptr[jj], ej = a.row_ptr[jj + 1];
+            ++attempts;
+            ++tests;
+            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, a.c_sim)) {
+                ++passed;
+                if (lane == 0) a.parent[w] = jj;
+                done = true;
+                break;
+            }
         }
+        if (done) break;
     }
-    if (lane == 0 && tests) {
-        atomicAdd(&a.stats[0], (unsigned long long)tests);
-        atomicAdd(&a.stats[1], (unsigned long long)passed);
-    }
+    add_stats(a, tests, passed);
 }
 
+// Root hooking: a core point that is still a root tests core neighbours whose
+// current root is smaller than itself (either side of it in the list) and
+// re-points itself at that root on the first pass.  Only roots write their own
+// entry and only to smaller values, so the parents stay a forest.
+__global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= a.n || !a.core[w] || a.parent[w] != (int)w) return;
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    unsigned tests = 0, passed = 0;
+    int attempts = 0;
+    for (int64_t t0 = b; t0 < e && attempts < max_tests; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const int j = t < e ? a.col[t] : -1;
+        const int rj = (t < e && a.core[j]) ? a.parent[j] : INT_MAX;  // parents are flattened
+        uint64_t bits = __ballot(rj < (int)w);
+        bool done = false;
+        while (bits && attempts < max_tests) {
+            const int src = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int jj = __shfl(j, src, 64);
+            const int rr = __shfl(rj, src, 64);
+            const int64_t bj = a.row_ptr[jj], ej = a.row_ptr[jj + 1];
+            ++attempts;
+            ++tests;
+            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, a.c_sim)) {
+                ++passed;
+                if (lane == 0) a.parent[w] = rr;
+                done = true;
+                break;
+            }
+            bits &= ~__ballot(rj == rr);  // one test per neighbouring component
+        }
+        if (done) break;
+    }
+    add_stats(a, tests, passed);
+}
+
+// One in-place pointer-jumping pass: parent[w] <- parent[parent[w]].  Parents
+// only ever point to smaller indices, so ceil(log2(depth)) passes flatten
+// every tree; reading values already updated in the same pass only helps.
+__global__ void k_jump(int *parent, int64_t n) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int p = parent[w];
+    const int gp = parent[p];
+    if (gp != p) parent[w] = gp;
+}
+
+// Full round.  Every edge between two different snapshot components is owned
+// by the endpoint whose component ranks lower (rank = size << 32 | root, all
+// distinct), and only the owner examines it.  A row whose rank is at least the
+// largest rank among its candidate tiles owns no edge and is skipped without
+// reading its list.  Candidates are filtered with the read-only snapshot (one
+// 8-byte gather per entry); a stale snapshot only yields redundant candidates,
+// which test_and_link's find() check discards.
 __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= a.n || !a.core[w] || a.inL[w]) return;
+    if (w >= a.n) return;
+    const uint64_t rw = a.rank[w];
+    if (rw == 0) return;
+    if (a.cand_max && rw >= a.cand_max[w >> 6]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     unsigned tests = 0, passed = 0;
+    int resolved = -1;  // last snapshot component found joined to w
     for (int64_t t0 = b; t0 < e; t0 += 64) {
         const int64_t t = t0 + lane;
         const int j = t < e ? a.col[t] : -1;
-        bool cand = (t < e) && j != (int)w && a.core[j] && (j > (int)w || a.inL[j]);
-        if (cand) cand = uf_find(a.parent, (int)w) != uf_find(a.parent, j);
+        const uint64_t rj = t < e ? a.rank[j] : 0ull;
+        const int kj = (int)(uint32_t)rj;
+        const bool cand = rj > rw && kj != resolved;
         uint64_t bits = __ballot(cand);
         while (bits) {
             const int src = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
             const int jj = __shfl(j, src, 64);
-            test_and_link(a, (int)w, jj, b, e, tests, passed);
+            const int kjj = __shfl(kj, src, 64);
+            if (test_and_link(a, (int)w, jj, b, e, tests, passed)) {
+                // w has joined snapshot component kjj: its other entries are joined too
+                resolved = kjj;
+                bits &= ~__ballot(kj == kjj);
+            }
         }
     }
-    if (lane == 0 && tests) {
-        atomicAdd(&a.stats[0], (unsigned long long)tests);
-        atomicAdd(&a.stats[1], (unsigned long long)passed);
+    add_stats(a, tests, passed, 1);
+}
+
+// Largest snapshot rank among the core points of each tile (0: none).
+__global__ void k_tile_rank_max(const uint64_t *__restrict__ rank, int64_t n, int64_t n_tiles,
+                                uint64_t *__restrict__ tmax) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= n_tiles) return;
+    const int64_t w = t * 64 + lane;
+    uint64_t v = w < n ? rank[w] : 0ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
     }
+    if (lane == 0) tmax[t] = v;
+}
+
+// Largest rank over the candidate column tiles of each row tile.
+__global__ void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__restrict__ tp_ptr,
+                           const int32_t *__restrict__ tp_col, int64_t n_tiles, uint64_t *__restrict__ cmax) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    const int64_t kb = tp_ptr ? tp_ptr[R] : 0, ke = tp_ptr ? tp_ptr[R + 1] : n_tiles;
+    uint64_t v = 0;
+    for (int64_t k = kb + lane; k < ke; k += 64) {
+        const int64_t C = tp_ptr ? (int64_t)tp_col[k] : k;
+        const uint64_t x = tmax[C];
+        v = x > v ? x : v;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if (lane == 0) cmax[R] = v;
+}
+
+__global__ void k_snapshot(const uint8_t *core, const int *parent, const int32_t *cnt, int64_t n, uint64_t *rank) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int r = parent[w];
+    rank[w] = core[w] ? (((uint64_t)(uint32_t)cnt[r] << 32) | (uint32_t)r) : 0ull;
 }
 
 __global__ void k_compress(int *parent, int64_t n) {
@@ -210,32 +334,36 @@ __global__ void k_compress(int *parent, int64_t n) {
     parent[w] = uf_find(parent, (int)w);
 }
 
+// Lanes sharing a key are combined before one atomic per distinct key (points
+// of a component are mostly adjacent in the working order).
+__device__ __forceinline__ void wave_add_by_key(int32_t *cnt, int32_t *mins, bool active, int key, int val) {
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int k = __shfl(key, leader, 64);
+        const bool mine = active && key == k;
+        const uint64_t match = __ballot(mine);
+        if (mins) {
+            int v = mine ? val : INT_MAX;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+            if (__lane_id() == leader) atomicMin(&mins[k], v);
+        }
+        if (__lane_id() == leader) atomicAdd(&cnt[k], __popcll(match));
+        pending &= ~match;
+    }
+}
+
 __global__ void k_root_count(const uint8_t *core, const int *parent, int64_t n, int32_t *cnt) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n || !core[w]) return;
-    atomicAdd(&cnt[parent[w]], 1);
+    const bool active = w < n && core[w];
+    wave_add_by_key(cnt, nullptr, active, active ? parent[w] : 0, 0);
 }
 
-__global__ void k_largest(const int32_t *cnt, int64_t n, unsigned long long *best) {
+__global__ void k_count_roots(const int32_t *cnt, int64_t n, unsigned long long *roots) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long v = 0;
-    if (w < n && cnt[w] > 0) v = ((unsigned long long)(uint32_t)cnt[w] << 32) | (uint32_t)(~(uint32_t)w);
-    // wave max then one atomic per wave
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    if ((threadIdx.x & 63) == 0 && v) atomicMax(best, v);
-}
-
-__global__ void k_mark_largest(const uint8_t *core, const int *parent, int64_t n, const unsigned long long *best,
-                               uint8_t *inL) {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n) return;
-    const unsigned long long b = *best;
-    const int L = b ? (int)(~(uint32_t)(b & 0xffffffffull)) : -1;
-    inL[w] = (core[w] && parent[w] == L) ? 1 : 0;
+    const uint64_t b = __ballot(w < n && cnt[w] > 0);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(roots, (unsigned long long)__popcll(b));
 }
 
 // ------------------------------------------------------------------ labels
@@ -244,13 +372,14 @@ __global__ void k_comp_stats(const uint8_t *core, const int *parent, const int32
                              int32_t *csize, unsigned long long *counters) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool is_core = false, is_root = false;
+    int r = 0, orig = 0;
     if (w < n && core[w]) {
         is_core = true;
-        const int r = parent[w];
+        r = parent[w];
         is_root = r == (int)w;
-        atomicMin(&cmin[r], perm ? perm[w] : (int32_t)w);
-        atomicAdd(&csize[r], 1);
+        orig = perm ? perm[w] : (int32_t)w;
     }
+    wave_add_by_key(csize, cmin, is_core, r, orig);
     const uint64_t bc = __ballot(is_core), br = __ballot(is_root);
     if ((threadIdx.x & 63) == 0) {
         if (bc) atomicAdd(&counters[0], (unsigned long long)__popcll(bc));
@@ -314,25 +443,42 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         if (st) *st = local;
         return;
     }
-    Scratch<uint8_t> core(ctx, n), inL(ctx, n);
+    Scratch<uint8_t> core(ctx, n);
+    Scratch<uint64_t> rank(ctx, n);
     Scratch<int> parent(ctx, n);
-    Scratch<unsigned long long> counters(ctx, 8);
+    Scratch<unsigned long long> counters(ctx, 8), tstats(ctx, 4 * kStatSlots);
     CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
-    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, inL.p, parent.p, counters.p};
+    CNN_HIP(hipMemsetAsync(tstats.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
+    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
     CNN_HIP(hipGetLastError());
-    k_round_a<<<blocks_for(n, 4), 256, 0, s>>>(a);
+    k_hook<<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     CNN_HIP(hipGetLastError());
-    k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    int rounds = 1;
+    while (((int64_t)1 << rounds) < n + 1) ++rounds;
+    for (int it = 0; it <= rounds; ++it) k_jump<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    k_hook_roots<<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
+    for (int it = 0; it <= rounds; ++it) k_jump<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    CNN_HIP(hipGetLastError());
     {
         Scratch<int32_t> cnt(ctx, n);
         fill_i32(ctx, cnt.p, 0, n);
         k_root_count<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, n, cnt.p);
-        k_largest<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters.p + 4);
-        k_mark_largest<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, n, counters.p + 4, inL.p);
+        k_count_roots<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters.p + 6);
+        k_snapshot<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cnt.p, n, rank.p);
         CNN_HIP(hipGetLastError());
+    }
+    std::unique_ptr<Scratch<uint64_t>> cand_max;
+    if (req.n_tiles > 0) {
+        Scratch<uint64_t> tmax(ctx, req.n_tiles);
+        cand_max.reset(new Scratch<uint64_t>(ctx, req.n_tiles));
+        k_tile_rank_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(rank.p, n, req.n_tiles, tmax.p);
+        k_cand_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(tmax.p, req.tp_ptr, req.tp_col, req.n_tiles,
+                                                              cand_max->p);
+        CNN_HIP(hipGetLastError());
+        a.cand_max = cand_max->p;
     }
     k_round_b<<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipGetLastError());
@@ -371,8 +517,19 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     if (req.consider) fill_u8(ctx, req.consider, 0, n);
     CNN_HIP(hipEventRecord(ctx->ev[2], s));
     unsigned long long h[8];
+    std::vector<unsigned long long> ht(4 * kStatSlots);
     CNN_HIP(hipMemcpyAsync(h, counters.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipMemcpyAsync(ht.data(), tstats.p, ht.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     CNN_HIP(hipStreamSynchronize(s));
+    h[0] = h[1] = 0;
+    unsigned long long rows = 0;
+    for (int i = 0; i < kStatSlots; ++i) {
+        h[0] += ht[4 * i];
+        h[1] += ht[4 * i + 1];
+        rows += ht[4 * i + 2];
+    }
+    local.sampled_components = (int64_t)h[6];
+    local.rows_scanned = (int64_t)rows;
     float t1 = 0, t2 = 0;
     CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
     CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
@@ -547,7 +704,7 @@ int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, consider, current_start, sim_mode,
-                       false, false, 2, -1, labels};
+                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles};
         labels_run(ctx, r, stats_out);
         return (int)CNN_OK;
     });
@@ -573,7 +730,7 @@ int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_cor
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles};
         labels_run(ctx, r, lstats);
         return (int)CNN_OK;
     });
@@ -591,7 +748,7 @@ int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, do
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles};
         labels_run(ctx, r, lstats);
         if (graph_out) *graph_out = hold.release();
         return (int)CNN_OK;

@@ -155,35 +155,63 @@ __global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, 
     }
 }
 
-__device__ __forceinline__ bool tiles_close(const double *__restrict__ blo, const double *__restrict__ bhi, int d,
-                                            int64_t R, int64_t C, double thr) {
+// Boxes of super-tiles (64 consecutive tiles = 4096 points of the working order).
+__global__ void k_super_boxes(const double *__restrict__ blo, const double *__restrict__ bhi, int d, int64_t n_tiles,
+                              int64_t n_super, double *__restrict__ slo, double *__restrict__ shi) {
+    const int lane = threadIdx.x & 63;
+    const int64_t S = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (S >= n_super) return;
+    const int64_t t = S * 64 + lane;
+    for (int k = 0; k < d; ++k) {
+        double lo = (t < n_tiles) ? blo[t * d + k] : INFINITY;
+        double hi = (t < n_tiles) ? bhi[t * d + k] : -INFINITY;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, off, 64));
+            hi = fmax(hi, __shfl_xor(hi, off, 64));
+        }
+        if (lane == 0) { slo[S * d + k] = lo; shi[S * d + k] = hi; }
+    }
+}
+
+__device__ __forceinline__ bool boxes_close(const double *__restrict__ alo, const double *__restrict__ ahi,
+                                            const double *__restrict__ blo, const double *__restrict__ bhi, int d,
+                                            double thr) {
     double s = 0.0;
     for (int k = 0; k < d; ++k) {
-        double g = fmax(blo[C * d + k] - bhi[R * d + k], blo[R * d + k] - bhi[C * d + k]);
+        double g = fmax(blo[k] - ahi[k], alo[k] - bhi[k]);
         g = fmax(g, 0.0);
         s += g * g;
     }
     return s <= thr;
 }
 
-// Count (fill == false) or list (fill == true) the candidate column tiles of each row tile.
+// Count (FILL == false) or list (FILL == true) the candidate column tiles of
+// each row tile: super-tiles are culled first (lane per super-tile), then the
+// 64 tiles of every surviving super-tile (lane per tile), in ascending order.
 template <bool FILL>
-__global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi, int d, int64_t n_tiles,
-                             double thr, int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
+__global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi,
+                             const double *__restrict__ slo, const double *__restrict__ shi, int d, int64_t n_tiles,
+                             int64_t n_super, double thr, int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
                              int32_t *__restrict__ tp_col) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
+    const double *rlo = blo + R * d, *rhi = bhi + R * d;
     int64_t out = FILL ? tp_ptr[R] : 0;
-    for (int64_t C0 = 0; C0 < n_tiles; C0 += 64) {
-        const int64_t C = C0 + lane;
-        bool ok = (C < n_tiles) && tiles_close(blo, bhi, d, R, C, thr);
-        uint64_t b = __ballot(ok);
-        if (FILL && ok) {
-            int before = __popcll(b & ((1ull << lane) - 1ull));
-            tp_col[out + before] = (int32_t)C;
+    for (int64_t S0 = 0; S0 < n_super; S0 += 64) {
+        const int64_t Sl = S0 + lane;
+        const bool sok = (Sl < n_super) && boxes_close(rlo, rhi, slo + Sl * d, shi + Sl * d, d, thr);
+        uint64_t sbits = __ballot(sok);
+        while (sbits) {
+            const int64_t S = S0 + __ffsll((unsigned long long)sbits) - 1;
+            sbits &= sbits - 1;
+            const int64_t C = S * 64 + lane;
+            const bool ok = (C < n_tiles) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
+            const uint64_t b = __ballot(ok);
+            if (FILL && ok) tp_col[out + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)C;
+            out += __popcll(b);
         }
-        out += __popcll(b);
     }
     if (!FILL && lane == 0) cnt[R] = (int32_t)out;
 }
@@ -341,29 +369,47 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
     finish_deg(a, i, mydeg);
 }
 
-// Resolve FP32 decisions: bits certain below lo; band pairs recomputed in FP64.
-__device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, int64_t i, int64_t C, const float *acc,
-                                                 unsigned &hits, unsigned &flips, double &maxerr) {
-    uint64_t m = 0;
-#pragma unroll
-    for (int jj = 0; jj < 64; ++jj) {
-        float s = acc[jj];
-        bool acc_ok = s < a.bp.lo;
-        bool rej = s > a.bp.hi;
-        m |= (uint64_t)acc_ok << jj;
-        if (!acc_ok && !rej) {
-            int64_t j = C * 64 + jj;
-            if (i < a.n && j < a.n) {
-                double e = exact_d2(a.xw64, a.d, i, j);
-                bool in = e <= a.bp.r2;
-                m |= (uint64_t)in << jj;
-                ++hits;
-                flips += (in != ((double)s <= a.bp.r2));
-                double err = fabs((double)s - e);
-                if (err == err) maxerr = fmax(maxerr, err);
+// Resolve the band bits of one row: pairs whose FP32 value lies in [lo, hi]
+// (or is NaN) are recomputed in FP64 in reference order.
+__device__ __forceinline__ uint64_t band_fix(const DistArgs &a, int64_t i, int64_t C, uint64_t m, uint64_t band,
+                                             unsigned &hits, unsigned &flips, double &maxerr,
+                                             const float *__restrict__ xc32) {
+    while (band) {
+        const int jj = __ffsll((unsigned long long)band) - 1;
+        band &= band - 1;
+        const int64_t j = C * 64 + jj;
+        if (i < a.n && j < a.n) {
+            const double e = exact_d2(a.xw64, a.d, i, j);
+            const bool in = e <= a.bp.r2;
+            m |= (uint64_t)in << jj;
+            // recompute the FP32 value for the statistics
+            float s32 = 0.0f;
+            for (int k = 0; k < a.d; ++k) {
+                float t = a.xw32[i * a.d + k] - xc32[jj * a.d + k];
+                s32 = __builtin_fmaf(t, t, s32);
             }
+            ++hits;
+            flips += (in != ((double)s32 <= a.bp.r2));
+            const double err = fabs((double)s32 - e);
+            if (err == err) maxerr = fmax(maxerr, err);
         }
     }
+    return m;
+}
+
+// Resolve FP32 decisions from 64 accumulators (runtime-d kernel).
+__device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, int64_t i, int64_t C, const float *acc,
+                                                 unsigned &hits, unsigned &flips, double &maxerr,
+                                                 const float *__restrict__ xc32) {
+    uint64_t m = 0, band = 0;
+#pragma unroll
+    for (int jj = 0; jj < 64; ++jj) {
+        const float s = acc[jj];
+        const bool acc_ok = s < a.bp.lo;
+        m |= (uint64_t)acc_ok << jj;
+        band |= (uint64_t)(!acc_ok && !(s > a.bp.hi)) << jj;
+    }
+    if (band) m = band_fix(a, i, C, m, band, hits, flips, maxerr, xc32);
     return m;
 }
 
@@ -380,7 +426,9 @@ __device__ __forceinline__ void flush_band(const DistArgs &a, unsigned hits, uns
     }
 }
 
-// FP32 direct form, compile-time d: t = a - b; acc = fma(t, t, acc).
+// FP32 direct form, compile-time d: t = a - b; acc = fma(t, t, acc).  The
+// certain bits and the band bits are built inline; band bits are resolved in
+// FP64 only for lanes that have any.
 template <int D>
 __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
     const int64_t R = a.R0 + blockIdx.x;
@@ -394,11 +442,12 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
+    const float lo = a.bp.lo, hi = a.bp.hi;
     for (int64_t k = kb + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float *__restrict__ xc = a.xw32 + C * 64 * D;
-        float acc[64];
-#pragma unroll
+        uint64_t m = 0, band = 0;
+#pragma unroll 16
         for (int jj = 0; jj < 64; ++jj) {
             float s = 0.0f;
 #pragma unroll
@@ -406,9 +455,11 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
                 float t = xi[k2] - xc[jj * D + k2];
                 s = __builtin_fmaf(t, t, s);
             }
-            acc[jj] = s;
+            const bool ok = s < lo;
+            m |= (uint64_t)ok << jj;
+            band |= (uint64_t)(!ok && !(s > hi)) << jj;
         }
-        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr);
+        if (band) m = band_fix(a, i, C, m, band, hits, flips, maxerr, xc);
         mydeg += store_mask(a, k, i, C, m);
     }
     flush_band(a, hits, flips, maxerr);
@@ -451,7 +502,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
                 acc[jj] = s;
             }
         }
-        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr);
+        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr, xc);
         mydeg += store_mask(a, k, i, C, m);
     }
     flush_band(a, hits, flips, maxerr);
@@ -460,32 +511,41 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
 
 // ------------------------------------------------------------------ CSR emit
 
-// One wave per working row: lanes take consecutive tile pairs of the row tile,
-// a wave prefix sum places each lane's set bits.
-__global__ __launch_bounds__(256) void k_emit(DistArgs a, int64_t row_lo, int64_t row_hi,
-                                             const int64_t *__restrict__ row_ptr, int64_t col_base,
+// One wave per row tile, lane = row: the mask words of a tile pair are one
+// coalesced 512-byte load, and each lane appends its set bits to its own row
+// (work proportional to the row's neighbours).
+__global__ __launch_bounds__(256) void k_emit(DistArgs a, int64_t n_rt, const int64_t *__restrict__ row_ptr,
                                              int32_t *__restrict__ col) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = row_lo + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= row_hi) return;
-    const int64_t R = w >> 6;
-    const int r = (int)(w & 63);
+    const int64_t rt = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (rt >= n_rt) return;
+    const int64_t R = a.R0 + rt;
+    const int64_t w = R * 64 + lane;
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
-    int64_t out = row_ptr[w] - col_base;
-    for (int64_t k0 = kb; k0 < ke; k0 += 64) {
-        const int64_t k = k0 + lane;
-        uint64_t m = (k < ke) ? a.mask[(k - a.k_base) * 64 + r] : 0ull;
-        const int c = __popcll(m);
-        const int incl = wave_incl_scan(c);
-        int64_t pos = out + incl - c;
-        const int64_t C = (k < ke) ? tile_col(a, k, kb) : 0;
-        while (m) {
-            int b = __ffsll((unsigned long long)m) - 1;
-            col[pos++] = (int32_t)(C * 64 + b);
-            m &= m - 1;
+    int64_t pos = (w < a.n) ? row_ptr[w] : 0;
+    int64_t k = kb;
+    for (; k + 1 < ke; k += 2) {
+        uint64_t m0 = a.mask[(k - a.k_base) * 64 + lane];
+        uint64_t m1 = a.mask[(k + 1 - a.k_base) * 64 + lane];
+        const int32_t c0 = (int32_t)uniform64(tile_col(a, k, kb)) * 64;
+        const int32_t c1 = (int32_t)uniform64(tile_col(a, k + 1, kb)) * 64;
+        while (m0) {
+            col[pos++] = c0 + (__ffsll((unsigned long long)m0) - 1);
+            m0 &= m0 - 1;
         }
-        out += __shfl(incl, 63, 64);
+        while (m1) {
+            col[pos++] = c1 + (__ffsll((unsigned long long)m1) - 1);
+            m1 &= m1 - 1;
+        }
+    }
+    if (k < ke) {
+        uint64_t m0 = a.mask[(k - a.k_base) * 64 + lane];
+        const int32_t c0 = (int32_t)uniform64(tile_col(a, k, kb)) * 64;
+        while (m0) {
+            col[pos++] = c0 + (__ffsll((unsigned long long)m0) - 1);
+            m0 &= m0 - 1;
+        }
     }
 }
 
@@ -589,6 +649,8 @@ void graph_free(cnn_graph *g) {
     if (g->xw32) (void)hipFree(g->xw32);
     if (g->row_ptr) (void)hipFree(g->row_ptr);
     if (g->col) (void)hipFree(g->col);
+    if (g->tp_ptr) (void)hipFree(g->tp_ptr);
+    if (g->tp_col) (void)hipFree(g->tp_col);
     delete g;
 }
 
@@ -695,28 +757,31 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     // candidate tile pairs
     const int64_t T = g->n_tiles;
     int64_t *tp_ptr = nullptr;
-    Scratch<int64_t> tp_ptr_s(ctx, g->layout == 1 ? T + 1 : 1);
     int32_t *tp_col = nullptr;
-    std::unique_ptr<Scratch<int32_t>> tp_col_s;
     int64_t total_pairs = T * T;
     const double thr = g->r2 * (1.0 + 1e-6);
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
         k_tile_boxes<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(g->xw64, n, d, T, blo.p, bhi.p);
         CNN_HIP(hipGetLastError());
-        Scratch<int32_t> cnt(ctx, T);
-        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, thr, cnt.p, nullptr, nullptr);
+        const int64_t NS = ceil_div(T, 64);
+        Scratch<double> slo(ctx, NS * d), shi(ctx, NS * d);
+        k_super_boxes<<<(unsigned)ceil_div(NS, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, NS, slo.p, shi.p);
         CNN_HIP(hipGetLastError());
-        tp_ptr = tp_ptr_s.p;
+        Scratch<int32_t> cnt(ctx, T);
+        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, cnt.p,
+                                                                     nullptr, nullptr);
+        CNN_HIP(hipGetLastError());
+        tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
         // tp_ptr[T] = tp_ptr[T-1] + cnt[T-1]
         int64_t last_excl = read_scalar(ctx, tp_ptr + (T - 1));
         int32_t last_cnt = read_scalar(ctx, cnt.p + (T - 1));
         total_pairs = last_excl + last_cnt;
         CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        tp_col_s.reset(new Scratch<int32_t>(ctx, std::max<int64_t>(total_pairs, 1)));
-        tp_col = tp_col_s->p;
-        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, thr, nullptr, tp_ptr, tp_col);
+        tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
+        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, nullptr,
+                                                                    tp_ptr, tp_col);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipStreamSynchronize(s));
     }
@@ -816,8 +881,10 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
             g->col = col;
         }
         CNN_HIP(hipEventRecord(ctx->ev[4], s));
-        if (rows > 0)
-            k_emit<<<(unsigned)ceil_div(rows, 4), 256, 0, s>>>(a, row_lo, row_hi, g->row_ptr, 0, col);
+        if (rows > 0) {
+            const int64_t n_rt = b.second - b.first;
+            k_emit<<<(unsigned)ceil_div(n_rt, 4), 256, 0, s>>>(a, n_rt, g->row_ptr, col);
+        }
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
         CNN_HIP(hipEventSynchronize(ctx->ev[5]));
@@ -835,6 +902,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     g->nnz = nnz_total;
     float ms_prep = 0.0f;
     CNN_HIP(hipEventElapsedTime(&ms_prep, ctx->ev[0], ctx->ev[1]));
+    g->n_tile_pairs = total_pairs;
     cnn_graph_stats &st = g->stats;
     st.tile_pairs = total_pairs;
     st.pairs_evaluated = total_pairs * 64 * 64;
@@ -934,6 +1002,16 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     if (src->xw32) {
         g->xw32 = dev_alloc<float>(g, n_pad * dd);
         CNN_HIP(hipMemcpyAsync(g->xw32, src->xw32, n_pad * dd * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    if (src->tp_ptr) {
+        // candidate tiles at r stay valid (a superset) at r' <= r
+        const int64_t T = g->n_tiles;
+        const int64_t pairs = src->n_tile_pairs;
+        g->n_tile_pairs = pairs;
+        g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
+        g->tp_col = dev_alloc<int32_t>(g, pairs);
+        CNN_HIP(hipMemcpyAsync(g->tp_ptr, src->tp_ptr, (T + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        CNN_HIP(hipMemcpyAsync(g->tp_col, src->tp_col, pairs * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     }
     if (n == 0) {
         fill_i64(ctx, g->row_ptr, 0, 1);

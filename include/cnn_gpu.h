@@ -96,6 +96,8 @@ typedef struct {
     int64_t n_clusters;        /* clusters after sort_by_size */
     int64_t tests;             /* similarity tests executed */
     int64_t tests_passed;
+    int64_t sampled_components;/* components among core points after the sampling round */
+    int64_t rows_scanned;      /* rows whose candidate edges the full round scanned */
     float ms_similarity;       /* similarity + union-find kernels */
     float ms_labels;           /* compress, label assignment, sort_by_size */
 } cnn_label_stats;
