@@ -62,6 +62,7 @@ struct cnn_graph {
     double radius = 0.0;
     double r2 = 0.0;
     int32_t layout = 0;          // 0 identity, 1 spatial order
+    int32_t dp = 0;              // row stride of xw32 (d, or d padded to 8 for the MFMA form)
     int64_t n_tiles = 0;
     int64_t nnz = 0;
     // device buffers owned by the graph

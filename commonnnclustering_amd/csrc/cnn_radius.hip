@@ -116,6 +116,25 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
 }
 
 template <typename Tin>
+__global__ void k_gather_padded(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n, int64_t n_pad,
+                                int d, int dp, double *__restrict__ xw64, float *__restrict__ xw32) {
+    const int64_t total = n_pad * d;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t w = idx / d;
+        int k = (int)(idx - w * d);
+        if (w < n) {
+            int64_t src = perm ? (int64_t)perm[w] : w;
+            Tin v = X[src * d + k];
+            xw64[idx] = (double)v;
+            xw32[w * dp + k] = (float)v;
+        } else {
+            xw64[idx] = 0.0;
+        }
+    }
+}
+
+template <typename Tin>
 __global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n, int64_t n_pad,
                          int d, double *__restrict__ xw64, float *__restrict__ xw32) {
     const int64_t total = n_pad * d;
@@ -238,6 +257,10 @@ struct DistArgs {
     int32_t *deg;
     BandParams bp;
     unsigned long long *band_stats;  // [0] hits, [1] flips, [2] max err bits (double as u64)
+    // Gram form (MFMA): padded FP32 row stride, squared norms, band = ga * (w_i + w_j) + gb
+    int dp;
+    const float *wnorm;
+    float ga, gb;
 };
 
 __device__ __forceinline__ void tile_range(const DistArgs &a, int64_t R, int64_t &kb, int64_t &ke) {
@@ -509,6 +532,111 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
     finish_deg(a, i, mydeg);
 }
 
+// ------------------------------------------------------------------ FP32 Gram form on MFMA (d >= 16)
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Squared norms in FP32 (sequential FMA over the padded row).
+__global__ void k_norms(const float *__restrict__ xw32, int64_t n_pad, int dp, float *__restrict__ wnorm) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    const float *x = xw32 + i * dp;
+    float s = 0.0f;
+    for (int k = 0; k < dp; ++k) s = __builtin_fmaf(x[k], x[k], s);
+    wnorm[i] = s;
+}
+
+// One wave per 64x64 tile pair (4 waves of a workgroup share the row tile).
+// Dot products x_i . x_j accumulate in FP32 on v_mfma_f32_32x32x2f32 as a 2x2
+// grid of 32x32 blocks; lane l supplies row l%32 at k-slot l/32, and a float4
+// load per lane feeds four k-steps (dims 8q+4h .. 8q+4h+3, the same order for
+// rows and columns).  Accumulator element v of lane l is D[8(v/4) + 4(l/32) +
+// v%4][l%32].  Each element becomes d2 = (w_i + w_j) - 2 dot (one FMA); pairs
+// with |d2 - r2| <= ga (w_i + w_j) + gb are recomputed in FP64 in reference
+// order.  Row words are assembled from ballots (one ballot = two half rows).
+__global__ __launch_bounds__(256) void k_dist_mfma(DistArgs a) {
+    __shared__ float s_wr[64];
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 31, h = lane >> 5;
+    const int64_t i = R * 64 + lane;
+    if (threadIdx.x < 64) s_wr[threadIdx.x] = a.wnorm[R * 64 + threadIdx.x];
+    __syncthreads();
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int dp = a.dp;
+    const float r2f = (float)a.bp.r2;
+    const float *__restrict__ xa0 = a.xw32 + (R * 64 + r) * dp + 4 * h;
+    const float *__restrict__ xa1 = xa0 + (int64_t)32 * dp;
+    int mydeg = 0;
+    unsigned hits = 0, flips = 0;
+    double maxerr = 0.0;
+    for (int64_t k = kb + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const float *__restrict__ xb0 = a.xw32 + (C * 64 + r) * dp + 4 * h;
+        const float *__restrict__ xb1 = xb0 + (int64_t)32 * dp;
+        v16f c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
+        for (int q = 0; q < dp; q += 8) {
+            const float4 A0 = *(const float4 *)(xa0 + q);
+            const float4 A1 = *(const float4 *)(xa1 + q);
+            const float4 B0 = *(const float4 *)(xb0 + q);
+            const float4 B1 = *(const float4 *)(xb1 + q);
+            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B0.x, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B1.x, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B0.x, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B1.x, c11, 0, 0, 0);
+            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B0.y, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B1.y, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B0.y, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B1.y, c11, 0, 0, 0);
+            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B0.z, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B1.z, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B0.z, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B1.z, c11, 0, 0, 0);
+            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B0.w, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B1.w, c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B0.w, c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B1.w, c11, 0, 0, 0);
+        }
+        const float wc0 = a.wnorm[C * 64 + r];
+        const float wc1 = a.wnorm[C * 64 + 32 + r];
+        uint64_t word = 0;
+        // rb, cb: block row/column; v: accumulator element
+#define CNN_MFMA_EPILOGUE(ACC, RB, CB, WC)                                                           \
+        _Pragma("unroll") for (int v = 0; v < 16; ++v) {                                              \
+            const int row = (RB) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);                               \
+            const float wsum = s_wr[row] + (WC);                                                      \
+            const float d2 = __builtin_fmaf(-2.0f, (ACC)[v], wsum);                                   \
+            const float delta = __builtin_fmaf(a.ga, wsum, a.gb);                                     \
+            bool in = d2 < r2f - delta;                                                               \
+            const bool band = !in && !(d2 > r2f + delta);                                             \
+            if (band) {                                                                               \
+                const int64_t gi = R * 64 + row, gj = C * 64 + (CB) * 32 + r;                         \
+                if (gi < a.n && gj < a.n) {                                                           \
+                    const double e = exact_d2(a.xw64, a.d, gi, gj);                                   \
+                    in = e <= a.bp.r2;                                                                \
+                    ++hits;                                                                           \
+                    flips += (in != ((double)d2 <= a.bp.r2));                                         \
+                    const double err = fabs((double)d2 - e);                                          \
+                    if (err == err) maxerr = fmax(maxerr, err);                                       \
+                }                                                                                     \
+            }                                                                                         \
+            const uint64_t b = __ballot(in);                                                          \
+            const int row0 = (RB) * 32 + 8 * (v >> 2) + (v & 3);                                      \
+            if (lane == row0) word |= (b & 0xffffffffull) << (32 * (CB));                             \
+            if (lane == row0 + 4) word |= (b >> 32) << (32 * (CB));                                   \
+        }
+        CNN_MFMA_EPILOGUE(c00, 0, 0, wc0)
+        CNN_MFMA_EPILOGUE(c01, 0, 1, wc1)
+        CNN_MFMA_EPILOGUE(c10, 1, 0, wc0)
+        CNN_MFMA_EPILOGUE(c11, 1, 1, wc1)
+#undef CNN_MFMA_EPILOGUE
+        mydeg += store_mask(a, k, i, C, word);
+    }
+    flush_band(a, hits, flips, maxerr);
+    finish_deg(a, i, mydeg);
+}
+
 // ------------------------------------------------------------------ CSR emit
 
 // One wave per row tile, lane = row: the mask words of a tile pair are one
@@ -605,10 +733,36 @@ static BandParams make_band_direct(double r2, int d, bool rounded_coords, double
     return bp;
 }
 
+// Guard band of the Gram form d2 = (w_i + w_j) - 2 x.y in FP32 (w = squared
+// norm by sequential FMA, x.y by MFMA in any summation order):
+//   |w_hat - w| <= gamma_d w,  |dot_hat - dot| <= gamma_d sqrt(w_i w_j) <= gamma_d (w_i + w_j) / 2,
+//   the sum w_i + w_j and the final FMA add u (w_i + w_j) and u d2,
+// so |d2_hat - d2| <= (2 gamma_d + 2u)(w_i + w_j) + u D; the reference's FP64
+// rounding adds gamma_{2d+2}(u64) D, D = 2 r2 near the threshold.  For f64
+// input rounded to f32, rounding the coordinates adds
+// 2u ||x_i - x_j|| (||x_i|| + ||x_j||) + u^2 (..)^2 <= u (2 (w_i + w_j) + D) + 2 u^2 (w_i + w_j).
+// Safety factor 4; gb also absorbs rounding r2 to FP32 and subnormal flushes.
+static void make_band_gram(int d, bool rounded_coords, double r2, float &ga, float &gb) {
+    const double u = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
+    const int dd = ((d + 7) / 8) * 8;
+    double A = 2.0 * gamma_m(dd, u) + 2.0 * u;
+    double B = (u + gamma_m(2 * d + 2, u64)) * 2.0 * r2;
+    if (rounded_coords) {
+        A += 2.0 * u + 2.0 * u * u;
+        B += u * 2.0 * r2;
+    }
+    A *= 4.0;
+    B = 4.0 * B + 4.0 * u * r2 + 1e-30;
+    ga = (float)(A * (1.0 + 1e-6));
+    gb = (float)(B * (1.0 + 1e-6));
+}
+
 static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band) {
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
-    if (!band) {
+    if (band && a.d >= kMfmaMinD) {
+        k_dist_mfma<<<grid, block, 0, s>>>(a);
+    } else if (!band) {
         switch (a.d) {
             case 1: k_dist_f64_small<1><<<grid, block, 0, s>>>(a); break;
             case 2: k_dist_f64_small<2><<<grid, block, 0, s>>>(a); break;
@@ -743,14 +897,30 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         CNN_HIP(hipGetLastError());
         sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
     }
+    const bool gram = band && d >= kMfmaMinD;
+    const int dp = gram ? ((d + 7) / 8) * 8 : d;  // MFMA rows padded to 8 dims (zeros)
     g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
-    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
+    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(dp, 1));
+    g->dp = dp;
     {
         const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * std::max(d, 1), 256), 16384);
-        if (dtype == CNN_F32)
+        if (gram) {
+            CNN_HIP(hipMemsetAsync(g->xw32, 0, (size_t)n_pad * dp * sizeof(float), s));
+            if (dtype == CNN_F32)
+                k_gather_padded<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, dp, g->xw64, g->xw32);
+            else
+                k_gather_padded<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, dp, g->xw64, g->xw32);
+        } else if (dtype == CNN_F32) {
             k_gather<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
-        else
+        } else {
             k_gather<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
+        }
+        CNN_HIP(hipGetLastError());
+    }
+    std::unique_ptr<Scratch<float>> wnorm;
+    if (gram) {
+        wnorm.reset(new Scratch<float>(ctx, n_pad));
+        k_norms<<<(unsigned)ceil_div(n_pad, 256), 256, 0, s>>>(g->xw32, n_pad, dp, wnorm->p);
         CNN_HIP(hipGetLastError());
     }
 
@@ -841,6 +1011,9 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         a.deg = deg.p;
         a.bp = bp;
         a.band_stats = bstats.p;
+        a.dp = dp;
+        a.wnorm = wnorm ? wnorm->p : nullptr;
+        if (gram) make_band_gram(d, dtype == CNN_F64, g->r2, a.ga, a.gb);
         CNN_HIP(hipEventRecord(ctx->ev[2], s));
         launch_dist(ctx, a, b.second - b.first, band);
         CNN_HIP(hipEventRecord(ctx->ev[3], s));
@@ -999,9 +1172,11 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     }
     g->xw64 = dev_alloc<double>(g, n_pad * dd);
     CNN_HIP(hipMemcpyAsync(g->xw64, src->xw64, n_pad * dd * sizeof(double), hipMemcpyDeviceToDevice, s));
+    g->dp = src->dp;
     if (src->xw32) {
-        g->xw32 = dev_alloc<float>(g, n_pad * dd);
-        CNN_HIP(hipMemcpyAsync(g->xw32, src->xw32, n_pad * dd * sizeof(float), hipMemcpyDeviceToDevice, s));
+        const int64_t words = n_pad * std::max(src->dp, 1);
+        g->xw32 = dev_alloc<float>(g, words);
+        CNN_HIP(hipMemcpyAsync(g->xw32, src->xw32, words * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
     if (src->tp_ptr) {
         // candidate tiles at r stay valid (a superset) at r' <= r

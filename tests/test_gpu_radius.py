@@ -94,3 +94,20 @@ def test_empty(cuda):
     X = np.zeros((0, 3), dtype=np.float32)
     rp, col, st = gpu_csr(X, 1.0)
     assert rp.tolist() == [0] and col.size == 0
+
+
+@pytest.mark.parametrize("d,dtype", [(16, np.float32), (64, np.float32), (33, np.float64)])
+def test_gram_band_hits_forced(cuda, d, dtype):
+    """High-d pairs placed within ~1e-6 relative of r: the MFMA Gram form must
+    send them to the FP64 recheck and still match the oracle exactly."""
+    rng = np.random.default_rng(d)
+    m = 1500
+    A = rng.normal(size=(m, d)) * 3.0 + 5.0   # offset: large norms relative to r
+    u = rng.normal(size=(m, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 2.0
+    eps = rng.uniform(-1e-5, 1e-5, size=(m, 1))
+    B = A + u * (r + eps)
+    X = np.concatenate([A, B]).astype(dtype)
+    st = assert_same(X, r, "f32_band")
+    assert st["band_hits"] >= m // 2
