@@ -116,25 +116,6 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
 }
 
 template <typename Tin>
-__global__ void k_gather_padded(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n, int64_t n_pad,
-                                int d, int dp, double *__restrict__ xw64, float *__restrict__ xw32) {
-    const int64_t total = n_pad * d;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        int64_t w = idx / d;
-        int k = (int)(idx - w * d);
-        if (w < n) {
-            int64_t src = perm ? (int64_t)perm[w] : w;
-            Tin v = X[src * d + k];
-            xw64[idx] = (double)v;
-            xw32[w * dp + k] = (float)v;
-        } else {
-            xw64[idx] = 0.0;
-        }
-    }
-}
-
-template <typename Tin>
 __global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n, int64_t n_pad,
                          int d, double *__restrict__ xw64, float *__restrict__ xw32) {
     const int64_t total = n_pad * d;
@@ -257,10 +238,18 @@ struct DistArgs {
     int32_t *deg;
     BandParams bp;
     unsigned long long *band_stats;  // [0] hits, [1] flips, [2] max err bits (double as u64)
-    // Gram form (MFMA): padded FP32 row stride, squared norms, band = ga * (w_i + w_j) + gb
-    int dp;
-    const float *wnorm;
-    float ga, gb;
+    int dp;                  // row stride of the augmented Gram vectors
+    // symmetric evaluation: only tile pairs with C >= R are computed; the
+    // transposed words go to the mirror pair (C, R)
+    int sym;
+    const int64_t *mirror;   // explicit lists: index of (C, R) for each k (nullptr for dense)
+    // augmented Gram form: row-role [x, 1, c] and column-role [x, c, 1] vectors
+    // (dp floats each) whose dot product is T' = (r2 - d2)/2 + e_i + e_j; e = margin
+    const float *xra;
+    const float *xca;
+    const float *emarg;
+    // column chunking of the dense wide FP64 kernel (tiles per workgroup)
+    int64_t col_chunk;
 };
 
 __device__ __forceinline__ void tile_range(const DistArgs &a, int64_t R, int64_t &kb, int64_t &ke) {
@@ -301,20 +290,70 @@ __device__ __forceinline__ void atomic_max_double(unsigned long long *p, double 
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-// Finish one wave's tile pair: apply validity, store the mask word, count.
-__device__ __forceinline__ int store_mask(const DistArgs &a, int64_t k, int64_t i, int64_t C, uint64_t m) {
+// First tile pair of row tile R to evaluate: all pairs, or those with C >= R.
+__device__ __forceinline__ int64_t first_pair(const DistArgs &a, int64_t R, int64_t kb, int64_t ke) {
+    if (!a.sym) return kb;
+    if (!a.tp_ptr) return kb + R;
+    int64_t lo = kb, hi = ke;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a.tp_col[mid] < R) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Bit transpose across the wave: lane j receives the word whose bit i is bit j
+// of lane i's input.  Six butterfly stages swap the off-diagonal blocks of
+// width 32, 16, ..., 1 between partner lanes (lane ^ s).
+__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int sft = 32 >> t;
+        const uint64_t m = masks[t];
+        const uint64_t y = __shfl_xor(x, sft, 64);
+        x = (lane & sft) ? ((x & ~m) | ((y >> sft) & m)) : ((x & m) | ((y & m) << sft));
+    }
+    return x;
+}
+
+// Store a row word (and, for an off-diagonal symmetric pair, the column word
+// of the mirror pair).  Returns the row word's popcount.
+__device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t R, int64_t i, int64_t C, uint64_t m,
+                                          bool have_col, uint64_t colword) {
     m &= valid_cols(a.n, C);
     if (i >= a.n) m = 0;
-    a.mask[(k - a.k_base) * 64 + (threadIdx.x & 63)] = m;
+    const int lane = threadIdx.x & 63;
+    a.mask[(k - a.k_base) * 64 + lane] = m;
+    if (a.sym && C != R) {
+        if (!have_col) colword = transpose_bits(m);
+        else {
+            colword &= valid_cols(a.n, R);       // rows of R beyond n
+            if (C * 64 + lane >= a.n) colword = 0;  // columns of C beyond n
+        }
+        const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
+        a.mask[(km - a.k_base) * 64 + lane] = colword;
+        const int cd = __popcll(colword);
+        if (cd) atomicAdd(&a.deg[C * 64 + lane], cd);  // column side of the symmetric pair
+    }
     return __popcll(m);
 }
 
+// Row degrees of this workgroup's tile: written directly (all pairs evaluated
+// here) or added to the column contributions of other tiles (symmetric).
 __device__ __forceinline__ void finish_deg(const DistArgs &a, int64_t i, int mydeg) {
     __shared__ int sdeg[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     sdeg[wave][lane] = mydeg;
     __syncthreads();
-    if (wave == 0 && i < a.n) a.deg[i] = sdeg[0][lane] + sdeg[1][lane] + sdeg[2][lane] + sdeg[3][lane];
+    if (wave == 0 && i < a.n) {
+        const int total = sdeg[0][lane] + sdeg[1][lane] + sdeg[2][lane] + sdeg[3][lane];
+        if (a.sym) atomicAdd(&a.deg[i], total);
+        else a.deg[i] = total;
+    }
 }
 
 // Plain FP64, compile-time d (small d): row point in VGPRs, column points from
@@ -329,8 +368,9 @@ __global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
     for (int k = 0; k < D; ++k) xi[k] = a.xw64[i * D + k];
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     int mydeg = 0;
-    for (int64_t k = kb + wave; k < ke; k += 4) {
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const double *__restrict__ xc = a.xw64 + C * 64 * D;
         uint64_t m = 0;
@@ -344,7 +384,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
             }
             m |= (uint64_t)(s <= a.bp.r2) << jj;
         }
-        mydeg += store_mask(a, k, i, C, m);
+        mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
 }
@@ -358,21 +398,22 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
     const int d = a.d;
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     int mydeg = 0;
-    for (int64_t k = kb + wave; k < ke; k += 4) {
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const double *__restrict__ xc = a.xw64 + C * 64 * d;
         double acc[64];
 #pragma unroll
         for (int jj = 0; jj < 64; ++jj) acc[jj] = 0.0;
-        for (int k0 = 0; k0 < d; k0 += 8) {
+        for (int q = 0; q < d; q += 8) {
             double xi[8];
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) xi[kk] = (k0 + kk < d) ? a.xw64[i * d + k0 + kk] : 0.0;
-            const int kn = min(8, d - k0);
+            for (int kk = 0; kk < 8; ++kk) xi[kk] = (q + kk < d) ? a.xw64[i * d + q + kk] : 0.0;
+            const int kn = min(8, d - q);
 #pragma unroll
             for (int jj = 0; jj < 64; ++jj) {
-                const double *__restrict__ xj = xc + (int64_t)jj * d + k0;
+                const double *__restrict__ xj = xc + (int64_t)jj * d + q;
                 double s = acc[jj];
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
@@ -387,7 +428,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
         uint64_t m = 0;
 #pragma unroll
         for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= a.bp.r2) << jj;
-        mydeg += store_mask(a, k, i, C, m);
+        mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
 }
@@ -462,11 +503,12 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
     for (int k = 0; k < D; ++k) xi[k] = a.xw32[i * D + k];
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
     const float lo = a.bp.lo, hi = a.bp.hi;
-    for (int64_t k = kb + wave; k < ke; k += 4) {
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float *__restrict__ xc = a.xw32 + C * 64 * D;
         uint64_t m = 0, band = 0;
@@ -483,7 +525,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
             band |= (uint64_t)(!ok && !(s > hi)) << jj;
         }
         if (band) m = band_fix(a, i, C, m, band, hits, flips, maxerr, xc);
-        mydeg += store_mask(a, k, i, C, m);
+        mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i, mydeg);
@@ -497,23 +539,24 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
     const int d = a.d;
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
-    for (int64_t k = kb + wave; k < ke; k += 4) {
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float *__restrict__ xc = a.xw32 + C * 64 * d;
         float acc[64];
 #pragma unroll
         for (int jj = 0; jj < 64; ++jj) acc[jj] = 0.0f;
-        for (int k0 = 0; k0 < d; k0 += 8) {
+        for (int q = 0; q < d; q += 8) {
             float xi[8];
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) xi[kk] = (k0 + kk < d) ? a.xw32[i * d + k0 + kk] : 0.0f;
-            const int kn = min(8, d - k0);
+            for (int kk = 0; kk < 8; ++kk) xi[kk] = (q + kk < d) ? a.xw32[i * d + q + kk] : 0.0f;
+            const int kn = min(8, d - q);
 #pragma unroll
             for (int jj = 0; jj < 64; ++jj) {
-                const float *__restrict__ xj = xc + (int64_t)jj * d + k0;
+                const float *__restrict__ xj = xc + (int64_t)jj * d + q;
                 float s = acc[jj];
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
@@ -526,7 +569,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
             }
         }
         uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr, xc);
-        mydeg += store_mask(a, k, i, C, m);
+        mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i, mydeg);
@@ -536,51 +579,81 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-// Squared norms in FP32 (sequential FMA over the padded row).
-__global__ void k_norms(const float *__restrict__ xw32, int64_t n_pad, int dp, float *__restrict__ wnorm) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Augmented FP32 rows for the Gram form.  With w = |x|^2 (sequential FMA) and
+// e = ka * w + kb (margin), c = -w/2 + r2/4 + e:
+//   row role    xra = [x_1 .. x_d, 1, c, 0 ...]
+//   column role xca = [x_1 .. x_d, c, 1, 0 ...]
+// so xra_i . xca_j = x_i.x_j + c_i + c_j = (r2 - d2_ij)/2 + e_i + e_j.
+__global__ void k_augment(const float *__restrict__ xw32, int64_t n, int64_t n_pad, int d, int dp, float r2q,
+                          float ka, float kb, float *__restrict__ xra, float *__restrict__ xca,
+                          float *__restrict__ emarg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
-    const float *x = xw32 + i * dp;
-    float s = 0.0f;
-    for (int k = 0; k < dp; ++k) s = __builtin_fmaf(x[k], x[k], s);
-    wnorm[i] = s;
+    const float *x = xw32 + i * d;
+    float *ra = xra + i * dp, *ca = xca + i * dp;
+    float w = 0.0f;
+    for (int k = 0; k < d; ++k) {
+        const float v = i < n ? x[k] : 0.0f;
+        w = __builtin_fmaf(v, v, w);
+        ra[k] = v;
+        ca[k] = v;
+    }
+    const float e = __builtin_fmaf(ka, w, kb);
+    const float c = (r2q - 0.5f * w) + e;
+    ra[d] = 1.0f;
+    ra[d + 1] = c;
+    ca[d] = c;
+    ca[d + 1] = 1.0f;
+    for (int k = d + 2; k < dp; ++k) {
+        ra[k] = 0.0f;
+        ca[k] = 0.0f;
+    }
+    emarg[i] = e;
 }
 
-// One wave per 64x64 tile pair (4 waves of a workgroup share the row tile).
-// Dot products x_i . x_j accumulate in FP32 on v_mfma_f32_32x32x2f32 as a 2x2
+// One wave per 64x64 tile pair (the 4 waves of a workgroup share the row tile).
+// T' = xra_i . xca_j accumulates in FP32 on v_mfma_f32_32x32x2f32 as a 2x2
 // grid of 32x32 blocks; lane l supplies row l%32 at k-slot l/32, and a float4
 // load per lane feeds four k-steps (dims 8q+4h .. 8q+4h+3, the same order for
-// rows and columns).  Accumulator element v of lane l is D[8(v/4) + 4(l/32) +
-// v%4][l%32].  Each element becomes d2 = (w_i + w_j) - 2 dot (one FMA); pairs
-// with |d2 - r2| <= ga (w_i + w_j) + gb are recomputed in FP64 in reference
-// order.  Row words are assembled from ballots (one ballot = two half rows).
+// rows and columns); the next block's operands are loaded while the current
+// block's MFMAs run.  Accumulator element v of lane l is T'[8(v/4) + 4(l/32) +
+// v%4][l%32].  T' < 0: certainly not a neighbour -- a tile pair with no
+// element >= 0 (the common case) costs one compare per element.  Otherwise the
+// accumulators go to LDS and each lane decides its own row: T' >= 2(e_i + e_j)
+// is certainly a neighbour, the rest is recomputed in FP64 in reference order.
+// The mirror pair's column words come from a bit transpose of the row words.
 __global__ __launch_bounds__(256) void k_dist_mfma(DistArgs a) {
-    __shared__ float s_wr[64];
+    __shared__ float sT[4][64 * 65];
+    __shared__ float s_e[64];
+    __shared__ float s_ec[4][64];
     const int64_t R = a.R0 + blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 31, h = lane >> 5;
     const int64_t i = R * 64 + lane;
-    if (threadIdx.x < 64) s_wr[threadIdx.x] = a.wnorm[R * 64 + threadIdx.x];
+    if (threadIdx.x < 64) s_e[threadIdx.x] = a.emarg[R * 64 + threadIdx.x];
     __syncthreads();
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     const int dp = a.dp;
-    const float r2f = (float)a.bp.r2;
-    const float *__restrict__ xa0 = a.xw32 + (R * 64 + r) * dp + 4 * h;
+    const float *__restrict__ xa0 = a.xra + (R * 64 + r) * dp + 4 * h;
     const float *__restrict__ xa1 = xa0 + (int64_t)32 * dp;
+    float *T = sT[wave];
+    const float er = s_e[lane];
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
-    for (int64_t k = kb + wave; k < ke; k += 4) {
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
-        const float *__restrict__ xb0 = a.xw32 + (C * 64 + r) * dp + 4 * h;
+        const float *__restrict__ xb0 = a.xca + (C * 64 + r) * dp + 4 * h;
         const float *__restrict__ xb1 = xb0 + (int64_t)32 * dp;
         v16f c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
+        float4 A0 = *(const float4 *)(xa0), A1 = *(const float4 *)(xa1);
+        float4 B0 = *(const float4 *)(xb0), B1 = *(const float4 *)(xb1);
         for (int q = 0; q < dp; q += 8) {
-            const float4 A0 = *(const float4 *)(xa0 + q);
-            const float4 A1 = *(const float4 *)(xa1 + q);
-            const float4 B0 = *(const float4 *)(xb0 + q);
-            const float4 B1 = *(const float4 *)(xb1 + q);
+            const int qn = (q + 8 < dp) ? q + 8 : q;  // last block reloads itself (in bounds, unused)
+            const float4 nA0 = *(const float4 *)(xa0 + qn), nA1 = *(const float4 *)(xa1 + qn);
+            const float4 nB0 = *(const float4 *)(xb0 + qn), nB1 = *(const float4 *)(xb1 + qn);
             c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B0.x, c00, 0, 0, 0);
             c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B1.x, c01, 0, 0, 0);
             c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B0.x, c10, 0, 0, 0);
@@ -597,44 +670,126 @@ __global__ __launch_bounds__(256) void k_dist_mfma(DistArgs a) {
             c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B1.w, c01, 0, 0, 0);
             c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B0.w, c10, 0, 0, 0);
             c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B1.w, c11, 0, 0, 0);
+            A0 = nA0; A1 = nA1; B0 = nB0; B1 = nB1;
         }
-        const float wc0 = a.wnorm[C * 64 + r];
-        const float wc1 = a.wnorm[C * 64 + 32 + r];
-        uint64_t word = 0;
-        // rb, cb: block row/column; v: accumulator element
-#define CNN_MFMA_EPILOGUE(ACC, RB, CB, WC)                                                           \
-        _Pragma("unroll") for (int v = 0; v < 16; ++v) {                                              \
-            const int row = (RB) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);                               \
-            const float wsum = s_wr[row] + (WC);                                                      \
-            const float d2 = __builtin_fmaf(-2.0f, (ACC)[v], wsum);                                   \
-            const float delta = __builtin_fmaf(a.ga, wsum, a.gb);                                     \
-            bool in = d2 < r2f - delta;                                                               \
-            const bool band = !in && !(d2 > r2f + delta);                                             \
-            if (band) {                                                                               \
-                const int64_t gi = R * 64 + row, gj = C * 64 + (CB) * 32 + r;                         \
-                if (gi < a.n && gj < a.n) {                                                           \
-                    const double e = exact_d2(a.xw64, a.d, gi, gj);                                   \
-                    in = e <= a.bp.r2;                                                                \
-                    ++hits;                                                                           \
-                    flips += (in != ((double)d2 <= a.bp.r2));                                         \
-                    const double err = fabs((double)d2 - e);                                          \
-                    if (err == err) maxerr = fmax(maxerr, err);                                       \
-                }                                                                                     \
-            }                                                                                         \
-            const uint64_t b = __ballot(in);                                                          \
-            const int row0 = (RB) * 32 + 8 * (v >> 2) + (v & 3);                                      \
-            if (lane == row0) word |= (b & 0xffffffffull) << (32 * (CB));                             \
-            if (lane == row0 + 4) word |= (b >> 32) << (32 * (CB));                                   \
+        // fast path: no element with T' >= 0 (NaN compares as a candidate)
+        bool cand = false;
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            cand |= !(c00[v] < 0.0f) || !(c01[v] < 0.0f) || !(c10[v] < 0.0f) || !(c11[v] < 0.0f);
+        uint64_t word = 0, colword = 0;
+        if (__ballot(cand)) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = 8 * (v >> 2) + 4 * h + (v & 3);
+                T[row * 65 + r] = c00[v];
+                T[row * 65 + 32 + r] = c01[v];
+                T[(row + 32) * 65 + r] = c10[v];
+                T[(row + 32) * 65 + 32 + r] = c11[v];
+            }
+            s_ec[wave][lane] = a.emarg[C * 64 + lane];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int64_t gi = R * 64 + lane;
+            for (int j = 0; j < 64; ++j) {
+                const float t = T[lane * 65 + j];
+                if (!(t < 0.0f)) {
+                    const float ej = s_ec[wave][j];
+                    bool in = t >= 2.0f * (er + ej);
+                    if (!in) {
+                        const int64_t gj = C * 64 + j;
+                        if (gi < a.n && gj < a.n) {
+                            const double e = exact_d2(a.xw64, a.d, gi, gj);
+                            in = e <= a.bp.r2;
+                            ++hits;
+                            const double d2 = a.bp.r2 - 2.0 * ((double)t - (double)(er + ej));
+                            flips += (in != (d2 <= a.bp.r2));
+                            const double err = fabs(d2 - e);
+                            if (err == err) maxerr = fmax(maxerr, err);
+                        }
+                    }
+                    word |= (uint64_t)in << j;
+                }
+            }
+            if (a.sym && C != R) colword = transpose_bits(word & valid_cols(a.n, C));
+            __builtin_amdgcn_wave_barrier();
         }
-        CNN_MFMA_EPILOGUE(c00, 0, 0, wc0)
-        CNN_MFMA_EPILOGUE(c01, 0, 1, wc1)
-        CNN_MFMA_EPILOGUE(c10, 1, 0, wc0)
-        CNN_MFMA_EPILOGUE(c11, 1, 1, wc1)
-#undef CNN_MFMA_EPILOGUE
-        mydeg += store_mask(a, k, i, C, word);
+        mydeg += store_pair(a, k, R, i, C, word, true, colword);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i, mydeg);
+}
+
+// Plain FP64 for the dense layout with 4 < d <= KMAX, on a copy of the
+// coordinates zero-padded to KMAX dims: the padded terms add (0 - 0)^2 = +0 at
+// the end of the reference-order sum, which leaves every s_ij unchanged.  The
+// row point lives in VGPRs, column points stream through wave-uniform (scalar)
+// loads.  A workgroup covers 4 consecutive row tiles (one per wave) and a chunk
+// of column tiles, so the 4 waves read the same column rows (scalar-cache
+// reuse); the chunking balances the triangle of the symmetric evaluation.
+// Degrees are accumulated atomically (several chunks contribute to a row).
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double *__restrict__ xp, int64_t n_chunks) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t rg = blockIdx.x / n_chunks, cc = blockIdx.x % n_chunks;
+    const int64_t R = rg * 4 + wave;
+    const int64_t T = a.n_tiles;
+    if (R >= T) return;
+    const int64_t i = R * 64 + lane;
+    double xi[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) xi[k] = xp[i * KMAX + k];
+    const int64_t c_lo = cc * a.col_chunk, c_hi = min(T, (cc + 1) * a.col_chunk);
+    const int64_t c_begin = a.sym ? max(c_lo, R) : c_lo;
+    int mydeg = 0;
+    for (int64_t C = c_begin; C < c_hi; ++C) {
+        const double *__restrict__ xc = xp + uniform64(C) * 64 * KMAX;
+        uint64_t m = 0;
+#pragma unroll 2
+        for (int jj = 0; jj < 64; jj += 2) {
+            const double *__restrict__ x0 = xc + jj * KMAX;
+            const double *__restrict__ x1 = x0 + KMAX;
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                const double t0 = xi[k] - x0[k];
+                const double t1 = xi[k] - x1[k];
+                s0 = s0 + t0 * t0;
+                s1 = s1 + t1 * t1;
+            }
+            m |= (uint64_t)(s0 <= a.bp.r2) << jj;
+            m |= (uint64_t)(s1 <= a.bp.r2) << (jj + 1);
+        }
+        mydeg += store_pair(a, R * T + C, R, i, C, m, false, 0);
+    }
+    if (i < a.n && mydeg) atomicAdd(&a.deg[i], mydeg);
+}
+
+__global__ void k_pad_rows(const double *__restrict__ xw64, int64_t n_pad, int d, int kmax, double *__restrict__ xp) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_pad * kmax) return;
+    const int64_t w = idx / kmax;
+    const int k = (int)(idx - w * kmax);
+    xp[idx] = k < d ? xw64[w * d + k] : 0.0;
+}
+
+// Mirror index of every candidate pair (R, C): position of R in C's list.
+__global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col, int64_t n_tiles,
+                         int64_t *__restrict__ mirror) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    for (int64_t k = tp_ptr[R] + lane; k < tp_ptr[R + 1]; k += 64) {
+        const int64_t C = tp_col[k];
+        int64_t lo = tp_ptr[C], hi = tp_ptr[C + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (tp_col[mid] < R) lo = mid + 1;
+            else hi = mid;
+        }
+        mirror[k] = lo;
+    }
 }
 
 // ------------------------------------------------------------------ CSR emit
@@ -733,35 +888,55 @@ static BandParams make_band_direct(double r2, int d, bool rounded_coords, double
     return bp;
 }
 
-// Guard band of the Gram form d2 = (w_i + w_j) - 2 x.y in FP32 (w = squared
-// norm by sequential FMA, x.y by MFMA in any summation order):
-//   |w_hat - w| <= gamma_d w,  |dot_hat - dot| <= gamma_d sqrt(w_i w_j) <= gamma_d (w_i + w_j) / 2,
-//   the sum w_i + w_j and the final FMA add u (w_i + w_j) and u d2,
-// so |d2_hat - d2| <= (2 gamma_d + 2u)(w_i + w_j) + u D; the reference's FP64
-// rounding adds gamma_{2d+2}(u64) D, D = 2 r2 near the threshold.  For f64
-// input rounded to f32, rounding the coordinates adds
-// 2u ||x_i - x_j|| (||x_i|| + ||x_j||) + u^2 (..)^2 <= u (2 (w_i + w_j) + D) + 2 u^2 (w_i + w_j).
-// Safety factor 4; gb also absorbs rounding r2 to FP32 and subnormal flushes.
-static void make_band_gram(int d, bool rounded_coords, double r2, float &ga, float &gb) {
+// Margin of the augmented Gram form.  T' = xra_i . xca_j is an FP32 sum of
+// dp products in any order (MFMA), so with Sabs = sum |x_ik x_jk| + |c_i| + |c_j|
+//   |T'_hat - (x_i.x_j + c_i_hat + c_j_hat)| <= gamma_dp Sabs,
+//   Sabs <= (w_i + w_j) + r2/2 + e_i + e_j,
+// and c_hat = -w_hat/2 + r2/4 + e carries gamma_d w/2 from the norm plus
+// 2u |c| from its two FP32 additions.  In T units (T = (r2 - d2)/2) the
+// reference's FP64 rounding adds gamma_{2d+2}(u64) r2 near the threshold, and
+// rounding f64 coordinates to f32 adds u (w_i + w_j) + u r2.  Choosing
+//   e_i = S (A w_i + B r2 / 2),  A = gamma_dp + gamma_d/2 + 2u (+u),
+//                                B = gamma_dp/2 + 2u + gamma64 (+u),
+// with S = 4 makes e_i + e_j exceed the total error, so T' < 0 proves
+// d2 > r2 and T' >= 2 (e_i + e_j) proves d2 < r2 in the reference's FP64.
+static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float &ka, float &kb) {
     const double u = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
-    const int dd = ((d + 7) / 8) * 8;
-    double A = 2.0 * gamma_m(dd, u) + 2.0 * u;
-    double B = (u + gamma_m(2 * d + 2, u64)) * 2.0 * r2;
+    double A = gamma_m(dp, u) + 0.5 * gamma_m(d, u) + 2.0 * u;
+    double B = 0.5 * gamma_m(dp, u) + 2.0 * u + gamma_m(2 * d + 2, u64);
     if (rounded_coords) {
-        A += 2.0 * u + 2.0 * u * u;
-        B += u * 2.0 * r2;
+        A += u;
+        B += u;
     }
-    A *= 4.0;
-    B = 4.0 * B + 4.0 * u * r2 + 1e-30;
-    ga = (float)(A * (1.0 + 1e-6));
-    gb = (float)(B * (1.0 + 1e-6));
+    const double S = 4.0;
+    ka = (float)(S * A);
+    kb = (float)(S * B * 0.5 * r2 + 1e-30);
+}
+
+// Padded width of the wide FP64 kernel (dense layout, one batch), 0 if unused.
+static int wide_kmax(const DistArgs &a) {
+    if (a.tp_ptr || a.d <= 4 || a.d > 64 || a.R0 != 0) return 0;
+    return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
 static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band) {
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
     if (band && a.d >= kMfmaMinD) {
+        (void)0;
         k_dist_mfma<<<grid, block, 0, s>>>(a);
+    } else if (!band && wide_kmax(a) > 0) {
+        const int kmax = wide_kmax(a);
+        const int64_t n_pad = a.n_tiles * 64;
+        Scratch<double> xp(ctx, n_pad * kmax);
+        k_pad_rows<<<(unsigned)ceil_div(n_pad * kmax, 256), 256, 0, s>>>(a.xw64, n_pad, a.d, kmax, xp.p);
+        const int64_t n_rg = ceil_div(a.n_tiles, 4);
+        const int64_t n_chunks = ceil_div(a.n_tiles, a.col_chunk);
+        dim3 g2((unsigned)(n_rg * n_chunks));
+        if (kmax == 8) k_dist_f64_wide<8><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
+        else if (kmax == 16) k_dist_f64_wide<16><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
+        else if (kmax == 32) k_dist_f64_wide<32><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
+        else k_dist_f64_wide<64><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
     } else if (!band) {
         switch (a.d) {
             case 1: k_dist_f64_small<1><<<grid, block, 0, s>>>(a); break;
@@ -898,29 +1073,27 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
     }
     const bool gram = band && d >= kMfmaMinD;
-    const int dp = gram ? ((d + 7) / 8) * 8 : d;  // MFMA rows padded to 8 dims (zeros)
+    const int dp = gram ? ((d + 2 + 7) / 8) * 8 : d;  // augmented MFMA rows: d + 2 dims, padded to 8
     g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
-    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(dp, 1));
-    g->dp = dp;
+    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
+    g->dp = d;
     {
         const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * std::max(d, 1), 256), 16384);
-        if (gram) {
-            CNN_HIP(hipMemsetAsync(g->xw32, 0, (size_t)n_pad * dp * sizeof(float), s));
-            if (dtype == CNN_F32)
-                k_gather_padded<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, dp, g->xw64, g->xw32);
-            else
-                k_gather_padded<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, dp, g->xw64, g->xw32);
-        } else if (dtype == CNN_F32) {
+        if (dtype == CNN_F32)
             k_gather<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
-        } else {
+        else
             k_gather<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
-        }
         CNN_HIP(hipGetLastError());
     }
-    std::unique_ptr<Scratch<float>> wnorm;
+    std::unique_ptr<Scratch<float>> xra, xca, emarg;
+    float gka = 0.0f, gkb = 0.0f;
     if (gram) {
-        wnorm.reset(new Scratch<float>(ctx, n_pad));
-        k_norms<<<(unsigned)ceil_div(n_pad, 256), 256, 0, s>>>(g->xw32, n_pad, dp, wnorm->p);
+        make_band_gram(d, dp, dtype == CNN_F64, g->r2, gka, gkb);
+        xra.reset(new Scratch<float>(ctx, n_pad * dp));
+        xca.reset(new Scratch<float>(ctx, n_pad * dp));
+        emarg.reset(new Scratch<float>(ctx, n_pad));
+        k_augment<<<(unsigned)ceil_div(n_pad, 256), 256, 0, s>>>(g->xw32, n, n_pad, d, dp, (float)(0.25 * g->r2), gka,
+                                                                 gkb, xra->p, xca->p, emarg->p);
         CNN_HIP(hipGetLastError());
     }
 
@@ -954,6 +1127,12 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
                                                                     tp_ptr, tp_col);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipStreamSynchronize(s));
+    }
+    std::unique_ptr<Scratch<int64_t>> mirror;
+    if (g->layout == 1) {
+        mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        k_mirror<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p);
+        CNN_HIP(hipGetLastError());
     }
     std::vector<int64_t> h_tp_ptr;
     if (g->layout == 1) {
@@ -991,6 +1170,8 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     for (auto &b : batches) max_batch_pairs = std::max(max_batch_pairs, pair_begin(b.second) - pair_begin(b.first));
     Scratch<uint64_t> mask(ctx, std::max<int64_t>(max_batch_pairs, 1) * 64);
 
+    // symmetric evaluation needs every mask of the graph in one buffer
+    const int sym = batches.size() == 1 ? 1 : 0;
     float ms_dist = 0.0f, ms_emit = 0.0f;
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
@@ -1012,9 +1193,14 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         a.bp = bp;
         a.band_stats = bstats.p;
         a.dp = dp;
-        a.wnorm = wnorm ? wnorm->p : nullptr;
-        if (gram) make_band_gram(d, dtype == CNN_F64, g->r2, a.ga, a.gb);
+        a.sym = sym;
+        a.mirror = mirror ? mirror->p : nullptr;
+        a.xra = xra ? xra->p : nullptr;
+        a.xca = xca ? xca->p : nullptr;
+        a.emarg = emarg ? emarg->p : nullptr;
+        a.col_chunk = 32;
         CNN_HIP(hipEventRecord(ctx->ev[2], s));
+        fill_i32(ctx, deg.p, 0, n);
         launch_dist(ctx, a, b.second - b.first, band);
         CNN_HIP(hipEventRecord(ctx->ev[3], s));
         // row_ptr for this batch's rows: exclusive scan + running offset
