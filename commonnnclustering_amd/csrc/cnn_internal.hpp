@@ -55,7 +55,8 @@ struct cnn_ctx {
 
 struct cnn_graph {
     int device = 0;
-    int64_t n = 0;
+    int64_t n = 0;               // rows of the working order (packed batches include padding rows)
+    int64_t n_points = 0;        // caller points
     int32_t d = 0;
     int32_t dtype = CNN_F32;
     int32_t precision = CNN_PLAIN_F64;
@@ -167,6 +168,17 @@ void graph_free(cnn_graph *g);
 void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col);
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);
 
+// Host-side description of a packed batch graph.
+struct PackedHost {
+    std::vector<int64_t> tile_start;  // first tile of each dataset (n_sets + 1 entries)
+    std::vector<int32_t> tile_ds;     // dataset of each tile
+};
+// Many datasets (rows offsets[b] .. offsets[b+1] of X, own r2[b]) in one graph:
+// each dataset is Morton-ordered and padded to whole tiles (padding rows have
+// perm -1 and NaN coordinates); candidate tile pairs never cross datasets.
+cnn_graph *graph_build_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d,
+                              int32_t dtype, const double *r2, int32_t precision, PackedHost &ph);
+
 struct LabelRequest {
     const int64_t *row_ptr;
     const int32_t *col;
@@ -186,6 +198,22 @@ struct LabelRequest {
     const int64_t *tp_ptr = nullptr;  // candidate column tiles per row tile
     const int32_t *tp_col = nullptr;
     int64_t n_tiles = 0;              // 0: no tile structure (caller CSR)
+    // packed batches: per-tile cutoffs and the per-dataset cluster ordering
+    const int32_t *tile_csim = nullptr;
+    const int32_t *tile_mcore = nullptr;
+    const struct PackedSort *packed = nullptr;
+};
+
+// Sort key layout of a packed batch: dataset << ds_shift | (size_max - size)
+// << size_shift | smallest original index (relative to ds_offset when set).
+struct PackedSort {
+    int n_sets;
+    int ds_shift;
+    int size_shift;
+    uint32_t size_max;
+    int key_bits;
+    const int32_t *tile_ds;     // device, per tile
+    const int64_t *ds_offset;   // device, per dataset (nullable)
 };
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st);
 int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,

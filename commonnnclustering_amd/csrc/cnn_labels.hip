@@ -112,7 +112,13 @@ struct LabelArgs {
     unsigned long long *stats;  // [slot][0] tests, [slot][1] passed
     const uint64_t *rank;       // snapshot rank of core points: (component size << 32) | root; 0 otherwise
     const uint64_t *cand_max;   // per row tile: max rank over its candidate tiles (nullable)
+    const int32_t *tile_csim;   // packed batches: c_sim / m_core per row tile (nullable)
+    const int32_t *tile_mcore;
 };
+
+__device__ __forceinline__ int csim_of(const LabelArgs &a, int64_t w) {
+    return a.tile_csim ? a.tile_csim[w >> 6] : a.c_sim;
+}
 
 __device__ __forceinline__ void add_stats(const LabelArgs &a, unsigned tests, unsigned passed, unsigned rows = 0) {
     if ((threadIdx.x & 63) == 0 && (tests || rows)) {
@@ -129,21 +135,23 @@ __global__ void k_init(LabelArgs a, const uint8_t *__restrict__ consider, const 
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n) return;
     const int64_t deg = a.row_ptr[w + 1] - a.row_ptr[w];
-    bool ok = deg > (int64_t)a.m_core;
-    if (consider) ok = ok && consider[perm ? perm[w] : w] != 0;
+    const int m_core = a.tile_mcore ? a.tile_mcore[w >> 6] : a.m_core;
+    const int32_t o = perm ? perm[w] : (int32_t)w;  // -1: padding row of a packed batch
+    bool ok = deg > (int64_t)m_core && o >= 0;
+    if (consider && ok) ok = consider[o] != 0;
     a.core[w] = ok ? 1 : 0;
     a.parent[w] = (int)w;
 }
 
 // Test edge (w, j) unless the endpoints are already joined; link on a pass.
 // Returns true when w and j are in one component afterwards.
-__device__ __forceinline__ bool test_and_link(const LabelArgs &a, int w, int j, int64_t bw, int64_t ew,
+__device__ __forceinline__ bool test_and_link(const LabelArgs &a, int w, int j, int64_t bw, int64_t ew, int c,
                                               unsigned &tests, unsigned &passed) {
     const int rw = uf_find(a.parent, w);
     const int rj = uf_find(a.parent, j);
     if (rw == rj) return true;
     const int64_t bj = a.row_ptr[j], ej = a.row_ptr[j + 1];
-    const bool pass = wave_sim(a.col + bw, ew - bw, a.col + bj, ej - bj, a.c_sim);
+    const bool pass = wave_sim(a.col + bw, ew - bw, a.col + bj, ej - bj, c);
     ++tests;
     if (pass) {
         ++passed;
@@ -184,7 +192,7 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
             const int64_t bj = a.row_ptr[jj], ej = a.row_ptr[jj + 1];
             ++attempts;
             ++tests;
-            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, a.c_sim)) {
+            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, csim_of(a, w))) {
                 ++passed;
                 if (lane == 0) a.parent[w] = jj;
                 done = true;
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
             const int64_t bj = a.row_ptr[jj], ej = a.row_ptr[jj + 1];
             ++attempts;
             ++tests;
-            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, a.c_sim)) {
+            if (wave_sim(a.col + b, e - b, a.col + bj, ej - bj, csim_of(a, w))) {
                 ++passed;
                 if (lane == 0) a.parent[w] = rr;
                 done = true;
@@ -260,6 +268,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     if (rw == 0) return;
     if (a.cand_max && rw >= a.cand_max[w >> 6]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
     for (int64_t t0 = b; t0 < e; t0 += 64) {
@@ -274,7 +283,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
             bits &= bits - 1;
             const int jj = __shfl(j, src, 64);
             const int kjj = __shfl(kj, src, 64);
-            if (test_and_link(a, (int)w, jj, b, e, tests, passed)) {
+            if (test_and_link(a, (int)w, jj, b, e, c, tests, passed)) {
                 // w has joined snapshot component kjj: its other entries are joined too
                 resolved = kjj;
                 bits &= ~__ballot(kj == kjj);
@@ -405,12 +414,57 @@ __global__ void k_rank_clusters(const int32_t *vals, int64_t K, int64_t max_clus
     newlab[vals[t]] = (max_clusters < 0 || t < max_clusters) ? (int32_t)(t + 1) : 0;
 }
 
+// Packed batches: clusters of all datasets sorted together by
+// (dataset, size descending, smallest original index); ranks restart at 1 in
+// every dataset.
+struct PackedKey {
+    int ds_shift, size_shift;
+    uint32_t size_max;
+    const int32_t *tile_ds;
+    const int64_t *ds_offset;  // cmin is stored relative to the dataset's first row when set
+};
+
+__global__ void k_collect_clusters_packed(const uint8_t *core, const int *parent, const int32_t *cmin,
+                                          const int32_t *csize, int64_t n, int64_t member_cutoff, PackedKey pk,
+                                          unsigned long long *K, uint64_t *keys, int32_t *vals) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n || !core[w] || parent[w] != (int)w) return;
+    const int32_t sz = csize[w];
+    if ((int64_t)sz < member_cutoff) return;
+    const int ds = pk.tile_ds[w >> 6];
+    const uint64_t cm = (uint64_t)(cmin[w] - (pk.ds_offset ? pk.ds_offset[ds] : 0));
+    const unsigned long long idx = atomicAdd(K, 1ull);
+    keys[idx] = ((uint64_t)ds << pk.ds_shift) | ((uint64_t)(pk.size_max - (uint32_t)sz) << pk.size_shift) | cm;
+    vals[idx] = (int32_t)w;
+}
+
+__global__ void k_first_of_set(const uint64_t *keys, int64_t K, int ds_shift, int32_t *first) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K) return;
+    const uint64_t ds = keys[t] >> ds_shift;
+    if (t == 0 || (keys[t - 1] >> ds_shift) != ds) first[ds] = (int32_t)t;
+}
+
+__global__ void k_rank_clusters_packed(const uint64_t *keys, const int32_t *vals, int64_t K, int ds_shift,
+                                       const int32_t *first, int64_t max_clusters, int32_t *newlab,
+                                       unsigned long long *kept) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    if (t < K) {
+        const int64_t r = t - first[keys[t] >> ds_shift] + 1;
+        keep = max_clusters < 0 || r <= max_clusters;
+        newlab[vals[t]] = keep ? (int32_t)r : 0;
+    }
+    const uint64_t b = __ballot(keep);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(kept, (unsigned long long)__popcll(b));
+}
+
 __global__ void k_write_sorted(const uint8_t *core, const int *parent, const int32_t *newlab, const int32_t *perm,
                                int64_t n, int64_t *labels) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     const int64_t o = perm ? perm[w] : w;
-    labels[o] = core[w] ? (int64_t)newlab[parent[w]] : 0;
+    if (o >= 0) labels[o] = core[w] ? (int64_t)newlab[parent[w]] : 0;
 }
 
 __global__ void k_mark_first(const uint8_t *core, const int *parent, const int32_t *cmin, int64_t n,
@@ -426,6 +480,7 @@ __global__ void k_write_unsorted(const uint8_t *core, const int *parent, const i
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     const int64_t o = perm ? perm[w] : w;
+    if (o < 0) return;
     if (core[w]) labels[o] = current_start + (int64_t)rank_incl[cmin[parent[w]]] - 1;
     else if (write_all) labels[o] = 0;
 }
@@ -449,7 +504,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     Scratch<unsigned long long> counters(ctx, 8), tstats(ctx, 4 * kStatSlots);
     CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
     CNN_HIP(hipMemsetAsync(tstats.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
-    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr};
+    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr,
+                req.tile_csim, req.tile_mcore};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
@@ -497,12 +553,33 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n, req.member_cutoff,
                                                          counters.p + 5, keys.p, vals.p);
         CNN_HIP(hipGetLastError());
-        const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
-        sort_pairs_u64_i32(ctx, keys.p, vals.p, K, 63);
-        if (K > 0) k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(vals.p, K, req.max_clusters, newlab.p);
-        k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
-        CNN_HIP(hipGetLastError());
-        local.n_clusters = (req.max_clusters >= 0) ? std::min<int64_t>(K, req.max_clusters) : K;
+        if (req.packed) {
+            const PackedSort &ps = *req.packed;
+            PackedKey pk{ps.ds_shift, ps.size_shift, ps.size_max, ps.tile_ds, ps.ds_offset};
+            k_collect_clusters_packed<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
+                                                                    req.member_cutoff, pk, counters.p + 5, keys.p,
+                                                                    vals.p);
+            CNN_HIP(hipGetLastError());
+            const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
+            sort_pairs_u64_i32(ctx, keys.p, vals.p, K, ps.key_bits);
+            if (K > 0) {
+                Scratch<int32_t> first(ctx, ps.n_sets);
+                k_first_of_set<<<blocks_for(K), 256, 0, s>>>(keys.p, K, ps.ds_shift, first.p);
+                k_rank_clusters_packed<<<blocks_for(K), 256, 0, s>>>(keys.p, vals.p, K, ps.ds_shift, first.p,
+                                                                     req.max_clusters, newlab.p, counters.p + 7);
+                CNN_HIP(hipGetLastError());
+            }
+            k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
+            CNN_HIP(hipGetLastError());
+            local.n_clusters = (int64_t)read_scalar(ctx, counters.p + 7);
+        } else {
+            const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
+            sort_pairs_u64_i32(ctx, keys.p, vals.p, K, 63);
+            if (K > 0) k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(vals.p, K, req.max_clusters, newlab.p);
+            k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
+            CNN_HIP(hipGetLastError());
+            local.n_clusters = (req.max_clusters >= 0) ? std::min<int64_t>(K, req.max_clusters) : K;
+        }
     } else {
         Scratch<uint8_t> isfirst(ctx, n);
         Scratch<int32_t> rank(ctx, n);

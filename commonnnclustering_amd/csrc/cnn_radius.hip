@@ -192,22 +192,26 @@ __device__ __forceinline__ bool boxes_close(const double *__restrict__ alo, cons
 template <bool FILL>
 __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi,
                              const double *__restrict__ slo, const double *__restrict__ shi, int d, int64_t n_tiles,
-                             int64_t n_super, double thr, int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
+                             int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
+                             const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
+                             int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
                              int32_t *__restrict__ tp_col) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
     const double *rlo = blo + R * d, *rhi = bhi + R * d;
+    const double thr = tile_thr ? tile_thr[R] : thr_all;
+    const int64_t clo = seg_lo ? seg_lo[R] : 0, chi = seg_hi ? seg_hi[R] : n_tiles;
     int64_t out = FILL ? tp_ptr[R] : 0;
-    for (int64_t S0 = 0; S0 < n_super; S0 += 64) {
+    for (int64_t S0 = clo / 64; S0 * 64 < chi; S0 += 64) {
         const int64_t Sl = S0 + lane;
-        const bool sok = (Sl < n_super) && boxes_close(rlo, rhi, slo + Sl * d, shi + Sl * d, d, thr);
+        const bool sok = (Sl < n_super) && (Sl * 64 < chi) && boxes_close(rlo, rhi, slo + Sl * d, shi + Sl * d, d, thr);
         uint64_t sbits = __ballot(sok);
         while (sbits) {
             const int64_t S = S0 + __ffsll((unsigned long long)sbits) - 1;
             sbits &= sbits - 1;
             const int64_t C = S * 64 + lane;
-            const bool ok = (C < n_tiles) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
+            const bool ok = (C >= clo) && (C < chi) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
             const uint64_t b = __ballot(ok);
             if (FILL && ok) tp_col[out + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)C;
             out += __popcll(b);
@@ -250,7 +254,14 @@ struct DistArgs {
     const float *emarg;
     // column chunking of the dense wide FP64 kernel (tiles per workgroup)
     int64_t col_chunk;
+    // packed batches: radius and FP32 band per row tile (nullptr: bp for all)
+    const double *tile_r2;
+    const float *tile_lo;
+    const float *tile_hi;
+    const int32_t *tile_valid;  // packed batches: real rows per tile (the first ones); nullptr: rows < n
 };
+
+__device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
 
 __device__ __forceinline__ void tile_range(const DistArgs &a, int64_t R, int64_t &kb, int64_t &ke) {
     if (a.tp_ptr) {
@@ -271,6 +282,17 @@ __device__ __forceinline__ uint64_t valid_cols(int64_t n, int64_t C) {
     if (v >= 64) return ~0ull;
     if (v <= 0) return 0ull;
     return (1ull << v) - 1ull;
+}
+
+// Real rows of tile C as a bit mask, and whether working row i is real.
+__device__ __forceinline__ uint64_t tile_mask(const DistArgs &a, int64_t C) {
+    if (!a.tile_valid) return valid_cols(a.n, C);
+    const int v = a.tile_valid[C];
+    return v >= 64 ? ~0ull : ((1ull << v) - 1ull);
+}
+
+__device__ __forceinline__ bool row_real(const DistArgs &a, int64_t i) {
+    return a.tile_valid ? (int)(i & 63) < a.tile_valid[i >> 6] : i < a.n;
 }
 
 // Exact FP64 reduced distance in reference order (rows of the working layout).
@@ -324,15 +346,15 @@ __device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
 // of the mirror pair).  Returns the row word's popcount.
 __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t R, int64_t i, int64_t C, uint64_t m,
                                           bool have_col, uint64_t colword) {
-    m &= valid_cols(a.n, C);
-    if (i >= a.n) m = 0;
+    m &= tile_mask(a, C);
+    if (!row_real(a, i)) m = 0;
     const int lane = threadIdx.x & 63;
     a.mask[(k - a.k_base) * 64 + lane] = m;
     if (a.sym && C != R) {
         if (!have_col) colword = transpose_bits(m);
         else {
-            colword &= valid_cols(a.n, R);       // rows of R beyond n
-            if (C * 64 + lane >= a.n) colword = 0;  // columns of C beyond n
+            colword &= tile_mask(a, R);                   // padding rows of R
+            if (!row_real(a, C * 64 + lane)) colword = 0;  // padding columns of C
         }
         const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
         a.mask[(km - a.k_base) * 64 + lane] = colword;
@@ -369,6 +391,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
     const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const double r2 = r2_of(a, R);
     int mydeg = 0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
@@ -382,7 +405,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
                 double t = xi[k2] - xc[jj * D + k2];
                 s = s + t * t;
             }
-            m |= (uint64_t)(s <= a.bp.r2) << jj;
+            m |= (uint64_t)(s <= r2) << jj;
         }
         mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
@@ -399,6 +422,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
     const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const double r2 = r2_of(a, R);
     int mydeg = 0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
@@ -427,7 +451,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
         }
         uint64_t m = 0;
 #pragma unroll
-        for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= a.bp.r2) << jj;
+        for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= r2) << jj;
         mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
@@ -435,16 +459,17 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
 
 // Resolve the band bits of one row: pairs whose FP32 value lies in [lo, hi]
 // (or is NaN) are recomputed in FP64 in reference order.
-__device__ __forceinline__ uint64_t band_fix(const DistArgs &a, int64_t i, int64_t C, uint64_t m, uint64_t band,
-                                             unsigned &hits, unsigned &flips, double &maxerr,
+__device__ __forceinline__ uint64_t band_fix(const DistArgs &a, double r2, int64_t i, int64_t C, uint64_t m,
+                                             uint64_t band, unsigned &hits, unsigned &flips, double &maxerr,
                                              const float *__restrict__ xc32) {
+    band &= tile_mask(a, C);
+    if (!row_real(a, i)) band = 0;
     while (band) {
         const int jj = __ffsll((unsigned long long)band) - 1;
         band &= band - 1;
-        const int64_t j = C * 64 + jj;
-        if (i < a.n && j < a.n) {
-            const double e = exact_d2(a.xw64, a.d, i, j);
-            const bool in = e <= a.bp.r2;
+        {
+            const double e = exact_d2(a.xw64, a.d, i, C * 64 + jj);
+            const bool in = e <= r2;
             m |= (uint64_t)in << jj;
             // recompute the FP32 value for the statistics
             float s32 = 0.0f;
@@ -453,7 +478,7 @@ __device__ __forceinline__ uint64_t band_fix(const DistArgs &a, int64_t i, int64
                 s32 = __builtin_fmaf(t, t, s32);
             }
             ++hits;
-            flips += (in != ((double)s32 <= a.bp.r2));
+            flips += (in != ((double)s32 <= r2));
             const double err = fabs((double)s32 - e);
             if (err == err) maxerr = fmax(maxerr, err);
         }
@@ -462,18 +487,18 @@ __device__ __forceinline__ uint64_t band_fix(const DistArgs &a, int64_t i, int64
 }
 
 // Resolve FP32 decisions from 64 accumulators (runtime-d kernel).
-__device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, int64_t i, int64_t C, const float *acc,
-                                                 unsigned &hits, unsigned &flips, double &maxerr,
-                                                 const float *__restrict__ xc32) {
+__device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, double r2, float lo, float hi, int64_t i,
+                                                 int64_t C, const float *acc, unsigned &hits, unsigned &flips,
+                                                 double &maxerr, const float *__restrict__ xc32) {
     uint64_t m = 0, band = 0;
 #pragma unroll
     for (int jj = 0; jj < 64; ++jj) {
         const float s = acc[jj];
-        const bool acc_ok = s < a.bp.lo;
+        const bool acc_ok = s < lo;
         m |= (uint64_t)acc_ok << jj;
-        band |= (uint64_t)(!acc_ok && !(s > a.bp.hi)) << jj;
+        band |= (uint64_t)(!acc_ok && !(s > hi)) << jj;
     }
-    if (band) m = band_fix(a, i, C, m, band, hits, flips, maxerr, xc32);
+    if (band) m = band_fix(a, r2, i, C, m, band, hits, flips, maxerr, xc32);
     return m;
 }
 
@@ -507,7 +532,8 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
-    const float lo = a.bp.lo, hi = a.bp.hi;
+    const float lo = a.tile_lo ? a.tile_lo[R] : a.bp.lo, hi = a.tile_hi ? a.tile_hi[R] : a.bp.hi;
+    const double r2 = r2_of(a, R);
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float *__restrict__ xc = a.xw32 + C * 64 * D;
@@ -524,7 +550,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
             m |= (uint64_t)ok << jj;
             band |= (uint64_t)(!ok && !(s > hi)) << jj;
         }
-        if (band) m = band_fix(a, i, C, m, band, hits, flips, maxerr, xc);
+        if (band) m = band_fix(a, r2, i, C, m, band, hits, flips, maxerr, xc);
         mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     flush_band(a, hits, flips, maxerr);
@@ -540,6 +566,8 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
     const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const float lo = a.tile_lo ? a.tile_lo[R] : a.bp.lo, hi = a.tile_hi ? a.tile_hi[R] : a.bp.hi;
+    const double r2 = r2_of(a, R);
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
@@ -568,7 +596,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
                 acc[jj] = s;
             }
         }
-        uint64_t m = band_resolve(a, i, C, acc, hits, flips, maxerr, xc);
+        uint64_t m = band_resolve(a, r2, lo, hi, i, C, acc, hits, flips, maxerr, xc);
         mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     flush_band(a, hits, flips, maxerr);
@@ -988,54 +1016,35 @@ struct GraphGuard {
     ~GraphGuard() { if (g) graph_free(g); }
 };
 
-cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
-                       int32_t precision) {
-    if (n < 0 || d < 0) throw InvalidArg{"negative size"};
-    if (n > 0 && d > 0 && !X) throw InvalidArg{"X is null"};
-    if (n >= (int64_t)INT32_MAX - 64) throw InvalidArg{"n exceeds the int32 index range"};
-    if (dtype != CNN_F32 && dtype != CNN_F64) throw InvalidArg{"dtype must be CNN_F32 or CNN_F64"};
-    if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
-    CNN_HIP(hipSetDevice(ctx->device));
+// Per-tile parameters of a packed batch (device arrays, one entry per tile).
+struct PackedTiles {
+    const int32_t *tile_valid = nullptr;
+    const double *tile_r2 = nullptr;
+    const double *tile_thr = nullptr;
+    const int32_t *seg_lo = nullptr;
+    const int32_t *seg_hi = nullptr;
+    const float *tile_lo = nullptr;
+    const float *tile_hi = nullptr;
+};
+
+// Bounding box of all finite coordinates and the largest squared norm.
+static void bbox(cnn_ctx *ctx, const void *X, int64_t n, int d, int dtype, std::vector<double> &lo,
+                 std::vector<double> &hi, double &max_norm2) {
     hipStream_t s = ctx->stream;
-    GraphGuard guard{new cnn_graph()};
-    cnn_graph *g = guard.g;
-    g->device = ctx->device;
-    g->n = n;
-    g->d = d;
-    g->dtype = dtype;
-    g->precision = precision;
-    g->r2 = r2;
-    g->radius = std::sqrt(r2);
-    g->n_tiles = ceil_div(n, 64);
-    const int64_t n_pad = g->n_tiles * 64;
-    const bool band = precision == CNN_F32_BAND;
-
-    CNN_HIP(hipEventRecord(ctx->ev[0], s));
-    g->row_ptr = dev_alloc<int64_t>(g, n + 1);
-    if (n == 0) {
-        fill_i64(ctx, g->row_ptr, 0, 1);
-        g->col = dev_alloc<int32_t>(g, 1);
-        CNN_HIP(hipStreamSynchronize(s));
-        g->stats = cnn_graph_stats{};
-        return std::exchange(guard.g, nullptr);
-    }
-
-    // bounding box and largest norm (host needs them for ordering and the band)
     const int nb = (int)std::min<int64_t>(256, ceil_div(n, 256));
     std::vector<double> part((size_t)nb * std::max(d, 1) * 2), pnorm(nb);
-    {
-        Scratch<double> dpart(ctx, part.size()), dnorm(ctx, nb);
-        if (dtype == CNN_F32)
-            k_bbox_partial<float><<<nb, 256, 0, s>>>((const float *)X, n, d, dpart.p, dnorm.p);
-        else
-            k_bbox_partial<double><<<nb, 256, 0, s>>>((const double *)X, n, d, dpart.p, dnorm.p);
-        CNN_HIP(hipGetLastError());
-        CNN_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipMemcpyAsync(pnorm.data(), dnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipStreamSynchronize(s));
-    }
-    std::vector<double> lo(d, INFINITY), hi(d, -INFINITY);
-    double max_norm2 = 0.0;
+    Scratch<double> dpart(ctx, part.size()), dnorm(ctx, nb);
+    if (dtype == CNN_F32)
+        k_bbox_partial<float><<<nb, 256, 0, s>>>((const float *)X, n, d, dpart.p, dnorm.p);
+    else
+        k_bbox_partial<double><<<nb, 256, 0, s>>>((const double *)X, n, d, dpart.p, dnorm.p);
+    CNN_HIP(hipGetLastError());
+    CNN_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipMemcpyAsync(pnorm.data(), dnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipStreamSynchronize(s));
+    lo.assign(d, INFINITY);
+    hi.assign(d, -INFINITY);
+    max_norm2 = 0.0;
     for (int b = 0; b < nb; ++b) {
         for (int k = 0; k < d; ++k) {
             lo[k] = std::fmin(lo[k], part[((size_t)b * d + k) * 2]);
@@ -1043,51 +1052,43 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         }
         max_norm2 = std::fmax(max_norm2, pnorm[b]);
     }
+}
 
-    // ordering: Morton order over the (up to) three widest dims for small d
-    g->layout = (d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
-    if (g->layout == 1) {
-        KeySpec ks{};
-        std::vector<int> dims(d);
-        for (int k = 0; k < d; ++k) dims[k] = k;
-        auto ext = [&](int k) { return (hi[k] >= lo[k]) ? hi[k] - lo[k] : 0.0; };
-        std::stable_sort(dims.begin(), dims.end(), [&](int x, int y) { return ext(x) > ext(y); });
-        ks.nk = std::min(d, 3);
-        const int bits = ks.nk == 1 ? 62 : (ks.nk == 2 ? 32 : 21);
-        ks.qmax = std::ldexp(1.0, bits) - 1.0;
-        for (int t = 0; t < ks.nk; ++t) {
-            int k = dims[t];
-            ks.dim[t] = k;
-            double e = ext(k);
-            ks.lo[t] = std::isfinite(lo[k]) ? lo[k] : 0.0;
-            ks.scale[t] = (e > 0.0 && std::isfinite(e)) ? ks.qmax / e : 0.0;
-        }
-        g->perm = dev_alloc<int32_t>(g, n);
-        Scratch<uint64_t> keys(ctx, n);
-        const unsigned blocks = (unsigned)ceil_div(n, 256);
-        if (dtype == CNN_F32)
-            k_morton<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
-        else
-            k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
-        CNN_HIP(hipGetLastError());
-        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
+// Morton key over the (up to) three widest dims, `bits` bits per dim.
+static KeySpec make_keyspec(int d, const std::vector<double> &lo, const std::vector<double> &hi, int max_bits) {
+    KeySpec ks{};
+    std::vector<int> dims(d);
+    for (int k = 0; k < d; ++k) dims[k] = k;
+    auto ext = [&](int k) { return (hi[k] >= lo[k]) ? hi[k] - lo[k] : 0.0; };
+    std::stable_sort(dims.begin(), dims.end(), [&](int x, int y) { return ext(x) > ext(y); });
+    ks.nk = std::min(d, 3);
+    const int bits = std::min(max_bits / ks.nk, ks.nk == 1 ? 62 : (ks.nk == 2 ? 32 : 21));
+    ks.qmax = std::ldexp(1.0, bits) - 1.0;
+    for (int t = 0; t < ks.nk; ++t) {
+        int k = dims[t];
+        ks.dim[t] = k;
+        double e = ext(k);
+        ks.lo[t] = std::isfinite(lo[k]) ? lo[k] : 0.0;
+        ks.scale[t] = (e > 0.0 && std::isfinite(e)) ? ks.qmax / e : 0.0;
     }
+    return ks;
+}
+
+// Candidate tile pairs, distance masks and CSR for a graph whose working
+// coordinates (xw64, xw32), perm and n_tiles are set.  pk: per-tile
+// parameters of a packed batch (nullptr: one dataset with radius g->r2).
+static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype, double max_norm2,
+                                const PackedTiles *pk, const BandParams *bp_in) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = g->n;
+    const int d = g->d;
+    const int64_t T = g->n_tiles;
+    const int64_t n_pad = T * 64;
     const bool gram = band && d >= kMfmaMinD;
     const int dp = gram ? ((d + 2 + 7) / 8) * 8 : d;  // augmented MFMA rows: d + 2 dims, padded to 8
-    g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
-    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
-    g->dp = d;
-    {
-        const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * std::max(d, 1), 256), 16384);
-        if (dtype == CNN_F32)
-            k_gather<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
-        else
-            k_gather<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
-        CNN_HIP(hipGetLastError());
-    }
     std::unique_ptr<Scratch<float>> xra, xca, emarg;
-    float gka = 0.0f, gkb = 0.0f;
     if (gram) {
+        float gka = 0.0f, gkb = 0.0f;
         make_band_gram(d, dp, dtype == CNN_F64, g->r2, gka, gkb);
         xra.reset(new Scratch<float>(ctx, n_pad * dp));
         xca.reset(new Scratch<float>(ctx, n_pad * dp));
@@ -1098,7 +1099,6 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     }
 
     // candidate tile pairs
-    const int64_t T = g->n_tiles;
     int64_t *tp_ptr = nullptr;
     int32_t *tp_col = nullptr;
     int64_t total_pairs = T * T;
@@ -1112,19 +1112,20 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         k_super_boxes<<<(unsigned)ceil_div(NS, 4), 256, 0, s>>>(blo.p, bhi.p, d, T, NS, slo.p, shi.p);
         CNN_HIP(hipGetLastError());
         Scratch<int32_t> cnt(ctx, T);
-        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, cnt.p,
-                                                                     nullptr, nullptr);
+        const double *tthr = pk ? pk->tile_thr : nullptr;
+        const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
+        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
+                                                                     sl, sh, cnt.p, nullptr, nullptr);
         CNN_HIP(hipGetLastError());
         tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
-        // tp_ptr[T] = tp_ptr[T-1] + cnt[T-1]
-        int64_t last_excl = read_scalar(ctx, tp_ptr + (T - 1));
-        int32_t last_cnt = read_scalar(ctx, cnt.p + (T - 1));
+        const int64_t last_excl = read_scalar(ctx, tp_ptr + (T - 1));
+        const int32_t last_cnt = read_scalar(ctx, cnt.p + (T - 1));
         total_pairs = last_excl + last_cnt;
         CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
         tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
-        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, nullptr,
-                                                                    tp_ptr, tp_col);
+        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
+                                                                    sl, sh, nullptr, tp_ptr, tp_col);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipStreamSynchronize(s));
     }
@@ -1144,7 +1145,9 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     CNN_HIP(hipEventRecord(ctx->ev[1], s));
 
     BandParams bp;
-    if (band) {
+    if (bp_in) {
+        bp = *bp_in;
+    } else if (band) {
         bp = make_band_direct(g->r2, d, dtype == CNN_F64, std::sqrt(max_norm2));
     } else {
         bp.r2 = g->r2;
@@ -1154,7 +1157,6 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     Scratch<unsigned long long> bstats(ctx, 4);
     CNN_HIP(hipMemsetAsync(bstats.p, 0, 4 * sizeof(unsigned long long), s));
     Scratch<int32_t> deg(ctx, n);
-    Scratch<int64_t> row_ptr_tmp(ctx, 1);
 
     // row-tile batches bounded by the mask budget (512 B per tile pair)
     const int64_t mask_budget = (int64_t)8 << 30;
@@ -1176,7 +1178,6 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
-    std::vector<int64_t> batch_nnz;
     for (auto &b : batches) {
         DistArgs a{};
         a.xw64 = g->xw64;
@@ -1199,6 +1200,12 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         a.xca = xca ? xca->p : nullptr;
         a.emarg = emarg ? emarg->p : nullptr;
         a.col_chunk = 32;
+        if (pk) {
+            a.tile_r2 = pk->tile_r2;
+            a.tile_lo = pk->tile_lo;
+            a.tile_hi = pk->tile_hi;
+            a.tile_valid = pk->tile_valid;
+        }
         CNN_HIP(hipEventRecord(ctx->ev[2], s));
         fill_i32(ctx, deg.p, 0, n);
         launch_dist(ctx, a, b.second - b.first, band);
@@ -1207,8 +1214,8 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
         const int64_t rows = row_hi - row_lo;
         exclusive_scan_i32_to_i64(ctx, deg.p + row_lo, g->row_ptr + row_lo, rows);
-        int64_t last_excl = read_scalar(ctx, g->row_ptr + row_hi - 1);
-        int32_t last_deg = read_scalar(ctx, deg.p + row_hi - 1);
+        const int64_t last_excl = read_scalar(ctx, g->row_ptr + row_hi - 1);
+        const int32_t last_deg = read_scalar(ctx, deg.p + row_hi - 1);
         const int64_t batch_total = last_excl + last_deg;
         if (nnz_total) {
             // shift by the entries of earlier batches
@@ -1221,7 +1228,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         }
         const int64_t need = nnz_total + batch_total;
         if (need > col_cap) {
-            int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
+            const int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
             int32_t *nc = nullptr;
             hipError_t e = hipMalloc(&nc, (size_t)std::max<int64_t>(new_cap, 1) * sizeof(int32_t));
             if (e != hipSuccess) {
@@ -1274,6 +1281,250 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     st.ms_emit = ms_emit;
     st.distance_launches = (int32_t)batches.size();
     st.layout = g->layout;
+}
+
+static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype, int32_t precision) {
+    if (n < 0 || d < 0) throw InvalidArg{"negative size"};
+    if (n > 0 && d > 0 && !X) throw InvalidArg{"X is null"};
+    if (n >= (int64_t)INT32_MAX - 64 * 4096) throw InvalidArg{"n exceeds the int32 index range"};
+    if (dtype != CNN_F32 && dtype != CNN_F64) throw InvalidArg{"dtype must be CNN_F32 or CNN_F64"};
+    if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
+}
+
+cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
+                       int32_t precision) {
+    check_build_args(X, n, d, dtype, precision);
+    CNN_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    GraphGuard guard{new cnn_graph()};
+    cnn_graph *g = guard.g;
+    g->device = ctx->device;
+    g->n = n;
+    g->n_points = n;
+    g->d = d;
+    g->dtype = dtype;
+    g->precision = precision;
+    g->r2 = r2;
+    g->radius = std::sqrt(r2);
+    g->n_tiles = ceil_div(n, 64);
+    const int64_t n_pad = g->n_tiles * 64;
+    const bool band = precision == CNN_F32_BAND;
+
+    CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    g->row_ptr = dev_alloc<int64_t>(g, n + 1);
+    if (n == 0) {
+        fill_i64(ctx, g->row_ptr, 0, 1);
+        g->col = dev_alloc<int32_t>(g, 1);
+        CNN_HIP(hipStreamSynchronize(s));
+        g->stats = cnn_graph_stats{};
+        return std::exchange(guard.g, nullptr);
+    }
+    std::vector<double> lo, hi;
+    double max_norm2 = 0.0;
+    bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
+
+    // ordering: Morton order over the (up to) three widest dims for small d
+    g->layout = (d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
+    if (g->layout == 1) {
+        const KeySpec ks = make_keyspec(d, lo, hi, 64);
+        g->perm = dev_alloc<int32_t>(g, n);
+        Scratch<uint64_t> keys(ctx, n);
+        const unsigned blocks = (unsigned)ceil_div(n, 256);
+        if (dtype == CNN_F32)
+            k_morton<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
+        else
+            k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
+        CNN_HIP(hipGetLastError());
+        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
+    }
+    g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
+    if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
+    g->dp = d;
+    {
+        const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * std::max(d, 1), 256), 16384);
+        if (dtype == CNN_F32)
+            k_gather<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
+        else
+            k_gather<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n, n_pad, d, g->xw64, g->xw32);
+        CNN_HIP(hipGetLastError());
+    }
+    build_pairs_and_csr(ctx, g, band, dtype, max_norm2, nullptr, nullptr);
+    return std::exchange(guard.g, nullptr);
+}
+
+// ------------------------------------------------------------------ packed batches
+
+template <typename Tin>
+__global__ void k_morton_packed(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks, const int64_t *__restrict__ offs,
+                                int n_sets, int shift, uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int lo = 0, hi = n_sets;  // dataset: last b with offs[b] <= p
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offs[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    uint64_t q[3] = {0, 0, 0};
+    for (int t = 0; t < ks.nk; ++t) {
+        double v = (double)X[p * d + ks.dim[t]];
+        double f = (v - ks.lo[t]) * ks.scale[t];
+        if (!(f >= 0.0)) f = 0.0;
+        if (f > ks.qmax) f = ks.qmax;
+        q[t] = (uint64_t)f;
+    }
+    uint64_t key;
+    if (ks.nk == 1) key = q[0];
+    else if (ks.nk == 2) key = spread2(q[0]) | (spread2(q[1]) << 1);
+    else key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    keys[p] = ((uint64_t)lo << shift) | key;
+    vals[p] = (int32_t)p;
+}
+
+// Sorted position -> padded working index: dataset b occupies tiles
+// tstart[b] .. tstart[b+1]; its points keep the sorted order.
+__global__ void k_place_packed(const uint64_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n,
+                               int shift, const int64_t *__restrict__ offs, const int64_t *__restrict__ tstart,
+                               int32_t *__restrict__ perm) {
+    const int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= n) return;
+    const int b = (int)(keys[sidx] >> shift);
+    perm[tstart[b] * 64 + (sidx - offs[b])] = vals[sidx];
+}
+
+template <typename Tin>
+__global__ void k_gather_packed(const Tin *__restrict__ X, const int32_t *__restrict__ perm, int64_t n_pad, int d,
+                                double *__restrict__ xw64, float *__restrict__ xw32) {
+    const int64_t total = n_pad * d;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = idx / d;
+        const int k = (int)(idx - w * d);
+        const int32_t src = perm[w];
+        if (src >= 0) {
+            const Tin v = X[(int64_t)src * d + k];
+            xw64[idx] = (double)v;
+            if (xw32) xw32[idx] = (float)v;
+        } else {  // padding rows: NaN is never within any radius, not even of itself
+            xw64[idx] = NAN;
+            if (xw32) xw32[idx] = NAN;
+        }
+    }
+}
+
+cnn_graph *graph_build_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d,
+                              int32_t dtype, const double *r2, int32_t precision, PackedHost &ph) {
+    const int64_t N = offsets[n_sets];
+    check_build_args(X, N, d, dtype, precision);
+    if (d < 1 || d > kSpatialMaxD) throw InvalidArg{"packed batches need 1 <= d <= 8"};
+    CNN_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    GraphGuard guard{new cnn_graph()};
+    cnn_graph *g = guard.g;
+    const bool band = precision == CNN_F32_BAND;
+    // tiles per dataset
+    ph.tile_start.assign(n_sets + 1, 0);
+    double r2max = 0.0;
+    for (int b = 0; b < n_sets; ++b) {
+        ph.tile_start[b + 1] = ph.tile_start[b] + ceil_div(offsets[b + 1] - offsets[b], 64);
+        r2max = std::fmax(r2max, r2[b]);
+    }
+    const int64_t T = ph.tile_start[n_sets];
+    if (T * 64 >= (int64_t)INT32_MAX - 64 * 4096) throw InvalidArg{"padded batch exceeds the int32 index range"};
+    g->device = ctx->device;
+    g->n = T * 64;
+    g->n_points = N;
+    g->d = d;
+    g->dtype = dtype;
+    g->precision = precision;
+    g->r2 = r2max;
+    g->radius = std::sqrt(r2max);
+    g->n_tiles = T;
+    g->layout = 1;
+    g->dp = d;
+    const int64_t n_pad = T * 64;
+    CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    g->row_ptr = dev_alloc<int64_t>(g, n_pad + 1);
+    if (N == 0 || T == 0) {
+        fill_i64(ctx, g->row_ptr, 0, n_pad + 1);
+        g->col = dev_alloc<int32_t>(g, 1);
+        CNN_HIP(hipStreamSynchronize(s));
+        return std::exchange(guard.g, nullptr);
+    }
+    std::vector<double> lo, hi;
+    double max_norm2 = 0.0;
+    bbox(ctx, X, N, d, dtype, lo, hi, max_norm2);
+    int ds_bits = 1;
+    while ((1 << ds_bits) < n_sets) ++ds_bits;
+    const KeySpec ks = make_keyspec(d, lo, hi, 63 - ds_bits);
+    const int key_bits = (int)std::log2(ks.qmax + 1.0) * ks.nk;
+    Scratch<int64_t> doffs(ctx, n_sets + 1), dtstart(ctx, n_sets + 1);
+    CNN_HIP(hipMemcpyAsync(doffs.p, offsets, (n_sets + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(dtstart.p, ph.tile_start.data(), (n_sets + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    {
+        Scratch<uint64_t> keys(ctx, N);
+        Scratch<int32_t> vals(ctx, N);
+        const unsigned blocks = (unsigned)ceil_div(N, 256);
+        if (dtype == CNN_F32)
+            k_morton_packed<float><<<blocks, 256, 0, s>>>((const float *)X, N, d, ks, doffs.p, n_sets, key_bits, keys.p, vals.p);
+        else
+            k_morton_packed<double><<<blocks, 256, 0, s>>>((const double *)X, N, d, ks, doffs.p, n_sets, key_bits, keys.p, vals.p);
+        CNN_HIP(hipGetLastError());
+        sort_pairs_u64_i32(ctx, keys.p, vals.p, N, std::min(64, key_bits + ds_bits));
+        g->perm = dev_alloc<int32_t>(g, n_pad);
+        fill_i32(ctx, g->perm, -1, n_pad);
+        k_place_packed<<<blocks, 256, 0, s>>>(keys.p, vals.p, N, key_bits, doffs.p, dtstart.p, g->perm);
+        CNN_HIP(hipGetLastError());
+    }
+    g->xw64 = dev_alloc<double>(g, n_pad * d);
+    if (band) g->xw32 = dev_alloc<float>(g, n_pad * d);
+    {
+        const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n_pad * d, 256), 16384);
+        if (dtype == CNN_F32)
+            k_gather_packed<float><<<blocks, 256, 0, s>>>((const float *)X, g->perm, n_pad, d, g->xw64, g->xw32);
+        else
+            k_gather_packed<double><<<blocks, 256, 0, s>>>((const double *)X, g->perm, n_pad, d, g->xw64, g->xw32);
+        CNN_HIP(hipGetLastError());
+    }
+    // per-tile parameters
+    std::vector<double> h_r2(T), h_thr(T);
+    std::vector<int32_t> h_lo(T), h_hi(T), h_valid(T);
+    std::vector<float> h_blo(T), h_bhi(T);
+    ph.tile_ds.resize(T);
+    for (int b = 0; b < n_sets; ++b) {
+        const BandParams bp = band ? make_band_direct(r2[b], d, dtype == CNN_F64, std::sqrt(max_norm2))
+                                   : BandParams{r2[b], 0.0f, 0.0f};
+        for (int64_t t = ph.tile_start[b]; t < ph.tile_start[b + 1]; ++t) {
+            h_r2[t] = r2[b];
+            h_thr[t] = r2[b] * (1.0 + 1e-6);
+            h_lo[t] = (int32_t)ph.tile_start[b];
+            h_hi[t] = (int32_t)ph.tile_start[b + 1];
+            h_blo[t] = bp.lo;
+            h_bhi[t] = bp.hi;
+            h_valid[t] = (int32_t)std::min<int64_t>(64, offsets[b + 1] - offsets[b] - (t - ph.tile_start[b]) * 64);
+            ph.tile_ds[t] = b;
+        }
+    }
+    Scratch<double> d_r2(ctx, T), d_thr(ctx, T);
+    Scratch<int32_t> d_lo(ctx, T), d_hi(ctx, T), d_valid(ctx, T);
+    Scratch<float> d_blo(ctx, T), d_bhi(ctx, T);
+    CNN_HIP(hipMemcpyAsync(d_r2.p, h_r2.data(), T * sizeof(double), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_thr.p, h_thr.data(), T * sizeof(double), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_lo.p, h_lo.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_hi.p, h_hi.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_valid.p, h_valid.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_blo.p, h_blo.data(), T * sizeof(float), hipMemcpyHostToDevice, s));
+    CNN_HIP(hipMemcpyAsync(d_bhi.p, h_bhi.data(), T * sizeof(float), hipMemcpyHostToDevice, s));
+    PackedTiles pk;
+    pk.tile_valid = d_valid.p;
+    pk.tile_r2 = d_r2.p;
+    pk.tile_thr = d_thr.p;
+    pk.seg_lo = d_lo.p;
+    pk.seg_hi = d_hi.p;
+    pk.tile_lo = band ? d_blo.p : nullptr;
+    pk.tile_hi = band ? d_bhi.p : nullptr;
+    BandParams bp0{r2max, 0.0f, 0.0f};
+    build_pairs_and_csr(ctx, g, band, dtype, max_norm2, &pk, &bp0);
     return std::exchange(guard.g, nullptr);
 }
 
