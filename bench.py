@@ -2,11 +2,20 @@
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload md_basins_3d|features_64d|blobs_2d]
 
-A step is one full fit (neighbour graph, similarity test, union-find,
-sort_by_size) of the whole synthetic dataset, inputs already resident in HBM.
-The timed K steps use the FP32-banded form (value); the plain FP64 form is
-timed the same way and reported beside it, and both label vectors must be
-identical.  One JSON line is printed.
+A step is one full fit (radius graph, similarity test, union-find,
+sort_by_size) of the whole synthetic dataset; the coordinates are already
+resident in HBM when the timed region starts.  The timed K steps use the
+FP32-banded form (``value``); the plain FP64 form is timed the same way and
+reported beside it, and the two label vectors must be identical.
+
+``roofline`` describes the distance kernel (the largest single kernel of the
+fit): its duration comes from HIP events recorded by the library on the stream
+the kernel runs on, averaged over the timed fits.  ``cpu_baseline`` times the C
+restatement of the reference fit (brute-force FP64 neighbours, components,
+sort_by_size) on all host threads over a seeded subsample; the time is
+extrapolated to the full workload by the brute force's n^2 cost.
+
+One JSON line is printed on stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -22,23 +31,30 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "fit points/s (plain FP64 vs FP32-banded) bit-exact labels; served p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3     # vector and dense MFMA FP32
+FP64_PEAK_TFLOPS = 78.6      # FMA-counted; separately rounded sub/mul/add reach half of it
+
+# HBM traffic of the distance kernel per launch, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs scaled by a copy kernel of known size (profiles/*traffic*.json).
+TRAFFIC_FILES = {
+    "md_basins_3d": "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def workload(name):
     from commonnnclustering_amd import synthetic
 
-    if name == "md_basins_3d":
-        X, p = synthetic.md_basins_3d()
-    elif name == "features_64d":
-        X, p = synthetic.features_64d()
-    elif name == "blobs_2d":
-        X, p = synthetic.blobs_2d()
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return X, p
+    gens = {"md_basins_3d": synthetic.md_basins_3d, "features_64d": synthetic.features_64d,
+            "blobs_2d": synthetic.blobs_2d}
+    if name not in gens:
+        raise SystemExit(f"unknown workload {name}; choose from {sorted(gens)}")
+    return gens[name]()
 
 
 def run_fits(Xd, p, precision, steps, warmup):
@@ -67,21 +83,62 @@ def mean(stats, key):
     return float(np.mean([s[key] for s in stats]))
 
 
-def cpu_baseline(X, p, sample_n, seed=0):
-    """The C restatement of the reference fit (brute-force FP64 neighbours, as
-    NeighboursGetterExtBruteForce; components labelling; sort_by_size) on a
-    seeded subsample, all host threads."""
+def phases(gs, ls):
+    return {"prep": mean(gs, "ms_prep"), "distance": mean(gs, "ms_distance"), "emit": mean(gs, "ms_emit"),
+            "similarity_unionfind": mean(ls, "ms_similarity"), "labels_sort": mean(ls, "ms_labels")}
+
+
+def distance_roofline(gs, d, precision, workload_name):
+    """Algorithmic work of the distance kernel per launch: 3 d flops (subtract,
+    multiply, add) per evaluated point pair; pairs = computed 64x64 tile pairs
+    x 4096 (symmetric evaluation computes each unordered tile pair once)."""
+    launches = max(mean(gs, "distance_launches"), 1.0)
+    ms = mean(gs, "ms_distance") / launches
+    pairs = mean(gs, "tile_pairs_computed") * 4096.0 / launches
+    flops = 3.0 * d * pairs
+    gram = precision == "f32_band" and d >= 16
+    if precision == "f32_band":
+        kernel = "k_dist_mfma" if gram else ("k_dist_f32_cols" if d <= 8 else "k_dist_f32_generic")
+        bound, peak = ("mfma" if gram else "valu"), FP32_PEAK_TFLOPS
+    else:
+        kernel = "k_dist_f64_cols" if d <= 8 else "k_dist_f64_wide"
+        bound, peak = "valu", FP64_PEAK_TFLOPS
+    achieved = flops / (ms * 1e-3) / 1e12
+    traffic = None
+    path = TRAFFIC_FILES.get(workload_name)
+    if path and precision == "f32_band" and os.path.exists(os.path.join(ROOT, path)):
+        with open(os.path.join(ROOT, path)) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    return {"kernel": kernel, "bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic, "ms_per_launch": ms, "flops_per_launch": flops,
+            "pairs_per_launch": pairs, "timing": "hipEvent pair around the launch on the library stream"}
+
+
+def cpu_baseline(X, p, target_s=15.0, seed=0):
+    """C restatement of the reference fit on all host threads, brute-force FP64
+    neighbours as NeighboursGetterExtBruteForce (cost ~ n^2): a seeded
+    subsample sized for about target_s seconds; points/s is extrapolated to the
+    full workload by the n^2 cost."""
     from oracle import c_oracle
 
+    n = X.shape[0]
     rng = np.random.default_rng(seed)
-    idx = np.sort(rng.choice(X.shape[0], size=min(sample_n, X.shape[0]), replace=False))
-    Xs = np.ascontiguousarray(X[idx], dtype=np.float64)
-    t0 = time.perf_counter()
-    c_oracle.fit(Xs, p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"], use_grid=False)
-    dt = time.perf_counter() - t0
-    return {"value": len(idx) / dt, "unit": "points/s", "cores": c_oracle.num_threads(), "kind": "port",
-            "sample": f"{len(idx)} points drawn (seed {seed}) from the same workload; brute-force FP64 "
-                      f"neighbour lists + labelling + sort_by_size took {dt:.2f} s"}
+
+    def run(m):
+        idx = np.sort(rng.choice(n, size=min(m, n), replace=False))
+        Xs = np.ascontiguousarray(X[idx], dtype=np.float64)
+        t0 = time.perf_counter()
+        c_oracle.fit(Xs, p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"], use_grid=False)
+        return len(idx), time.perf_counter() - t0
+
+    m0, t0 = run(min(8000, n))
+    m = min(n, int(m0 * np.sqrt(target_s / max(t0, 1e-3))))
+    ms, ts = run(m) if m > m0 else (m0, t0)
+    t_full = ts * (n / ms) ** 2
+    return {"value": n / t_full, "unit": "points/s", "cores": c_oracle.num_threads(), "kind": "port",
+            "sample": f"{ms} of {n} points (seed {seed}), brute-force FP64 neighbour lists + components + "
+                      f"sort_by_size in {ts:.2f} s ({ms / ts:.0f} points/s on the sample); full-size time "
+                      f"extrapolated by (n/{ms})^2 to {t_full:.0f} s"}
 
 
 def main():
@@ -90,55 +147,52 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="md_basins_3d")
-    ap.add_argument("--cpu-sample", type=int, default=60_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 fits (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run a plain device-to-device copy of MIB MiB three times and exit; its rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU only: every box in this setup has a single MI355X "
+              f"(got --gpus {args.gpus}); run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 copies of {nbytes} bytes")
+        return
+
     X, p = workload(args.workload)
     n, d = X.shape
     Xd = torch.from_numpy(X).cuda()
-    print(f"[bench] workload {args.workload}: n={n} d={d}", file=sys.stderr, flush=True)
+    log(f"workload {args.workload}: n={n} d={d} r={p['radius_cutoff']:.6g} c={p['cnn_cutoff']}")
     t_band, lab_band, gs_band, ls_band = run_fits(Xd, p, "f32_band", args.steps, args.warmup)
-    print(f"[bench] f32_band: {t_band * 1e3:.2f} ms/fit", file=sys.stderr, flush=True)
-    t_plain, lab_plain, gs_plain, ls_plain = run_fits(Xd, p, "f64", args.steps, max(1, args.warmup // 2))
-    print(f"[bench] plain f64: {t_plain * 1e3:.2f} ms/fit", file=sys.stderr, flush=True)
-    identical = bool(np.array_equal(lab_band, lab_plain))
-
-    # dominant kernel of the banded fit and its roofline
-    phases = {
-        "distance": mean(gs_band, "ms_distance"),
-        "emit": mean(gs_band, "ms_emit"),
-        "prep": mean(gs_band, "ms_prep"),
-        "similarity_unionfind": mean(ls_band, "ms_similarity"),
-        "labels_sort": mean(ls_band, "ms_labels"),
-    }
-    pairs = mean(gs_band, "pairs_evaluated")
-    launches = mean(gs_band, "distance_launches")
-    ms_dist = phases["distance"]
-    flops = 3.0 * d * pairs  # per pair and dim: subtract + FMA (2 flops)
-    roofline = {
-        "kernel": "k_dist_f32 (phase N distance, FP32 direct form)" if d < 16 else "phase N distance",
-        "bound": "mfma" if d >= 16 else "valu",
-        "achieved": flops / (ms_dist * 1e-3) / 1e12 / max(launches, 1) * max(launches, 1),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "traffic": None,
-    }
-    roofline["frac"] = roofline["achieved"] / roofline["peak"]
+    log(f"f32_band: {t_band * 1e3:.2f} ms/fit")
+    out_plain, identical, ph_plain = None, None, None
+    if not args.no_plain:
+        t_plain, lab_plain, gs_plain, ls_plain = run_fits(Xd, p, "f64", args.steps, max(1, args.warmup // 2))
+        log(f"plain f64: {t_plain * 1e3:.2f} ms/fit")
+        identical = bool(np.array_equal(lab_band, lab_plain))
+        out_plain = {"points_per_s": n / t_plain, "ms_per_step": t_plain * 1e3,
+                     "distance_roofline": distance_roofline(gs_plain, d, "f64", args.workload)}
+        ph_plain = phases(gs_plain, ls_plain)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(X, p, args.cpu_sample)
-        print(f"[bench] cpu baseline: {cpu['sample']}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(X, p, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['sample']}")
 
     out = {
-        "metric": "fit points/s (FP32-banded, bit-exact labels)",
+        "metric": METRIC,
         "value": n / t_band,
         "unit": "points/s",
         "n_gpus": 1,
@@ -149,24 +203,22 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "n": int(n), "d": int(d), "radius_cutoff": float(p["radius_cutoff"]),
-                   "cnn_cutoff": int(p["cnn_cutoff"]), "member_cutoff": int(p["member_cutoff"])},
-        "plain_f64": {"points_per_s": n / t_plain, "ms_per_step": t_plain * 1e3},
-        "band_vs_plain": t_plain / t_band,
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": args.workload, "n": int(n), "d": int(d), "dtype_in": str(X.dtype),
+                   "radius_cutoff": float(p["radius_cutoff"]), "cnn_cutoff": int(p["cnn_cutoff"]),
+                   "member_cutoff": int(p["member_cutoff"])},
+        "plain_f64": out_plain,
+        "band_vs_plain": (out_plain["ms_per_step"] / (t_band * 1e3)) if out_plain else None,
         "labels_identical": identical,
         "n_clusters": int(ls_band[-1]["n_clusters"]),
-        "label_stats": ls_band[-1],
         "nnz": int(gs_band[-1]["nnz"]),
         "band_hits": int(gs_band[-1]["band_hits"]),
         "band_flips": int(gs_band[-1]["band_flips"]),
         "max_fp32_abs_err_band": float(gs_band[-1]["max_abs_err_band"]),
-        "phase_ms": phases,
-        "phase_ms_plain": {"distance": mean(gs_plain, "ms_distance"), "emit": mean(gs_plain, "ms_emit"),
-                           "prep": mean(gs_plain, "ms_prep"),
-                           "similarity_unionfind": mean(ls_plain, "ms_similarity"),
-                           "labels_sort": mean(ls_plain, "ms_labels")},
-        "roofline": roofline,
+        "phase_ms": phases(gs_band, ls_band),
+        "phase_ms_plain": ph_plain,
+        "label_stats": ls_band[-1],
+        "roofline": distance_roofline(gs_band, d, "f32_band", args.workload),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -72,6 +72,7 @@ class GraphStats(ctypes.Structure):
         ("ms_emit", ctypes.c_float),
         ("distance_launches", ctypes.c_int32),
         ("layout", ctypes.c_int32),
+        ("tile_pairs_computed", ctypes.c_int64),
     ]
 
     def to_dict(self) -> dict:

@@ -35,6 +35,7 @@ static void add_graph_stats(cnn_graph_stats &gs, const cnn_graph_stats &o) {
     gs.ms_distance += o.ms_distance;
     gs.ms_emit += o.ms_emit;
     gs.distance_launches += o.distance_launches;
+    gs.tile_pairs_computed += o.tile_pairs_computed;
     gs.layout = o.layout;
 }
 

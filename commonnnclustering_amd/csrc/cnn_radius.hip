@@ -18,6 +18,7 @@
 // (this file is compiled with -ffp-contract=off); the FP32 kernels decide pairs
 // outside a guard band around r2 and recompute the band in that FP64 order.
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cmath>
 #include <memory>
@@ -378,40 +379,6 @@ __device__ __forceinline__ void finish_deg(const DistArgs &a, int64_t i, int myd
     }
 }
 
-// Plain FP64, compile-time d (small d): row point in VGPRs, column points from
-// wave-uniform (scalar) loads.
-template <int D>
-__global__ __launch_bounds__(256) void k_dist_f64_small(DistArgs a) {
-    const int64_t R = a.R0 + blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t i = R * 64 + lane;
-    double xi[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = a.xw64[i * D + k];
-    int64_t kb, ke;
-    tile_range(a, R, kb, ke);
-    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
-    const double r2 = r2_of(a, R);
-    int mydeg = 0;
-    for (int64_t k = k0 + wave; k < ke; k += 4) {
-        const int64_t C = uniform64(tile_col(a, k, kb));
-        const double *__restrict__ xc = a.xw64 + C * 64 * D;
-        uint64_t m = 0;
-#pragma unroll 16
-        for (int jj = 0; jj < 64; ++jj) {
-            double s = 0.0;
-#pragma unroll
-            for (int k2 = 0; k2 < D; ++k2) {
-                double t = xi[k2] - xc[jj * D + k2];
-                s = s + t * t;
-            }
-            m |= (uint64_t)(s <= r2) << jj;
-        }
-        mydeg += store_pair(a, k, R, i, C, m, false, 0);
-    }
-    finish_deg(a, i, mydeg);
-}
-
 // Plain FP64, runtime d: 64 per-column accumulators in registers, dims in
 // chunks of 8 (the per-pair summation stays in ascending k).
 __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
@@ -515,46 +482,161 @@ __device__ __forceinline__ void flush_band(const DistArgs &a, unsigned hits, uns
     }
 }
 
-// FP32 direct form, compile-time d: t = a - b; acc = fma(t, t, acc).  The
-// certain bits and the band bits are built inline; band bits are resolved in
-// FP64 only for lanes that have any.
+// Column-lane kernels (d <= 8).  Lane j holds column point j of the tile pair
+// in registers; the row tile's coordinates sit in LDS and the wave walks its
+// 64 rows, reading each row as a broadcast.  Lane j accumulates its column
+// word (bit i = pair (row i, column j)) with one compare and an or/select per
+// row; one butterfly transpose per tile pair turns the 64 column words into
+// the row words, and the column words themselves are the mirror pair's row
+// words.  The arithmetic is the reference metric's (difference, square, sum in
+// ascending dimension), so every decision equals the row-lane kernels'.
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 in order.
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// A zero the compiler cannot see through: indexing the LDS rows with it inside
+// the tile-pair loop keeps the row reads in the loop (otherwise all 64 rows
+// are hoisted into registers).
+__device__ __forceinline__ int opaque_zero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+// Set bit I of (hi:lo) when `on`.
+template <int I>
+__device__ __forceinline__ void set_bit_if(uint32_t &lo, uint32_t &hi, bool on) {
+    if constexpr (I < 32) lo = on ? (lo | (1u << I)) : lo;
+    else hi = on ? (hi | (1u << (I - 32))) : hi;
+}
+
+// LDS row stride: rows padded to 16-byte multiples, so each row is one or two
+// 16-byte LDS reads at an immediate offset from one base register.
+template <typename T, int D>
+constexpr int lds_row_stride() {
+    return (int)(((D * sizeof(T) + 15) / 16) * 16 / sizeof(T));
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void load_row_tile(T *srow, const T *__restrict__ xw, int64_t R) {
+    constexpr int DP = lds_row_stride<T, D>();
+    for (int t = threadIdx.x; t < 64 * DP; t += 256) {
+        const int r = t / DP, q = t - (t / DP) * DP;
+        srow[t] = q < D ? xw[(R * 64 + r) * D + q] : (T)0;
+    }
+    __syncthreads();
+}
+
 template <int D>
-__global__ __launch_bounds__(256) void k_dist_f32_small(DistArgs a) {
+__global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
+    constexpr int DP = lds_row_stride<double, D>();
+    __shared__ double srow[64 * DP];
     const int64_t R = a.R0 + blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t i = R * 64 + lane;
-    float xi[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = a.xw32[i * D + k];
+    const int64_t i_lane = R * 64 + lane;
+    load_row_tile<double, D>(srow, a.xw64, R);
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
     const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const double r2 = r2_of(a, R);
+    int mydeg = 0;
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        double xc[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) xc[q] = a.xw64[(C * 64 + lane) * D + q];
+        uint32_t clo = 0, chi = 0;
+        const double *rows = srow + opaque_zero();
+        static_for<0, 64>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            double t = rows[i * DP] - xc[0];
+            double s = t * t;  // 0 + t*t == t*t exactly (t*t is never -0)
+#pragma unroll
+            for (int q = 1; q < D; ++q) {
+                t = rows[i * DP + q] - xc[q];
+                s = s + t * t;
+            }
+            set_bit_if<i>(clo, chi, s <= r2);
+        });
+        const uint64_t colword = ((uint64_t)chi << 32) | clo;
+        mydeg += store_pair(a, k, R, i_lane, C, transpose_bits(colword), true, colword);
+    }
+    finish_deg(a, i_lane, mydeg);
+}
+
+template <int D>
+__device__ __forceinline__ float f32_d2_row(const float *srow, int i, const float *xc) {
+    constexpr int DP = lds_row_stride<float, D>();
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+        const float t = srow[i * DP + q] - xc[q];
+        s = __builtin_fmaf(t, t, s);
+    }
+    return s;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
+    constexpr int DP = lds_row_stride<float, D>();
+    __shared__ float srow[64 * DP];
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i_lane = R * 64 + lane;
+    load_row_tile<float, D>(srow, a.xw32, R);
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const float lo = a.tile_lo ? a.tile_lo[R] : a.bp.lo, hi = a.tile_hi ? a.tile_hi[R] : a.bp.hi;
+    const double r2 = r2_of(a, R);
     int mydeg = 0;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
-    const float lo = a.tile_lo ? a.tile_lo[R] : a.bp.lo, hi = a.tile_hi ? a.tile_hi[R] : a.bp.hi;
-    const double r2 = r2_of(a, R);
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
-        const float *__restrict__ xc = a.xw32 + C * 64 * D;
-        uint64_t m = 0, band = 0;
-#pragma unroll 16
-        for (int jj = 0; jj < 64; ++jj) {
-            float s = 0.0f;
+        const int64_t j = C * 64 + lane;
+        float xc[D];
 #pragma unroll
-            for (int k2 = 0; k2 < D; ++k2) {
-                float t = xi[k2] - xc[jj * D + k2];
-                s = __builtin_fmaf(t, t, s);
+        for (int q = 0; q < D; ++q) xc[q] = a.xw32[j * D + q];
+        const uint64_t cmask = tile_mask(a, C);
+        uint32_t clo = 0, chi = 0;
+        uint64_t band_cols = 0;  // columns with an FP32 value inside the guard band (any row)
+        const float *rows = srow + opaque_zero();
+        static_for<0, 64>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const float s = f32_d2_row<D>(rows, i, xc);
+            const bool in = s < lo;
+            band_cols |= ~(__ballot(in) | __ballot(s > hi));
+            set_bit_if<i>(clo, chi, in);
+        });
+        uint64_t colword = ((uint64_t)chi << 32) | clo;
+        if (band_cols & cmask) {
+            // rare: those columns recompute their rows and settle the band pairs in FP64
+            const bool col_band = ((band_cols & cmask) >> lane) & 1ull;
+            for (int i = 0; i < 64; ++i) {
+                const float s = f32_d2_row<D>(rows, i, xc);
+                const int64_t gi = R * 64 + i;
+                if (col_band && !(s < lo) && !(s > hi) && row_real(a, gi)) {
+                    const double e = exact_d2(a.xw64, D, gi, j);
+                    const bool inx = e <= r2;
+                    if (inx) colword |= 1ull << i;
+                    ++hits;
+                    flips += (inx != ((double)s <= r2));
+                    const double err = fabs((double)s - e);
+                    if (err == err) maxerr = fmax(maxerr, err);
+                }
             }
-            const bool ok = s < lo;
-            m |= (uint64_t)ok << jj;
-            band |= (uint64_t)(!ok && !(s > hi)) << jj;
         }
-        if (band) m = band_fix(a, r2, i, C, m, band, hits, flips, maxerr, xc);
-        mydeg += store_pair(a, k, R, i, C, m, false, 0);
+        mydeg += store_pair(a, k, R, i_lane, C, transpose_bits(colword), true, colword);
     }
     flush_band(a, hits, flips, maxerr);
-    finish_deg(a, i, mydeg);
+    finish_deg(a, i_lane, mydeg);
 }
 
 // FP32 direct form, runtime d.
@@ -822,40 +904,74 @@ __global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__re
 
 // ------------------------------------------------------------------ CSR emit
 
-// One wave per row tile, lane = row: the mask words of a tile pair are one
-// coalesced 512-byte load, and each lane appends its set bits to its own row
-// (work proportional to the row's neighbours).
-__global__ __launch_bounds__(256) void k_emit(DistArgs a, int64_t n_rt, const int64_t *__restrict__ row_ptr,
-                                             int32_t *__restrict__ col) {
-    const int lane = threadIdx.x & 63;
-    const int64_t rt = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (rt >= n_rt) return;
-    const int64_t R = a.R0 + rt;
-    const int64_t w = R * 64 + lane;
+// Coalesced CSR emit: one workgroup per row tile.  Chunks of 64 tile pairs
+// are staged in LDS (pair-major, rows padded to 65 words).  For each of its 16
+// rows a wave takes the row's 64 words of the chunk, scans their popcounts and
+// writes the row's entries in 64-wide contiguous windows: lane t finds its word
+// as the first prefix sum above t (binary lifting over the scanned lanes) and
+// its bit by a 6-step select on popcounts.  Column order within a row follows
+// the ascending candidate-tile list, then the bit position, so rows come out
+// sorted.
+__device__ __forceinline__ int select_bit(uint64_t x, int k) {
+    int bit = 0;
+    int c = __popc((uint32_t)x);
+    if (k >= c) { k -= c; x >>= 32; bit += 32; }
+    c = __popc((uint32_t)x & 0xffffu);
+    if (k >= c) { k -= c; x >>= 16; bit += 16; }
+    c = __popc((uint32_t)x & 0xffu);
+    if (k >= c) { k -= c; x >>= 8; bit += 8; }
+    c = __popc((uint32_t)x & 0xfu);
+    if (k >= c) { k -= c; x >>= 4; bit += 4; }
+    c = __popc((uint32_t)x & 0x3u);
+    if (k >= c) { k -= c; x >>= 2; bit += 2; }
+    if (k >= (int)(x & 1ull)) bit += 1;
+    return bit;
+}
+
+__global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
+                                                   int32_t *__restrict__ col) {
+    __shared__ uint64_t sm[64 * 65];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t R = a.R0 + blockIdx.x;
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
-    int64_t pos = (w < a.n) ? row_ptr[w] : 0;
-    int64_t k = kb;
-    for (; k + 1 < ke; k += 2) {
-        uint64_t m0 = a.mask[(k - a.k_base) * 64 + lane];
-        uint64_t m1 = a.mask[(k + 1 - a.k_base) * 64 + lane];
-        const int32_t c0 = (int32_t)uniform64(tile_col(a, k, kb)) * 64;
-        const int32_t c1 = (int32_t)uniform64(tile_col(a, k + 1, kb)) * 64;
-        while (m0) {
-            col[pos++] = c0 + (__ffsll((unsigned long long)m0) - 1);
-            m0 &= m0 - 1;
+    // lane rr < 16 holds the write position of row wave * 16 + rr
+    const int64_t wl = R * 64 + wave * 16 + lane;
+    int64_t mypos = (lane < 16 && wl < a.n) ? row_ptr[wl] : 0;
+    for (int64_t c0 = kb; c0 < ke; c0 += 64) {
+        const int nk = (int)min<int64_t>(64, ke - c0);
+        __syncthreads();
+#pragma unroll 4
+        for (int st = 0; st < 16; ++st) {
+            const int idx = (int)threadIdx.x + 256 * st;
+            const int p = idx >> 6, r = idx & 63;
+            sm[p * 65 + r] = p < nk ? a.mask[(c0 + p - a.k_base) * 64 + r] : 0ull;
         }
-        while (m1) {
-            col[pos++] = c1 + (__ffsll((unsigned long long)m1) - 1);
-            m1 &= m1 - 1;
-        }
-    }
-    if (k < ke) {
-        uint64_t m0 = a.mask[(k - a.k_base) * 64 + lane];
-        const int32_t c0 = (int32_t)uniform64(tile_col(a, k, kb)) * 64;
-        while (m0) {
-            col[pos++] = c0 + (__ffsll((unsigned long long)m0) - 1);
-            m0 &= m0 - 1;
+        __syncthreads();
+        const int cbase = lane < nk ? (int)tile_col(a, c0 + lane, kb) * 64 : 0;
+        for (int rr = 0; rr < 16; ++rr) {
+            const int r = wave * 16 + rr;
+            const uint64_t v = sm[lane * 65 + r];
+            const int pc = __popcll(v);
+            const int incl = wave_incl_scan(pc);
+            const int total = __shfl(incl, 63, 64);
+            if (total == 0) continue;
+            const int64_t base = __shfl(mypos, rr, 64);
+            const int excl = incl - pc;
+            for (int t0 = 0; t0 < total; t0 += 64) {
+                const int t = min(t0 + lane, total - 1);  // every lane takes part in the shuffles
+                int q = 0;
+#pragma unroll
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int iv = __shfl(incl, q + step - 1, 64);
+                    if (iv <= t) q += step;
+                }
+                const int eq = __shfl(excl, q, 64);
+                const uint64_t wq = __shfl(v, q, 64);
+                const int cq = __shfl(cbase, q, 64);
+                if (t0 + lane < total) col[base + t] = cq + select_bit(wq, t - eq);
+            }
+            if (lane == rr) mypos += total;
         }
     }
 }
@@ -943,7 +1059,7 @@ static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float 
 
 // Padded width of the wide FP64 kernel (dense layout, one batch), 0 if unused.
 static int wide_kmax(const DistArgs &a) {
-    if (a.tp_ptr || a.d <= 4 || a.d > 64 || a.R0 != 0) return 0;
+    if (a.tp_ptr || a.d <= 8 || a.d > 64 || a.R0 != 0) return 0;
     return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
@@ -967,18 +1083,26 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         else k_dist_f64_wide<64><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
     } else if (!band) {
         switch (a.d) {
-            case 1: k_dist_f64_small<1><<<grid, block, 0, s>>>(a); break;
-            case 2: k_dist_f64_small<2><<<grid, block, 0, s>>>(a); break;
-            case 3: k_dist_f64_small<3><<<grid, block, 0, s>>>(a); break;
-            case 4: k_dist_f64_small<4><<<grid, block, 0, s>>>(a); break;
+            case 1: k_dist_f64_cols<1><<<grid, block, 0, s>>>(a); break;
+            case 2: k_dist_f64_cols<2><<<grid, block, 0, s>>>(a); break;
+            case 3: k_dist_f64_cols<3><<<grid, block, 0, s>>>(a); break;
+            case 4: k_dist_f64_cols<4><<<grid, block, 0, s>>>(a); break;
+            case 5: k_dist_f64_cols<5><<<grid, block, 0, s>>>(a); break;
+            case 6: k_dist_f64_cols<6><<<grid, block, 0, s>>>(a); break;
+            case 7: k_dist_f64_cols<7><<<grid, block, 0, s>>>(a); break;
+            case 8: k_dist_f64_cols<8><<<grid, block, 0, s>>>(a); break;
             default: k_dist_f64_generic<<<grid, block, 0, s>>>(a); break;
         }
     } else {
         switch (a.d) {
-            case 1: k_dist_f32_small<1><<<grid, block, 0, s>>>(a); break;
-            case 2: k_dist_f32_small<2><<<grid, block, 0, s>>>(a); break;
-            case 3: k_dist_f32_small<3><<<grid, block, 0, s>>>(a); break;
-            case 4: k_dist_f32_small<4><<<grid, block, 0, s>>>(a); break;
+            case 1: k_dist_f32_cols<1><<<grid, block, 0, s>>>(a); break;
+            case 2: k_dist_f32_cols<2><<<grid, block, 0, s>>>(a); break;
+            case 3: k_dist_f32_cols<3><<<grid, block, 0, s>>>(a); break;
+            case 4: k_dist_f32_cols<4><<<grid, block, 0, s>>>(a); break;
+            case 5: k_dist_f32_cols<5><<<grid, block, 0, s>>>(a); break;
+            case 6: k_dist_f32_cols<6><<<grid, block, 0, s>>>(a); break;
+            case 7: k_dist_f32_cols<7><<<grid, block, 0, s>>>(a); break;
+            case 8: k_dist_f32_cols<8><<<grid, block, 0, s>>>(a); break;
             default: k_dist_f32_generic<<<grid, block, 0, s>>>(a); break;
         }
     }
@@ -1206,8 +1330,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
             a.tile_hi = pk->tile_hi;
             a.tile_valid = pk->tile_valid;
         }
-        CNN_HIP(hipEventRecord(ctx->ev[2], s));
         fill_i32(ctx, deg.p, 0, n);
+        CNN_HIP(hipEventRecord(ctx->ev[2], s));
         launch_dist(ctx, a, b.second - b.first, band);
         CNN_HIP(hipEventRecord(ctx->ev[3], s));
         // row_ptr for this batch's rows: exclusive scan + running offset
@@ -1247,10 +1371,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
             g->col = col;
         }
         CNN_HIP(hipEventRecord(ctx->ev[4], s));
-        if (rows > 0) {
-            const int64_t n_rt = b.second - b.first;
-            k_emit<<<(unsigned)ceil_div(n_rt, 4), 256, 0, s>>>(a, n_rt, g->row_ptr, col);
-        }
+        if (rows > 0) k_emit_rows<<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
         CNN_HIP(hipEventSynchronize(ctx->ev[5]));
@@ -1281,6 +1402,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     st.ms_emit = ms_emit;
     st.distance_launches = (int32_t)batches.size();
     st.layout = g->layout;
+    // lists are symmetric and hold the diagonal: the symmetric kernels compute C >= R only
+    st.tile_pairs_computed = sym ? (total_pairs + T) / 2 : total_pairs;
 }
 
 static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype, int32_t precision) {
