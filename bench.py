@@ -8,9 +8,11 @@ resident in HBM when the timed region starts.  The timed K steps use the
 FP32-banded form (``value``); the plain FP64 form is timed the same way and
 reported beside it, and the two label vectors must be identical.
 
-``roofline`` describes the distance kernel (the largest single kernel of the
-fit): its duration comes from HIP events recorded by the library on the stream
-the kernel runs on, averaged over the timed fits.  ``cpu_baseline`` times the C
+``roofline`` describes the longer of the two largest single kernels of the
+fit, the distance kernel (VALU-bound) and the CSR emit kernel (HBM-bound);
+both are in ``kernel_rooflines``.  Durations come from HIP events the library
+records around each launch on the stream the kernel runs on, averaged over the
+timed fits.  ``cpu_baseline`` times the C
 restatement of the reference fit (brute-force FP64 neighbours, components,
 sort_by_size) on all host threads over a seeded subsample; the time is
 extrapolated to the full workload by the brute force's n^2 cost.
@@ -36,10 +38,11 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3     # vector and dense MFMA FP32
 FP64_PEAK_TFLOPS = 78.6      # FMA-counted; separately rounded sub/mul/add reach half of it
 
-# HBM traffic of the distance kernel per launch, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs scaled by a copy kernel of known size (profiles/*traffic*.json).
+# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by a copy kernel of known size (profiles/*_traffic.json).
 TRAFFIC_FILES = {
-    "md_basins_3d": "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
+    ("k_dist_f32_cols", "md_basins_3d"): "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
+    ("k_emit_rows", "md_basins_3d"): "profiles/k_emit_rows_md_basins_3d_traffic.json",
 }
 
 
@@ -88,6 +91,24 @@ def phases(gs, ls):
             "similarity_unionfind": mean(ls, "ms_similarity"), "labels_sort": mean(ls, "ms_labels")}
 
 
+def emit_roofline(gs, n, workload_name):
+    """CSR emit kernel: compulsory HBM bytes per launch are the neighbour
+    lists written (4 B per entry) and the row offsets read (8 B per row)."""
+    launches = max(mean(gs, "distance_launches"), 1.0)
+    ms = mean(gs, "ms_emit") / launches
+    nbytes = (4.0 * mean(gs, "nnz") + 8.0 * (n + 1)) / launches
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    traffic = None
+    path = TRAFFIC_FILES.get(("k_emit_rows", workload_name))
+    if path and os.path.exists(os.path.join(ROOT, path)):
+        with open(os.path.join(ROOT, path)) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    return {"kernel": "k_emit_rows", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms,
+            "algorithmic_bytes_per_launch": nbytes,
+            "timing": "hipEvent pair around the launch on the library stream"}
+
+
 def distance_roofline(gs, d, precision, workload_name):
     """Algorithmic work of the distance kernel per launch: 3 d flops (subtract,
     multiply, add) per evaluated point pair; pairs = computed 64x64 tile pairs
@@ -105,8 +126,8 @@ def distance_roofline(gs, d, precision, workload_name):
         bound, peak = "valu", FP64_PEAK_TFLOPS
     achieved = flops / (ms * 1e-3) / 1e12
     traffic = None
-    path = TRAFFIC_FILES.get(workload_name)
-    if path and precision == "f32_band" and os.path.exists(os.path.join(ROOT, path)):
+    path = TRAFFIC_FILES.get((kernel, workload_name))
+    if path and os.path.exists(os.path.join(ROOT, path)):
         with open(os.path.join(ROOT, path)) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     return {"kernel": kernel, "bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -191,6 +212,11 @@ def main():
         cpu = cpu_baseline(X, p, args.cpu_seconds)
         log(f"cpu baseline: {cpu['sample']}")
 
+    # the roofline line describes the longer of the two single-kernel phases
+    dist_rf = distance_roofline(gs_band, d, "f32_band", args.workload)
+    emit_rf = emit_roofline(gs_band, n, args.workload)
+    dominant = dist_rf if dist_rf["ms_per_launch"] >= emit_rf["ms_per_launch"] else emit_rf
+
     out = {
         "metric": METRIC,
         "value": n / t_band,
@@ -218,7 +244,8 @@ def main():
         "phase_ms": phases(gs_band, ls_band),
         "phase_ms_plain": ph_plain,
         "label_stats": ls_band[-1],
-        "roofline": distance_roofline(gs_band, d, "f32_band", args.workload),
+        "roofline": dominant,
+        "kernel_rooflines": {"distance": dist_rf, "emit": emit_rf},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

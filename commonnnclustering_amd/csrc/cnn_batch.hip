@@ -136,4 +136,79 @@ int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n
     });
 }
 
+// Parameter scan on one dataset: the graph is built once at the largest
+// radius; each smaller radius filters the previous (larger) lists exactly in
+// FP64, and every cutoff pair runs the labelling pass on the lists of its
+// radius.  Radii are visited in descending order; results land at the
+// caller's (radius, cutoff) positions.
+int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, const double *r2, int32_t n_r,
+                 const int32_t *c_sim, const int32_t *m_core, int32_t n_c, int32_t precision, int64_t member_cutoff,
+                 int64_t max_clusters, int64_t *labels, cnn_graph_stats *gstats, cnn_label_stats *lstats) {
+    if (!ctx || n_r < 0 || n_c < 0) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        if (n_r == 0 || n_c == 0) return (int)CNN_OK;
+        if (!r2 || !c_sim || !m_core) throw InvalidArg{"null parameter array"};
+        if (n > 0 && !labels) throw InvalidArg{"labels is null"};
+        for (int32_t ci = 0; ci < n_c; ++ci)
+            if (c_sim[ci] < 0 || m_core[ci] < 0) throw InvalidArg{"c_sim and m_core must be >= 0"};
+        std::vector<int32_t> order(n_r);
+        for (int32_t i = 0; i < n_r; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return r2[x] > r2[y]; });
+        CNN_HIP(hipSetDevice(ctx->device));
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2[order[0]], precision);
+        std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        if (gstats) *gstats = g->stats;
+        // lists of the radius being labelled; the graph's own for the largest
+        const int64_t *rp = g->row_ptr;
+        const int32_t *col = g->col;
+        int64_t *own_rp = nullptr;
+        int32_t *own_col = nullptr;
+        auto release = [&]() {
+            if (own_rp) (void)hipFree(own_rp);
+            if (own_col) (void)hipFree(own_col);
+            own_rp = nullptr;
+            own_col = nullptr;
+        };
+        try {
+            double cur_r2 = r2[order[0]];
+            for (int32_t oi = 0; oi < n_r; ++oi) {
+                const int32_t ri = order[oi];
+                if (r2[ri] < cur_r2) {
+                    int64_t *nrp = nullptr;
+                    int32_t *ncol = nullptr;
+                    if (hipMalloc(&nrp, (size_t)(n + 1) * sizeof(int64_t)) != hipSuccess) {
+                        (void)hipGetLastError();
+                        throw OutOfMemory{"row offsets for the scan"};
+                    }
+                    try {
+                        csr_filter(ctx, g->xw64, n, d, rp, col, r2[ri], nrp, &ncol);
+                    } catch (...) {
+                        (void)hipFree(nrp);
+                        throw;
+                    }
+                    release();
+                    own_rp = nrp;
+                    own_col = ncol;
+                    rp = own_rp;
+                    col = own_col;
+                    cur_r2 = r2[ri];
+                }
+                for (int32_t ci = 0; ci < n_c; ++ci) {
+                    LabelRequest req{rp, col, n, g->perm, c_sim[ci], m_core[ci], nullptr, 1, CNN_SIM_AUTO, true, true,
+                                     member_cutoff, max_clusters, labels + ((int64_t)ri * n_c + ci) * n,
+                                     g->tp_ptr, g->tp_col, g->n_tiles};
+                    cnn_label_stats one{};
+                    labels_run(ctx, req, &one);
+                    if (lstats) lstats[(int64_t)ri * n_c + ci] = one;
+                }
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+        return (int)CNN_OK;
+    });
+}
+
 }  // extern "C"

@@ -167,6 +167,8 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
 void graph_free(cnn_graph *g);
 void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col);
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);
+int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int64_t *rp, const int32_t *col,
+                   double r2, int64_t *out_rp, int32_t **out_col);
 
 // Host-side description of a packed batch graph.
 struct PackedHost {
@@ -232,14 +234,19 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
 // Inclusive prefix sum across the 64 lanes of a wave.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        int t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
-    return v;
+// Inclusive prefix sum over the 64 lanes with DPP moves (no LDS round trips):
+// shifts by 1, 2, 4, 8 inside each row of 16 lanes (out-of-row sources read
+// 0), then the last lane of row 0 is added to row 1 and row 3 (row_bcast:15)
+// and the last lane of row 1 to rows 2 and 3 (row_bcast:31).  Every lane of
+// the wave must be active.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
 }
 
 __device__ __forceinline__ int wave_sum(int v) {

@@ -906,28 +906,11 @@ __global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__re
 
 // Coalesced CSR emit: one workgroup per row tile.  Chunks of 64 tile pairs
 // are staged in LDS (pair-major, rows padded to 65 words).  For each of its 16
-// rows a wave takes the row's 64 words of the chunk, scans their popcounts and
-// writes the row's entries in 64-wide contiguous windows: lane t finds its word
-// as the first prefix sum above t (binary lifting over the scanned lanes) and
-// its bit by a 6-step select on popcounts.  Column order within a row follows
-// the ascending candidate-tile list, then the bit position, so rows come out
-// sorted.
-__device__ __forceinline__ int select_bit(uint64_t x, int k) {
-    int bit = 0;
-    int c = __popc((uint32_t)x);
-    if (k >= c) { k -= c; x >>= 32; bit += 32; }
-    c = __popc((uint32_t)x & 0xffffu);
-    if (k >= c) { k -= c; x >>= 16; bit += 16; }
-    c = __popc((uint32_t)x & 0xffu);
-    if (k >= c) { k -= c; x >>= 8; bit += 8; }
-    c = __popc((uint32_t)x & 0xfu);
-    if (k >= c) { k -= c; x >>= 4; bit += 4; }
-    c = __popc((uint32_t)x & 0x3u);
-    if (k >= c) { k -= c; x >>= 2; bit += 2; }
-    if (k >= (int)(x & 1ull)) bit += 1;
-    return bit;
-}
-
+// rows a wave scans the popcounts of the row's 64 words of the chunk (lane =
+// word), then walks the non-zero words: the word is broadcast to the wave,
+// lane b holds bit b, and the set bits go out as one contiguous store, each
+// at its rank among the word's set bits (mbcnt) after the word's prefix.
+// Rows come out sorted: candidate tiles ascend, then bit positions.
 __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
                                                    int32_t *__restrict__ col) {
     __shared__ uint64_t sm[64 * 65];
@@ -958,18 +941,19 @@ __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__
             if (total == 0) continue;
             const int64_t base = __shfl(mypos, rr, 64);
             const int excl = incl - pc;
-            for (int t0 = 0; t0 < total; t0 += 64) {
-                const int t = min(t0 + lane, total - 1);  // every lane takes part in the shuffles
-                int q = 0;
-#pragma unroll
-                for (int step = 32; step >= 1; step >>= 1) {
-                    const int iv = __shfl(incl, q + step - 1, 64);
-                    if (iv <= t) q += step;
+            uint64_t nz = __ballot(v != 0ull);
+            while (nz) {
+                const int q = __ffsll((unsigned long long)nz) - 1;
+                nz &= nz - 1;
+                const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)v, q);
+                const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), q);
+                const int eq = __builtin_amdgcn_readlane(excl, q);
+                const int cq = __builtin_amdgcn_readlane(cbase, q);
+                const uint64_t w = ((uint64_t)whi << 32) | wlo;
+                if ((w >> lane) & 1ull) {
+                    const int rank = (int)__builtin_amdgcn_mbcnt_hi(whi, __builtin_amdgcn_mbcnt_lo(wlo, 0u));
+                    col[base + eq + rank] = cq + lane;
                 }
-                const int eq = __shfl(excl, q, 64);
-                const uint64_t wq = __shfl(v, q, 64);
-                const int cq = __shfl(cbase, q, 64);
-                if (t0 + lane < total) col[base + t] = cq + select_bit(wq, t - eq);
             }
             if (lane == rr) mypos += total;
         }
@@ -1778,6 +1762,38 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     g->stats.ms_distance = ms;
     g->stats.layout = g->layout;
     return std::exchange(guard.g, nullptr);
+}
+
+// Neighbour lists at r2 from lists at a larger radius (rows of the working
+// order): each entry is re-evaluated exactly in FP64.  out_rp has n+1
+// entries; out_col is allocated here (hipMalloc) and owned by the caller.
+int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int64_t *rp, const int32_t *col,
+                   double r2, int64_t *out_rp, int32_t **out_col) {
+    hipStream_t s = ctx->stream;
+    *out_col = nullptr;
+    if (n == 0) {
+        fill_i64(ctx, out_rp, 0, 1);
+        CNN_HIP(hipMalloc(out_col, sizeof(int32_t)));
+        return 0;
+    }
+    Scratch<int32_t> deg(ctx, n);
+    k_rethreshold<false><<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(rp, col, xw64, n, d, r2, deg.p, nullptr, nullptr);
+    CNN_HIP(hipGetLastError());
+    exclusive_scan_i32_to_i64(ctx, deg.p, out_rp, n);
+    const int64_t last_excl = read_scalar(ctx, out_rp + (n - 1));
+    const int32_t last_deg = read_scalar(ctx, deg.p + (n - 1));
+    const int64_t nnz = last_excl + last_deg;
+    CNN_HIP(hipMemcpyAsync(out_rp + n, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    hipError_t e = hipMalloc(out_col, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out_col = nullptr;
+        throw OutOfMemory{"neighbour list allocation failed (" + std::to_string(nnz) + " entries)"};
+    }
+    k_rethreshold<true><<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(rp, col, xw64, n, d, r2, nullptr, out_rp, *out_col);
+    CNN_HIP(hipGetLastError());
+    CNN_HIP(hipStreamSynchronize(s));  // nnz lives on this stack frame
+    return nnz;
 }
 
 }  // namespace cnn

@@ -266,3 +266,33 @@ def fit_batch(X: torch.Tensor, offsets, r2, c_sim, m_core, precision="f64", memb
                                        _max_clusters(max_clusters), ptr(labels), ctypes.byref(gs),
                                        ctypes.byref(ls)), "cnn_fit_batch")
     return labels, gs.to_dict(), ls.to_dict()
+
+
+def fit_scan(X: torch.Tensor, radii, cnn_cutoffs, precision="f32_band", member_cutoff=None, max_clusters=None,
+             labels=None):
+    """Fits for every (radius_cutoff, cnn_cutoff) pair of one dataset.
+
+    radii are radius cutoffs r (squared here, r2 = r*r as make_parameters,
+    _fit.pyx:143); cnn_cutoffs are c (c_sim = c + 2, m_core = c + 1).  The
+    graph is built once at max(r); returns (labels [n_r, n_c, n] int64 on the
+    device, graph stats at max(r), list of label stats per pair).
+    """
+    n, d, dt = _check_points(X)
+    r2 = np.ascontiguousarray([float(r) * float(r) for r in radii], dtype=np.float64)
+    cs = np.asarray(list(cnn_cutoffs), dtype=np.int64)
+    c_sim = np.ascontiguousarray(cs + 2, dtype=np.int32)
+    m_core = np.ascontiguousarray(cs + 1, dtype=np.int32)
+    n_r, n_c = len(r2), len(cs)
+    if labels is None:
+        labels = torch.empty((n_r, n_c, n), dtype=torch.int64, device=X.device)
+    elif tuple(labels.shape) != (n_r, n_c, n) or labels.dtype != torch.int64 or not labels.is_contiguous():
+        raise ValueError(f"labels must be a contiguous int64 tensor of shape {(n_r, n_c, n)}")
+    p = _lib.precision_code(precision)
+    ctx = get_context(X.device)
+    gs = GraphStats()
+    ls = (LabelStats * max(n_r * n_c, 1))()
+    ctx.check(_lib.lib().cnn_fit_scan(ctx.handle, ptr(X), n, d, dt, r2.ctypes.data, n_r, c_sim.ctypes.data,
+                                      m_core.ctypes.data, n_c, p, _member_cutoff(member_cutoff),
+                                      _max_clusters(max_clusters), ptr(labels), ctypes.byref(gs),
+                                      ctypes.cast(ls, ctypes.c_void_p)), "cnn_fit_scan")
+    return labels, gs.to_dict(), [ls[i].to_dict() for i in range(n_r * n_c)]

@@ -181,16 +181,31 @@ int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_cor
                   cnn_label_stats *lstats);
 
 /* ------------------------------------------------------------------ batched */
-/* Many independent datasets in packed launches.  Dataset b has offsets[b] ..
- * offsets[b+1] rows of X (device, row-major, common d and dtype; offsets is a
- * HOST int64[n_sets+1]) and its own r2 / c_sim / m_core (HOST arrays).
- * labels (device int64[offsets[n_sets]]) receives every dataset's sorted labels
- * (sort_by_size per dataset with member_cutoff / max_clusters). */
+/* Many independent datasets, each fitted as by cnn_fit.  Dataset b has
+ * offsets[b] .. offsets[b+1] rows of X (device, row-major, common d and dtype;
+ * offsets is a HOST int64[n_sets+1]) and its own r2 / c_sim / m_core (HOST
+ * arrays).  labels (device int64[offsets[n_sets]]) receives every dataset's
+ * sorted labels (sort_by_size per dataset with member_cutoff / max_clusters).
+ * Replaces a loop of Clustering.fit calls over datasets (cluster.pyx:256-392). */
 int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d,
                   int32_t dtype, const double *r2, const int32_t *c_sim,
                   const int32_t *m_core, int32_t precision, int64_t member_cutoff,
                   int64_t max_clusters, int64_t *labels, cnn_graph_stats *gstats,
                   cnn_label_stats *lstats);
+
+/* Parameter scan on one dataset: fits for every (r2[ri], c_sim[ci] / m_core[ci])
+ * pair, as a loop of Clustering.fit calls or HierarchicalFitterRepeat._fit
+ * steps (_fit.pyx:1058-1187) would produce them.  The neighbour graph is built
+ * once at the largest radius; smaller radii filter the lists of the next larger
+ * one exactly in FP64 (the radius cache of NeighboursGetterRecomputeLookup,
+ * _types.pyx:1867-1873, without recomputation).  labels: device
+ * int64[n_r * n_c * n], block (ri, ci) at offset (ri * n_c + ci) * n, sorted by
+ * size with member_cutoff / max_clusters.  gstats: the graph at the largest
+ * radius; lstats (nullable): HOST array of n_r * n_c entries. */
+int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+                 const double *r2, int32_t n_r, const int32_t *c_sim, const int32_t *m_core,
+                 int32_t n_c, int32_t precision, int64_t member_cutoff, int64_t max_clusters,
+                 int64_t *labels, cnn_graph_stats *gstats, cnn_label_stats *lstats);
 
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
