@@ -1,6 +1,6 @@
 """Benchmark: CommonNN fit throughput on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload md_basins_3d|features_64d|blobs_2d]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload md_basins_3d|features_64d|blobs_2d|serving]
 
 A step is one full fit (radius graph, similarity test, union-find,
 sort_by_size) of the whole synthetic dataset; the coordinates are already
@@ -111,13 +111,15 @@ def emit_roofline(gs, n, workload_name):
 
 def distance_roofline(gs, d, precision, workload_name):
     """Algorithmic work of the distance kernel per launch: 3 d flops (subtract,
-    multiply, add) per evaluated point pair; pairs = computed 64x64 tile pairs
-    x 4096 (symmetric evaluation computes each unordered tile pair once)."""
+    multiply, add) per evaluated point pair for the direct forms, 2 d for the
+    Gram form on MFMA; pairs = computed 64x64 tile pairs x 4096 (symmetric
+    evaluation computes each unordered tile pair once)."""
     launches = max(mean(gs, "distance_launches"), 1.0)
     ms = mean(gs, "ms_distance") / launches
     pairs = mean(gs, "tile_pairs_computed") * 4096.0 / launches
-    flops = 3.0 * d * pairs
     gram = precision == "f32_band" and d >= 16
+    # direct forms: subtract, multiply, add per dimension; Gram form: one FMA (2 flops) per dimension
+    flops = (2.0 if gram else 3.0) * d * pairs
     if precision == "f32_band":
         kernel = "k_dist_mfma" if gram else ("k_dist_f32_cols" if d <= 8 else "k_dist_f32_generic")
         bound, peak = ("mfma" if gram else "valu"), FP32_PEAK_TFLOPS
@@ -162,6 +164,92 @@ def cpu_baseline(X, p, target_s=15.0, seed=0):
                       f"extrapolated by (n/{ms})^2 to {t_full:.0f} s"}
 
 
+def serving_requests(count, seed=5000, refit_frac=0.3):
+    """Requests of the serving workload: fresh fits of synthetic datasets
+    (n log-uniform in [1e3, 3e5], d in {2, 3, 16, 64}) and, for about
+    refit_frac of them, re-fits of an earlier dataset at a smaller radius or
+    another cnn_cutoff."""
+    from commonnnclustering_amd import synthetic
+    from commonnnclustering_amd.serving import FitRequest
+
+    rng = np.random.default_rng(seed)
+    reqs, fresh = [], []
+    for k in range(count):
+        if fresh and rng.uniform() < refit_frac:
+            base = fresh[int(rng.integers(0, len(fresh)))]
+            if rng.uniform() < 0.5:
+                reqs.append(FitRequest(base.radius_cutoff * float(rng.uniform(0.6, 1.0)), base.cnn_cutoff,
+                                       dataset_id=base.dataset_id, member_cutoff=base.member_cutoff))
+            else:
+                reqs.append(FitRequest(base.radius_cutoff, int(rng.integers(2, 51)), dataset_id=base.dataset_id,
+                                       member_cutoff=base.member_cutoff))
+            continue
+        X, p = synthetic.serving_request(seed + k)
+        r = FitRequest(p["radius_cutoff"], p["cnn_cutoff"], X=X, dataset_id=f"req{k}",
+                       member_cutoff=p["member_cutoff"])
+        reqs.append(r)
+        fresh.append(r)
+    return reqs
+
+
+def serving_bench(args):
+    """Closed loop (all requests queued at once) for capacity, then Poisson
+    arrivals at --rate-frac of that capacity for latency."""
+    from commonnnclustering_amd.serving import Server
+
+    reqs = serving_requests(args.requests)
+    n_pts = [r.X.shape[0] if r.X is not None else None for r in reqs]
+    log(f"serving: {len(reqs)} requests, {sum(r.X is None for r in reqs)} re-fits, "
+        f"n in [{min(x for x in n_pts if x)}, {max(x for x in n_pts if x)}]")
+    # warm-up: one small request per d compiles nothing but pages in code and pools
+    with Server(slots=args.slots) as srv:
+        for r in reqs[:4]:
+            if r.X is not None:
+                srv.submit(type(r)(r.radius_cutoff, r.cnn_cutoff, X=r.X[:2000])).result()
+    with Server(slots=args.slots) as srv:
+        t0 = time.perf_counter()
+        futs = [srv.submit(r) for r in reqs]
+        for f in futs:
+            f.result()
+        t_closed = time.perf_counter() - t0
+        closed = srv.latency_summary()
+    cap = len(reqs) / t_closed
+    rate = args.rate_frac * cap
+    log(f"closed loop: {t_closed * 1e3:.1f} ms, capacity {cap:.1f} req/s; open loop at {rate:.1f} req/s")
+    rng = np.random.default_rng(1)
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    with Server(slots=args.slots) as srv:
+        futs = []
+        t_next = time.perf_counter()
+        for r, g in zip(reqs, gaps):
+            t_next += g
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(srv.submit(r))
+        for f in futs:
+            f.result()
+        opened = srv.latency_summary()
+    return {
+        "metric": METRIC,
+        "value": closed["points_per_s"],
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 1,
+        "ms_per_step": t_closed * 1e3 / len(reqs),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": "serving", "requests": len(reqs), "slots": args.slots,
+                   "n_range": [1000, 300000], "dims": [2, 3, 16, 64], "refit_frac": 0.3},
+        "served": {"capacity_req_per_s": cap, "closed_loop": closed, "poisson_rate_req_per_s": rate,
+                   "open_loop": opened},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +259,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 fits (profiling runs)")
+    ap.add_argument("--requests", type=int, default=48, help="serving workload: number of requests")
+    ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
+    ap.add_argument("--rate-frac", type=float, default=0.7,
+                    help="serving workload: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device-to-device copy of MIB MiB three times and exit; its rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters")
@@ -190,6 +282,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copy probe: 3 copies of {nbytes} bytes")
+        return
+
+    if args.workload == "serving":
+        print(json.dumps(serving_bench(args)))
         return
 
     X, p = workload(args.workload)
