@@ -260,6 +260,14 @@ struct DistArgs {
     const float *tile_lo;
     const float *tile_hi;
     const int32_t *tile_valid;  // packed batches: real rows per tile (the first ones); nullptr: rows < n
+    // MFMA form: tile pairs with guard-band elements, settled by a second
+    // kernel -- a record (k, R, 64 per-lane band flags) per pair while the list
+    // has room, else the pair alone (k, R) in the redo list
+    unsigned long long *band_cnt;
+    int64_t *band_list;         // 66 int64 per record
+    int64_t band_cap;           // records
+    unsigned long long *redo_cnt;
+    int64_t *redo;              // (k, R) per entry
 };
 
 __device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
@@ -304,6 +312,31 @@ __device__ __forceinline__ double exact_d2(const double *__restrict__ xw64, int 
     for (int k = 0; k < d; ++k) {
         double t = a[k] - b[k];
         s = s + t * t;
+    }
+    return s;
+}
+
+// Exact FP64 reduced distance in reference order, computed by the whole wave:
+// lane k forms the rounded square of dimension k's rounded difference, then the
+// squares are added one after another in ascending k (a chain of readlanes),
+// exactly the reference's sequence.  Wave-uniform result; all lanes active.
+__device__ __forceinline__ double exact_d2_wave(const double *__restrict__ xw64, int d, int64_t gi, int64_t gj) {
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int k0 = 0; k0 < d; k0 += 64) {
+        const int k = k0 + lane;
+        double sq = 0.0;
+        if (k < d) {
+            const double t = xw64[gi * d + k] - xw64[gj * d + k];
+            sq = t * t;
+        }
+        const int m = min(64, d - k0);
+        const uint64_t bits = __double_as_longlong(sq);
+        for (int q = 0; q < m; ++q) {
+            const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)bits, q);
+            const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(bits >> 32), q);
+            s = s + __longlong_as_double(((int64_t)hi << 32) | lo);
+        }
     }
     return s;
 }
@@ -709,7 +742,7 @@ __global__ void k_augment(const float *__restrict__ xw32, int64_t n, int64_t n_p
         ca[k] = v;
     }
     const float e = __builtin_fmaf(ka, w, kb);
-    const float c = (r2q - 0.5f * w) + e;
+    const float c = r2q - 0.5f * w;
     ra[d] = 1.0f;
     ra[d + 1] = c;
     ca[d] = c;
@@ -721,114 +754,307 @@ __global__ void k_augment(const float *__restrict__ xw32, int64_t n, int64_t n_p
     emarg[i] = e;
 }
 
-// One wave per 64x64 tile pair (the 4 waves of a workgroup share the row tile).
-// T' = xra_i . xca_j accumulates in FP32 on v_mfma_f32_32x32x2f32 as a 2x2
-// grid of 32x32 blocks; lane l supplies row l%32 at k-slot l/32, and a float4
-// load per lane feeds four k-steps (dims 8q+4h .. 8q+4h+3, the same order for
-// rows and columns); the next block's operands are loaded while the current
-// block's MFMAs run.  Accumulator element v of lane l is T'[8(v/4) + 4(l/32) +
-// v%4][l%32].  T' < 0: certainly not a neighbour -- a tile pair with no
-// element >= 0 (the common case) costs one compare per element.  Otherwise the
-// accumulators go to LDS and each lane decides its own row: T' >= 2(e_i + e_j)
-// is certainly a neighbour, the rest is recomputed in FP64 in reference order.
-// The mirror pair's column words come from a bit transpose of the row words.
-__global__ __launch_bounds__(256) void k_dist_mfma(DistArgs a) {
-    __shared__ float sT[4][64 * 65];
-    __shared__ float s_e[64];
-    __shared__ float s_ec[4][64];
-    const int64_t R = a.R0 + blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// T = xra_i . xca_j = (r2 - d2)/2 for the 64x64 tile pair (R, C), accumulated
+// in FP32 on v_mfma_f32_32x32x2f32 as a 2x2 grid of 32x32 blocks; lane l
+// supplies row l%32 at k-slot l/32, and a float4 load per lane feeds four
+// k-steps (dims 8q+4h .. 8q+4h+3, the same order for rows and columns); the
+// next block's operands are loaded while the current block's MFMAs run.
+// Accumulator element v of lane l is T[8(v/4) + 4(l/32) + v%4][l%32].
+struct MfmaTile {
+    v16f c00, c01, c10, c11;
+};
+
+__device__ __forceinline__ MfmaTile mfma_tile(const DistArgs &a, int64_t R, int64_t C) {
+    const int lane = threadIdx.x & 63;
     const int r = lane & 31, h = lane >> 5;
-    const int64_t i = R * 64 + lane;
-    if (threadIdx.x < 64) s_e[threadIdx.x] = a.emarg[R * 64 + threadIdx.x];
-    __syncthreads();
-    int64_t kb, ke;
-    tile_range(a, R, kb, ke);
-    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     const int dp = a.dp;
     const float *__restrict__ xa0 = a.xra + (R * 64 + r) * dp + 4 * h;
     const float *__restrict__ xa1 = xa0 + (int64_t)32 * dp;
-    float *T = sT[wave];
-    const float er = s_e[lane];
-    int mydeg = 0;
+    const float *__restrict__ xb0 = a.xca + (C * 64 + r) * dp + 4 * h;
+    const float *__restrict__ xb1 = xb0 + (int64_t)32 * dp;
+    MfmaTile t;
+    t.c00 = v16f{0};
+    t.c01 = v16f{0};
+    t.c10 = v16f{0};
+    t.c11 = v16f{0};
+    float4 A0 = *(const float4 *)(xa0), A1 = *(const float4 *)(xa1);
+    float4 B0 = *(const float4 *)(xb0), B1 = *(const float4 *)(xb1);
+    for (int q = 0; q < dp; q += 8) {
+        const int qn = (q + 8 < dp) ? q + 8 : q;  // last block reloads itself (in bounds, unused)
+        const float4 nA0 = *(const float4 *)(xa0 + qn), nA1 = *(const float4 *)(xa1 + qn);
+        const float4 nB0 = *(const float4 *)(xb0 + qn), nB1 = *(const float4 *)(xb1 + qn);
+        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B0.x, t.c00, 0, 0, 0);
+        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B1.x, t.c01, 0, 0, 0);
+        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B0.x, t.c10, 0, 0, 0);
+        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B1.x, t.c11, 0, 0, 0);
+        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B0.y, t.c00, 0, 0, 0);
+        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B1.y, t.c01, 0, 0, 0);
+        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B0.y, t.c10, 0, 0, 0);
+        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B1.y, t.c11, 0, 0, 0);
+        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B0.z, t.c00, 0, 0, 0);
+        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B1.z, t.c01, 0, 0, 0);
+        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B0.z, t.c10, 0, 0, 0);
+        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B1.z, t.c11, 0, 0, 0);
+        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B0.w, t.c00, 0, 0, 0);
+        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B1.w, t.c01, 0, 0, 0);
+        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B0.w, t.c10, 0, 0, 0);
+        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B1.w, t.c11, 0, 0, 0);
+        A0 = nA0; A1 = nA1; B0 = nB0; B1 = nB1;
+    }
+    return t;
+}
+
+// Words of the tile pair from per-lane row bits: lane (r, h) sets, for its
+// rows 8(v/4) + v%4 (+32 for the second row block, shifted by 4h), the bits of
+// its two columns r (blocks c00, c10) and 32 + r (c01, c11); the lane halves
+// are merged with one exchange and lane j ends up with column j's word.
+__device__ __forceinline__ uint64_t merge_col_words(uint32_t a_lo, uint32_t a_hi, uint32_t b_lo, uint32_t b_hi) {
+    const int h = (threadIdx.x & 63) >> 5;
+    const uint64_t wa = ((((uint64_t)a_hi) << 32) | a_lo) << (4 * h);
+    const uint64_t wb = ((((uint64_t)b_hi) << 32) | b_lo) << (4 * h);
+    const uint64_t fa = wa | __shfl_xor(wa, 32, 64);
+    const uint64_t fb = wb | __shfl_xor(wb, 32, 64);
+    return h == 0 ? fa : fb;
+}
+
+// Lane l's 64 elements of a tile pair are numbered 4v + b: v the accumulator
+// slot, b the block (0: c00, 1: c10, 2: c01, 3: c11); element 4v + b is row
+// 8(v/4) + 4(l/32) + v%4 (+32 for b odd) and column l%32 (+32 for b >= 2).
+__device__ __forceinline__ void band_element(int lane, int e, int &row, int &col) {
+    const int v = e >> 2, b = e & 3;
+    row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3) + ((b & 1) ? 32 : 0);
+    col = (lane & 31) + ((b & 2) ? 32 : 0);
+}
+
+// Settles the guard-band elements of listed tile pairs (one wave per record):
+// the flagged elements are taken one at a time and decided by the exact FP64
+// distance in reference order, evaluated by the whole wave; neighbours are
+// or-ed into the stored words of the pair (and of its mirror for a symmetric
+// evaluation) and counted in both degrees.  The FP32 statistics use a wave
+// sum of the augmented vectors' products.
+__device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hits, unsigned &flips,
+                                   double &maxerr);
+
+__global__ __launch_bounds__(256) void k_dist_band_flags(DistArgs a, int64_t n_rec) {
+    const int lane = threadIdx.x & 63;
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
+    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    for (int64_t rec = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); rec < n_rec; rec += n_waves)
+        settle_band_record(a, rec, hits, flips, maxerr);
+    flush_band(a, hits, flips, maxerr);
+}
+
+__device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hits, unsigned &flips,
+                                   double &maxerr) {
+    const int lane = threadIdx.x & 63;
+    const int64_t *p = a.band_list + rec * 66;
+    const int64_t k = p[0], R = p[1];
+    uint64_t flags = (uint64_t)p[2 + lane];
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int64_t C = uniform64(tile_col(a, k, kb));
+    uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
+    for (uint64_t todo = __ballot(flags != 0ull); todo; todo = __ballot(flags != 0ull)) {
+        const int src = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t flo = __builtin_amdgcn_readlane((uint32_t)flags, src);
+        const uint32_t fhi = __builtin_amdgcn_readlane((uint32_t)(flags >> 32), src);
+        const uint64_t fs = ((uint64_t)fhi << 32) | flo;
+        const int e = __ffsll((unsigned long long)fs) - 1;
+        if (lane == src) flags &= flags - 1;
+        int row, col;
+        band_element(src, e, row, col);
+        const int64_t gi = R * 64 + row, gj = C * 64 + col;
+        const double ex = exact_d2_wave(a.xw64, a.d, gi, gj);
+        const bool in = ex <= a.bp.r2;
+        float prod = 0.0f;
+        for (int q = lane; q < a.dp; q += 64) prod = __builtin_fmaf(a.xra[gi * a.dp + q], a.xca[gj * a.dp + q], prod);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) prod += __shfl_xor(prod, off, 64);
+        if (lane == src) {
+            const double d2 = a.bp.r2 - 2.0 * (double)prod;
+            ++hits;
+            flips += (in != (d2 <= a.bp.r2));
+            const double err = fabs(d2 - ex);
+            if (err == err) maxerr = fmax(maxerr, err);
+            if (in) {
+                const int v = e >> 2, b = e & 3;
+                const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
+                if (b == 0) a_lo |= bit;
+                else if (b == 1) a_hi |= bit;
+                else if (b == 2) b_lo |= bit;
+                else b_hi |= bit;
+            }
+        }
+    }
+    const uint64_t colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
+    const uint64_t word = transpose_bits(colword);
+    if (word) {
+        atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
+        atomicAdd(&a.deg[R * 64 + lane], __popcll(word));
+    }
+    if (a.sym && C != R && colword) {
+        const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
+        atomicOr(&a.mask[(km - a.k_base) * 64 + lane], (unsigned long long)colword);
+        atomicAdd(&a.deg[C * 64 + lane], __popcll(colword));
+    }
+}
+
+// One wave per 64x64 tile pair (the 4 waves of a workgroup share the row
+// tile).  With E = (largest margin of the row tile) + (largest margin of the
+// column tile): T >= E is a neighbour for certain, T < -E is not.  Elements in
+// between (and NaN) are left out here and the tile pair is queued for
+// k_dist_mfma_band, which settles them in FP64 in reference order.  No LDS:
+// each lane sets its rows' bits in its column words, one butterfly transpose
+// gives the row words, and the column words are the mirror pair's row words.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_dist_mfma(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    const float e_row = wave_max(a.emarg[R * 64 + lane]);
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    int mydeg = 0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
-        const float *__restrict__ xb0 = a.xca + (C * 64 + r) * dp + 4 * h;
-        const float *__restrict__ xb1 = xb0 + (int64_t)32 * dp;
-        v16f c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
-        float4 A0 = *(const float4 *)(xa0), A1 = *(const float4 *)(xa1);
-        float4 B0 = *(const float4 *)(xb0), B1 = *(const float4 *)(xb1);
-        for (int q = 0; q < dp; q += 8) {
-            const int qn = (q + 8 < dp) ? q + 8 : q;  // last block reloads itself (in bounds, unused)
-            const float4 nA0 = *(const float4 *)(xa0 + qn), nA1 = *(const float4 *)(xa1 + qn);
-            const float4 nB0 = *(const float4 *)(xb0 + qn), nB1 = *(const float4 *)(xb1 + qn);
-            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B0.x, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B1.x, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B0.x, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B1.x, c11, 0, 0, 0);
-            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B0.y, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B1.y, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B0.y, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B1.y, c11, 0, 0, 0);
-            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B0.z, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B1.z, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B0.z, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B1.z, c11, 0, 0, 0);
-            c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B0.w, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B1.w, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B0.w, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B1.w, c11, 0, 0, 0);
-            A0 = nA0; A1 = nA1; B0 = nB0; B1 = nB1;
-        }
-        // fast path: no element with T' >= 0 (NaN compares as a candidate)
+        const float e_col = a.emarg[C * 64 + lane];
+        const MfmaTile t = mfma_tile(a, R, C);
+        const float E = e_row + wave_max(e_col);
+        const float nE = -E;
+        // fast path: every element below -E (NaN counts as a candidate)
         bool cand = false;
 #pragma unroll
         for (int v = 0; v < 16; ++v)
-            cand |= !(c00[v] < 0.0f) || !(c01[v] < 0.0f) || !(c10[v] < 0.0f) || !(c11[v] < 0.0f);
+            cand |= !(t.c00[v] < nE) || !(t.c01[v] < nE) || !(t.c10[v] < nE) || !(t.c11[v] < nE);
         uint64_t word = 0, colword = 0;
         if (__ballot(cand)) {
+            uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
+            uint32_t f_lo = 0, f_hi = 0;  // band flags of elements 4v + b (see band_element)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int row = 8 * (v >> 2) + 4 * h + (v & 3);
-                T[row * 65 + r] = c00[v];
-                T[row * 65 + 32 + r] = c01[v];
-                T[(row + 32) * 65 + r] = c10[v];
-                T[(row + 32) * 65 + 32 + r] = c11[v];
+                const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
+                a_lo = t.c00[v] >= E ? (a_lo | bit) : a_lo;
+                a_hi = t.c10[v] >= E ? (a_hi | bit) : a_hi;
+                b_lo = t.c01[v] >= E ? (b_lo | bit) : b_lo;
+                b_hi = t.c11[v] >= E ? (b_hi | bit) : b_hi;
+                uint32_t &f = v < 8 ? f_lo : f_hi;
+                const int sh = 4 * (v & 7);
+                f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
+                f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
+                f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
+                f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
             }
-            s_ec[wave][lane] = a.emarg[C * 64 + lane];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int64_t gi = R * 64 + lane;
-            for (int j = 0; j < 64; ++j) {
-                const float t = T[lane * 65 + j];
-                if (!(t < 0.0f)) {
-                    const float ej = s_ec[wave][j];
-                    bool in = t >= 2.0f * (er + ej);
-                    if (!in) {
-                        const int64_t gj = C * 64 + j;
-                        if (gi < a.n && gj < a.n) {
-                            const double e = exact_d2(a.xw64, a.d, gi, gj);
-                            in = e <= a.bp.r2;
-                            ++hits;
-                            const double d2 = a.bp.r2 - 2.0 * ((double)t - (double)(er + ej));
-                            flips += (in != (d2 <= a.bp.r2));
-                            const double err = fabs(d2 - e);
-                            if (err == err) maxerr = fmax(maxerr, err);
-                        }
+            uint64_t flags = ((uint64_t)f_hi << 32) | f_lo;
+            {  // only real rows and columns
+                const uint64_t cm = valid_cols(a.n, C);
+                const bool ca = (cm >> (lane & 31)) & 1ull, cb = (cm >> (32 + (lane & 31))) & 1ull;
+                const uint64_t colmask = (ca ? 0x3333333333333333ull : 0ull) | (cb ? 0xccccccccccccccccull : 0ull);
+                flags &= colmask;
+                if (R * 64 + 64 > a.n) {
+                    for (int e = 0; e < 64; ++e) {
+                        int row, col;
+                        band_element(lane, e, row, col);
+                        if (R * 64 + row >= a.n) flags &= ~(1ull << e);
                     }
-                    word |= (uint64_t)in << j;
                 }
             }
-            if (a.sym && C != R) colword = transpose_bits(word & valid_cols(a.n, C));
-            __builtin_amdgcn_wave_barrier();
+            if (__ballot(flags != 0ull)) {
+                unsigned long long slot = 0;
+                if (lane == 0) slot = atomicAdd(a.band_cnt, 1ull);
+                slot = __shfl(slot, 0, 64);
+                if ((int64_t)slot < a.band_cap) {
+                    int64_t *p = a.band_list + (int64_t)slot * 66;
+                    if (lane == 0) {
+                        p[0] = k;
+                        p[1] = R;
+                    }
+                    p[2 + lane] = (int64_t)flags;
+                } else if (lane == 0) {
+                    const unsigned long long rs = atomicAdd(a.redo_cnt, 1ull);
+                    a.redo[2 * rs] = k;
+                    a.redo[2 * rs + 1] = R;
+                }
+            }
+            colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
+            word = transpose_bits(colword);
         }
         mydeg += store_pair(a, k, R, i, C, word, true, colword);
     }
-    flush_band(a, hits, flips, maxerr);
     finish_deg(a, i, mydeg);
+}
+
+// Band elements of the queued tile pairs (one wave per pair): the FP32 values
+// are recomputed exactly as in k_dist_mfma; every element with -E <= T < E is
+// decided in FP64 in reference order (one wave-wide evaluation per element),
+// and its bits are added to the stored words (and to the degrees) of the pair
+// and of its mirror.
+__global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_redo) {
+    const int64_t slot = blockIdx.x;
+    if (slot >= n_redo) return;
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 31, h = lane >> 5;
+    const int64_t k = a.redo[2 * slot], R = a.redo[2 * slot + 1];
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int64_t C = uniform64(tile_col(a, k, kb));
+    const float e_row = wave_max(a.emarg[R * 64 + lane]);
+    const float E = e_row + wave_max(a.emarg[C * 64 + lane]);
+    const float nE = -E;
+    const MfmaTile t = mfma_tile(a, R, C);
+    unsigned hits = 0, flips = 0;
+    double maxerr = 0.0;
+    const uint64_t cm = valid_cols(a.n, C);
+    const bool col_a = (cm >> r) & 1ull, col_b = (cm >> (32 + r)) & 1ull;
+    uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
+    // element (row 4h + rr + rb, column cb + r) of lane (r, h) with value x
+    auto settle = [&](float x, int rr, int rb, int cb, bool col_ok, uint32_t &part) {
+        const bool band = !(x >= E) && !(x < nE) && col_ok && (R * 64 + 4 * h + rr + rb) < a.n;
+        uint64_t todo = __ballot(band);
+        while (todo) {
+            const int src = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int64_t gi = R * 64 + 4 * (src >> 5) + rr + rb;
+            const int64_t gj = C * 64 + cb + (src & 31);
+            const double e = exact_d2_wave(a.xw64, a.d, gi, gj);
+            const bool in = e <= a.bp.r2;
+            if (lane == src) {
+                if (in) part |= 1u << rr;
+                ++hits;
+                const double d2 = a.bp.r2 - 2.0 * (double)x;
+                flips += (in != (d2 <= a.bp.r2));
+                const double err = fabs(d2 - e);
+                if (err == err) maxerr = fmax(maxerr, err);
+            }
+        }
+    };
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int rr = 8 * (v >> 2) + (v & 3);
+        settle(t.c00[v], rr, 0, 0, col_a, a_lo);
+        settle(t.c10[v], rr, 32, 0, col_a, a_hi);
+        settle(t.c01[v], rr, 0, 32, col_b, b_lo);
+        settle(t.c11[v], rr, 32, 32, col_b, b_hi);
+    }
+    const uint64_t colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
+    const uint64_t word = transpose_bits(colword);  // lane i: row i's extra bits
+    const int64_t i = R * 64 + lane;
+    if (word) {
+        atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
+        atomicAdd(&a.deg[i], __popcll(word));
+    }
+    if (a.sym && C != R && colword) {
+        const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
+        atomicOr(&a.mask[(km - a.k_base) * 64 + lane], (unsigned long long)colword);
+        atomicAdd(&a.deg[C * 64 + lane], __popcll(colword));
+    }
+    flush_band(a, hits, flips, maxerr);
 }
 
 // Plain FP64 for the dense layout with 4 < d <= KMAX, on a copy of the
@@ -1036,7 +1262,7 @@ static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float 
         A += u;
         B += u;
     }
-    const double S = 4.0;
+    const double S = 1.0;  // the bound above is already a worst case (gamma_n per sum)
     ka = (float)(S * A);
     kb = (float)(S * B * 0.5 * r2 + 1e-30);
 }
@@ -1047,12 +1273,33 @@ static int wide_kmax(const DistArgs &a) {
     return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
-static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band) {
+static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band, int64_t batch_pairs) {
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
     if (band && a.d >= kMfmaMinD) {
-        (void)0;
-        k_dist_mfma<<<grid, block, 0, s>>>(a);
+        // guard-band elements are listed (and, when the list is full, whole tile pairs,
+        // at most once each) and settled afterwards
+        const int64_t cap = std::min<int64_t>(std::max<int64_t>(batch_pairs / 4, 1 << 14), (int64_t)1 << 21);
+        Scratch<int64_t> cnt(ctx, 2);
+        Scratch<int64_t> list(ctx, 66 * cap);
+        Scratch<int64_t> redo(ctx, 2 * std::max<int64_t>(batch_pairs, 1));
+        fill_i64(ctx, cnt.p, 0, 2);
+        DistArgs b = a;
+        b.band_cnt = (unsigned long long *)cnt.p;
+        b.band_list = list.p;
+        b.band_cap = cap;
+        b.redo_cnt = (unsigned long long *)(cnt.p + 1);
+        b.redo = redo.p;
+        k_dist_mfma<<<grid, block, 0, s>>>(b);
+        CNN_HIP(hipGetLastError());
+        int64_t counts[2];
+        CNN_HIP(hipMemcpyAsync(counts, cnt.p, sizeof(counts), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+        const int64_t n_elems = std::min(counts[0], cap), n_redo = counts[1];
+        if (n_redo > batch_pairs) throw HipFailure{hipErrorUnknown, "band pair queue overflow"};
+        if (n_elems > 0)
+            k_dist_band_flags<<<(unsigned)std::min<int64_t>(ceil_div(n_elems, 4), 4096), 256, 0, s>>>(b, n_elems);
+        if (n_redo > 0) k_dist_mfma_band<<<(unsigned)n_redo, 64, 0, s>>>(b, n_redo);
     } else if (!band && wide_kmax(a) > 0) {
         const int kmax = wide_kmax(a);
         const int64_t n_pad = a.n_tiles * 64;
@@ -1316,7 +1563,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         }
         fill_i32(ctx, deg.p, 0, n);
         CNN_HIP(hipEventRecord(ctx->ev[2], s));
-        launch_dist(ctx, a, b.second - b.first, band);
+        launch_dist(ctx, a, b.second - b.first, band, pair_begin(b.second) - pair_begin(b.first));
         CNN_HIP(hipEventRecord(ctx->ev[3], s));
         // row_ptr for this batch's rows: exclusive scan + running offset
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
