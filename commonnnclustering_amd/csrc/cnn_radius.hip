@@ -268,6 +268,7 @@ struct DistArgs {
     int64_t band_cap;           // records
     unsigned long long *redo_cnt;
     int64_t *redo;              // (k, R) per entry
+    uint8_t *pair_nz;           // per pair of the batch: 1 if its words hold any bit (nullable)
 };
 
 __device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
@@ -384,6 +385,11 @@ __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t 
     if (!row_real(a, i)) m = 0;
     const int lane = threadIdx.x & 63;
     a.mask[(k - a.k_base) * 64 + lane] = m;
+    const bool any = __ballot(m != 0ull) != 0ull;
+    if (a.pair_nz && lane == 0) {
+        a.pair_nz[k - a.k_base] = any ? 1 : 0;
+        if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = any ? 1 : 0;
+    }
     if (a.sym && C != R) {
         if (!have_col) colword = transpose_bits(m);
         else {
@@ -754,6 +760,11 @@ __global__ void k_augment(const float *__restrict__ xw32, int64_t n, int64_t n_p
     emarg[i] = e;
 }
 
+__device__ __forceinline__ void mark_pair_nz(const DistArgs &a, int64_t k, int64_t R, int64_t C) {
+    a.pair_nz[k - a.k_base] = 1;
+    if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = 1;
+}
+
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -895,6 +906,8 @@ __device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hit
     }
     const uint64_t colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
     const uint64_t word = transpose_bits(colword);
+    const bool any_new = __ballot(word != 0ull) != 0ull;
+    if (a.pair_nz && lane == 0 && any_new) mark_pair_nz(a, k, R, C);
     if (word) {
         atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
         atomicAdd(&a.deg[R * 64 + lane], __popcll(word));
@@ -1045,6 +1058,8 @@ __global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_red
     const uint64_t colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
     const uint64_t word = transpose_bits(colword);  // lane i: row i's extra bits
     const int64_t i = R * 64 + lane;
+    const bool any_new = __ballot(word != 0ull) != 0ull;
+    if (a.pair_nz && lane == 0 && any_new) mark_pair_nz(a, k, R, C);
     if (word) {
         atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
         atomicAdd(&a.deg[i], __popcll(word));
@@ -1130,16 +1145,19 @@ __global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__re
 
 // ------------------------------------------------------------------ CSR emit
 
-// Coalesced CSR emit: one workgroup per row tile.  Chunks of 64 tile pairs
-// are staged in LDS (pair-major, rows padded to 65 words).  For each of its 16
-// rows a wave scans the popcounts of the row's 64 words of the chunk (lane =
-// word), then walks the non-zero words: the word is broadcast to the wave,
-// lane b holds bit b, and the set bits go out as one contiguous store, each
-// at its rank among the word's set bits (mbcnt) after the word's prefix.
-// Rows come out sorted: candidate tiles ascend, then bit positions.
+// Coalesced CSR emit: one workgroup per row tile.  The row tile's non-empty
+// tile pairs (all pairs when no flags are kept) are gathered in ascending
+// order, 64 at a time, and their words staged in LDS (pair-major, rows padded
+// to 65 words).  For each of its 16 rows a wave scans the popcounts of the
+// row's 64 words (lane = word), then walks the non-zero words: the word is
+// broadcast to the wave, lane b holds bit b, and the set bits go out as one
+// contiguous store, each at its rank among the word's set bits (mbcnt) after
+// the word's prefix.  Rows come out sorted: candidate tiles ascend, then bits.
 __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
                                                    int32_t *__restrict__ col) {
     __shared__ uint64_t sm[64 * 65];
+    __shared__ int64_t s_k[64];
+    __shared__ int s_nk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t R = a.R0 + blockIdx.x;
     int64_t kb, ke;
@@ -1147,17 +1165,42 @@ __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__
     // lane rr < 16 holds the write position of row wave * 16 + rr
     const int64_t wl = R * 64 + wave * 16 + lane;
     int64_t mypos = (lane < 16 && wl < a.n) ? row_ptr[wl] : 0;
-    for (int64_t c0 = kb; c0 < ke; c0 += 64) {
-        const int nk = (int)min<int64_t>(64, ke - c0);
+    int64_t next = kb;  // gathering cursor (wave 0)
+    for (;;) {
         __syncthreads();
+        if (wave == 0) {
+            int cnt = 0;
+            while (cnt < 64 && next < ke) {
+                const int64_t kk = next + lane;
+                const bool nz = kk < ke && (a.pair_nz == nullptr || a.pair_nz[kk - a.k_base] != 0);
+                const uint64_t m = __ballot(nz);
+                const int room = 64 - cnt;
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (nz && rank < room) s_k[cnt + rank] = kk;
+                if (__popcll(m) > room) {
+                    // the window holds more than fits: resume after the last pair taken
+                    const uint64_t last = __ballot(nz && rank == room - 1);
+                    next += __ffsll((unsigned long long)last);
+                    cnt = 64;
+                } else {
+                    cnt += __popcll(m);
+                    next = min(next + 64, ke);
+                }
+            }
+            if (lane == 0) s_nk = cnt;
+        }
+        __syncthreads();
+        const int nk = s_nk;
+        if (nk == 0) break;
 #pragma unroll 4
         for (int st = 0; st < 16; ++st) {
             const int idx = (int)threadIdx.x + 256 * st;
             const int p = idx >> 6, r = idx & 63;
-            sm[p * 65 + r] = p < nk ? a.mask[(c0 + p - a.k_base) * 64 + r] : 0ull;
+            sm[p * 65 + r] = p < nk ? a.mask[(s_k[p] - a.k_base) * 64 + r] : 0ull;
         }
         __syncthreads();
-        const int cbase = lane < nk ? (int)tile_col(a, c0 + lane, kb) * 64 : 0;
+        const int cbase = lane < nk ? (int)tile_col(a, s_k[lane], kb) * 64 : 0;
         for (int rr = 0; rr < 16; ++rr) {
             const int r = wave * 16 + rr;
             const uint64_t v = sm[lane * 65 + r];
@@ -1526,6 +1569,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int64_t max_batch_pairs = 0;
     for (auto &b : batches) max_batch_pairs = std::max(max_batch_pairs, pair_begin(b.second) - pair_begin(b.first));
     Scratch<uint64_t> mask(ctx, std::max<int64_t>(max_batch_pairs, 1) * 64);
+    Scratch<uint8_t> pair_nz(ctx, std::max<int64_t>(max_batch_pairs, 1));
 
     // symmetric evaluation needs every mask of the graph in one buffer
     const int sym = batches.size() == 1 ? 1 : 0;
@@ -1555,6 +1599,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.xca = xca ? xca->p : nullptr;
         a.emarg = emarg ? emarg->p : nullptr;
         a.col_chunk = 32;
+        a.pair_nz = pair_nz.p;
         if (pk) {
             a.tile_r2 = pk->tile_r2;
             a.tile_lo = pk->tile_lo;
