@@ -137,8 +137,11 @@ __global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ 
 }
 
 // Per-tile bounding boxes over all d dims (finite coordinates of valid rows).
+// Also the boxes of the tile's four row groups (rows 16g .. 16g+15), used to
+// skip row groups of a candidate tile pair (glo/ghi nullable).
 __global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, int64_t n_tiles,
-                             double *__restrict__ blo, double *__restrict__ bhi) {
+                             double *__restrict__ blo, double *__restrict__ bhi, double *__restrict__ glo,
+                             double *__restrict__ ghi) {
     const int lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (t >= n_tiles) return;
@@ -148,7 +151,16 @@ __global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, 
         double lo = isfinite(v) ? v : INFINITY;
         double hi = isfinite(v) ? v : -INFINITY;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
+        for (int off = 8; off >= 1; off >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, off, 64));
+            hi = fmax(hi, __shfl_xor(hi, off, 64));
+        }
+        if (glo && (lane & 15) == 0) {
+            glo[(t * 4 + (lane >> 4)) * d + k] = lo;
+            ghi[(t * 4 + (lane >> 4)) * d + k] = hi;
+        }
+#pragma unroll
+        for (int off = 32; off >= 16; off >>= 1) {
             lo = fmin(lo, __shfl_xor(lo, off, 64));
             hi = fmax(hi, __shfl_xor(hi, off, 64));
         }
@@ -196,7 +208,8 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
                              int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
                              const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
                              int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
-                             int32_t *__restrict__ tp_col) {
+                             int32_t *__restrict__ tp_col, const double *__restrict__ glo,
+                             const double *__restrict__ ghi, uint8_t *__restrict__ gmask) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -214,7 +227,17 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
             const int64_t C = S * 64 + lane;
             const bool ok = (C >= clo) && (C < chi) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
             const uint64_t b = __ballot(ok);
-            if (FILL && ok) tp_col[out + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)C;
+            if (FILL && ok) {
+                const int64_t slot = out + __popcll(b & ((1ull << lane) - 1ull));
+                tp_col[slot] = (int32_t)C;
+                if (gmask) {  // row groups of R within reach of tile C
+                    uint8_t gm = 0;
+                    for (int gi = 0; gi < 4; ++gi)
+                        gm |= (uint8_t)(boxes_close(glo + (R * 4 + gi) * d, ghi + (R * 4 + gi) * d, blo + C * d,
+                                                    bhi + C * d, d, thr) << gi);
+                    gmask[slot] = gm;
+                }
+            }
             out += __popcll(b);
         }
     }
@@ -269,6 +292,7 @@ struct DistArgs {
     unsigned long long *redo_cnt;
     int64_t *redo;              // (k, R) per entry
     uint8_t *pair_nz;           // per pair of the batch: 1 if its words hold any bit (nullable)
+    const uint8_t *pair_groups; // per pair (global index): row groups of R (16 rows each) in reach of C
 };
 
 __device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
@@ -592,16 +616,22 @@ __global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
         for (int q = 0; q < D; ++q) xc[q] = a.xw64[(C * 64 + lane) * D + q];
         uint32_t clo = 0, chi = 0;
         const double *rows = srow + opaque_zero();
-        static_for<0, 64>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            double t = rows[i * DP] - xc[0];
-            double s = t * t;  // 0 + t*t == t*t exactly (t*t is never -0)
+        const int groups = a.pair_groups ? a.pair_groups[k] : 0xf;  // row groups within reach of C
+        static_for<0, 4>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            if (groups & (1 << g)) {
+                static_for<16 * g, 16 * g + 16>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    double t = rows[i * DP] - xc[0];
+                    double s = t * t;  // 0 + t*t == t*t exactly (t*t is never -0)
 #pragma unroll
-            for (int q = 1; q < D; ++q) {
-                t = rows[i * DP + q] - xc[q];
-                s = s + t * t;
+                    for (int q = 1; q < D; ++q) {
+                        t = rows[i * DP + q] - xc[q];
+                        s = s + t * t;
+                    }
+                    set_bit_if<i>(clo, chi, s <= r2);
+                });
             }
-            set_bit_if<i>(clo, chi, s <= r2);
         });
         const uint64_t colword = ((uint64_t)chi << 32) | clo;
         mydeg += store_pair(a, k, R, i_lane, C, transpose_bits(colword), true, colword);
@@ -647,18 +677,25 @@ __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
         uint32_t clo = 0, chi = 0;
         uint64_t band_cols = 0;  // columns with an FP32 value inside the guard band (any row)
         const float *rows = srow + opaque_zero();
-        static_for<0, 64>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            const float s = f32_d2_row<D>(rows, i, xc);
-            const bool in = s < lo;
-            band_cols |= ~(__ballot(in) | __ballot(s > hi));
-            set_bit_if<i>(clo, chi, in);
+        const int groups = a.pair_groups ? a.pair_groups[k] : 0xf;  // row groups within reach of C
+        static_for<0, 4>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            if (groups & (1 << g)) {
+                static_for<16 * g, 16 * g + 16>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const float s = f32_d2_row<D>(rows, i, xc);
+                    const bool in = s < lo;
+                    band_cols |= ~(__ballot(in) | __ballot(s > hi));
+                    set_bit_if<i>(clo, chi, in);
+                });
+            }
         });
         uint64_t colword = ((uint64_t)chi << 32) | clo;
         if (band_cols & cmask) {
             // rare: those columns recompute their rows and settle the band pairs in FP64
             const bool col_band = ((band_cols & cmask) >> lane) & 1ull;
             for (int i = 0; i < 64; ++i) {
+                if (!(groups & (1 << (i >> 4)))) continue;
                 const float s = f32_d2_row<D>(rows, i, xc);
                 const int64_t gi = R * 64 + i;
                 if (col_band && !(s < lo) && !(s > hi) && row_real(a, gi)) {
@@ -1501,9 +1538,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int32_t *tp_col = nullptr;
     int64_t total_pairs = T * T;
     const double thr = g->r2 * (1.0 + 1e-6);
+    std::unique_ptr<Scratch<double>> glo, ghi;
+    std::unique_ptr<Scratch<uint8_t>> gmask;
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
-        k_tile_boxes<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(g->xw64, n, d, T, blo.p, bhi.p);
+        glo.reset(new Scratch<double>(ctx, T * 4 * d));
+        ghi.reset(new Scratch<double>(ctx, T * 4 * d));
+        k_tile_boxes<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(g->xw64, n, d, T, blo.p, bhi.p, glo->p, ghi->p);
         CNN_HIP(hipGetLastError());
         const int64_t NS = ceil_div(T, 64);
         Scratch<double> slo(ctx, NS * d), shi(ctx, NS * d);
@@ -1513,7 +1554,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const double *tthr = pk ? pk->tile_thr : nullptr;
         const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
         k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                     sl, sh, cnt.p, nullptr, nullptr);
+                                                                     sl, sh, cnt.p, nullptr, nullptr, nullptr, nullptr,
+                                                                     nullptr);
         CNN_HIP(hipGetLastError());
         tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
@@ -1522,8 +1564,10 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         total_pairs = last_excl + last_cnt;
         CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
         tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
+        gmask.reset(new Scratch<uint8_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                    sl, sh, nullptr, tp_ptr, tp_col);
+                                                                    sl, sh, nullptr, tp_ptr, tp_col, glo->p, ghi->p,
+                                                                    gmask->p);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipStreamSynchronize(s));
     }
@@ -1600,6 +1644,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.emarg = emarg ? emarg->p : nullptr;
         a.col_chunk = 32;
         a.pair_nz = pair_nz.p;
+        a.pair_groups = gmask ? gmask->p : nullptr;
         if (pk) {
             a.tile_r2 = pk->tile_r2;
             a.tile_lo = pk->tile_lo;
