@@ -15,7 +15,7 @@
 //   k_hook      each core row tests its nearest smaller-index core neighbours
 //               and hooks its parent to the first that passes (a forest, no
 //               atomics); links most of every dense region cheaply
-//   k_jump      pointer jumping until every parent is a root
+//   k_compress  every parent becomes its root
 //   k_hook_roots roots re-point at a smaller neighbouring root on a pass
 //   k_snapshot  rank[i] = (component size << 32 | root) of core points
 //   k_round_b   every edge between different snapshot components is tested by
@@ -242,17 +242,6 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
     add_stats(a, tests, passed);
 }
 
-// One in-place pointer-jumping pass: parent[w] <- parent[parent[w]].  Parents
-// only ever point to smaller indices, so ceil(log2(depth)) passes flatten
-// every tree; reading values already updated in the same pass only helps.
-__global__ void k_jump(int *parent, int64_t n) {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n) return;
-    const int p = parent[w];
-    const int gp = parent[p];
-    if (gp != p) parent[w] = gp;
-}
-
 // Full round.  Every edge between two different snapshot components is owned
 // by the endpoint whose component ranks lower (rank = size << 32 | root, all
 // distinct), and only the owner examines it.  A row whose rank is at least the
@@ -337,10 +326,20 @@ __global__ void k_snapshot(const uint8_t *core, const int *parent, const int32_t
     rank[w] = core[w] ? (((uint64_t)(uint32_t)cnt[r] << 32) | (uint32_t)r) : 0ull;
 }
 
+// Flatten: every entry points at its root afterwards.  The walk only reads
+// (entries of other threads may change meanwhile, but only towards their
+// roots) and each thread writes only its own entry, so no write can leave an
+// entry below its root.
 __global__ void k_compress(int *parent, int64_t n) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
-    parent[w] = uf_find(parent, (int)w);
+    int x = (int)w;
+    int p = atomic_load_relaxed(&parent[x]);
+    while (p != x) {
+        x = p;
+        p = atomic_load_relaxed(&parent[x]);
+    }
+    __hip_atomic_store(&parent[w], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Lanes sharing a key are combined before one atomic per distinct key (points
@@ -512,11 +511,13 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     CNN_HIP(hipGetLastError());
     k_hook<<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     CNN_HIP(hipGetLastError());
-    int rounds = 1;
-    while (((int64_t)1 << rounds) < n + 1) ++rounds;
-    for (int it = 0; it <= rounds; ++it) k_jump<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
-    k_hook_roots<<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
-    for (int it = 0; it <= rounds; ++it) k_jump<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    // flatten (every parent becomes its root), then let roots re-point to smaller
+    // neighbouring roots twice, flattening after each pass
+    k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    for (int pass = 0; pass < 2; ++pass) {
+        k_hook_roots<<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
+        k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    }
     CNN_HIP(hipGetLastError());
     {
         Scratch<int32_t> cnt(ctx, n);
