@@ -17,6 +17,8 @@ constexpr int kTile = 64;        // points per tile: one wave64 lane per row
 constexpr int kWave = 64;
 constexpr int kSpatialMaxD = 8;  // d <= 8: spatial (Morton) order + tile-box culling
 constexpr int kMfmaMinD = 16;    // d >= 16: FP32 Gram form on MFMA in CNN_F32_BAND
+constexpr int kMfmaColChunk = 64;   // column tiles per workgroup of the dense LDS MFMA kernel
+constexpr int64_t kPivotMinN = 2048;  // d > 8: smallest n ordered by nearest-pivot cells
 
 struct Error {
     int code;

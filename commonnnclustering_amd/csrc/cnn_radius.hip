@@ -18,6 +18,7 @@
 // (this file is compiled with -ffp-contract=off); the FP32 kernels decide pairs
 // outside a guard band around r2 and recompute the band in that FP64 order.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <cfloat>
 #include <cmath>
@@ -114,6 +115,59 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
     else key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
     keys[i] = key;
     vals[i] = (int32_t)i;
+}
+
+// Working order for d > 8 (all tile pairs are evaluated, in any order): points
+// grouped by their nearest of L pivots (the points at caller indices j*n/L),
+// measured in FP32 over the first min(d, 64) dimensions.  Only the order comes
+// from this; clustered data then fills tiles from one cluster each, so most
+// tile pairs lie entirely outside the radius and take the distance kernels'
+// early exits.  Key = pivot index; the sort is stable (caller order within a
+// cell).
+constexpr int kPivotDims = 64;
+constexpr int kPivotChunk = 128;
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, int64_t n, int d, int L,
+                                                     uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    __shared__ float4 sp[kPivotChunk * kPivotDims / 4];
+    const int dd = min(d, kPivotDims);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float x[kPivotDims];
+#pragma unroll
+    for (int k = 0; k < kPivotDims; ++k) x[k] = (i < n && k < dd) ? (float)X[i * d + k] : 0.0f;
+    float best = INFINITY;
+    int bi = 0;
+    for (int p0 = 0; p0 < L; p0 += kPivotChunk) {
+        __syncthreads();
+        float *spf = (float *)sp;
+        for (int q = threadIdx.x; q < kPivotChunk * kPivotDims; q += 256) {
+            const int pj = p0 + q / kPivotDims, k = q % kPivotDims;
+            spf[q] = (pj < L && k < dd) ? (float)X[((int64_t)pj * n / L) * d + k] : 0.0f;
+        }
+        __syncthreads();
+        const int m = min(kPivotChunk, L - p0);
+        for (int j = 0; j < m; ++j) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k4 = 0; k4 < kPivotDims / 4; ++k4) {
+                const float4 v = sp[j * (kPivotDims / 4) + k4];
+                const float t0 = x[4 * k4] - v.x, t1 = x[4 * k4 + 1] - v.y;
+                const float t2 = x[4 * k4 + 2] - v.z, t3 = x[4 * k4 + 3] - v.w;
+                acc = __builtin_fmaf(t0, t0, acc);
+                acc = __builtin_fmaf(t1, t1, acc);
+                acc = __builtin_fmaf(t2, t2, acc);
+                acc = __builtin_fmaf(t3, t3, acc);
+            }
+            if (acc < best) {
+                best = acc;
+                bi = p0 + j;
+            }
+        }
+    }
+    if (i < n) {
+        keys[i] = (uint64_t)bi;
+        vals[i] = (int32_t)i;
+    }
 }
 
 template <typename Tin>
@@ -266,13 +320,15 @@ struct DistArgs {
     int32_t *deg;
     BandParams bp;
     unsigned long long *band_stats;  // [0] hits, [1] flips, [2] max err bits (double as u64)
-    int dp;                  // row stride of the augmented Gram vectors
+    int dp;                  // padded width of the augmented Gram vectors (8 per group)
+    int mfma_k;              // MFMA steps (two dimensions each): ceil((d + 2) / 2)
     // symmetric evaluation: only tile pairs with C >= R are computed; the
     // transposed words go to the mirror pair (C, R)
     int sym;
     const int64_t *mirror;   // explicit lists: index of (C, R) for each k (nullptr for dense)
     // augmented Gram form: row-role [x, 1, c] and column-role [x, c, 1] vectors
-    // (dp floats each) whose dot product is T' = (r2 - d2)/2 + e_i + e_j; e = margin
+    // whose dot product is T = (r2 - d2)/2, stored in the packed MFMA operand
+    // layout (see packed_at); e = error margin per point
     const float *xra;
     const float *xca;
     const float *emarg;
@@ -402,18 +458,22 @@ __device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
 }
 
 // Store a row word (and, for an off-diagonal symmetric pair, the column word
-// of the mirror pair).  Returns the row word's popcount.
+// of the mirror pair).  Returns the row word's popcount.  With per-pair flags,
+// the words of a pair without any bit are not stored: readers skip flagged-
+// empty pairs (the mirror of an empty pair is empty too).  `force` stores them
+// anyway, for pairs whose guard-band bits a later kernel ors in.
 __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t R, int64_t i, int64_t C, uint64_t m,
-                                          bool have_col, uint64_t colword) {
+                                          bool have_col, uint64_t colword, bool force = false) {
     m &= tile_mask(a, C);
     if (!row_real(a, i)) m = 0;
     const int lane = threadIdx.x & 63;
-    a.mask[(k - a.k_base) * 64 + lane] = m;
     const bool any = __ballot(m != 0ull) != 0ull;
     if (a.pair_nz && lane == 0) {
         a.pair_nz[k - a.k_base] = any ? 1 : 0;
         if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = any ? 1 : 0;
     }
+    if (a.pair_nz && !any && !force) return 0;
+    a.mask[(k - a.k_base) * 64 + lane] = m;
     if (a.sym && C != R) {
         if (!have_col) colword = transpose_bits(m);
         else {
@@ -765,36 +825,56 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-// Augmented FP32 rows for the Gram form.  With w = |x|^2 (sequential FMA) and
-// e = ka * w + kb (margin), c = -w/2 + r2/4 + e:
-//   row role    xra = [x_1 .. x_d, 1, c, 0 ...]
-//   column role xca = [x_1 .. x_d, c, 1, 0 ...]
-// so xra_i . xca_j = x_i.x_j + c_i + c_j = (r2 - d2_ij)/2 + e_i + e_j.
-__global__ void k_augment(const float *__restrict__ xw32, int64_t n, int64_t n_pad, int d, int dp, float r2q,
-                          float ka, float kb, float *__restrict__ xra, float *__restrict__ xca,
-                          float *__restrict__ emarg) {
+// Augmented FP32 rows for the Gram form.  With w = |x|^2 (sequential FMA),
+// margin e = ka * w + kb and c = r2/4 - w/2:
+//   row role    [x_1 .. x_d, 1, c]
+//   column role [x_1 .. x_d, c, 1]
+// so row_i . col_j = x_i.x_j + c_i + c_j = (r2 - d2_ij)/2.
+//
+// Packed operand layout (G = dp/8 groups of 8 dimensions per point): the
+// float4 at ((t*G + g)*2 + h)*64 + r holds dimensions 8g + 2j + h (j = 0..3)
+// of point 64t + r.  MFMA step m = 4g + j multiplies dimension 2m (lanes
+// 0-31, k-slot 0) and 2m + 1 (lanes 32-63, k-slot 1), so one wave-wide float4
+// load of a 32-row block is 512 contiguous bytes per lane half and feeds four
+// steps; dimensions past d + 1 are zero and steps past mfma_k are skipped.
+__device__ __forceinline__ int64_t packed_at(int64_t i, int k, int G) {
+    const int m = k >> 1;
+    return (((i >> 6) * G + (m >> 2)) * 2 + (k & 1)) * 256 + (i & 63) * 4 + (m & 3);
+}
+
+__global__ void k_augment_rows(const float *__restrict__ xw32, int64_t n, int64_t n_pad, int d, float r2q, float ka,
+                               float kb, float *__restrict__ emarg, float *__restrict__ caug) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
-    const float *x = xw32 + i * d;
-    float *ra = xra + i * dp, *ca = xca + i * dp;
     float w = 0.0f;
-    for (int k = 0; k < d; ++k) {
-        const float v = i < n ? x[k] : 0.0f;
-        w = __builtin_fmaf(v, v, w);
-        ra[k] = v;
-        ca[k] = v;
+    if (i < n) {
+        const float *x = xw32 + i * d;
+        for (int k = 0; k < d; ++k) w = __builtin_fmaf(x[k], x[k], w);
     }
-    const float e = __builtin_fmaf(ka, w, kb);
-    const float c = r2q - 0.5f * w;
-    ra[d] = 1.0f;
-    ra[d + 1] = c;
-    ca[d] = c;
-    ca[d + 1] = 1.0f;
-    for (int k = d + 2; k < dp; ++k) {
-        ra[k] = 0.0f;
-        ca[k] = 0.0f;
+    emarg[i] = __builtin_fmaf(ka, w, kb);
+    caug[i] = r2q - 0.5f * w;
+}
+
+// One thread per packed float4 (n_pad / 64 * G * 128 of them, in layout order).
+__global__ void k_augment_pack(const float *__restrict__ xw32, int64_t n, int64_t n_pad, int d, int G,
+                               const float *__restrict__ caug, float4 *__restrict__ xra, float4 *__restrict__ xca) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_pad * 2 * G) return;
+    const int r = (int)(idx & 63), h = (int)((idx >> 6) & 1);
+    const int64_t tg = idx >> 7;
+    const int g = (int)(tg % G);
+    const int64_t i = (tg / G) * 64 + r;
+    const float c = caug[i];
+    float vr[4], vc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 8 * g + 2 * j + h;
+        const float x = (k < d && i < n) ? xw32[i * d + k] : 0.0f;
+        vr[j] = k < d ? x : k == d ? 1.0f : k == d + 1 ? c : 0.0f;
+        vc[j] = k < d ? x : k == d ? c : k == d + 1 ? 1.0f : 0.0f;
     }
-    emarg[i] = e;
+    xra[idx] = make_float4(vr[0], vr[1], vr[2], vr[3]);
+    xca[idx] = make_float4(vc[0], vc[1], vc[2], vc[3]);
 }
 
 __device__ __forceinline__ void mark_pair_nz(const DistArgs &a, int64_t k, int64_t R, int64_t C) {
@@ -808,52 +888,57 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// T = xra_i . xca_j = (r2 - d2)/2 for the 64x64 tile pair (R, C), accumulated
-// in FP32 on v_mfma_f32_32x32x2f32 as a 2x2 grid of 32x32 blocks; lane l
-// supplies row l%32 at k-slot l/32, and a float4 load per lane feeds four
-// k-steps (dims 8q+4h .. 8q+4h+3, the same order for rows and columns); the
-// next block's operands are loaded while the current block's MFMAs run.
-// Accumulator element v of lane l is T[8(v/4) + 4(l/32) + v%4][l%32].
+// T = row_i . col_j = (r2 - d2)/2 for a 64x64 tile pair, accumulated in FP32
+// on v_mfma_f32_32x32x2f32 as a 2x2 grid of 32x32 blocks; lane l supplies row
+// l%32 at k-slot l/32 from the packed layout.  Accumulator element v of lane l
+// is T[8(v/4) + 4(l/32) + v%4][l%32].
 struct MfmaTile {
     v16f c00, c01, c10, c11;
 };
 
-__device__ __forceinline__ MfmaTile mfma_tile(const DistArgs &a, int64_t R, int64_t C) {
-    const int lane = threadIdx.x & 63;
-    const int r = lane & 31, h = lane >> 5;
-    const int dp = a.dp;
-    const float *__restrict__ xa0 = a.xra + (R * 64 + r) * dp + 4 * h;
-    const float *__restrict__ xa1 = xa0 + (int64_t)32 * dp;
-    const float *__restrict__ xb0 = a.xca + (C * 64 + r) * dp + 4 * h;
-    const float *__restrict__ xb1 = xb0 + (int64_t)32 * dp;
-    MfmaTile t;
+__device__ __forceinline__ void mfma_zero(MfmaTile &t) {
     t.c00 = v16f{0};
     t.c01 = v16f{0};
     t.c10 = v16f{0};
     t.c11 = v16f{0};
-    float4 A0 = *(const float4 *)(xa0), A1 = *(const float4 *)(xa1);
-    float4 B0 = *(const float4 *)(xb0), B1 = *(const float4 *)(xb1);
-    for (int q = 0; q < dp; q += 8) {
-        const int qn = (q + 8 < dp) ? q + 8 : q;  // last block reloads itself (in bounds, unused)
-        const float4 nA0 = *(const float4 *)(xa0 + qn), nA1 = *(const float4 *)(xa1 + qn);
-        const float4 nB0 = *(const float4 *)(xb0 + qn), nB1 = *(const float4 *)(xb1 + qn);
-        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B0.x, t.c00, 0, 0, 0);
-        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.x, B1.x, t.c01, 0, 0, 0);
-        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B0.x, t.c10, 0, 0, 0);
-        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.x, B1.x, t.c11, 0, 0, 0);
-        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B0.y, t.c00, 0, 0, 0);
-        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.y, B1.y, t.c01, 0, 0, 0);
-        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B0.y, t.c10, 0, 0, 0);
-        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.y, B1.y, t.c11, 0, 0, 0);
-        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B0.z, t.c00, 0, 0, 0);
-        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.z, B1.z, t.c01, 0, 0, 0);
-        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B0.z, t.c10, 0, 0, 0);
-        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.z, B1.z, t.c11, 0, 0, 0);
-        t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B0.w, t.c00, 0, 0, 0);
-        t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0.w, B1.w, t.c01, 0, 0, 0);
-        t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B0.w, t.c10, 0, 0, 0);
-        t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1.w, B1.w, t.c11, 0, 0, 0);
-        A0 = nA0; A1 = nA1; B0 = nB0; B1 = nB1;
+}
+
+__device__ __forceinline__ void mfma_step(MfmaTile &t, float a0, float a1, float b0, float b1) {
+    t.c00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, t.c00, 0, 0, 0);
+    t.c01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, t.c01, 0, 0, 0);
+    t.c10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, t.c10, 0, 0, 0);
+    t.c11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, t.c11, 0, 0, 0);
+}
+
+// The steps of one group of four (all four, or the first m of the last group).
+__device__ __forceinline__ void mfma_group(MfmaTile &t, const float4 &A0, const float4 &A1, const float4 &B0,
+                                           const float4 &B1, int m) {
+    mfma_step(t, A0.x, A1.x, B0.x, B1.x);
+    if (m > 1) mfma_step(t, A0.y, A1.y, B0.y, B1.y);
+    if (m > 2) mfma_step(t, A0.z, A1.z, B0.z, B1.z);
+    if (m > 3) mfma_step(t, A0.w, A1.w, B0.w, B1.w);
+}
+
+// Both operands from global memory (the general form, any d); the next
+// group's operands are loaded while the current group's MFMAs run.
+__device__ __forceinline__ MfmaTile mfma_tile(const DistArgs &a, int64_t R, int64_t C) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 31, h = lane >> 5;
+    const int G = a.dp >> 3;
+    const float4 *__restrict__ pa = (const float4 *)a.xra + R * G * 128 + h * 64 + r;
+    const float4 *__restrict__ pb = (const float4 *)a.xca + C * G * 128 + h * 64 + r;
+    MfmaTile t;
+    mfma_zero(t);
+    float4 A0 = pa[0], A1 = pa[32], B0 = pb[0], B1 = pb[32];
+    for (int g = 0; g < G; ++g) {
+        const int gn = (g + 1 < G) ? g + 1 : g;  // the last group reloads itself (in bounds, unused)
+        const float4 nA0 = pa[gn * 128], nA1 = pa[gn * 128 + 32];
+        const float4 nB0 = pb[gn * 128], nB1 = pb[gn * 128 + 32];
+        mfma_group(t, A0, A1, B0, B1, a.mfma_k - 4 * g);
+        A0 = nA0;
+        A1 = nA1;
+        B0 = nB0;
+        B1 = nB1;
     }
     return t;
 }
@@ -922,7 +1007,8 @@ __device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hit
         const double ex = exact_d2_wave(a.xw64, a.d, gi, gj);
         const bool in = ex <= a.bp.r2;
         float prod = 0.0f;
-        for (int q = lane; q < a.dp; q += 64) prod = __builtin_fmaf(a.xra[gi * a.dp + q], a.xca[gj * a.dp + q], prod);
+        for (int q = lane; q < a.dp; q += 64)
+            prod = __builtin_fmaf(a.xra[packed_at(gi, q, a.dp >> 3)], a.xca[packed_at(gj, q, a.dp >> 3)], prod);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) prod += __shfl_xor(prod, off, 64);
         if (lane == src) {
@@ -956,13 +1042,83 @@ __device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hit
     }
 }
 
-// One wave per 64x64 tile pair (the 4 waves of a workgroup share the row
-// tile).  With E = (largest margin of the row tile) + (largest margin of the
-// column tile): T >= E is a neighbour for certain, T < -E is not.  Elements in
-// between (and NaN) are left out here and the tile pair is queued for
-// k_dist_mfma_band, which settles them in FP64 in reference order.  No LDS:
-// each lane sets its rows' bits in its column words, one butterfly transpose
-// gives the row words, and the column words are the mirror pair's row words.
+// Decides the elements of one tile pair from its FP32 values.  With E = (largest
+// margin of the row tile) + (largest margin of the column tile): T >= E is a
+// neighbour for certain, T < -E is not.  Elements in between (and NaN) are left
+// out here and the pair is queued (a record of per-lane band flags, or the pair
+// alone when the record list is full) for settling in FP64 in reference order.
+// No LDS: each lane sets its rows' bits in its column words, one butterfly
+// transpose gives the row words, and the column words are the mirror pair's
+// row words.  Stores both and returns the row word's popcount.
+__device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTile &t, int64_t k, int64_t R,
+                                                 int64_t C, int64_t i, float E) {
+    const int lane = threadIdx.x & 63;
+    const float nE = -E;
+    // fast path: every element below -E (NaN counts as a candidate)
+    bool cand = false;
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+        cand |= !(t.c00[v] < nE) || !(t.c01[v] < nE) || !(t.c10[v] < nE) || !(t.c11[v] < nE);
+    uint64_t word = 0, colword = 0;
+    bool queued = false;  // guard-band elements settled later: the words must be stored
+    if (__ballot(cand)) {
+        uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
+        uint32_t f_lo = 0, f_hi = 0;  // band flags of elements 4v + b (see band_element)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
+            a_lo = t.c00[v] >= E ? (a_lo | bit) : a_lo;
+            a_hi = t.c10[v] >= E ? (a_hi | bit) : a_hi;
+            b_lo = t.c01[v] >= E ? (b_lo | bit) : b_lo;
+            b_hi = t.c11[v] >= E ? (b_hi | bit) : b_hi;
+            uint32_t &f = v < 8 ? f_lo : f_hi;
+            const int sh = 4 * (v & 7);
+            f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
+            f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
+            f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
+            f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
+        }
+        uint64_t flags = ((uint64_t)f_hi << 32) | f_lo;
+        {  // only real rows and columns
+            const uint64_t cm = valid_cols(a.n, C);
+            const bool ca = (cm >> (lane & 31)) & 1ull, cb = (cm >> (32 + (lane & 31))) & 1ull;
+            const uint64_t colmask = (ca ? 0x3333333333333333ull : 0ull) | (cb ? 0xccccccccccccccccull : 0ull);
+            flags &= colmask;
+            if (R * 64 + 64 > a.n) {
+                for (int e = 0; e < 64; ++e) {
+                    int row, col;
+                    band_element(lane, e, row, col);
+                    if (R * 64 + row >= a.n) flags &= ~(1ull << e);
+                }
+            }
+        }
+        if (__ballot(flags != 0ull)) {
+            queued = true;
+            unsigned long long slot = 0;
+            if (lane == 0) slot = atomicAdd(a.band_cnt, 1ull);
+            slot = __shfl(slot, 0, 64);
+            if ((int64_t)slot < a.band_cap) {
+                int64_t *p = a.band_list + (int64_t)slot * 66;
+                if (lane == 0) {
+                    p[0] = k;
+                    p[1] = R;
+                }
+                p[2 + lane] = (int64_t)flags;
+            } else if (lane == 0) {
+                const unsigned long long rs = atomicAdd(a.redo_cnt, 1ull);
+                a.redo[2 * rs] = k;
+                a.redo[2 * rs + 1] = R;
+            }
+        }
+        colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
+        word = transpose_bits(colword);
+    }
+    return store_pair(a, k, R, i, C, word, true, colword, queued);
+}
+
+// General MFMA form (any d, explicit or dense pair lists): one wave per 64x64
+// tile pair, the 4 waves of a workgroup share the row tile; both operands are
+// read from global memory.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_dist_mfma(DistArgs a) {
     const int64_t R = a.R0 + blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -976,68 +1132,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float e_col = a.emarg[C * 64 + lane];
         const MfmaTile t = mfma_tile(a, R, C);
-        const float E = e_row + wave_max(e_col);
-        const float nE = -E;
-        // fast path: every element below -E (NaN counts as a candidate)
-        bool cand = false;
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-            cand |= !(t.c00[v] < nE) || !(t.c01[v] < nE) || !(t.c10[v] < nE) || !(t.c11[v] < nE);
-        uint64_t word = 0, colword = 0;
-        if (__ballot(cand)) {
-            uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
-            uint32_t f_lo = 0, f_hi = 0;  // band flags of elements 4v + b (see band_element)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
-                a_lo = t.c00[v] >= E ? (a_lo | bit) : a_lo;
-                a_hi = t.c10[v] >= E ? (a_hi | bit) : a_hi;
-                b_lo = t.c01[v] >= E ? (b_lo | bit) : b_lo;
-                b_hi = t.c11[v] >= E ? (b_hi | bit) : b_hi;
-                uint32_t &f = v < 8 ? f_lo : f_hi;
-                const int sh = 4 * (v & 7);
-                f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
-                f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
-                f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
-                f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
-            }
-            uint64_t flags = ((uint64_t)f_hi << 32) | f_lo;
-            {  // only real rows and columns
-                const uint64_t cm = valid_cols(a.n, C);
-                const bool ca = (cm >> (lane & 31)) & 1ull, cb = (cm >> (32 + (lane & 31))) & 1ull;
-                const uint64_t colmask = (ca ? 0x3333333333333333ull : 0ull) | (cb ? 0xccccccccccccccccull : 0ull);
-                flags &= colmask;
-                if (R * 64 + 64 > a.n) {
-                    for (int e = 0; e < 64; ++e) {
-                        int row, col;
-                        band_element(lane, e, row, col);
-                        if (R * 64 + row >= a.n) flags &= ~(1ull << e);
-                    }
-                }
-            }
-            if (__ballot(flags != 0ull)) {
-                unsigned long long slot = 0;
-                if (lane == 0) slot = atomicAdd(a.band_cnt, 1ull);
-                slot = __shfl(slot, 0, 64);
-                if ((int64_t)slot < a.band_cap) {
-                    int64_t *p = a.band_list + (int64_t)slot * 66;
-                    if (lane == 0) {
-                        p[0] = k;
-                        p[1] = R;
-                    }
-                    p[2 + lane] = (int64_t)flags;
-                } else if (lane == 0) {
-                    const unsigned long long rs = atomicAdd(a.redo_cnt, 1ull);
-                    a.redo[2 * rs] = k;
-                    a.redo[2 * rs + 1] = R;
-                }
-            }
-            colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
-            word = transpose_bits(colword);
-        }
-        mydeg += store_pair(a, k, R, i, C, word, true, colword);
+        mydeg += mfma_settle_store(a, t, k, R, C, i, e_row + wave_max(e_col));
     }
     finish_deg(a, i, mydeg);
+}
+
+// Row blocks of the dense LDS MFMA kernel with work in column chunk cc: all of
+// them, or (symmetric evaluation, C >= R only) those starting before the
+// chunk's end.
+__host__ __device__ inline int64_t mfma_chunk_blocks(int sym, int64_t cc, int64_t chunk, int64_t n_tiles,
+                                                     int64_t R0, int64_t n_rb) {
+    if (!sym) return n_rb;
+    const int64_t c_hi = cc * chunk + chunk < n_tiles ? cc * chunk + chunk : n_tiles;
+    const int64_t b = (c_hi - R0 + 3) / 4;
+    return b < 0 ? 0 : (b < n_rb ? b : n_rb);
+}
+
+// Dense MFMA form for d + 2 <= 8G.  A workgroup takes 4 consecutive row tiles
+// (one per wave, its operands held in registers for the whole launch) and a
+// chunk of column tiles; each column tile is staged once in LDS (double
+// buffered: the next tile's loads are in flight during the current tile's
+// MFMAs) and read by all 4 waves.  The work items (column chunk, row block)
+// that hold any pair are numbered chunk-major and dealt to the 8 XCDs in
+// contiguous ranges of equal length (workgroup b runs on XCD b % 8), so every
+// XCD gets the same work and workgroups that share an L2 read the same column
+// tiles.  Degrees are added atomically (several chunks contribute to a row).
+template <int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_dist_mfma_lds(
+    DistArgs a, int64_t n_rt, int64_t n_rb, int64_t n_cc, int64_t per_xcd, int64_t n_items) {
+    constexpr int kTileF4 = G * 128;  // float4 per packed tile
+    constexpr int kPer = (kTileF4 + 255) / 256;
+    __shared__ float4 sB[2][kTileF4];
+    __shared__ float sE[2];  // largest margin of the staged column tile
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 31, h = lane >> 5;
+    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (item >= n_items) return;
+    int64_t cc = 0, rb = item;
+    for (; cc < n_cc; ++cc) {
+        const int64_t nb = mfma_chunk_blocks(a.sym, cc, a.col_chunk, a.n_tiles, a.R0, n_rb);
+        if (rb < nb) break;
+        rb -= nb;
+    }
+    if (cc >= n_cc) return;  // not reached: item < n_items
+    const int64_t R_lo = a.R0 + 4 * rb;
+    const int64_t R = R_lo + wave;
+    const bool rvalid = R < a.R0 + n_rt;
+    const int64_t c_lo = cc * a.col_chunk;
+    const int64_t c_hi = min(a.n_tiles, c_lo + a.col_chunk);
+    const int64_t c_begin = a.sym ? max(c_lo, R_lo) : c_lo;
+    if (c_begin >= c_hi) return;  // the whole workgroup
+    const int tail = a.mfma_k - 4 * (G - 1);
+
+    float4 A0[G], A1[G];
+    float e_row = 0.0f;
+    if (rvalid) {
+        const float4 *__restrict__ pa = (const float4 *)a.xra + R * kTileF4 + h * 64 + r;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            A0[g] = pa[g * 128];
+            A1[g] = pa[g * 128 + 32];
+        }
+        e_row = wave_max(a.emarg[R * 64 + lane]);
+    } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) A0[g] = A1[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const float4 *__restrict__ xb = (const float4 *)a.xca;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        if (q < kTileF4) sB[0][q] = xb[c_begin * kTileF4 + q];
+    }
+    if (wave == 0) {
+        const float e = wave_max(a.emarg[c_begin * 64 + lane]);
+        if (lane == 0) sE[0] = e;
+    }
+    __syncthreads();
+    const int64_t i = R * 64 + lane;
+    int mydeg = 0;
+    for (int64_t C = c_begin; C < c_hi; ++C) {
+        const int buf = (int)((C - c_begin) & 1);
+        const bool more = C + 1 < c_hi;
+        // the next tile's operands and margins (the last tile reloads itself, in
+        // bounds, unused): loads in flight during this tile's MFMAs, no branch
+        const int64_t Cn = more ? C + 1 : C;
+        // named registers, not an array: a private array here is placed in LDS or
+        // scratch memory by the compiler, which waits for the loads right away
+        static_assert(kPer <= 5, "staging registers");
+        const float4 *src = xb + Cn * kTileF4;
+        float4 n0, n1, n2, n3, n4;
+        n0 = src[min((int)threadIdx.x, kTileF4 - 1)];
+        if constexpr (kPer > 1) n1 = src[min((int)threadIdx.x + 256, kTileF4 - 1)];
+        if constexpr (kPer > 2) n2 = src[min((int)threadIdx.x + 512, kTileF4 - 1)];
+        if constexpr (kPer > 3) n3 = src[min((int)threadIdx.x + 768, kTileF4 - 1)];
+        if constexpr (kPer > 4) n4 = src[min((int)threadIdx.x + 1024, kTileF4 - 1)];
+        const float e_nx = a.emarg[Cn * 64 + lane];
+        const bool work = rvalid && (!a.sym || C >= R);
+        MfmaTile t;
+        mfma_zero(t);
+        if (work) {
+            const float4 *sb = &sB[buf][h * 64 + r];
+            float4 B0 = sb[0], B1 = sb[32];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                float4 nB0 = B0, nB1 = B1;
+                if (g + 1 < G) {
+                    nB0 = sb[(g + 1) * 128];
+                    nB1 = sb[(g + 1) * 128 + 32];
+                }
+                mfma_group(t, A0[g], A1[g], B0, B1, g + 1 < G ? 4 : tail);
+                B0 = nB0;
+                B1 = nB1;
+            }
+        }
+        const float e_col = sE[buf];
+        // Stage the next tile between the MFMAs and the epilogue: the loads have
+        // had the MFMAs' time to arrive, and no store of this tile is in flight
+        // yet (waiting for the loads would also wait for those).  The empty asm
+        // keeps the compiler from moving the LDS stores up next to their loads.
+        asm volatile("" ::: "memory");
+        if (more) {
+            float4 *dst = sB[buf ^ 1];
+            const int q = threadIdx.x;
+            dst[q] = n0;  // kTileF4 >= 384
+            if constexpr (kPer > 1) { if (q + 256 < kTileF4) dst[q + 256] = n1; }
+            if constexpr (kPer > 2) { if (q + 512 < kTileF4) dst[q + 512] = n2; }
+            if constexpr (kPer > 3) { if (q + 768 < kTileF4) dst[q + 768] = n3; }
+            if constexpr (kPer > 4) { if (q + 1024 < kTileF4) dst[q + 1024] = n4; }
+            if (wave == 0) {
+                const float e = wave_max(e_nx);
+                if (lane == 0) sE[buf ^ 1] = e;
+            }
+        }
+        asm volatile("" ::: "memory");
+        if (work) mydeg += mfma_settle_store(a, t, R * a.n_tiles + C, R, C, i, e_row + e_col);
+        __syncthreads();
+    }
+    if (rvalid && mydeg && i < a.n) atomicAdd(&a.deg[i], mydeg);
 }
 
 // Band elements of the queued tile pairs (one wave per pair): the FP32 values
@@ -1118,12 +1350,13 @@ __global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_red
 // reuse); the chunking balances the triangle of the symmetric evaluation.
 // Degrees are accumulated atomically (several chunks contribute to a row).
 template <int KMAX>
-__global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double *__restrict__ xp, int64_t n_chunks) {
+__global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double *__restrict__ xp, int64_t n_chunks,
+                                                       int64_t n_rt) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t rg = blockIdx.x / n_chunks, cc = blockIdx.x % n_chunks;
-    const int64_t R = rg * 4 + wave;
+    const int64_t R = a.R0 + rg * 4 + wave;  // row tiles R0 .. R0 + n_rt - 1 of this batch
     const int64_t T = a.n_tiles;
-    if (R >= T) return;
+    if (R >= a.R0 + n_rt) return;
     const int64_t i = R * 64 + lane;
     double xi[KMAX];
 #pragma unroll
@@ -1347,9 +1580,9 @@ static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float 
     kb = (float)(S * B * 0.5 * r2 + 1e-30);
 }
 
-// Padded width of the wide FP64 kernel (dense layout, one batch), 0 if unused.
+// Padded width of the wide FP64 kernel (dense layout), 0 if unused.
 static int wide_kmax(const DistArgs &a) {
-    if (a.tp_ptr || a.d <= 8 || a.d > 64 || a.R0 != 0) return 0;
+    if (a.tp_ptr || a.d <= 8 || a.d > 64) return 0;
     return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
@@ -1359,7 +1592,8 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
     if (band && a.d >= kMfmaMinD) {
         // guard-band elements are listed (and, when the list is full, whole tile pairs,
         // at most once each) and settled afterwards
-        const int64_t cap = std::min<int64_t>(std::max<int64_t>(batch_pairs / 4, 1 << 14), (int64_t)1 << 21);
+        int64_t cap = std::min<int64_t>(std::max<int64_t>(batch_pairs / 4, 1 << 14), (int64_t)1 << 21);
+        if (const char *ev = std::getenv("CNN_HIP_BAND_RECORDS")) cap = std::max<int64_t>(1, std::atoll(ev));
         Scratch<int64_t> cnt(ctx, 2);
         Scratch<int64_t> list(ctx, 66 * cap);
         Scratch<int64_t> redo(ctx, 2 * std::max<int64_t>(batch_pairs, 1));
@@ -1370,7 +1604,33 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         b.band_cap = cap;
         b.redo_cnt = (unsigned long long *)(cnt.p + 1);
         b.redo = redo.p;
-        k_dist_mfma<<<grid, block, 0, s>>>(b);
+        const int G = a.dp / 8;
+        if (!a.tp_ptr && !a.tile_valid && G >= 3 && G <= 10) {
+            // dense pairs, row operands in registers, column tiles shared through LDS
+            b.col_chunk = kMfmaColChunk;
+            const int64_t n_rb = ceil_div(n_rt, 4), n_cc = ceil_div(a.n_tiles, b.col_chunk);
+            int64_t n_items = 0;
+            for (int64_t cc = 0; cc < n_cc; ++cc)
+                n_items += mfma_chunk_blocks(b.sym, cc, b.col_chunk, b.n_tiles, b.R0, n_rb);
+            const int64_t per_xcd = ceil_div(n_items, 8);
+            if (8 * per_xcd > (int64_t)INT32_MAX) throw HipFailure{hipErrorInvalidValue, "MFMA grid too large"};
+            const dim3 g8((unsigned)std::max<int64_t>(8 * per_xcd, 1));
+            switch (G) {
+#define CNN_MFMA_LDS(GG) \
+    case GG: k_dist_mfma_lds<GG><<<g8, block, 0, s>>>(b, n_rt, n_rb, n_cc, per_xcd, n_items); break;
+                CNN_MFMA_LDS(3)
+                CNN_MFMA_LDS(4)
+                CNN_MFMA_LDS(5)
+                CNN_MFMA_LDS(6)
+                CNN_MFMA_LDS(7)
+                CNN_MFMA_LDS(8)
+                CNN_MFMA_LDS(9)
+                default: k_dist_mfma_lds<10><<<g8, block, 0, s>>>(b, n_rt, n_rb, n_cc, per_xcd, n_items); break;
+#undef CNN_MFMA_LDS
+            }
+        } else {
+            k_dist_mfma<<<grid, block, 0, s>>>(b);
+        }
         CNN_HIP(hipGetLastError());
         int64_t counts[2];
         CNN_HIP(hipMemcpyAsync(counts, cnt.p, sizeof(counts), hipMemcpyDeviceToHost, s));
@@ -1385,13 +1645,13 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         const int64_t n_pad = a.n_tiles * 64;
         Scratch<double> xp(ctx, n_pad * kmax);
         k_pad_rows<<<(unsigned)ceil_div(n_pad * kmax, 256), 256, 0, s>>>(a.xw64, n_pad, a.d, kmax, xp.p);
-        const int64_t n_rg = ceil_div(a.n_tiles, 4);
+        const int64_t n_rg = ceil_div(n_rt, 4);
         const int64_t n_chunks = ceil_div(a.n_tiles, a.col_chunk);
         dim3 g2((unsigned)(n_rg * n_chunks));
-        if (kmax == 8) k_dist_f64_wide<8><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
-        else if (kmax == 16) k_dist_f64_wide<16><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
-        else if (kmax == 32) k_dist_f64_wide<32><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
-        else k_dist_f64_wide<64><<<g2, block, 0, s>>>(a, xp.p, n_chunks);
+        if (kmax == 8) k_dist_f64_wide<8><<<g2, block, 0, s>>>(a, xp.p, n_chunks, n_rt);
+        else if (kmax == 16) k_dist_f64_wide<16><<<g2, block, 0, s>>>(a, xp.p, n_chunks, n_rt);
+        else if (kmax == 32) k_dist_f64_wide<32><<<g2, block, 0, s>>>(a, xp.p, n_chunks, n_rt);
+        else k_dist_f64_wide<64><<<g2, block, 0, s>>>(a, xp.p, n_chunks, n_rt);
     } else if (!band) {
         switch (a.d) {
             case 1: k_dist_f64_cols<1><<<grid, block, 0, s>>>(a); break;
@@ -1528,8 +1788,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         xra.reset(new Scratch<float>(ctx, n_pad * dp));
         xca.reset(new Scratch<float>(ctx, n_pad * dp));
         emarg.reset(new Scratch<float>(ctx, n_pad));
-        k_augment<<<(unsigned)ceil_div(n_pad, 256), 256, 0, s>>>(g->xw32, n, n_pad, d, dp, (float)(0.25 * g->r2), gka,
-                                                                 gkb, xra->p, xca->p, emarg->p);
+        Scratch<float> caug(ctx, n_pad);
+        k_augment_rows<<<(unsigned)ceil_div(n_pad, 256), 256, 0, s>>>(g->xw32, n, n_pad, d, (float)(0.25 * g->r2), gka,
+                                                                      gkb, emarg->p, caug.p);
+        k_augment_pack<<<(unsigned)ceil_div(n_pad * 2 * (dp / 8), 256), 256, 0, s>>>(
+            g->xw32, n, n_pad, d, dp / 8, caug.p, (float4 *)xra->p, (float4 *)xca->p);
         CNN_HIP(hipGetLastError());
     }
 
@@ -1601,7 +1864,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     Scratch<int32_t> deg(ctx, n);
 
     // row-tile batches bounded by the mask budget (512 B per tile pair)
-    const int64_t mask_budget = (int64_t)8 << 30;
+    int64_t mask_budget = (int64_t)8 << 30;
+    if (const char *ev = std::getenv("CNN_HIP_MASK_BUDGET")) mask_budget = std::max<int64_t>(512, std::atoll(ev));
     const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
     std::vector<std::pair<int64_t, int64_t>> batches;
     for (int64_t R = 0; R < T;) {
@@ -1637,6 +1901,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.bp = bp;
         a.band_stats = bstats.p;
         a.dp = dp;
+        a.mfma_k = (d + 3) / 2;
         a.sym = sym;
         a.mirror = mirror ? mirror->p : nullptr;
         a.xra = xra ? xra->p : nullptr;
@@ -1767,7 +2032,8 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     double max_norm2 = 0.0;
     bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
 
-    // ordering: Morton order over the (up to) three widest dims for small d
+    // ordering: Morton order over the (up to) three widest dims for small d,
+    // nearest-pivot cells for larger d
     g->layout = (d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
     if (g->layout == 1) {
         const KeySpec ks = make_keyspec(d, lo, hi, 64);
@@ -1780,6 +2046,17 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
             k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
         CNN_HIP(hipGetLastError());
         sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
+    } else if (d > kSpatialMaxD && n >= kPivotMinN) {
+        const int L = (int)std::min<int64_t>(256, n / 64);
+        g->perm = dev_alloc<int32_t>(g, n);
+        Scratch<uint64_t> keys(ctx, n);
+        const unsigned blocks = (unsigned)ceil_div(n, 256);
+        if (dtype == CNN_F32)
+            k_pivot_cells<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, keys.p, g->perm);
+        else
+            k_pivot_cells<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, keys.p, g->perm);
+        CNN_HIP(hipGetLastError());
+        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, 8);
     }
     g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
     if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));

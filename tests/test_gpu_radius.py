@@ -37,7 +37,10 @@ def test_blobs_2d_cfg(cuda, precision):
 @pytest.mark.parametrize("n,d,dtype", [(1, 2, np.float64), (63, 3, np.float32), (64, 1, np.float64),
                                        (65, 2, np.float32), (500, 3, np.float64), (1000, 5, np.float32),
                                        (777, 8, np.float64), (600, 16, np.float32), (300, 64, np.float32),
-                                       (257, 70, np.float64), (2000, 2, np.float32)])
+                                       (257, 70, np.float64), (2000, 2, np.float32), (700, 17, np.float32),
+                                       (400, 22, np.float32), (450, 24, np.float32), (500, 40, np.float32),
+                                       (300, 78, np.float32), (300, 100, np.float32), (3000, 10, np.float32),
+                                       (2500, 100, np.float64), (2100, 33, np.float32)])
 def test_random_shapes(cuda, precision, n, d, dtype):
     rng = np.random.default_rng(n * 100 + d)
     X = np.concatenate([rng.normal(0, 1, size=(n // 2, d)), rng.uniform(-3, 3, size=(n - n // 2, d))])
@@ -110,4 +113,47 @@ def test_gram_band_hits_forced(cuda, d, dtype):
     B = A + u * (r + eps)
     X = np.concatenate([A, B]).astype(dtype)
     st = assert_same(X, r, "f32_band")
+    assert st["band_hits"] >= m // 2
+
+
+def _clustered(n, d, seed):
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(0.0, 4.0, size=(8, d))
+    X = centres[rng.integers(0, 8, size=n)] + rng.normal(size=(n, d))
+    D = np.sqrt(O.reduced_distances(X[:300], X[:300]))
+    return X, float(np.percentile(D[D > 0], 3))
+
+
+@pytest.mark.parametrize("d", [20, 64])
+def test_gram_many_column_chunks(cuda, d):
+    """More than one column chunk of the dense MFMA kernel (128 tiles each) and
+    a ragged last tile."""
+    X, r = _clustered(9000 + 37, d, d)
+    assert_same(X.astype(np.float32), r, "f32_band")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+@pytest.mark.parametrize("d", [3, 12, 24])
+def test_row_tile_batches(cuda, monkeypatch, precision, d):
+    """A small mask budget splits the distance pass into row-tile batches (each
+    batch evaluates all its tile pairs, no mirror writes); lists are unchanged."""
+    X, r = _clustered(3000 + 11, d, 100 + d)
+    monkeypatch.setenv("CNN_HIP_MASK_BUDGET", str(512 * 47 * 5))  # ~5 row tiles of 47 per batch
+    st = assert_same(X.astype(np.float32), r, precision)
+    assert st["distance_launches"] > 1
+
+
+@pytest.mark.parametrize("d", [16, 64])
+def test_gram_band_redo_path(cuda, monkeypatch, d):
+    """With room for a single band record, the other guard-band tile pairs are
+    recomputed whole and settled by the redo kernel; lists stay exact."""
+    monkeypatch.setenv("CNN_HIP_BAND_RECORDS", "1")
+    rng = np.random.default_rng(d + 1)
+    m = 800
+    A = rng.normal(size=(m, d)) * 3.0 + 5.0
+    u = rng.normal(size=(m, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 2.0
+    B = A + u * (r + rng.uniform(-1e-5, 1e-5, size=(m, 1)))
+    st = assert_same(np.concatenate([A, B]).astype(np.float32), r, "f32_band")
     assert st["band_hits"] >= m // 2
