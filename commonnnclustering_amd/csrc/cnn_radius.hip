@@ -851,7 +851,9 @@ __global__ void k_augment_rows(const float *__restrict__ xw32, int64_t n, int64_
         const float *x = xw32 + i * d;
         for (int k = 0; k < d; ++k) w = __builtin_fmaf(x[k], x[k], w);
     }
-    emarg[i] = __builtin_fmaf(ka, w, kb);
+    // non-finite or huge points (|T| could overflow in FP32): margin +inf, so
+    // every element of their tile pairs goes to the FP64 settling
+    emarg[i] = (w <= 1e36f) ? __builtin_fmaf(ka, w, kb) : INFINITY;
     caug[i] = r2q - 0.5f * w;
 }
 
@@ -1042,6 +1044,14 @@ __device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hit
     }
 }
 
+// Decision margin of a tile pair: NaN (every element fails both the
+// certain-in and the certain-out test and is settled in FP64) when a point of
+// either tile is non-finite or huge, whose margin is +inf.
+__device__ __forceinline__ float pair_margin(float e_row, float e_col) {
+    const float E = e_row + e_col;
+    return E < INFINITY ? E : NAN;
+}
+
 // Decides the elements of one tile pair from its FP32 values.  With E = (largest
 // margin of the row tile) + (largest margin of the column tile): T >= E is a
 // neighbour for certain, T < -E is not.  Elements in between (and NaN) are left
@@ -1054,16 +1064,20 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
                                                  int64_t C, int64_t i, float E) {
     const int lane = threadIdx.x & 63;
     const float nE = -E;
-    // fast path: every element below -E (NaN counts as a candidate)
-    bool cand = false;
+    // Fast path: the largest element is below -E.  fmaxf skips NaN elements, but
+    // those arise only from non-finite or huge inputs, whose pair margin is NaN,
+    // which fails this test and every certain-in / certain-out test below.
+    float mx = -INFINITY;
 #pragma unroll
-    for (int v = 0; v < 16; ++v)
-        cand |= !(t.c00[v] < nE) || !(t.c01[v] < nE) || !(t.c10[v] < nE) || !(t.c11[v] < nE);
+    for (int v = 0; v < 16; ++v) {
+        mx = fmaxf(fmaxf(mx, t.c00[v]), t.c01[v]);
+        mx = fmaxf(fmaxf(mx, t.c10[v]), t.c11[v]);
+    }
     uint64_t word = 0, colword = 0;
     bool queued = false;  // guard-band elements settled later: the words must be stored
-    if (__ballot(cand)) {
+    if (__ballot(!(mx < nE))) {
         uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
-        uint32_t f_lo = 0, f_hi = 0;  // band flags of elements 4v + b (see band_element)
+        float mn = INFINITY;  // smallest |T|: no element is in the band when it exceeds E
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
@@ -1071,14 +1085,23 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
             a_hi = t.c10[v] >= E ? (a_hi | bit) : a_hi;
             b_lo = t.c01[v] >= E ? (b_lo | bit) : b_lo;
             b_hi = t.c11[v] >= E ? (b_hi | bit) : b_hi;
-            uint32_t &f = v < 8 ? f_lo : f_hi;
-            const int sh = 4 * (v & 7);
-            f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
-            f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
-            f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
-            f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
+            mn = fminf(fminf(mn, fabsf(t.c00[v])), fabsf(t.c01[v]));
+            mn = fminf(fminf(mn, fabsf(t.c10[v])), fabsf(t.c11[v]));
         }
-        uint64_t flags = ((uint64_t)f_hi << 32) | f_lo;
+        uint64_t flags = 0;  // band flags of elements 4v + b (see band_element)
+        if (__ballot(!(mn > E))) {
+            uint32_t f_lo = 0, f_hi = 0;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                uint32_t &f = v < 8 ? f_lo : f_hi;
+                const int sh = 4 * (v & 7);
+                f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
+                f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
+                f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
+                f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
+            }
+            flags = ((uint64_t)f_hi << 32) | f_lo;
+        }
         {  // only real rows and columns
             const uint64_t cm = valid_cols(a.n, C);
             const bool ca = (cm >> (lane & 31)) & 1ull, cb = (cm >> (32 + (lane & 31))) & 1ull;
@@ -1132,7 +1155,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         const int64_t C = uniform64(tile_col(a, k, kb));
         const float e_col = a.emarg[C * 64 + lane];
         const MfmaTile t = mfma_tile(a, R, C);
-        mydeg += mfma_settle_store(a, t, k, R, C, i, e_row + wave_max(e_col));
+        mydeg += mfma_settle_store(a, t, k, R, C, i, pair_margin(e_row, wave_max(e_col)));
     }
     finish_deg(a, i, mydeg);
 }
@@ -1266,7 +1289,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
         asm volatile("" ::: "memory");
-        if (work) mydeg += mfma_settle_store(a, t, R * a.n_tiles + C, R, C, i, e_row + e_col);
+        if (work) mydeg += mfma_settle_store(a, t, R * a.n_tiles + C, R, C, i, pair_margin(e_row, e_col));
         __syncthreads();
     }
     if (rvalid && mydeg && i < a.n) atomicAdd(&a.deg[i], mydeg);
@@ -1287,7 +1310,7 @@ __global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_red
     tile_range(a, R, kb, ke);
     const int64_t C = uniform64(tile_col(a, k, kb));
     const float e_row = wave_max(a.emarg[R * 64 + lane]);
-    const float E = e_row + wave_max(a.emarg[C * 64 + lane]);
+    const float E = pair_margin(e_row, wave_max(a.emarg[C * 64 + lane]));
     const float nE = -E;
     const MfmaTile t = mfma_tile(a, R, C);
     unsigned hits = 0, flips = 0;

@@ -157,3 +157,23 @@ def test_gram_band_redo_path(cuda, monkeypatch, d):
     B = A + u * (r + rng.uniform(-1e-5, 1e-5, size=(m, 1)))
     st = assert_same(np.concatenate([A, B]).astype(np.float32), r, "f32_band")
     assert st["band_hits"] >= m // 2
+
+
+def test_gram_huge_and_nonfinite(cuda):
+    """MFMA form with points whose FP32 Gram terms overflow (|x| ~ 1e19, exact
+    duplicates among them) and with inf / NaN coordinates: their tile pairs are
+    settled in FP64, lists equal the oracle's."""
+    rng = np.random.default_rng(11)
+    d = 16
+    X = rng.normal(size=(300, d))
+    X[10] = 1e19
+    X[11] = 1e19            # duplicate of 10: distance 0
+    X[12] = -1e19
+    X[70, 3] = np.inf
+    X[71, 0] = np.nan
+    X[72] = X[5] * 3e18
+    for dtype in (np.float64, np.float32):
+        Xc = X.astype(dtype)
+        rp, col, _ = gpu_csr(Xc, 2.5, "f32_band")
+        rp0, col0 = O.neighbourhoods_csr(Xc.astype(np.float64), 2.5)
+        assert (rp == rp0).all() and (col == col0).all()
