@@ -171,11 +171,18 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    // first entry >= w: a 64-way search (each step probes 64 evenly spaced
+    // entries at once and keeps the gap that holds the boundary)
     int64_t lo = b, hi = e;
     while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (a.col[mid] < (int32_t)w) lo = mid + 1;
-        else hi = mid;
+        const int64_t step = (hi - lo + 63) >> 6;
+        const int64_t pos = lo + lane * step;
+        const bool below = pos < hi && a.col[pos] < (int32_t)w;
+        const int cnt = __popcll(__ballot(below));  // probes 0 .. cnt-1 lie below w
+        const int64_t nlo = cnt > 0 ? lo + (int64_t)(cnt - 1) * step + 1 : lo;
+        const int64_t nhi = min(hi, lo + (int64_t)cnt * step);
+        lo = nlo;
+        hi = nhi;
     }
     unsigned tests = 0, passed = 0;
     int attempts = 0;
