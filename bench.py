@@ -43,6 +43,8 @@ FP64_PEAK_TFLOPS = 78.6      # FMA-counted; separately rounded sub/mul/add reach
 TRAFFIC_FILES = {
     ("k_dist_f32_cols", "md_basins_3d"): "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
     ("k_emit_rows", "md_basins_3d"): "profiles/k_emit_rows_md_basins_3d_traffic.json",
+    ("k_dist_mfma_lds", "features_64d"): "profiles/k_dist_mfma_lds_features_64d_traffic.json",
+    ("k_emit_rows", "features_64d"): "profiles/k_emit_rows_features_64d_traffic.json",
 }
 
 
@@ -115,13 +117,17 @@ def distance_roofline(gs, d, precision, workload_name):
     Gram form on MFMA; pairs = computed 64x64 tile pairs x 4096 (symmetric
     evaluation computes each unordered tile pair once)."""
     launches = max(mean(gs, "distance_launches"), 1.0)
-    ms = mean(gs, "ms_distance") / launches
+    # the main kernel alone (the distance phase also holds guard-band settling)
+    ms = mean(gs, "ms_distance_kernel") / launches
     pairs = mean(gs, "tile_pairs_computed") * 4096.0 / launches
     gram = precision == "f32_band" and d >= 16
     # direct forms: subtract, multiply, add per dimension; Gram form: one FMA (2 flops) per dimension
     flops = (2.0 if gram else 3.0) * d * pairs
     if precision == "f32_band":
-        kernel = "k_dist_mfma" if gram else ("k_dist_f32_cols" if d <= 8 else "k_dist_f32_generic")
+        if gram:
+            kernel = "k_dist_mfma_lds" if (d + 2 + 7) // 8 <= 10 else "k_dist_mfma"
+        else:
+            kernel = "k_dist_f32_cols" if d <= 8 else "k_dist_f32_generic"
         bound, peak = ("mfma" if gram else "valu"), FP32_PEAK_TFLOPS
     else:
         kernel = "k_dist_f64_cols" if d <= 8 else "k_dist_f64_wide"

@@ -73,6 +73,7 @@ class GraphStats(ctypes.Structure):
         ("distance_launches", ctypes.c_int32),
         ("layout", ctypes.c_int32),
         ("tile_pairs_computed", ctypes.c_int64),
+        ("ms_distance_kernel", ctypes.c_float),
     ]
 
     def to_dict(self) -> dict:

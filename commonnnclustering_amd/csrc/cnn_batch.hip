@@ -34,6 +34,7 @@ static void add_graph_stats(cnn_graph_stats &gs, const cnn_graph_stats &o) {
     gs.ms_prep += o.ms_prep;
     gs.ms_distance += o.ms_distance;
     gs.ms_emit += o.ms_emit;
+    gs.ms_distance_kernel += o.ms_distance_kernel;
     gs.distance_launches += o.distance_launches;
     gs.tile_pairs_computed += o.tile_pairs_computed;
     gs.layout = o.layout;

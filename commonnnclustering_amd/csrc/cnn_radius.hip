@@ -1609,9 +1609,11 @@ static int wide_kmax(const DistArgs &a) {
     return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
+// Events 6 and 7 bracket the main distance kernel (without guard-band settling).
 static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band, int64_t batch_pairs) {
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
+    CNN_HIP(hipEventRecord(ctx->ev[6], s));
     if (band && a.d >= kMfmaMinD) {
         // guard-band elements are listed (and, when the list is full, whole tile pairs,
         // at most once each) and settled afterwards
@@ -1655,6 +1657,7 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
             k_dist_mfma<<<grid, block, 0, s>>>(b);
         }
         CNN_HIP(hipGetLastError());
+        CNN_HIP(hipEventRecord(ctx->ev[7], s));
         int64_t counts[2];
         CNN_HIP(hipMemcpyAsync(counts, cnt.p, sizeof(counts), hipMemcpyDeviceToHost, s));
         CNN_HIP(hipStreamSynchronize(s));
@@ -1701,6 +1704,7 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         }
     }
     CNN_HIP(hipGetLastError());
+    if (!(band && a.d >= kMfmaMinD)) CNN_HIP(hipEventRecord(ctx->ev[7], s));
 }
 
 template <typename T>
@@ -1904,7 +1908,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
 
     // symmetric evaluation needs every mask of the graph in one buffer
     const int sym = batches.size() == 1 ? 1 : 0;
-    float ms_dist = 0.0f, ms_emit = 0.0f;
+    float ms_dist = 0.0f, ms_emit = 0.0f, ms_kernel = 0.0f;
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
@@ -1984,11 +1988,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
         CNN_HIP(hipEventSynchronize(ctx->ev[5]));
-        float t1 = 0, t2 = 0;
+        float t1 = 0, t2 = 0, t3 = 0;
         CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
         CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[4], ctx->ev[5]));
+        CNN_HIP(hipEventElapsedTime(&t3, ctx->ev[6], ctx->ev[7]));
         ms_dist += t1;
         ms_emit += t2;
+        ms_kernel += t3;
         nnz_total = need;
     }
     CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &nnz_total, sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -2009,6 +2015,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     st.ms_prep = ms_prep;
     st.ms_distance = ms_dist;
     st.ms_emit = ms_emit;
+    st.ms_distance_kernel = ms_kernel;
     st.distance_launches = (int32_t)batches.size();
     st.layout = g->layout;
     // lists are symmetric and hold the diagonal: the symmetric kernels compute C >= R only

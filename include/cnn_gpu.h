@@ -89,6 +89,7 @@ typedef struct {
     int32_t distance_launches; /* launches of the distance kernel timed in ms_distance */
     int32_t layout;            /* 0 identity order, 1 spatial (Morton) order */
     int64_t tile_pairs_computed; /* 64x64 tile pairs the distance kernel evaluated (symmetric: C >= R) */
+    float ms_distance_kernel;  /* the main distance kernel alone (ms_distance also holds guard-band settling) */
 } cnn_graph_stats;
 
 typedef struct {
