@@ -1578,18 +1578,20 @@ static BandParams make_band_direct(double r2, int d, bool rounded_coords, double
     return bp;
 }
 
-// Margin of the augmented Gram form.  T' = xra_i . xca_j is an FP32 sum of
+// Margin of the augmented Gram form.  T_hat = row_i . col_j is an FP32 sum of
 // dp products in any order (MFMA), so with Sabs = sum |x_ik x_jk| + |c_i| + |c_j|
-//   |T'_hat - (x_i.x_j + c_i_hat + c_j_hat)| <= gamma_dp Sabs,
-//   Sabs <= (w_i + w_j) + r2/2 + e_i + e_j,
-// and c_hat = -w_hat/2 + r2/4 + e carries gamma_d w/2 from the norm plus
-// 2u |c| from its two FP32 additions.  In T units (T = (r2 - d2)/2) the
-// reference's FP64 rounding adds gamma_{2d+2}(u64) r2 near the threshold, and
-// rounding f64 coordinates to f32 adds u (w_i + w_j) + u r2.  Choosing
+//   |T_hat - (x_i.x_j + c_i_hat + c_j_hat)| <= gamma_dp Sabs,
+//   Sabs <= (w_i + w_j) + r2/2,
+// and c_hat = r2/4 - w_hat/2 carries gamma_d w/2 from the norm plus u r2/4 +
+// u |c| from rounding r2/4 and the subtraction.  In T units (T = (r2 - d2)/2)
+// the reference's FP64 rounding adds gamma_{2d+2}(u64) r2 near the threshold,
+// and rounding f64 coordinates to f32 adds u (w_i + w_j) + u r2.  Hence
 //   e_i = S (A w_i + B r2 / 2),  A = gamma_dp + gamma_d/2 + 2u (+u),
-//                                B = gamma_dp/2 + 2u + gamma64 (+u),
-// with S = 4 makes e_i + e_j exceed the total error, so T' < 0 proves
-// d2 > r2 and T' >= 2 (e_i + e_j) proves d2 < r2 in the reference's FP64.
+//                                B = gamma_dp/2 + 2u + gamma64 (+u)
+// gives e_i + e_j >= the total error with S = 1; S = 1 + 2 gamma_d + 8u also
+// covers evaluating e in FP32 (w_hat, ka, the FMA) and the FP32 sum E = max e
+// over the row tile + max e over the column tile >= e_i + e_j.  So T_hat >= E
+// proves d2 <= r2 and T_hat < -E proves d2 > r2 in the reference's FP64.
 static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float &ka, float &kb) {
     const double u = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
     double A = gamma_m(dp, u) + 0.5 * gamma_m(d, u) + 2.0 * u;
@@ -1598,7 +1600,7 @@ static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float 
         A += u;
         B += u;
     }
-    const double S = 1.0;  // the bound above is already a worst case (gamma_n per sum)
+    const double S = 1.0 + 2.0 * gamma_m(d, u) + 8.0 * u;
     ka = (float)(S * A);
     kb = (float)(S * B * 0.5 * r2 + 1e-30);
 }
