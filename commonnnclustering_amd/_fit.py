@@ -165,3 +165,86 @@ class FitterGPU(Fitter):
 
 
 FitterExtBFS = FitterGPU
+
+
+class Predictor(ABC):
+    """Predictor interface (_fit.pyx:164-216): ``predict`` labels the points of
+    a second data set from a labelled one; ``make_parameters`` as the fitter's."""
+
+    @abstractmethod
+    def predict(self, input_data, predictand_input_data, labels: Labels, predictand_labels: Labels,
+                cluster_params: ClusterParameters):
+        """Generic cluster label prediction"""
+
+    make_parameters = Fitter.make_parameters
+
+
+class PredictorGPU(Predictor):
+    """First-match prediction on the GPU, replacing PredictorFirstmatch.predict
+    (_fit.pyx:1194-1291) with brute-force getters: for every considered
+    predictand point with more than n_member_cutoff reference points within
+    the radius, the reference neighbours are visited in ascending index and the
+    first whose label is in ``predictand_labels.consider_set``, which has more
+    than n_member_cutoff neighbours itself and which shares at least
+    similarity_cutoff neighbours with the point gives it its label (its
+    consider flag becomes 0).
+
+    Args:
+        neighbours_getter: ``NeighboursGetterGPU``; its ``precision`` selects the
+            distance form (identical neighbour sets either way).
+        neighbours_getter_other, neighbours, neighbour_neighbours,
+        similarity_checker: accepted for recipe compatibility with
+            PredictorFirstmatch.get_builder_kwargs (_fit.pyx:1222-1230).
+    """
+
+    def __init__(self, neighbours_getter=None, neighbours_getter_other=None, neighbours=None,
+                 neighbour_neighbours=None, similarity_checker=None):
+        self._neighbours_getter = NeighboursGetterGPU() if neighbours_getter is None else neighbours_getter
+        self._neighbours_getter_other = (self._neighbours_getter if neighbours_getter_other is None
+                                         else neighbours_getter_other)
+        self._neighbours = neighbours
+        self._neighbour_neighbours = neighbour_neighbours
+        self._similarity_checker = SimilarityCheckerGPU() if similarity_checker is None else similarity_checker
+        self.last_tests = None
+
+    @classmethod
+    def get_builder_kwargs(cls):
+        return [
+            ("neighbours_getter", None),
+            ("neighbours_getter_other", "neighbours_getter"),
+            ("neighbours", None),
+            ("neighbour_neighbours", "neighbours"),
+            ("similarity_checker", None),
+        ]
+
+    def __str__(self):
+        return (f"{type(self).__name__}(ngetter={self._neighbours_getter}, "
+                f"checker={self._similarity_checker})")
+
+    def predict(self, input_data, predictand_input_data, labels: Labels, predictand_labels: Labels,
+                cluster_params: ClusterParameters):
+        import torch
+        from . import engine
+
+        if not isinstance(input_data, InputDataComponents) or not isinstance(predictand_input_data,
+                                                                             InputDataComponents):
+            raise NotImplementedError("GPU prediction needs point coordinates for both data sets")
+        if predictand_labels.n_points != predictand_input_data.n_points:
+            raise ValueError("predictand labels and input data differ in length")
+        X = input_data.device_tensor()
+        Y = predictand_input_data.device_tensor()
+        dev = X.device
+        lab_x = torch.from_numpy(np.ascontiguousarray(labels._labels, dtype=np.int64)).to(dev)
+        lab_y = torch.from_numpy(np.ascontiguousarray(predictand_labels._labels, dtype=np.int64)).to(dev)
+        con_y = torch.from_numpy(np.ascontiguousarray(predictand_labels._consider, dtype=np.uint8)).to(dev)
+        precision = getattr(self._neighbours_getter, "precision", "f64")
+        _, _, tests = engine.predict(X, lab_x, Y.to(dev), cluster_params.radius_cutoff,
+                                     cluster_params.similarity_cutoff, cluster_params.n_member_cutoff,
+                                     predictand_labels.consider_set, labels_y=lab_y, consider_y=con_y,
+                                     precision=precision)
+        self.last_tests = tests
+        predictand_labels._labels[:] = lab_y.cpu().numpy()
+        predictand_labels._consider[:] = con_y.cpu().numpy()
+
+
+PredictorFirstmatch = PredictorGPU

@@ -17,7 +17,7 @@ import numpy as np
 
 from ._primitive_types import P_AINDEX
 from ._types import InputData, Labels
-from ._fit import Fitter, FitterGPU
+from ._fit import Fitter, FitterGPU, Predictor
 from .report import Record, Summary, timed
 from . import recipes
 
@@ -90,9 +90,16 @@ class Clustering:
             self._fitter = None if f is object else f
         if hierarchical_fitter is not None or "hierarchical_fitter" in builder.recipe:
             raise NotImplementedError("hierarchical fitters are not part of this engine; "
-                                      "use commonnnclustering_amd.batch.scan for (r, c) scans")
+                                      "use commonnnclustering_amd.engine.fit_scan for (r, c) scans")
         self._hierarchical_fitter = None
-        self._predictor = predictor
+        if predictor is not None:
+            if not isinstance(predictor, Predictor):
+                raise TypeError(f"Can't use object of type {type(predictor).__name__} as predictor. "
+                                f"Expected type {Predictor.__name__}.")
+            self._predictor = predictor
+        else:
+            p = builder.make_component("predictor")
+            self._predictor = None if p is object else p
 
     @property
     def root(self):
@@ -210,3 +217,54 @@ class Clustering:
             if v:
                 print(rec)
             bundle._summary.append(rec)
+
+    def predict(self, other, radius_cutoff: float, cnn_cutoff: int, clusters=None, cnn_offset: int = None,
+                info: bool = True, record: bool = True, record_time: bool = True, v: bool = True,
+                purge: bool = False, bundle=None) -> None:
+        """Predict cluster labels for the points of ``other`` (a ``Bundle`` or a
+        ``Clustering``) from this clustering's labels (cluster.pyx:560-653).
+
+        Args:
+            other: the data to label.
+            radius_cutoff: neighbour search radius.
+            cnn_cutoff: similarity criterion (as in ``fit``).
+            clusters: labels that may be given (default: every label of this
+                clustering except 0).
+            cnn_offset: subtracted from ``cnn_cutoff`` (compatibility).
+            purge: re-initialise the labels of ``other`` (they are also reset
+                when missing or not frozen).
+        """
+        if bundle is None:
+            bundle = self._bundle
+        if isinstance(other, Clustering):
+            other = other._bundle
+        if self._predictor is None:
+            raise AttributeError("this clustering has no predictor; pass predictor= or use a recipe with one")
+        if cnn_offset is None:
+            cnn_offset = 0
+        cnn_cutoff -= cnn_offset
+        assert cnn_cutoff >= 0
+
+        if (other._labels is None) or purge or (not other._labels.meta.get("frozen", False)):
+            other._labels = Labels(np.zeros(other._input_data.n_points, order="C", dtype=P_AINDEX))
+
+        cluster_params = self._predictor.make_parameters(radius_cutoff, cnn_cutoff, 0)
+        if clusters is None:
+            clusters = bundle._labels.to_set() - {0}
+        other._labels.consider_set = clusters
+
+        args = (bundle._input_data, other._input_data, bundle._labels, other._labels, cluster_params)
+        if record_time:
+            _, execution_time = timed(self._predictor.predict)(*args)
+        else:
+            self._predictor.predict(*args)
+            execution_time = None
+
+        if info:
+            params = {k: (radius_cutoff, cnn_cutoff) for k in clusters if k != 0}
+            meta = {"params": params, "reference": weakref.proxy(bundle), "origin": "predict"}
+            old_params = other._labels.meta.get("params", {})
+            old_params.update(meta["params"])
+            meta["params"] = old_params
+            other._labels.meta.update(meta)
+        other._labels.meta["frozen"] = True

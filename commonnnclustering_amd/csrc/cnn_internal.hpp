@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
@@ -276,6 +277,47 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 
 __device__ __forceinline__ int atomic_load_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// |A & B| >= c for sorted, duplicate-free lists, evaluated by the whole wave:
+// 64-wide windows of both lists advance like the two-pointer merge; each lane
+// looks its B element up in the A window by a shuffle binary search.  Every
+// common element <= min(last A, last B) lies in both windows, so each is
+// counted once.  Wave-uniform result.
+__device__ inline bool wave_sim(const int32_t *__restrict__ A, int64_t na, const int32_t *__restrict__ B, int64_t nb,
+                         int c) {
+    if (c <= 0) return true;
+    if (na < c || nb < c) return false;
+    const int lane = __lane_id();
+    int64_t ia = 0, ib = 0;
+    int count = 0;
+    while (ia < na && ib < nb) {
+        const int va = (int)min<int64_t>(64, na - ia);
+        const int vb = (int)min<int64_t>(64, nb - ib);
+        const int a = lane < va ? A[ia + lane] : INT_MAX;
+        const int b = lane < vb ? B[ib + lane] : INT_MAX;
+        const int alast = __shfl(a, va - 1, 64);
+        const int blast = __shfl(b, vb - 1, 64);
+        const int m = min(alast, blast);
+        int lo = 0, hi = va;
+#pragma unroll
+        for (int step = 0; step < 7; ++step) {
+            const int mid = (lo + hi) >> 1;
+            const int am = __shfl(a, min(mid, 63), 64);
+            if (lo < hi) {
+                if (am < b) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        const int found = __shfl(a, min(lo, 63), 64);
+        const bool hit = (lane < vb) && (b <= m) && (lo < va) && (found == b);
+        count += __popcll(__ballot(hit));
+        if (count >= c) return true;
+        ia += __popcll(__ballot(lane < va && a <= m));
+        ib += __popcll(__ballot(lane < vb && b <= m));
+        if ((int64_t)count + min(na - ia, nb - ib) < (int64_t)c) return false;
+    }
+    return false;
 }
 
 }  // namespace cnn

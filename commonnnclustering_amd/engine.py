@@ -24,6 +24,8 @@ __all__ = [
     "sort_by_size",
     "fit",
     "fit_batch",
+    "fit_scan",
+    "predict",
     "to_device_points",
 ]
 
@@ -296,3 +298,44 @@ def fit_scan(X: torch.Tensor, radii, cnn_cutoffs, precision="f32_band", member_c
                                       _max_clusters(max_clusters), ptr(labels), ctypes.byref(gs),
                                       ctypes.cast(ls, ctypes.c_void_p)), "cnn_fit_scan")
     return labels, gs.to_dict(), [ls[i].to_dict() for i in range(n_r * n_c)]
+
+
+def predict(X: torch.Tensor, labels_x: torch.Tensor, Y: torch.Tensor, r2: float, c_sim: int, m_core: int,
+            clusters, labels_y: torch.Tensor = None, consider_y: torch.Tensor = None, precision="f64"):
+    """Labels for the points Y from the labelled reference points X, as
+    PredictorFirstmatch.predict (_fit.pyx:1241-1291): for every considered y,
+    the first reference neighbour (ascending index) whose label is in
+    ``clusters``, which has more than ``m_core`` reference neighbours and which
+    shares at least ``c_sim`` of y's reference neighbours gives y its label and
+    clears y's consider flag.  ``labels_y`` / ``consider_y`` (int64 / uint8 on
+    the device) are updated in place (zeros / ones when None).  Returns
+    (labels_y, consider_y, similarity tests executed).
+    """
+    n_x, d, dt = _check_points(X)
+    n_y, d_y, dt_y = _check_points(Y)
+    if d_y != d:
+        raise ValueError("X and Y differ in dimension")
+    if dt_y != dt:  # the reference works in float64: upcasting float32 is exact
+        X, Y = X.double().contiguous(), Y.double().contiguous()
+        n_x, d, dt = _check_points(X)
+    if labels_x.dtype != torch.int64 or not labels_x.is_cuda or labels_x.numel() != n_x:
+        raise ValueError("labels_x must be an int64 CUDA tensor of length n_x")
+    labels_x = labels_x.contiguous()
+    if labels_y is None:
+        labels_y = torch.zeros(n_y, dtype=torch.int64, device=Y.device)
+    if consider_y is None:
+        consider_y = torch.ones(n_y, dtype=torch.uint8, device=Y.device)
+    _check_labels(labels_y, n_y, consider_y)
+    consider_y = consider_y.contiguous()
+    wanted = sorted(int(c) for c in clusters if int(c) >= 0)
+    n_allowed = (wanted[-1] + 1) if wanted else 0
+    table = np.zeros(max(n_allowed, 1), dtype=np.uint8)
+    table[wanted] = 1
+    allowed = torch.from_numpy(table).to(X.device)
+    ctx = get_context(X.device)
+    tests = ctypes.c_int64(0)
+    ctx.check(_lib.lib().cnn_predict(ctx.handle, ptr(X), n_x, ptr(Y), n_y, d, dt, float(r2),
+                                     _lib.precision_code(precision), int(c_sim), int(m_core), ptr(labels_x),
+                                     ptr(allowed), n_allowed, ptr(labels_y), ptr(consider_y),
+                                     ctypes.byref(tests)), "cnn_predict")
+    return labels_y, consider_y, int(tests.value)

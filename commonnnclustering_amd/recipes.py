@@ -206,6 +206,9 @@ REGISTERED_RECIPES = {
         "fitter.queue": "fifo",
         "fitter.ngetter.dgetter": "metric",
         "fitter.ngetter.dgetter.metric": "euclidean_r",
+        "predictor": "firstmatch",
+        "predictor.ngetter": "brute_force",
+        "predictor.checker": "switch",
     },
     # FP32 distances with the FP64 guard-band recheck: identical neighbour sets
     "coordinates_f32": {
@@ -218,6 +221,9 @@ REGISTERED_RECIPES = {
         "fitter.queue": "fifo",
         "fitter.ngetter.dgetter": "metric",
         "fitter.ngetter.dgetter.metric": "euclidean_r",
+        "predictor": "firstmatch",
+        "predictor.ngetter": ("brute_force", (), {"precision": "f32_band"}),
+        "predictor.checker": "switch",
     },
     "neighbourhoods": {
         "input_data": "neighbourhoods_mview",
@@ -268,6 +274,9 @@ COMPONENT_ALT_KW_MAP = {
 COMPONENT_KW_TYPE_ALIAS_MAP = {
     "neighbour_neighbours": "neighbours",
     "neighbour_getter_other": "neighbours_getter",
+    # the reference's map spells this key "neighbour_getter_other" (recipes.py:388-392),
+    # so a string-named getter for the predictor's other-getter slot is not found there
+    "neighbours_getter_other": "neighbours_getter",
     "priority_queue_tree": "priority_queue",
 }
 
@@ -316,6 +325,10 @@ COMPONENT_NAME_TYPE_MAP = {
         "gpu": _fit.FitterGPU,
     },
     "hierarchical_fitter": {},
+    "predictor": {
+        "firstmatch": _fit.PredictorGPU,
+        "gpu": _fit.PredictorGPU,
+    },
     "priority_queue": {},
 }
 
@@ -331,7 +344,7 @@ UNSUPPORTED_COMPONENTS = {
     "distance_getter": {"lookup": "DistanceGetterExtLookup (precomputed distances) is not implemented"},
     "fitter": {"bfs_debug": "FitterExtBFSDebug (step-by-step debug generator) is not implemented"},
     "hierarchical_fitter": {
-        "repeat": "HierarchicalFitterRepeat is not implemented; use commonnnclustering_amd.batch.scan",
+        "repeat": "HierarchicalFitterRepeat is not implemented; use commonnnclustering_amd.engine.fit_scan",
         "mst": "HierarchicalFitterMSTPrim is not implemented",
     },
     "priority_queue": {"maxheap": "priority queues (MST hierarchy) are not implemented"},

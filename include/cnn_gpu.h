@@ -208,6 +208,25 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
                  int32_t n_c, int32_t precision, int64_t member_cutoff, int64_t max_clusters,
                  int64_t *labels, cnn_graph_stats *gstats, cnn_label_stats *lstats);
 
+/* ------------------------------------------------------------------ predict */
+/* Label prediction for a second point set, replacing PredictorFirstmatch.predict
+ * (_fit.pyx:1241-1291) as called by Clustering.predict (cluster.pyx:560-653)
+ * with brute-force neighbour getters (_get / _get_other, _types.pyx:1692-1731).
+ * X: reference points (n_x, d) with labels_x (device int64[n_x], caller order);
+ * Y: points to label (n_y, d), same dtype as X.  allowed: device uint8 table
+ * indexed by label (n_allowed entries; labels outside it are not allowed), the
+ * reference's consider_set.  For every y with consider_y != 0 and more than
+ * m_core reference points within r, the reference neighbours are visited in
+ * ascending index; the first with an allowed label, more than m_core reference
+ * neighbours of its own and at least c_sim reference neighbours shared with y
+ * gives y its label and clears consider_y.  Other entries are left unchanged.
+ * r2 = r*r, c_sim = c + 2, m_core = c + 1 as make_parameters (_fit.pyx:186-216).
+ * tests_out (nullable, host): similarity tests executed. */
+int cnn_predict(cnn_ctx *ctx, const void *X, int64_t n_x, const void *Y, int64_t n_y, int32_t d,
+                int32_t dtype, double r2, int32_t precision, int32_t c_sim, int32_t m_core,
+                const int64_t *labels_x, const uint8_t *allowed, int64_t n_allowed,
+                int64_t *labels_y, uint8_t *consider_y, int64_t *tests_out);
+
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
 

@@ -21,6 +21,8 @@ default ``"coordinates"`` recipe:
   fitter     _fit.pyx:408-491 (``FitterExtBFS._fit``)
   sort       _types.pyx:212-286 (``Labels.sort_by_size``)
   wrapper    cluster.pyx:256-392 (``Clustering.fit``), report.py:91-191 (``Record``)
+  predict    _fit.pyx:1241-1291 (``PredictorFirstmatch.predict``), _types.pyx:1712-1731
+             (``_get_other``), cluster.pyx:560-653 (``Clustering.predict``)
 
 How each part is pinned (tests/test_oracle.py):
   * metric: bit-identical to a literal scalar Python loop (IEEE double, the
@@ -35,6 +37,10 @@ How each part is pinned (tests/test_oracle.py):
     scikit-learn tutorial records (scikit_learn_datasets.ipynb cell 10 output).
   * sort_by_size: test/unit/test_types.py:98-117 known answers; tie order
     checked against ``collections.Counter.most_common`` semantics directly.
+  * predict: the reference's own integration case
+    (test/integration/test_procedures.py:246-277: half of a two-moons set
+    labelled by the generator predicts the whole set's generator labels) and
+    hand-built cases whose first match is fixed by construction.
 """
 
 from __future__ import annotations
@@ -391,6 +397,72 @@ def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
         sort_by_size_np(labels, member_cutoff, max_clusters)
     rec = make_record(labels, n, radius_cutoff, cnn_cutoff, member_cutoff, max_clusters)
     return labels, rec
+
+
+def neighbourhoods_other(X: np.ndarray, Y: np.ndarray, radius_cutoff: float, chunk: int = 1024) -> list:
+    """For each y of Y: {i of X : s(y, i) <= r*r}, ascending i (_types.pyx:1712-1731).
+
+    ``_calc_distance_other`` (_types.pyx:2269-2284) subtracts in the order
+    (y_k - x_k); the square of a difference does not depend on the order.
+    """
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    r2 = adjust_radius(radius_cutoff)
+    out = []
+    for s in range(0, Y.shape[0], chunk):
+        D = reduced_distances(Y[s:s + chunk], X)
+        for row in D:
+            out.append(np.nonzero(row <= r2)[0].astype(np.int64))
+    return out
+
+
+def predict_firstmatch(X, labels_x, Y, radius_cutoff, cnn_cutoff, clusters=None, labels_y=None,
+                       consider_y=None):
+    """Restates ``PredictorFirstmatch.predict`` (_fit.pyx:1241-1291) with
+    brute-force getters, as ``Clustering.predict`` calls it (cluster.pyx:600-630):
+    parameters from make_parameters (r -> r*r, similarity c+2, members c+1),
+    ``clusters`` defaults to the reference labels without 0.
+
+    For each y with consider != 0: its reference neighbours must be more than
+    n_member_cutoff; they are visited in ascending index, skipping labels not in
+    the set and members whose own neighbourhood is not larger than
+    n_member_cutoff; the first member passing the similarity check gives y its
+    label and clears y's consider flag.  Returns (labels_y, consider_y).
+    """
+    X = prepare_points(X)
+    Y = prepare_points(Y)
+    labels_x = np.asarray(labels_x, dtype=np.int64)
+    params = make_parameters(radius_cutoff, cnn_cutoff, 0)
+    if clusters is None:
+        clusters = set(int(v) for v in labels_x) - {0}
+    clusters = set(int(v) for v in clusters)
+    n_y = Y.shape[0]
+    labels_y = np.zeros(n_y, dtype=np.int64) if labels_y is None else np.array(labels_y, dtype=np.int64)
+    consider_y = np.ones(n_y, dtype=np.uint8) if consider_y is None else np.array(consider_y, dtype=np.uint8)
+    m = params["n_member_cutoff"]
+    c = params["similarity_cutoff"]
+    if X.shape[0] == 0 or n_y == 0:
+        return labels_y, consider_y
+    own = neighbourhoods(X, radius_cutoff)
+    other = neighbourhoods_other(X, Y, radius_cutoff)
+    for point in range(n_y):
+        if consider_y[point] == 0:
+            continue
+        na = other[point]
+        if not len(na) > m:
+            continue
+        for member in na:
+            label = int(labels_x[member])
+            if label not in clusters:
+                continue
+            nb = own[member]
+            if not len(nb) > m:
+                continue
+            if check_similarity(na, nb, c):
+                consider_y[point] = 0
+                labels_y[point] = label
+                break
+    return labels_y, consider_y
 
 
 def gamma(m: int, u: float) -> float:

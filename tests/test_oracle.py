@@ -223,3 +223,54 @@ def test_c_oracle_neighbourhoods_match_numpy(d, use_grid):
     rp, col = O.neighbourhoods_csr(X, 0.5)
     rp2, col2 = C.neighbours_csr(X, 0.5, use_grid=use_grid)
     assert (rp == rp2).all() and (col == col2).all()
+
+
+# ------------------------------------------------------------------ predict
+
+def test_predict_moons_reference_case():
+    """test_procedures.py:246-277: every second point of a two-moons set,
+    labelled with the generator's labels, predicts the generator's labels of
+    the whole set at r=0.2, c=2."""
+    from sklearn import datasets
+    from sklearn.preprocessing import StandardScaler
+
+    X, truth = datasets.make_moons(n_samples=1500, random_state=7)
+    X = StandardScaler().fit_transform(X)
+    truth = truth + 1
+    lab_y, con_y = O.predict_firstmatch(X[::2], truth[::2], X, 0.2, 2)
+    assert (equalise_labels(lab_y) == equalise_labels(truth)).all()
+    assert (con_y == 0).all()
+
+
+def _predict_toy():
+    # reference points: 0 at (-0.95, 0) and 11 at (-1.9, 0) (label 3), 1-5 a tight
+    # group at x = 0 (label 1), 6-10 a tight group at x = 0.3 (label 2); r = 1.
+    # Neighbourhoods: N(0) = {0..5, 11}, N(1..5) = {0..10}, N(6..10) = {1..10},
+    # N(11) = {0, 11}; y = (0.04, 0) sees {0..10}.
+    X = np.array([[-0.95, 0.0]] + [[0.0, 0.01 * k] for k in range(5)]
+                 + [[0.3, 0.01 * k] for k in range(5)] + [[-1.9, 0.0]])
+    labels = np.array([3] + [1] * 5 + [2] * 5 + [3])
+    return X, labels
+
+
+def test_predict_first_match_rules():
+    X, labels = _predict_toy()
+    y = np.array([[0.04, 0.0], [0.04, 0.0], [5.0, 5.0]])
+    # c = 3 (more than 4 neighbours, 5 shared): member 0 has 7 and shares 6 -> label 3
+    lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 3)
+    assert lab.tolist() == [3] and con.tolist() == [0]
+    # c = 5 (more than 6, 7 shared): member 0 has 7 but shares 6 -> member 1 gives 1
+    lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 5)
+    assert lab.tolist() == [1]
+    # c = 6 (more than 7): member 0 is skipped for its size -> member 1 gives 1
+    lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 6)
+    assert lab.tolist() == [1]
+    # only label 2 allowed: member 6, the first of the second group
+    lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 3, clusters={2})
+    assert lab.tolist() == [2]
+    # not considered: untouched; far point: no neighbours, untouched
+    lab, con = O.predict_firstmatch(X, labels, y[1:], 1.0, 3, labels_y=[7, 0], consider_y=[0, 1])
+    assert lab.tolist() == [7, 0] and con.tolist() == [0, 1]
+    # c = 10: y needs more than 11 neighbours -> nothing assigned
+    lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 10)
+    assert lab.tolist() == [0] and con.tolist() == [1]
