@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_objs")
 LIB = os.path.join(HERE, "libcnn_hip.so")
-SOURCES = ["cnn_util.hip", "cnn_radius.hip", "cnn_labels.hip", "cnn_batch.hip", "cnn_predict.hip"]
+SOURCES = ["cnn_util.hip", "cnn_radius.hip", "cnn_labels.hip", "cnn_batch.hip", "cnn_predict.hip", "cnn_metrics.hip"]
 
 # -ffp-contract=off: FP64 distance code must keep sub/mul/add separately
 # rounded (reference order); FP32 code requests FMA explicitly.
@@ -47,7 +47,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     """Compile every HIP source and link libcnn_hip.so; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 5)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 6)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]

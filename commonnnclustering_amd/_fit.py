@@ -121,6 +121,12 @@ class FitterGPU(Fitter):
                                            current_start=cluster_params.current_start, labels=lab,
                                            sim_mode=self._sim_mode())
             self.last_graph_stats = None
+        elif self._neighbours_getter.precomputed:
+            row_ptr, col = self._neighbours_getter.get_precomputed_csr(input_data, cluster_params)
+            _, st = engine.labels_from_csr(row_ptr, col, c_sim, m_core, consider=con,
+                                           current_start=cluster_params.current_start, labels=lab,
+                                           sim_mode=self._sim_mode())
+            self.last_graph_stats = None
         else:
             graph = self._neighbours_getter.get_graph(input_data, cluster_params)
             try:
@@ -153,10 +159,18 @@ class FitterGPU(Fitter):
             engine.sort_by_size(lab, member_cutoff, max_clusters)
             self.last_graph_stats, self.last_label_stats = None, st
             return lab.cpu().numpy()
-        metric = self._neighbours_getter._distance_getter._metric
-        if type(metric).__name__ != "MetricEuclideanReduced":
-            raise NotImplementedError(f"metric {type(metric).__name__} has no GPU kernel")
-        lab, _, gs, ls = engine.fit(input_data.device_tensor(), cluster_params.radius_cutoff,
+        if self._neighbours_getter.precomputed:
+            import torch
+
+            row_ptr, col = self._neighbours_getter.get_precomputed_csr(input_data, cluster_params)
+            lab = torch.zeros(input_data.n_points, dtype=torch.int64, device=row_ptr.device)
+            _, st = engine.labels_from_csr(row_ptr, col, cluster_params.similarity_cutoff,
+                                           cluster_params.n_member_cutoff, current_start=1, labels=lab,
+                                           sim_mode=self._sim_mode())
+            engine.sort_by_size(lab, member_cutoff, max_clusters)
+            self.last_graph_stats, self.last_label_stats = None, st
+            return lab.cpu().numpy()
+        lab, _, gs, ls = engine.fit(input_data.device_tensor(), self._neighbours_getter.squared_threshold(cluster_params),
                                     cluster_params.similarity_cutoff, cluster_params.n_member_cutoff,
                                     precision=self._neighbours_getter.precision, sort=True,
                                     member_cutoff=member_cutoff, max_clusters=max_clusters)
@@ -238,7 +252,7 @@ class PredictorGPU(Predictor):
         lab_y = torch.from_numpy(np.ascontiguousarray(predictand_labels._labels, dtype=np.int64)).to(dev)
         con_y = torch.from_numpy(np.ascontiguousarray(predictand_labels._consider, dtype=np.uint8)).to(dev)
         precision = getattr(self._neighbours_getter, "precision", "f64")
-        _, _, tests = engine.predict(X, lab_x, Y.to(dev), cluster_params.radius_cutoff,
+        _, _, tests = engine.predict(X, lab_x, Y.to(dev), self._neighbours_getter.squared_threshold(cluster_params),
                                      cluster_params.similarity_cutoff, cluster_params.n_member_cutoff,
                                      predictand_labels.consider_set, labels_y=lab_y, consider_y=con_y,
                                      precision=precision)

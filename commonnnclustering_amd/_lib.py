@@ -43,7 +43,7 @@ EXPORTED = [
     "cnn_graph_build", "cnn_graph_destroy", "cnn_graph_nnz", "cnn_graph_stats_get",
     "cnn_graph_bytes", "cnn_graph_export_csr", "cnn_radius_csr", "cnn_graph_rethreshold",
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
-    "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_version",
+    "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_version",
 ]
 
 
@@ -139,6 +139,8 @@ def _declare(L):
                              ctypes.POINTER(GraphStats), _vp]),
         "cnn_predict": (c, [_vp, _vp, _i64, _vp, _i64, _i32, _i32, ctypes.c_double, _i32, _i32, _i32, _vp, _vp,
                             _i64, _vp, _vp, ctypes.POINTER(_i64)]),
+        "cnn_precomputed_csr": (c, [_vp, _vp, _i64, _i32, ctypes.c_double, _vp, _vp, _i64,
+                                    ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
         "cnn_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

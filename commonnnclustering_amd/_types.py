@@ -335,11 +335,20 @@ InputDataExtNeighbourhoodsMemoryview = InputDataNeighbourhoods
 # ---------------------------------------------------------------- metric
 
 class Metric(ABC):
-    """Metric interface (_types.pyx:654-716)."""
+    """Metric interface (_types.pyx:654-716).
+
+    ``squared_threshold(adjusted_radius)`` is this build's addition: the
+    threshold T on the reduced sum s_ij (sequential FP64 sum of rounded squares
+    of rounded differences) such that the metric's decision equals s_ij <= T,
+    or None when the metric has no such form.
+    """
 
     @abstractmethod
     def adjust_radius(self, radius_cutoff: float) -> float:
         ...
+
+    def squared_threshold(self, adjusted_radius: float):
+        return None
 
 
 class MetricEuclideanReduced(Metric):
@@ -353,8 +362,44 @@ class MetricEuclideanReduced(Metric):
         r = float(radius_cutoff)
         return r * r
 
+    def squared_threshold(self, adjusted_radius: float):
+        return float(adjusted_radius)
+
     def __str__(self):
         return type(self).__name__
+
+
+class MetricEuclidean(Metric):
+    """Euclidean distance sqrt(s) (MetricExtEuclidean, _types.pyx:2170-2207),
+    radius used as given.  fl(sqrt(s)) <= r is decided exactly as s <= T with T
+    the largest double whose correctly rounded square root is <= r, so the
+    reduced-metric kernels serve it (both precision forms)."""
+
+    def adjust_radius(self, radius_cutoff: float) -> float:
+        return float(radius_cutoff)
+
+    def squared_threshold(self, adjusted_radius: float):
+        from .engine import sqrt_decision_threshold
+
+        return sqrt_decision_threshold(adjusted_radius)
+
+    def __str__(self):
+        return type(self).__name__
+
+
+class MetricPrecomputed(Metric):
+    """Distances read from an (n, n) matrix given as the input data
+    (MetricExtPrecomputed, _types.pyx:2105-2127), radius used as given."""
+
+    def adjust_radius(self, radius_cutoff: float) -> float:
+        return float(radius_cutoff)
+
+    def __str__(self):
+        return type(self).__name__
+
+
+MetricExtEuclidean = MetricEuclidean
+MetricExtPrecomputed = MetricPrecomputed
 
 
 MetricExtEuclideanReduced = MetricEuclideanReduced
@@ -410,13 +455,34 @@ class NeighboursGetterGPU(NeighboursGetter):
     def get_builder_kwargs(cls):
         return [("distance_getter", None)]
 
+    def squared_threshold(self, cluster_params: ClusterParameters) -> float:
+        """Threshold on the reduced sum for the kernels (r*r for the reduced
+        metric); NotImplementedError for metrics without that form."""
+        metric = self._distance_getter._metric
+        thr = metric.squared_threshold(cluster_params.radius_cutoff)
+        if thr is None:
+            raise NotImplementedError(f"metric {type(metric).__name__} has no reduced-sum form")
+        return thr
+
+    @property
+    def precomputed(self) -> bool:
+        return isinstance(self._distance_getter._metric, MetricPrecomputed)
+
     def get_graph(self, input_data: InputDataComponents, cluster_params: ClusterParameters):
         from . import engine
 
-        metric = self._distance_getter._metric
-        if not isinstance(metric, MetricEuclideanReduced):
-            raise NotImplementedError(f"metric {type(metric).__name__} has no GPU kernel")
-        return engine.build_graph(input_data.device_tensor(), cluster_params.radius_cutoff, self.precision)
+        return engine.build_graph(input_data.device_tensor(), self.squared_threshold(cluster_params),
+                                  self.precision)
+
+    def get_precomputed_csr(self, input_data: InputDataComponents, cluster_params: ClusterParameters):
+        """(row_ptr, col) from a distance matrix given as the input data."""
+        from . import engine
+
+        row_ptr, col, asym = engine.precomputed_csr(input_data.device_tensor(), cluster_params.radius_cutoff)
+        if asym:
+            raise NotImplementedError(f"{asym} point pairs of the distance matrix are neighbours in one "
+                                      "direction only; the labelling needs a symmetric neighbour relation")
+        return row_ptr, col
 
     def __str__(self):
         return (f"{type(self).__name__}(dgetter={self._distance_getter}, precision={self.precision}, "

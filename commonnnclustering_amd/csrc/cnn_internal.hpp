@@ -279,6 +279,23 @@ __device__ __forceinline__ int atomic_load_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bit transpose across the wave: lane j receives the word whose bit i is bit j
+// of lane i's input.  Six butterfly stages swap the off-diagonal blocks of
+// width 32, 16, ..., 1 between partner lanes (lane ^ s).
+__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int sft = 32 >> t;
+        const uint64_t m = masks[t];
+        const uint64_t y = __shfl_xor(x, sft, 64);
+        x = (lane & sft) ? ((x & ~m) | ((y >> sft) & m)) : ((x & m) | ((y & m) << sft));
+    }
+    return x;
+}
+
 // |A & B| >= c for sorted, duplicate-free lists, evaluated by the whole wave:
 // 64-wide windows of both lists advance like the two-pointer merge; each lane
 // looks its B element up in the A window by a shuffle binary search.  Every

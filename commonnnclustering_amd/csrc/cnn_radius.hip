@@ -440,23 +440,6 @@ __device__ __forceinline__ int64_t first_pair(const DistArgs &a, int64_t R, int6
     return lo;
 }
 
-// Bit transpose across the wave: lane j receives the word whose bit i is bit j
-// of lane i's input.  Six butterfly stages swap the off-diagonal blocks of
-// width 32, 16, ..., 1 between partner lanes (lane ^ s).
-__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int sft = 32 >> t;
-        const uint64_t m = masks[t];
-        const uint64_t y = __shfl_xor(x, sft, 64);
-        x = (lane & sft) ? ((x & ~m) | ((y >> sft) & m)) : ((x & m) | ((y & m) << sft));
-    }
-    return x;
-}
-
 // Store a row word (and, for an off-diagonal symmetric pair, the column word
 // of the mirror pair).  Returns the row word's popcount.  With per-pair flags,
 // the words of a pair without any bit are not stored: readers skip flagged-

@@ -26,6 +26,8 @@ __all__ = [
     "fit_batch",
     "fit_scan",
     "predict",
+    "precomputed_csr",
+    "sqrt_decision_threshold",
     "to_device_points",
 ]
 
@@ -339,3 +341,52 @@ def predict(X: torch.Tensor, labels_x: torch.Tensor, Y: torch.Tensor, r2: float,
                                      ptr(allowed), n_allowed, ptr(labels_y), ptr(consider_y),
                                      ctypes.byref(tests)), "cnn_predict")
     return labels_y, consider_y, int(tests.value)
+
+
+def precomputed_csr(D: torch.Tensor, r: float):
+    """Neighbour lists from a precomputed (n, n) distance matrix on the device:
+    N(i) = {j : D[i, j] <= r} (MetricExtPrecomputed under the brute-force
+    getter, _types.pyx:1692-1710, 2105-2127).  Returns (row_ptr int64[n+1],
+    col int32[nnz], number of pairs {i, j} whose two decisions differ)."""
+    n, n2, dt = _check_points(D)
+    if n != n2:
+        raise ValueError(f"a distance matrix must be square, got {tuple(D.shape)}")
+    ctx = get_context(D.device)
+    row_ptr = torch.empty(n + 1, dtype=torch.int64, device=D.device)
+    nnz, asym = ctypes.c_int64(0), ctypes.c_int64(0)
+    L = _lib.lib()
+    rc = L.cnn_precomputed_csr(ctx.handle, ptr(D), n, dt, float(r), ptr(row_ptr), None, 0,
+                               ctypes.byref(nnz), ctypes.byref(asym))
+    if rc == _lib.CNN_ERR_CAPACITY and nnz.value > 0:
+        col = torch.empty(nnz.value, dtype=torch.int32, device=D.device)
+        rc = L.cnn_precomputed_csr(ctx.handle, ptr(D), n, dt, float(r), ptr(row_ptr), ptr(col), nnz.value,
+                                   ctypes.byref(nnz), ctypes.byref(asym))
+    else:
+        col = torch.empty(0, dtype=torch.int32, device=D.device)
+    ctx.check(rc, "cnn_precomputed_csr")
+    return row_ptr, col, int(asym.value)
+
+
+def sqrt_decision_threshold(r: float) -> float:
+    """Largest double s with fl(sqrt(s)) <= r, for IEEE (correctly rounded)
+    sqrt.  The non-reduced Euclidean metric decides sqrt(s_ij) <= r
+    (MetricExtEuclidean, _types.pyx:2170-2207); since fl(sqrt(.)) is monotone,
+    that holds exactly when s_ij <= this threshold, so the reduced-metric
+    kernels (which evaluate s_ij in the reference's order) decide it exactly.
+    Negative or NaN r: -1 (no sum qualifies)."""
+    import math
+
+    r = float(r)
+    if r != r or r < 0.0:
+        return -1.0
+    if r == math.inf:
+        return math.inf
+    s = r * r
+    while s > 0.0 and math.sqrt(s) > r:
+        s = math.nextafter(s, -math.inf)
+    while True:
+        t = math.nextafter(s, math.inf)
+        if t != math.inf and math.sqrt(t) <= r:
+            s = t
+        else:
+            return s

@@ -225,6 +225,17 @@ REGISTERED_RECIPES = {
         "predictor.ngetter": ("brute_force", (), {"precision": "f32_band"}),
         "predictor.checker": "switch",
     },
+    "distances": {
+        "input_data": "components_mview",
+        "preparation_hook": "points_from_parts",
+        "fitter": "bfs",
+        "fitter.ngetter": "brute_force",
+        "fitter.na": "vuset",
+        "fitter.checker": "switch",
+        "fitter.queue": "fifo",
+        "fitter.ngetter.dgetter": "metric",
+        "fitter.ngetter.dgetter.metric": "precomputed",
+    },
     "neighbourhoods": {
         "input_data": "neighbourhoods_mview",
         "preparation_hook": "neighbourhoods",
@@ -309,6 +320,8 @@ COMPONENT_NAME_TYPE_MAP = {
     "metric": {
         "euclidean_r": _types.MetricEuclideanReduced,
         "euclidean_reduced": _types.MetricEuclideanReduced,
+        "euclidean": _types.MetricEuclidean,
+        "precomputed": _types.MetricPrecomputed,
     },
     "similarity_checker": {
         "contains": _types.SimilarityCheckerGPU,
@@ -335,8 +348,6 @@ COMPONENT_NAME_TYPE_MAP = {
 # Reference components without a GPU implementation in this engine, with the reason.
 UNSUPPORTED_COMPONENTS = {
     "metric": {
-        "precomputed": "the precomputed-distance metric (MetricExtPrecomputed) is not implemented on the GPU",
-        "euclidean": "the sqrt Euclidean metric (MetricExtEuclidean) is not implemented; use euclidean_r",
         "euclidean_periodic_r": "the periodic metric (MetricExtEuclideanPeriodicReduced) is not implemented",
         "euclidean_periodic_reduced": "the periodic metric is not implemented",
         "dummy": "MetricExtDummy is not implemented",

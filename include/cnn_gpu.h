@@ -227,6 +227,24 @@ int cnn_predict(cnn_ctx *ctx, const void *X, int64_t n_x, const void *Y, int64_t
                 const int64_t *labels_x, const uint8_t *allowed, int64_t n_allowed,
                 int64_t *labels_y, uint8_t *consider_y, int64_t *tests_out);
 
+/* ------------------------------------------------------------------ other metrics */
+/* Neighbour lists from a precomputed distance matrix, replacing
+ * NeighboursGetterExtBruteForce._get (_types.pyx:1692-1710) over
+ * MetricExtPrecomputed (_types.pyx:2105-2127): N(i) = { j : D[i, j] <= r },
+ * ascending j, r used as given.  D: device (n, n) row-major, CNN_F32 / CNN_F64
+ * (compared in FP64).  row_ptr: device int64[n+1]; col: device int32[col_cap].
+ * *nnz_out receives the entry count; when col_cap < nnz, CNN_ERR_CAPACITY is
+ * returned with row_ptr complete and col untouched.  *asymmetric_out (host)
+ * counts pairs {i, j} whose decisions for (i, j) and (j, i) differ: the labelling
+ * entry points treat lists as a symmetric relation, which the reference's BFS
+ * matches only when this count is 0.  (The non-reduced Euclidean metric needs
+ * no entry point of its own: fl(sqrt(s)) <= r is decided as s <= T with T the
+ * largest double whose correctly rounded square root is <= r, so the reduced-
+ * metric kernels take T in place of r*r.) */
+int cnn_precomputed_csr(cnn_ctx *ctx, const void *D, int64_t n, int32_t dtype, double r,
+                        int64_t *row_ptr, int32_t *col, int64_t col_cap, int64_t *nnz_out,
+                        int64_t *asymmetric_out);
+
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
 

@@ -21,6 +21,7 @@ default ``"coordinates"`` recipe:
   fitter     _fit.pyx:408-491 (``FitterExtBFS._fit``)
   sort       _types.pyx:212-286 (``Labels.sort_by_size``)
   wrapper    cluster.pyx:256-392 (``Clustering.fit``), report.py:91-191 (``Record``)
+  metrics    _types.pyx:2105-2127 (``MetricExtPrecomputed``), :2170-2207 (``MetricExtEuclidean``)
   predict    _fit.pyx:1241-1291 (``PredictorFirstmatch.predict``), _types.pyx:1712-1731
              (``_get_other``), cluster.pyx:560-653 (``Clustering.predict``)
 
@@ -37,6 +38,10 @@ How each part is pinned (tests/test_oracle.py):
     scikit-learn tutorial records (scikit_learn_datasets.ipynb cell 10 output).
   * sort_by_size: test/unit/test_types.py:98-117 known answers; tie order
     checked against ``collections.Counter.most_common`` semantics directly.
+  * other metrics: the sqrt metric's lists equal a literal scalar loop with
+    math.sqrt (IEEE, correctly rounded) and include pairs whose decision differs
+    from the reduced metric's at r*r; the precomputed metric's lists equal the
+    reduced metric's when fed the reduced distance matrix with r*r.
   * predict: the reference's own integration case
     (test/integration/test_procedures.py:246-277: half of a two-moons set
     labelled by the generator predicts the whole set's generator labels) and
@@ -161,9 +166,37 @@ def neighbourhoods(X: np.ndarray, radius_cutoff: float, chunk: int = 1024) -> li
     return out
 
 
-def neighbourhoods_csr(X: np.ndarray, radius_cutoff: float, chunk: int = 1024):
+def neighbourhoods_metric(X: np.ndarray, radius_cutoff: float, metric: str = "euclidean_r",
+                          chunk: int = 1024) -> list:
+    """Brute-force neighbourhoods under the reference's other metrics:
+
+    ``euclidean``    MetricExtEuclidean (_types.pyx:2170-2207): sqrt of the
+                     reduced sum (IEEE sqrt) compared with r as given;
+    ``precomputed``  MetricExtPrecomputed (_types.pyx:2105-2127): X is the
+                     (n, n) distance matrix, D[i, j] compared with r as given;
+    ``euclidean_r``  the reduced metric (``neighbourhoods``).
+    """
+    if metric == "euclidean_r":
+        return neighbourhoods(X, radius_cutoff, chunk)
+    X = np.asarray(X, dtype=np.float64)
+    r = float(radius_cutoff)
+    out = []
+    if metric == "precomputed":
+        for row in X:
+            out.append(np.nonzero(row <= r)[0].astype(np.int64))
+        return out
+    if metric != "euclidean":
+        raise ValueError(f"unknown metric {metric!r}")
+    for s in range(0, X.shape[0], chunk):
+        D = np.sqrt(reduced_distances(X[s:s + chunk], X))
+        for row in D:
+            out.append(np.nonzero(row <= r)[0].astype(np.int64))
+    return out
+
+
+def neighbourhoods_csr(X: np.ndarray, radius_cutoff: float, chunk: int = 1024, metric: str = "euclidean_r"):
     """The same neighbourhoods as CSR: (row_ptr int64[n+1], col int64[nnz])."""
-    lists = neighbourhoods(X, radius_cutoff, chunk)
+    lists = neighbourhoods_metric(X, radius_cutoff, metric, chunk)
     deg = np.array([len(x) for x in lists], dtype=np.int64)
     row_ptr = np.zeros(len(lists) + 1, dtype=np.int64)
     np.cumsum(deg, out=row_ptr[1:])
@@ -373,7 +406,7 @@ def make_record(labels: np.ndarray, n_points: int, radius_cutoff, cnn_cutoff,
 
 
 def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
-        cnn_offset=None, sort=True, method="components"):
+        cnn_offset=None, sort=True, method="components", metric="euclidean_r"):
     """End-to-end restatement of ``Clustering.fit`` (cluster.pyx:256-392).
 
     Returns ``(labels int64[n], record dict)``.  ``method`` selects the
@@ -389,9 +422,9 @@ def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
     consider = np.ones(n, dtype=np.uint8)
     params = make_parameters(radius_cutoff, cnn_cutoff, 1)
     if method == "bfs":
-        fit_bfs(neighbourhoods(X, radius_cutoff), params, labels, consider)
+        fit_bfs(neighbourhoods_metric(X, radius_cutoff, metric), params, labels, consider)
     else:
-        row_ptr, col = neighbourhoods_csr(X, radius_cutoff)
+        row_ptr, col = neighbourhoods_csr(X, radius_cutoff, metric=metric)
         fit_components(row_ptr, col, params, labels, consider)
     if sort:
         sort_by_size_np(labels, member_cutoff, max_clusters)

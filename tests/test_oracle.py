@@ -274,3 +274,69 @@ def test_predict_first_match_rules():
     # c = 10: y needs more than 11 neighbours -> nothing assigned
     lab, con = O.predict_firstmatch(X, labels, y[:1], 1.0, 10)
     assert lab.tolist() == [0] and con.tolist() == [1]
+
+
+# ------------------------------------------------------------------ other metrics
+
+def _sqrt_threshold(r):
+    from commonnnclustering_amd.engine import sqrt_decision_threshold
+    return sqrt_decision_threshold(r)
+
+
+def test_sqrt_threshold_exact():
+    """fl(sqrt(s)) <= r  <=>  s <= T(r) on every double in a window around r*r."""
+    import math
+    rng = np.random.default_rng(0)
+    radii = [0.0, 1.0, 0.1, 0.3, 2.5, 1e-160, 1e150, 3.0000000000000004] + list(rng.uniform(0, 10, 40))
+    for r in radii:
+        T = _sqrt_threshold(r)
+        s = r * r
+        for _ in range(300):
+            s = math.nextafter(s, -math.inf)
+        for _ in range(600):
+            if s >= 0.0:  # sums of squares are never negative
+                assert (math.sqrt(s) <= r) == (s <= T), (r, s, T)
+            s = math.nextafter(s, math.inf)
+    assert _sqrt_threshold(-1.0) < 0 and _sqrt_threshold(float("nan")) < 0
+    assert _sqrt_threshold(float("inf")) == float("inf")
+
+
+def test_sqrt_metric_equals_literal_loop():
+    import math
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(120, 3))
+    r = 0.9
+    got = O.neighbourhoods_metric(X, r, "euclidean")
+    for i in range(len(X)):
+        want = [j for j in range(len(X)) if math.sqrt(O.reduced_distance_scalar(X[i], X[j])) <= r]
+        assert got[i].tolist() == want
+
+
+def test_sqrt_metric_boundary_differs_from_reduced():
+    """Pairs with s just above fl(r*r) whose square root still rounds to r are
+    neighbours under the sqrt metric only."""
+    import math
+    r = 0.7
+    T = _sqrt_threshold(r)
+    assert T > r * r
+    # points on the circle of radius r around the origin: some rounded sums land in (r*r, T]
+    found = 0
+    for theta in np.linspace(0.01, 1.5, 4000):
+        p = np.array([r * math.cos(theta), r * math.sin(theta)])
+        s = O.reduced_distance_scalar(np.zeros(2), p)
+        if r * r < s <= T:
+            X = np.array([[0.0, 0.0], p])
+            assert O.neighbourhoods_metric(X, r, "euclidean")[0].tolist() == [0, 1]
+            assert O.neighbourhoods(X, r)[0].tolist() == [0]
+            found += 1
+    assert found > 10
+
+
+def test_precomputed_metric_equals_reduced():
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(200, 2))
+    r = 0.5
+    D = O.reduced_distances(X, X)
+    a = O.neighbourhoods_metric(D, r * r, "precomputed")
+    b = O.neighbourhoods(X, r)
+    assert all((x == y).all() for x, y in zip(a, b))
