@@ -170,6 +170,15 @@ class FitterGPU(Fitter):
             engine.sort_by_size(lab, member_cutoff, max_clusters)
             self.last_graph_stats, self.last_label_stats = None, st
             return lab.cpu().numpy()
+        if self._neighbours_getter.periodic_bounds is not None:
+            graph = self._neighbours_getter.get_graph(input_data, cluster_params)
+            try:
+                lab, ls = graph.fit(cluster_params.similarity_cutoff, cluster_params.n_member_cutoff, sort=True,
+                                    member_cutoff=member_cutoff, max_clusters=max_clusters)
+                self.last_graph_stats, self.last_label_stats = graph.stats, ls
+            finally:
+                graph.release()
+            return lab.cpu().numpy()
         lab, _, gs, ls = engine.fit(input_data.device_tensor(), self._neighbours_getter.squared_threshold(cluster_params),
                                     cluster_params.similarity_cutoff, cluster_params.n_member_cutoff,
                                     precision=self._neighbours_getter.precision, sort=True,
@@ -245,6 +254,10 @@ class PredictorGPU(Predictor):
             raise NotImplementedError("GPU prediction needs point coordinates for both data sets")
         if predictand_labels.n_points != predictand_input_data.n_points:
             raise ValueError("predictand labels and input data differ in length")
+        if getattr(self._neighbours_getter, "periodic_bounds", None) is not None:
+            raise NotImplementedError("prediction under the periodic metric is not offered: the reference's "
+                                      "_calc_distance_other for it sums (a_k - b_k)^2 without the periodic "
+                                      "reduction (_types.pyx:2332-2343)")
         X = input_data.device_tensor()
         Y = predictand_input_data.device_tensor()
         dev = X.device

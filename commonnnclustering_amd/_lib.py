@@ -43,7 +43,8 @@ EXPORTED = [
     "cnn_graph_build", "cnn_graph_destroy", "cnn_graph_nnz", "cnn_graph_stats_get",
     "cnn_graph_bytes", "cnn_graph_export_csr", "cnn_radius_csr", "cnn_graph_rethreshold",
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
-    "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_version",
+    "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_graph_build_periodic",
+    "cnn_version",
 ]
 
 
@@ -141,6 +142,7 @@ def _declare(L):
                             _i64, _vp, _vp, ctypes.POINTER(_i64)]),
         "cnn_precomputed_csr": (c, [_vp, _vp, _i64, _i32, ctypes.c_double, _vp, _vp, _i64,
                                     ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "cnn_graph_build_periodic": (c, [_vp, _vp, _i64, _i32, _i32, ctypes.c_double, _vp, ctypes.POINTER(_vp)]),
         "cnn_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

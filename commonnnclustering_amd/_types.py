@@ -398,8 +398,33 @@ class MetricPrecomputed(Metric):
         return type(self).__name__
 
 
+class MetricEuclideanPeriodicReduced(Metric):
+    """Squared Euclidean distance with periodic boundaries
+    (MetricExtEuclideanPeriodicReduced, _types.pyx:2290-2348): per dimension
+    the difference is reduced modulo its period (``bounds[k] > 0``) and to the
+    nearer image.  Radius squared as for the reduced metric."""
+
+    def __init__(self, bounds):
+        self._bounds = np.ascontiguousarray(bounds, dtype=np.float64).reshape(-1)
+
+    @property
+    def bounds(self):
+        return self._bounds
+
+    def adjust_radius(self, radius_cutoff: float) -> float:
+        r = float(radius_cutoff)
+        return r * r
+
+    def squared_threshold(self, adjusted_radius: float):
+        return float(adjusted_radius)
+
+    def __str__(self):
+        return f"{type(self).__name__}(bounds={self._bounds.tolist()})"
+
+
 MetricExtEuclidean = MetricEuclidean
 MetricExtPrecomputed = MetricPrecomputed
+MetricExtEuclideanPeriodicReduced = MetricEuclideanPeriodicReduced
 
 
 MetricExtEuclideanReduced = MetricEuclideanReduced
@@ -468,11 +493,16 @@ class NeighboursGetterGPU(NeighboursGetter):
     def precomputed(self) -> bool:
         return isinstance(self._distance_getter._metric, MetricPrecomputed)
 
+    @property
+    def periodic_bounds(self):
+        metric = self._distance_getter._metric
+        return metric.bounds if isinstance(metric, MetricEuclideanPeriodicReduced) else None
+
     def get_graph(self, input_data: InputDataComponents, cluster_params: ClusterParameters):
         from . import engine
 
         return engine.build_graph(input_data.device_tensor(), self.squared_threshold(cluster_params),
-                                  self.precision)
+                                  self.precision, bounds=self.periodic_bounds)
 
     def get_precomputed_csr(self, input_data: InputDataComponents, cluster_params: ClusterParameters):
         """(row_ptr, col) from a distance matrix given as the input data."""

@@ -349,6 +349,7 @@ struct DistArgs {
     int64_t *redo;              // (k, R) per entry
     uint8_t *pair_nz;           // per pair of the batch: 1 if its words hold any bit (nullable)
     const uint8_t *pair_groups; // per pair (global index): row groups of R (16 rows each) in reach of C
+    const double *bounds;       // periodic metric: period per dimension (<= 0: not periodic); nullptr: none
 };
 
 __device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
@@ -525,6 +526,46 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
         uint64_t m = 0;
 #pragma unroll
         for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= r2) << jj;
+        mydeg += store_pair(a, k, R, i, C, m, false, 0);
+    }
+    finish_deg(a, i, mydeg);
+}
+
+// Plain FP64 with the periodic reduced metric (MetricExtEuclideanPeriodicReduced
+// ._calc_distance, _types.pyx:2296-2317): per dimension the absolute
+// difference, reduced modulo a positive period and replaced by period - x when
+// above half of it, squared and added in ascending k, each step rounded as
+// there (fmod is exact).  Dense layout, symmetric evaluation; the column
+// point's coordinates are wave-uniform loads.
+__global__ __launch_bounds__(256) void k_dist_f64_periodic(DistArgs a) {
+    const int64_t R = a.R0 + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = R * 64 + lane;
+    const int d = a.d;
+    int64_t kb, ke;
+    tile_range(a, R, kb, ke);
+    const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
+    const double r2 = a.bp.r2;
+    const double *__restrict__ xi = a.xw64 + i * d;
+    int mydeg = 0;
+    for (int64_t k = k0 + wave; k < ke; k += 4) {
+        const int64_t C = uniform64(tile_col(a, k, kb));
+        const double *__restrict__ xc = a.xw64 + C * 64 * d;
+        uint64_t m = 0;
+        for (int jj = 0; jj < 64; ++jj) {
+            const double *__restrict__ xj = xc + (int64_t)jj * d;
+            double total = 0.0;
+            for (int q = 0; q < d; ++q) {
+                double dist = fabs(xi[q] - xj[q]);
+                const double bound = a.bounds[q];
+                if (bound > 0.0) {
+                    dist = fmod(dist, bound);
+                    if (dist > bound / 2.0) dist = bound - dist;
+                }
+                total = total + dist * dist;
+            }
+            m |= (uint64_t)(total <= r2) << jj;
+        }
         mydeg += store_pair(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
@@ -1599,6 +1640,12 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
     CNN_HIP(hipEventRecord(ctx->ev[6], s));
+    if (a.bounds) {
+        k_dist_f64_periodic<<<grid, block, 0, s>>>(a);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipEventRecord(ctx->ev[7], s));
+        return;
+    }
     if (band && a.d >= kMfmaMinD) {
         // guard-band elements are listed (and, when the list is full, whole tile pairs,
         // at most once each) and settled afterwards
@@ -1715,6 +1762,7 @@ void graph_free(cnn_graph *g) {
     if (g->col) (void)hipFree(g->col);
     if (g->tp_ptr) (void)hipFree(g->tp_ptr);
     if (g->tp_col) (void)hipFree(g->tp_col);
+    if (g->bounds) (void)hipFree(g->bounds);
     delete g;
 }
 
@@ -1922,6 +1970,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.col_chunk = 32;
         a.pair_nz = pair_nz.p;
         a.pair_groups = gmask ? gmask->p : nullptr;
+        a.bounds = g->bounds;
         if (pk) {
             a.tile_r2 = pk->tile_r2;
             a.tile_lo = pk->tile_lo;
@@ -2016,8 +2065,9 @@ static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype,
 }
 
 cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
-                       int32_t precision) {
+                       int32_t precision, const double *bounds_host) {
     check_build_args(X, n, d, dtype, precision);
+    if (bounds_host) precision = CNN_PLAIN_F64;  // the periodic metric has the FP64 form only
     CNN_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     GraphGuard guard{new cnn_graph()};
@@ -2048,8 +2098,14 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
 
     // ordering: Morton order over the (up to) three widest dims for small d,
-    // nearest-pivot cells for larger d
-    g->layout = (d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
+    // nearest-pivot cells for larger d; the caller's order (all tile pairs)
+    // for the periodic metric, whose neighbours wrap around the box
+    if (bounds_host) {
+        g->bounds = dev_alloc<double>(g, d);
+        CNN_HIP(hipMemcpyAsync(g->bounds, bounds_host, d * sizeof(double), hipMemcpyHostToDevice, s));
+        CNN_HIP(hipStreamSynchronize(s));
+    }
+    g->layout = (!bounds_host && d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
     if (g->layout == 1) {
         const KeySpec ks = make_keyspec(d, lo, hi, 64);
         g->perm = dev_alloc<int32_t>(g, n);
@@ -2061,7 +2117,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
             k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
         CNN_HIP(hipGetLastError());
         sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
-    } else if (d > kSpatialMaxD && n >= kPivotMinN) {
+    } else if (!bounds_host && d > kSpatialMaxD && n >= kPivotMinN) {
         const int L = (int)std::min<int64_t>(256, n / 64);
         g->perm = dev_alloc<int32_t>(g, n);
         Scratch<uint64_t> keys(ctx, n);
@@ -2442,6 +2498,17 @@ int cnn_graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t d
     });
 }
 
+int cnn_graph_build_periodic(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
+                             const double *bounds, cnn_graph **out) {
+    if (!ctx || !out) return CNN_ERR_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&]() {
+        if (!bounds) throw InvalidArg{"null bounds"};
+        *out = graph_build(ctx, X, n, d, dtype, r2, CNN_PLAIN_F64, bounds);
+        return (int)CNN_OK;
+    });
+}
+
 int cnn_graph_destroy(cnn_graph *g) {
     if (!g) return CNN_ERR_INVALID;
     graph_free(g);
@@ -2499,6 +2566,7 @@ int cnn_graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t p
     if (!ctx || !g || !out) return CNN_ERR_INVALID;
     *out = nullptr;
     return guarded(ctx, [&]() {
+        if (g->bounds) throw InvalidArg{"a periodic-metric graph cannot be re-thresholded; build it again"};
         *out = graph_rethreshold(ctx, g, r2, precision);
         return (int)CNN_OK;
     });

@@ -173,13 +173,24 @@ def _check_labels(labels, n, consider):
             raise ValueError("consider must be a uint8 CUDA tensor of length n")
 
 
-def build_graph(X: torch.Tensor, r2: float, precision="f64") -> NeighbourGraph:
+def build_graph(X: torch.Tensor, r2: float, precision="f64", bounds=None) -> NeighbourGraph:
+    """Radius graph of X at squared radius r2.  ``bounds`` (one period per
+    dimension, <= 0 for a non-periodic one) selects the periodic reduced
+    metric, which has the plain FP64 form only."""
     n, d, dt = _check_points(X)
     p = _lib.precision_code(precision)
     ctx = get_context(X.device)
     h = ctypes.c_void_p()
-    ctx.check(_lib.lib().cnn_graph_build(ctx.handle, ptr(X), n, d, dt, float(r2), p, ctypes.byref(h)),
-              "cnn_graph_build")
+    if bounds is None:
+        ctx.check(_lib.lib().cnn_graph_build(ctx.handle, ptr(X), n, d, dt, float(r2), p, ctypes.byref(h)),
+                  "cnn_graph_build")
+    else:
+        b = np.ascontiguousarray(bounds, dtype=np.float64).reshape(-1)
+        if b.size != d:
+            raise ValueError(f"bounds needs one entry per dimension ({d}), got {b.size}")
+        p = _lib.precision_code("f64")
+        ctx.check(_lib.lib().cnn_graph_build_periodic(ctx.handle, ptr(X), n, d, dt, float(r2), b.ctypes.data,
+                                                      ctypes.byref(h)), "cnn_graph_build_periodic")
     return NeighbourGraph(h, ctx, X, r2, p)
 
 

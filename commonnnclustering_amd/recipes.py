@@ -322,6 +322,8 @@ COMPONENT_NAME_TYPE_MAP = {
         "euclidean_reduced": _types.MetricEuclideanReduced,
         "euclidean": _types.MetricEuclidean,
         "precomputed": _types.MetricPrecomputed,
+        "euclidean_periodic_r": _types.MetricEuclideanPeriodicReduced,
+        "euclidean_periodic_reduced": _types.MetricEuclideanPeriodicReduced,
     },
     "similarity_checker": {
         "contains": _types.SimilarityCheckerGPU,
@@ -348,8 +350,6 @@ COMPONENT_NAME_TYPE_MAP = {
 # Reference components without a GPU implementation in this engine, with the reason.
 UNSUPPORTED_COMPONENTS = {
     "metric": {
-        "euclidean_periodic_r": "the periodic metric (MetricExtEuclideanPeriodicReduced) is not implemented",
-        "euclidean_periodic_reduced": "the periodic metric is not implemented",
         "dummy": "MetricExtDummy is not implemented",
     },
     "distance_getter": {"lookup": "DistanceGetterExtLookup (precomputed distances) is not implemented"},

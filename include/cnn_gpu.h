@@ -245,6 +245,20 @@ int cnn_precomputed_csr(cnn_ctx *ctx, const void *D, int64_t n, int32_t dtype, d
                         int64_t *row_ptr, int32_t *col, int64_t col_cap, int64_t *nnz_out,
                         int64_t *asymmetric_out);
 
+/* Radius graph under the periodic reduced metric, replacing
+ * NeighboursGetterExtBruteForce._get (_types.pyx:1692-1710) over
+ * MetricExtEuclideanPeriodicReduced._calc_distance (_types.pyx:2296-2317): per
+ * dimension k the difference |a_k - b_k| is reduced modulo bounds[k] and
+ * replaced by bounds[k] - x above bounds[k]/2 (dimensions with bounds[k] <= 0
+ * are not periodic), squared and summed in ascending k in FP64.  bounds: HOST
+ * double[d].  Plain FP64, all tile pairs (no spatial culling); use the graph
+ * with cnn_graph_labels / cnn_graph_fit / cnn_graph_export_csr.  (The
+ * reference's _calc_distance_other for this metric adds (a_k - b_k)^2 instead
+ * of the reduced difference, _types.pyx:2332-2343; prediction with it is not
+ * offered.) */
+int cnn_graph_build_periodic(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
+                             double r2, const double *bounds, cnn_graph **out);
+
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
 

@@ -21,7 +21,8 @@ default ``"coordinates"`` recipe:
   fitter     _fit.pyx:408-491 (``FitterExtBFS._fit``)
   sort       _types.pyx:212-286 (``Labels.sort_by_size``)
   wrapper    cluster.pyx:256-392 (``Clustering.fit``), report.py:91-191 (``Record``)
-  metrics    _types.pyx:2105-2127 (``MetricExtPrecomputed``), :2170-2207 (``MetricExtEuclidean``)
+  metrics    _types.pyx:2105-2127 (``MetricExtPrecomputed``), :2170-2207 (``MetricExtEuclidean``),
+             :2296-2317 (``MetricExtEuclideanPeriodicReduced._calc_distance``)
   predict    _fit.pyx:1241-1291 (``PredictorFirstmatch.predict``), _types.pyx:1712-1731
              (``_get_other``), cluster.pyx:560-653 (``Clustering.predict``)
 
@@ -41,7 +42,10 @@ How each part is pinned (tests/test_oracle.py):
   * other metrics: the sqrt metric's lists equal a literal scalar loop with
     math.sqrt (IEEE, correctly rounded) and include pairs whose decision differs
     from the reduced metric's at r*r; the precomputed metric's lists equal the
-    reduced metric's when fed the reduced distance matrix with r*r.
+    reduced metric's when fed the reduced distance matrix with r*r; the
+    periodic metric equals a literal scalar loop with math.fmod, gives the
+    minimum-image closed form on hand-placed points and is invariant under
+    shifting points by whole periods.
   * predict: the reference's own integration case
     (test/integration/test_procedures.py:246-277: half of a two-moons set
     labelled by the generator predicts the whole set's generator labels) and
@@ -166,8 +170,26 @@ def neighbourhoods(X: np.ndarray, radius_cutoff: float, chunk: int = 1024) -> li
     return out
 
 
+def periodic_reduced_distances(A: np.ndarray, B: np.ndarray, bounds) -> np.ndarray:
+    """MetricExtEuclideanPeriodicReduced._calc_distance (_types.pyx:2296-2317) for
+    all pairs: per dimension d = |a - b|; for a positive bound, d = fmod(d, bound)
+    (exact; Python's % on the non-negative d agrees) and, when d > bound / 2,
+    d = bound - d; total = total + d * d in ascending dimension."""
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    total = np.zeros((A.shape[0], B.shape[0]), dtype=np.float64)
+    for k in range(A.shape[1]):
+        dist = np.abs(A[:, k, None] - B[None, :, k])
+        bound = float(bounds[k])
+        if bound > 0:
+            dist = np.fmod(dist, bound)
+            dist = np.where(dist > bound / 2, bound - dist, dist)
+        total = total + dist * dist
+    return total
+
+
 def neighbourhoods_metric(X: np.ndarray, radius_cutoff: float, metric: str = "euclidean_r",
-                          chunk: int = 1024) -> list:
+                          chunk: int = 1024, bounds=None) -> list:
     """Brute-force neighbourhoods under the reference's other metrics:
 
     ``euclidean``    MetricExtEuclidean (_types.pyx:2170-2207): sqrt of the
@@ -185,6 +207,13 @@ def neighbourhoods_metric(X: np.ndarray, radius_cutoff: float, metric: str = "eu
         for row in X:
             out.append(np.nonzero(row <= r)[0].astype(np.int64))
         return out
+    if metric == "periodic":
+        r2 = adjust_radius(radius_cutoff)
+        for s in range(0, X.shape[0], chunk):
+            D = periodic_reduced_distances(X[s:s + chunk], X, bounds)
+            for row in D:
+                out.append(np.nonzero(row <= r2)[0].astype(np.int64))
+        return out
     if metric != "euclidean":
         raise ValueError(f"unknown metric {metric!r}")
     for s in range(0, X.shape[0], chunk):
@@ -194,9 +223,10 @@ def neighbourhoods_metric(X: np.ndarray, radius_cutoff: float, metric: str = "eu
     return out
 
 
-def neighbourhoods_csr(X: np.ndarray, radius_cutoff: float, chunk: int = 1024, metric: str = "euclidean_r"):
+def neighbourhoods_csr(X: np.ndarray, radius_cutoff: float, chunk: int = 1024, metric: str = "euclidean_r",
+                       bounds=None):
     """The same neighbourhoods as CSR: (row_ptr int64[n+1], col int64[nnz])."""
-    lists = neighbourhoods_metric(X, radius_cutoff, metric, chunk)
+    lists = neighbourhoods_metric(X, radius_cutoff, metric, chunk, bounds)
     deg = np.array([len(x) for x in lists], dtype=np.int64)
     row_ptr = np.zeros(len(lists) + 1, dtype=np.int64)
     np.cumsum(deg, out=row_ptr[1:])
@@ -406,7 +436,7 @@ def make_record(labels: np.ndarray, n_points: int, radius_cutoff, cnn_cutoff,
 
 
 def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
-        cnn_offset=None, sort=True, method="components", metric="euclidean_r"):
+        cnn_offset=None, sort=True, method="components", metric="euclidean_r", bounds=None):
     """End-to-end restatement of ``Clustering.fit`` (cluster.pyx:256-392).
 
     Returns ``(labels int64[n], record dict)``.  ``method`` selects the
@@ -422,9 +452,9 @@ def fit(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
     consider = np.ones(n, dtype=np.uint8)
     params = make_parameters(radius_cutoff, cnn_cutoff, 1)
     if method == "bfs":
-        fit_bfs(neighbourhoods_metric(X, radius_cutoff, metric), params, labels, consider)
+        fit_bfs(neighbourhoods_metric(X, radius_cutoff, metric, bounds=bounds), params, labels, consider)
     else:
-        row_ptr, col = neighbourhoods_csr(X, radius_cutoff, metric=metric)
+        row_ptr, col = neighbourhoods_csr(X, radius_cutoff, metric=metric, bounds=bounds)
         fit_components(row_ptr, col, params, labels, consider)
     if sort:
         sort_by_size_np(labels, member_cutoff, max_clusters)

@@ -4,6 +4,9 @@ Reads the reference's documentation and test files as TEXT (nothing is
 imported or executed from the reference).  The fixture holds data only:
 inputs, parameters and expected outputs, each with its source file:line.
 
+Also stores the backbone-dihedral data file of the periodic-metric test as
+tests/golden/backbone_dihedrals.npy (loaded with allow_pickle=False).
+
 Usage: python tests/golden/make_golden.py [/path/to/reference]
 """
 
@@ -147,6 +150,14 @@ def main():
     with open(OUT, "w") as fh:
         json.dump(kats, fh, indent=1)
     print(f"wrote {OUT}")
+    # data file of the reference's periodic-metric test (test_procedures.py:175-215),
+    # loaded without unpickling and stored as a fixture
+    import numpy as np
+
+    src = os.path.join(REF, "test/integration/test_procedures/backbone_dihedrals.npy")
+    dst = os.path.join(os.path.dirname(OUT), "backbone_dihedrals.npy")
+    np.save(dst, np.ascontiguousarray(np.load(src, allow_pickle=False)))
+    print(f"wrote {dst}")
 
 
 if __name__ == "__main__":

@@ -77,3 +77,39 @@ def test_precomputed_asymmetric_rejected(cuda):
     cl = Clustering(D, recipe="distances")
     with pytest.raises(NotImplementedError):
         cl.fit(0.1, 1, v=False)
+
+
+def _periodic_metric(bounds):
+    from commonnnclustering_amd._types import MetricExtEuclideanPeriodicReduced
+    return (MetricExtEuclideanPeriodicReduced, (np.asarray(bounds, dtype=float),), {})
+
+
+def test_periodic_dihedrals_fit(cuda):
+    """The reference's periodic-metric case (test_procedures.py:175-215):
+    backbone dihedrals in degrees, bounds (360, 360), fit(10, 15, member_cutoff=50)."""
+    import os
+    data = np.load(os.path.join(os.path.dirname(__file__), "golden", "backbone_dihedrals.npy"))
+    cl = Clustering(data, fitter__getter__dgetter__metric=_periodic_metric([360, 360]))
+    cl.fit(10, 15, member_cutoff=50, v=False)
+    want, _ = O.fit(data, 10, 15, member_cutoff=50, metric="periodic", bounds=[360.0, 360.0])
+    assert (cl.labels == want).all()
+    assert len(set(want.tolist()) - {0}) >= 2
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_periodic_lists_beyond_one_period(cuda, dtype):
+    rng = np.random.default_rng(9)
+    X = np.concatenate([rng.normal(170, 12, size=(400, 3)), rng.uniform(-800, 800, size=(300, 3))])
+    X[:, 2] = rng.normal(0, 3, size=len(X))      # a non-periodic axis
+    X = X.astype(dtype)
+    bounds = [360.0, 360.0, 0.0]
+    g = engine.build_graph(torch.from_numpy(X).cuda(), 18.0 ** 2, "f32_band", bounds=bounds)
+    try:
+        rp, col = g.to_csr()
+        with pytest.raises(Exception):
+            g.rethreshold(10.0 ** 2)   # periodic graphs are rebuilt, not re-thresholded
+    finally:
+        g.release()
+    rp0, col0 = O.neighbourhoods_csr(X.astype(np.float64), 18.0, metric="periodic", bounds=bounds)
+    assert (rp.cpu().numpy() == rp0).all()
+    assert (col.cpu().numpy().astype(np.int64) == col0).all()

@@ -340,3 +340,32 @@ def test_precomputed_metric_equals_reduced():
     a = O.neighbourhoods_metric(D, r * r, "precomputed")
     b = O.neighbourhoods(X, r)
     assert all((x == y).all() for x, y in zip(a, b))
+
+
+def test_periodic_metric_literal_loop_and_closed_form():
+    import math
+    bounds = [360.0, 360.0, 0.0]
+    rng = np.random.default_rng(8)
+    X = np.round(rng.uniform(-700, 700, size=(80, 3)), 1)
+    D = O.periodic_reduced_distances(X, X, bounds)
+    for i in range(0, 80, 7):
+        for j in range(80):
+            total = 0.0
+            for k in range(3):
+                dist = abs(X[i, k] - X[j, k])
+                if bounds[k] > 0:
+                    dist = math.fmod(dist, bounds[k])
+                    if dist > bounds[k] / 2:
+                        dist = bounds[k] - dist
+                total = total + dist * dist
+            assert D[i, j] == total
+    # minimum image: 179 and -179 are 2 apart, 10 and 370 coincide; the third axis is not periodic
+    P = np.array([[179.0, 0.0, 0.0], [-179.0, 0.0, 0.0], [10.0, 370.0, 5.0], [10.0, 10.0, 5.0]])
+    Dp = O.periodic_reduced_distances(P, P, bounds)
+    assert Dp[0, 1] == 4.0 and Dp[2, 3] == 0.0
+    # whole-period shifts of integer-valued points leave every neighbourhood unchanged
+    Y = np.round(rng.uniform(-180, 180, size=(150, 2)))
+    shift = 360.0 * rng.integers(-2, 3, size=(150, 2))
+    a = O.neighbourhoods_metric(Y, 25.0, "periodic", bounds=[360.0, 360.0])
+    b = O.neighbourhoods_metric(Y + shift, 25.0, "periodic", bounds=[360.0, 360.0])
+    assert all((x == y).all() for x, y in zip(a, b))
