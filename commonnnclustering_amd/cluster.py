@@ -268,3 +268,22 @@ class Clustering:
             meta["params"] = old_params
             other._labels.meta.update(meta)
         other._labels.meta["frozen"] = True
+
+    def to_dtrajs(self, bundle=None):
+        """Labels split into the parts the input was given in (``meta["edges"]``),
+        as Clustering.to_dtrajs (cluster.pyx:1166-1188): one array per part, a
+        trailing empty part dropped; the whole array when there are no edges."""
+        if bundle is None:
+            bundle = self._bundle
+        labels_array = bundle.labels
+        if labels_array is None:
+            return []
+        edges = None
+        if bundle._input_data is not None:
+            edges = bundle._input_data.meta.get("edges")
+        if edges is None:
+            return [labels_array]
+        dtrajs = np.split(labels_array, np.cumsum(edges))
+        if len(dtrajs[-1]) == 0:
+            dtrajs = dtrajs[:-1]
+        return dtrajs

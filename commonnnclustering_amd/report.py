@@ -82,9 +82,26 @@ class Summary(MutableSequence):
         return "\n".join(str(r) for r in self._list)
 
     def to_DataFrame(self):
+        """Records as a typed pandas.DataFrame (Summary.to_DataFrame, report.py:55-88):
+        nullable integer columns for counts and cutoffs, float64 for the rest."""
+        import numpy as np
         import pandas as pd
 
-        return pd.DataFrame([r.to_dict() for r in self._list])
+        dtypes = [pd.Int64Dtype(), np.float64, pd.Int64Dtype(), pd.Int64Dtype(), pd.Int64Dtype(),
+                  pd.Int64Dtype(), np.float64, np.float64, np.float64]
+        fields = Record.__slots__
+        content = [[getattr(r, f) for r in self._list] for f in fields]
+        return make_typed_DataFrame(fields, dtypes, content)
+
+
+def make_typed_DataFrame(columns, dtypes, content=None):
+    """pandas.DataFrame with typed columns (report.py:194-210)."""
+    import pandas as pd
+
+    assert len(columns) == len(dtypes)
+    if content is None:
+        content = [[] for _ in columns]
+    return pd.DataFrame({k: pd.array(c, dtype=v) for k, v, c in zip(columns, dtypes, content)})
 
 
 def timed(function):
