@@ -71,6 +71,9 @@ struct LabelArgs {
     unsigned long long *stats;  // [slot][0] tests, [slot][1] passed
     const uint64_t *rank;       // snapshot rank of core points: (component size << 32) | root; 0 otherwise
     const uint64_t *cand_max;   // per row tile: max rank over its candidate tiles (nullable)
+    // per tile: the rank every point of the tile has (0: no core point; all
+    // ones: ranks differ), so list entries in such tiles need no rank gather
+    const uint64_t *tile_rank;
     const int32_t *tile_csim;   // packed batches: c_sim / m_core per row tile (nullable)
     const int32_t *tile_mcore;
 };
@@ -229,7 +232,11 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     for (int64_t t0 = b; t0 < e; t0 += 64) {
         const int64_t t = t0 + lane;
         const int j = t < e ? a.col[t] : -1;
-        const uint64_t rj = t < e ? a.rank[j] : 0ull;
+        uint64_t rj = 0ull;
+        if (t < e) {
+            const uint64_t tr = a.tile_rank ? a.tile_rank[j >> 6] : ~0ull;
+            rj = tr != ~0ull ? tr : a.rank[j];
+        }
         const int kj = (int)(uint32_t)rj;
         const bool cand = rj > rw && kj != resolved;
         uint64_t bits = __ballot(cand);
@@ -248,20 +255,28 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     add_stats(a, tests, passed, 1);
 }
 
-// Largest snapshot rank among the core points of each tile (0: none).
+// Largest snapshot rank among the core points of each tile (0: none), and the
+// tile's common rank: the rank of all its points when they share one (a
+// non-core point has rank 0), all ones when they differ.
 __global__ void k_tile_rank_max(const uint64_t *__restrict__ rank, int64_t n, int64_t n_tiles,
-                                uint64_t *__restrict__ tmax) {
+                                uint64_t *__restrict__ tmax, uint64_t *__restrict__ tcommon) {
     const int lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= n_tiles) return;
     const int64_t w = t * 64 + lane;
-    uint64_t v = w < n ? rank[w] : 0ull;
+    const bool real = w < n;
+    const uint64_t mine = real ? rank[w] : 0ull;
+    uint64_t v = mine;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const uint64_t o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    if (lane == 0) tmax[t] = v;
+    const bool same = __ballot(real && mine != v) == 0ull;  // every real point has the largest rank
+    if (lane == 0) {
+        tmax[t] = v;
+        tcommon[t] = same ? v : ~0ull;
+    }
 }
 
 // Largest rank over the candidate column tiles of each row tile.
@@ -470,7 +485,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
     CNN_HIP(hipMemsetAsync(tstats.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
     LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr,
-                req.tile_csim, req.tile_mcore};
+                nullptr, req.tile_csim, req.tile_mcore};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
@@ -493,15 +508,18 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         k_snapshot<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cnt.p, n, rank.p);
         CNN_HIP(hipGetLastError());
     }
-    std::unique_ptr<Scratch<uint64_t>> cand_max;
+    std::unique_ptr<Scratch<uint64_t>> cand_max, tile_common;
     if (req.n_tiles > 0) {
         Scratch<uint64_t> tmax(ctx, req.n_tiles);
         cand_max.reset(new Scratch<uint64_t>(ctx, req.n_tiles));
-        k_tile_rank_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(rank.p, n, req.n_tiles, tmax.p);
+        tile_common.reset(new Scratch<uint64_t>(ctx, req.n_tiles));
+        k_tile_rank_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(rank.p, n, req.n_tiles, tmax.p,
+                                                                   tile_common->p);
         k_cand_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(tmax.p, req.tp_ptr, req.tp_col, req.n_tiles,
                                                               cand_max->p);
         CNN_HIP(hipGetLastError());
         a.cand_max = cand_max->p;
+        a.tile_rank = tile_common->p;
     }
     k_round_b<<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipGetLastError());

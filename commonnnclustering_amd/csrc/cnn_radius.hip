@@ -262,8 +262,7 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
                              int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
                              const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
                              int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
-                             int32_t *__restrict__ tp_col, const double *__restrict__ glo,
-                             const double *__restrict__ ghi, uint8_t *__restrict__ gmask) {
+                             int32_t *__restrict__ tp_col) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -281,17 +280,7 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
             const int64_t C = S * 64 + lane;
             const bool ok = (C >= clo) && (C < chi) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
             const uint64_t b = __ballot(ok);
-            if (FILL && ok) {
-                const int64_t slot = out + __popcll(b & ((1ull << lane) - 1ull));
-                tp_col[slot] = (int32_t)C;
-                if (gmask) {  // row groups of R within reach of tile C
-                    uint8_t gm = 0;
-                    for (int gi = 0; gi < 4; ++gi)
-                        gm |= (uint8_t)(boxes_close(glo + (R * 4 + gi) * d, ghi + (R * 4 + gi) * d, blo + C * d,
-                                                    bhi + C * d, d, thr) << gi);
-                    gmask[slot] = gm;
-                }
-            }
+            if (FILL && ok) tp_col[out + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)C;
             out += __popcll(b);
         }
     }
@@ -1443,11 +1432,18 @@ __global__ void k_pad_rows(const double *__restrict__ xw64, int64_t n_pad, int d
 }
 
 // Mirror index of every candidate pair (R, C): position of R in C's list.
-__global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col, int64_t n_tiles,
-                         int64_t *__restrict__ mirror) {
+// Per candidate pair k = (R, C), lane-parallel over the pairs of a row tile:
+// mirror[k] = index of (C, R) in C's list, and gmask[k] = the 16-row groups of
+// R whose boxes are within reach of tile C (bit g for group g).
+__global__ void k_pair_info(const int64_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col, int64_t n_tiles,
+                            int64_t *__restrict__ mirror, const double *__restrict__ glo,
+                            const double *__restrict__ ghi, const double *__restrict__ blo,
+                            const double *__restrict__ bhi, int d, double thr_all,
+                            const double *__restrict__ tile_thr, uint8_t *__restrict__ gmask) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
+    const double thr = tile_thr ? tile_thr[R] : thr_all;
     for (int64_t k = tp_ptr[R] + lane; k < tp_ptr[R + 1]; k += 64) {
         const int64_t C = tp_col[k];
         int64_t lo = tp_ptr[C], hi = tp_ptr[C + 1];
@@ -1457,6 +1453,11 @@ __global__ void k_mirror(const int64_t *__restrict__ tp_ptr, const int32_t *__re
             else hi = mid;
         }
         mirror[k] = lo;
+        uint8_t gm = 0;
+        for (int gi = 0; gi < 4; ++gi)
+            gm |= (uint8_t)(boxes_close(glo + (R * 4 + gi) * d, ghi + (R * 4 + gi) * d, blo + C * d, bhi + C * d, d,
+                                        thr) << gi);
+        gmask[k] = gm;
     }
 }
 
@@ -1863,6 +1864,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     const double thr = g->r2 * (1.0 + 1e-6);
     std::unique_ptr<Scratch<double>> glo, ghi;
     std::unique_ptr<Scratch<uint8_t>> gmask;
+    std::unique_ptr<Scratch<int64_t>> mirror;
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
         glo.reset(new Scratch<double>(ctx, T * 4 * d));
@@ -1877,8 +1879,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const double *tthr = pk ? pk->tile_thr : nullptr;
         const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
         k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                     sl, sh, cnt.p, nullptr, nullptr, nullptr, nullptr,
-                                                                     nullptr);
+                                                                     sl, sh, cnt.p, nullptr, nullptr);
         CNN_HIP(hipGetLastError());
         tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
@@ -1887,18 +1888,15 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         total_pairs = last_excl + last_cnt;
         CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
         tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
-        gmask.reset(new Scratch<uint8_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                    sl, sh, nullptr, tp_ptr, tp_col, glo->p, ghi->p,
-                                                                    gmask->p);
+                                                                    sl, sh, nullptr, tp_ptr, tp_col);
+        CNN_HIP(hipGetLastError());
+        gmask.reset(new Scratch<uint8_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        k_pair_info<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
+                                                             bhi.p, d, thr, tthr, gmask->p);
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipStreamSynchronize(s));
-    }
-    std::unique_ptr<Scratch<int64_t>> mirror;
-    if (g->layout == 1) {
-        mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
-        k_mirror<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p);
-        CNN_HIP(hipGetLastError());
     }
     std::vector<int64_t> h_tp_ptr;
     if (g->layout == 1) {
