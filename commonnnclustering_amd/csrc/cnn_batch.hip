@@ -120,7 +120,7 @@ int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n
             for (int32_t b = 0; b < n_sets; ++b) {
                 const int64_t n = offsets[b + 1] - offsets[b];
                 const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
-                cnn_graph *g = graph_build(ctx, xb, n, d, dtype, r2[b], precision);
+                cnn_graph *g = graph_build(ctx, xb, n, d, dtype, r2[b], precision, nullptr, true);
                 std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
                 LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
                                true, true, member_cutoff, max_clusters, labels + offsets[b],
@@ -156,7 +156,7 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
         for (int32_t i = 0; i < n_r; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return r2[x] > r2[y]; });
         CNN_HIP(hipSetDevice(ctx->device));
-        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2[order[0]], precision);
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2[order[0]], precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
         // lists of the radius being labelled; the graph's own for the largest

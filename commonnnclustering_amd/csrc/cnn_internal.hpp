@@ -66,6 +66,7 @@ struct cnn_graph {
     double radius = 0.0;
     double r2 = 0.0;
     double *bounds = nullptr;    // periodic metric: period per dimension (device, d entries)
+    cnn::Arena *arena = nullptr; // memory owner when the graph lives inside one call (else hipMalloc)
     int32_t layout = 0;          // 0 identity, 1 spatial order
     int32_t dp = 0;              // row stride of xw32 (d, or d padded to 8 for the MFMA form)
     int64_t n_tiles = 0;
@@ -166,8 +167,11 @@ T read_scalar(cnn_ctx *ctx, const T *dev) {
 }
 
 // --------------------------------------------------------------- graph / labels
+// transient: the graph is freed before the calling entry point returns, on the
+// same context; its memory then comes from the context's arena.
 cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
-                       double r2, int32_t precision, const double *bounds_host = nullptr);
+                       double r2, int32_t precision, const double *bounds_host = nullptr,
+                       bool transient = false);
 void graph_free(cnn_graph *g);
 void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col);
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);

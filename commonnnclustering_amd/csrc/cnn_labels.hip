@@ -806,7 +806,7 @@ int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, do
     return guarded(ctx, [&]() {
         if (n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
-        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision);
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision, nullptr, graph_out == nullptr);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,

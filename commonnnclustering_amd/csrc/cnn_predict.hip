@@ -157,7 +157,7 @@ int cnn_predict(cnn_ctx *ctx, const void *X, int64_t n_x, const void *Y, int64_t
         Scratch<uint8_t> Z(ctx, (size_t)n * d * es);
         CNN_HIP(hipMemcpyAsync(Z.p, X, (size_t)n_x * d * es, hipMemcpyDeviceToDevice, s));
         CNN_HIP(hipMemcpyAsync(Z.p + (size_t)n_x * d * es, Y, (size_t)n_y * d * es, hipMemcpyDeviceToDevice, s));
-        cnn_graph *g = graph_build(ctx, Z.p, n, d, dtype, r2, precision);
+        cnn_graph *g = graph_build(ctx, Z.p, n, d, dtype, r2, precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         const unsigned wblocks = (unsigned)ceil_div(n, 4);
         Scratch<int32_t> cnt(ctx, n);

@@ -1740,15 +1740,34 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
     if (!(band && a.d >= kMfmaMinD)) CNN_HIP(hipEventRecord(ctx->ev[7], s));
 }
 
+// Graph memory: from the building context's arena for a graph that lives only
+// inside one call on that context (stream-ordered reuse, no hipMalloc/hipFree
+// per fit), else hipMalloc.
+static void *graph_mem(cnn_graph *g, size_t bytes) {
+    void *p = nullptr;
+    if (g->arena) {
+        p = g->arena->get(bytes);
+    } else {
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+        }
+    }
+    if (!p) throw OutOfMemory{"graph allocation of " + std::to_string(bytes) + " bytes failed"};
+    return p;
+}
+
+static void graph_mem_free(cnn_graph *g, void *p) {
+    if (!p) return;
+    if (g->arena) g->arena->put(p);
+    else (void)hipFree(p);
+}
+
 template <typename T>
 static T *dev_alloc(cnn_graph *g, int64_t count) {
-    void *p = nullptr;
     size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw OutOfMemory{"graph allocation of " + std::to_string(bytes) + " bytes failed"};
-    }
+    void *p = graph_mem(g, bytes);
     g->bytes += (int64_t)bytes;
     return (T *)p;
 }
@@ -1756,14 +1775,14 @@ static T *dev_alloc(cnn_graph *g, int64_t count) {
 void graph_free(cnn_graph *g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
-    if (g->perm) (void)hipFree(g->perm);
-    if (g->xw64) (void)hipFree(g->xw64);
-    if (g->xw32) (void)hipFree(g->xw32);
-    if (g->row_ptr) (void)hipFree(g->row_ptr);
-    if (g->col) (void)hipFree(g->col);
-    if (g->tp_ptr) (void)hipFree(g->tp_ptr);
-    if (g->tp_col) (void)hipFree(g->tp_col);
-    if (g->bounds) (void)hipFree(g->bounds);
+    graph_mem_free(g, g->perm);
+    graph_mem_free(g, g->xw64);
+    graph_mem_free(g, g->xw32);
+    graph_mem_free(g, g->row_ptr);
+    graph_mem_free(g, g->col);
+    graph_mem_free(g, g->tp_ptr);
+    graph_mem_free(g, g->tp_col);
+    graph_mem_free(g, g->bounds);
     delete g;
 }
 
@@ -1999,15 +2018,15 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         if (need > col_cap) {
             const int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
             int32_t *nc = nullptr;
-            hipError_t e = hipMalloc(&nc, (size_t)std::max<int64_t>(new_cap, 1) * sizeof(int32_t));
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
+            try {
+                nc = (int32_t *)graph_mem(g, (size_t)std::max<int64_t>(new_cap, 1) * sizeof(int32_t));
+            } catch (const OutOfMemory &) {
                 throw OutOfMemory{"neighbour list allocation failed (" + std::to_string(need) + " entries)"};
             }
             if (col && nnz_total) CNN_HIP(hipMemcpyAsync(nc, col, nnz_total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
             if (col) {
                 CNN_HIP(hipStreamSynchronize(s));
-                (void)hipFree(col);
+                graph_mem_free(g, col);
                 g->bytes -= col_cap * (int64_t)sizeof(int32_t);
             }
             col = nc;
@@ -2063,13 +2082,14 @@ static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype,
 }
 
 cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, double r2,
-                       int32_t precision, const double *bounds_host) {
+                       int32_t precision, const double *bounds_host, bool transient) {
     check_build_args(X, n, d, dtype, precision);
     if (bounds_host) precision = CNN_PLAIN_F64;  // the periodic metric has the FP64 form only
     CNN_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     GraphGuard guard{new cnn_graph()};
     cnn_graph *g = guard.g;
+    if (transient) g->arena = ctx->arena;
     g->device = ctx->device;
     g->n = n;
     g->n_points = n;
@@ -2547,7 +2567,7 @@ int cnn_radius_csr(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dt
                    cnn_graph_stats *stats_out) {
     if (!ctx || !row_ptr || !nnz_out) return CNN_ERR_INVALID;
     return guarded(ctx, [&]() {
-        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision);
+        cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision, nullptr, true);
         GraphGuard guard{g};
         *nnz_out = g->nnz;
         if (stats_out) *stats_out = g->stats;
