@@ -1,6 +1,7 @@
 """Benchmark: CommonNN fit throughput on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload md_basins_3d|features_64d|blobs_2d|serving]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload md_basins_3d|features_64d|blobs_2d|serving|scan_2d|many_small]
 
 A step is one full fit (radius graph, similarity test, union-find,
 sort_by_size) of the whole synthetic dataset; the coordinates are already
@@ -256,6 +257,93 @@ def serving_bench(args):
     }
 
 
+def _timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / max(steps, 1)
+
+
+def _line(value, ms, steps, warmup, config, extra):
+    out = {"metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic (seeded numpy generator)", "config": config}
+    out.update(extra)
+    return out
+
+
+def scan_bench(args):
+    """(r, c) scan of one dataset (n=5e4, d=2, 8 radii x 8 cutoffs): one graph at
+    the largest radius, exact FP64 re-thresholding, a labelling pass per pair.
+    A step is the whole scan; points/s counts n per (r, c) pair.  Beside it:
+    the same 64 labels from independent fits."""
+    import torch
+    from commonnnclustering_amd import engine, synthetic
+
+    X, p = synthetic.scan_2d()
+    Xd = torch.from_numpy(X).cuda()
+    n = X.shape[0]
+    radii, cs = list(p["radii"]), list(p["cnn_cutoffs"])
+    pairs = len(radii) * len(cs)
+    t = _timed(lambda: engine.fit_scan(Xd, radii, cs, precision="f32_band"), args.steps, args.warmup)
+
+    def independent():
+        for r in radii:
+            for c in cs:
+                engine.fit(Xd, float(r) * float(r), c + 2, c + 1, precision="f32_band")
+
+    t_ind = _timed(independent, 1, 1)
+    log(f"scan: {t * 1e3:.2f} ms for {pairs} (r, c) pairs; independent fits {t_ind * 1e3:.2f} ms")
+    return _line(n * pairs / t, t * 1e3, args.steps, args.warmup,
+                 {"workload": "scan_2d", "n": n, "d": 2, "radii": [float(r) for r in radii], "cnn_cutoffs": cs},
+                 {"independent_fits_ms": t_ind * 1e3, "scan_vs_independent": t_ind / t})
+
+
+def batch_bench(args):
+    """512 small datasets (n log-uniform in [500, 2e4], d in {2, 3, 8}, r at the
+    2nd distance percentile, c = 5), one cnn_fit_batch call per dimension.
+    A step fits all 512; points/s counts every point once.  Beside it: the
+    same datasets fitted one call after another."""
+    import torch
+    from commonnnclustering_amd import engine, synthetic
+
+    sets = synthetic.many_small()
+    groups = []
+    for d in sorted({x.shape[1] for x, _ in sets}):
+        mine = [(x, q) for x, q in sets if x.shape[1] == d]
+        X = np.concatenate([x for x, _ in mine]).astype(np.float32)
+        off = np.zeros(len(mine) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(x) for x, _ in mine])
+        r2 = np.array([float(q["radius_cutoff"]) ** 2 for _, q in mine])
+        c = np.array([int(q["cnn_cutoff"]) for _, q in mine])
+        groups.append((torch.from_numpy(X).cuda(), off, r2, c))
+    total = sum(len(x) for x, _ in sets)
+
+    def batched():
+        for Xd, off, r2, c in groups:
+            engine.fit_batch(Xd, off, r2, c + 2, c + 1, precision="f32_band", member_cutoff=2)
+
+    singles = [(torch.from_numpy(x.astype(np.float32)).cuda(), q) for x, q in sets]
+
+    def one_by_one():
+        for Xd, q in singles:
+            r = float(q["radius_cutoff"])
+            engine.fit(Xd, r * r, q["cnn_cutoff"] + 2, q["cnn_cutoff"] + 1, precision="f32_band", member_cutoff=2)
+
+    t = _timed(batched, args.steps, args.warmup)
+    t_seq = _timed(one_by_one, 1, 1)
+    log(f"batch: {len(sets)} datasets, {total} points in {t * 1e3:.1f} ms; one call each {t_seq * 1e3:.1f} ms")
+    return _line(total / t, t * 1e3, args.steps, args.warmup,
+                 {"workload": "many_small", "datasets": len(sets), "points": total, "dims": [2, 3, 8]},
+                 {"one_call_each_ms": t_seq * 1e3, "batch_vs_one_call_each": t_seq / t})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +380,12 @@ def main():
 
     if args.workload == "serving":
         print(json.dumps(serving_bench(args)))
+        return
+    if args.workload == "scan_2d":
+        print(json.dumps(scan_bench(args)))
+        return
+    if args.workload == "many_small":
+        print(json.dumps(batch_bench(args)))
         return
 
     X, p = workload(args.workload)

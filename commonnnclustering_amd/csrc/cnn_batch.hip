@@ -8,6 +8,11 @@
 #include <memory>
 #include <vector>
 
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
+
 #include "cnn_internal.hpp"
 
 using namespace cnn;
@@ -88,6 +93,68 @@ static void fit_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int3
     CNN_HIP(hipStreamSynchronize(s));
 }
 
+// Datasets fitted independently, several at once: host threads take datasets
+// (largest first) from a shared counter, each on a worker context with its
+// own stream and arena, so the device overlaps the kernels, copies and host
+// round trips of different datasets.  Work already queued on the caller's
+// stream is finished first; every dataset's work is finished on return.
+static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d, int32_t dtype,
+                     const double *r2, const int32_t *c_sim, const int32_t *m_core, int32_t precision,
+                     int64_t member_cutoff, int64_t max_clusters, int64_t *labels, cnn_graph_stats &gs,
+                     cnn_label_stats &ls) {
+    if (n_sets == 0) return;
+    CNN_HIP(hipStreamSynchronize(ctx->stream));
+    const int n_workers = std::min<int>(kBatchWorkers, n_sets);
+    while ((int)ctx->workers.size() < n_workers) {
+        cnn_ctx *w = nullptr;
+        const int rc = cnn_ctx_create(ctx->device, &w);
+        if (rc != CNN_OK) throw HipFailure{hipErrorUnknown, "worker context creation failed"};
+        ctx->workers.push_back(w);
+    }
+    std::vector<int32_t> order(n_sets);
+    for (int32_t b = 0; b < n_sets; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+    });
+    const size_t esize = dtype == CNN_F64 ? sizeof(double) : sizeof(float);
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex mu;
+    std::exception_ptr first_error;
+    auto work = [&](cnn_ctx *wc) {
+        try {
+            CNN_HIP(hipSetDevice(wc->device));
+            for (;;) {
+                const int32_t i = next.fetch_add(1);
+                if (i >= n_sets || failed.load()) break;
+                const int32_t b = order[i];
+                const int64_t n = offsets[b + 1] - offsets[b];
+                const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
+                cnn_graph *g = graph_build(wc, xb, n, d, dtype, r2[b], precision, nullptr, true);
+                std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+                LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
+                               true, true, member_cutoff, max_clusters, labels + offsets[b],
+                               g->tp_ptr, g->tp_col, g->n_tiles};
+                cnn_label_stats one{};
+                labels_run(wc, r, &one);
+                CNN_HIP(hipStreamSynchronize(wc->stream));
+                std::lock_guard<std::mutex> lock(mu);
+                add_graph_stats(gs, g->stats);
+                add_label_stats(ls, one);
+            }
+        } catch (...) {
+            failed.store(true);
+            std::lock_guard<std::mutex> lock(mu);
+            if (!first_error) first_error = std::current_exception();
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int k = 1; k < n_workers; ++k) threads.emplace_back(work, ctx->workers[k]);
+    work(ctx->workers[0]);
+    for (auto &t : threads) t.join();
+    if (first_error) std::rethrow_exception(first_error);
+}
+
 extern "C" {
 
 int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d, int32_t dtype,
@@ -116,20 +183,8 @@ int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n
             fit_packed(ctx, X, offsets, n_sets, d, dtype, r2, c_sim, m_core, precision, member_cutoff,
                        max_clusters, labels, size_bits, ds_bits, gs, ls);
         } else {
-            const size_t esize = dtype == CNN_F64 ? sizeof(double) : sizeof(float);
-            for (int32_t b = 0; b < n_sets; ++b) {
-                const int64_t n = offsets[b + 1] - offsets[b];
-                const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
-                cnn_graph *g = graph_build(ctx, xb, n, d, dtype, r2[b], precision, nullptr, true);
-                std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
-                LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
-                               true, true, member_cutoff, max_clusters, labels + offsets[b],
-                               g->tp_ptr, g->tp_col, g->n_tiles};
-                cnn_label_stats one{};
-                labels_run(ctx, r, &one);
-                add_graph_stats(gs, g->stats);
-                add_label_stats(ls, one);
-            }
+            fit_each(ctx, X, offsets, n_sets, d, dtype, r2, c_sim, m_core, precision, member_cutoff, max_clusters,
+                     labels, gs, ls);
         }
         if (gstats) *gstats = gs;
         if (lstats) *lstats = ls;

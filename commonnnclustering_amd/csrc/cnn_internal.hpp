@@ -19,6 +19,7 @@ constexpr int kWave = 64;
 constexpr int kSpatialMaxD = 8;  // d <= 8: spatial (Morton) order + tile-box culling
 constexpr int kMfmaMinD = 16;    // d >= 16: FP32 Gram form on MFMA in CNN_F32_BAND
 constexpr int kMfmaColChunk = 64;   // column tiles per workgroup of the dense LDS MFMA kernel
+constexpr int kBatchWorkers = 6;    // datasets of a batch fitted at once (own stream each)
 constexpr int64_t kPivotMinN = 2048;  // d > 8: smallest n ordered by nearest-pivot cells
 
 struct Error {
@@ -54,6 +55,9 @@ struct cnn_ctx {
     std::string last_error;
     hipEvent_t ev[8] = {};
     int cu_count = 256;
+    // contexts (own stream and arena each) that fit independent datasets of a
+    // batch concurrently; created on first use, destroyed with this context
+    std::vector<cnn_ctx *> workers;
 };
 
 struct cnn_graph {

@@ -197,6 +197,8 @@ int cnn_ctx_create(int device, cnn_ctx **out) {
 
 int cnn_ctx_destroy(cnn_ctx *ctx) {
     if (!ctx) return CNN_ERR_INVALID;
+    for (cnn_ctx *w : ctx->workers) cnn_ctx_destroy(w);
+    ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     delete ctx->arena;

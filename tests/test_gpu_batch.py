@@ -115,3 +115,18 @@ def test_batch_of_one_equals_fit(cuda):
     lab, _, _, _ = engine.fit(Xd, r * r, p["cnn_cutoff"] + 2, p["cnn_cutoff"] + 1, precision="f32_band",
                               member_cutoff=10)
     assert (got[0] == lab.cpu().numpy()).all()
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_batch_many_sets_concurrent(cuda, d):
+    """More datasets than worker streams, uneven sizes, empty ones included:
+    every dataset's labels equal a fit of it alone."""
+    sets = _small_sets(d, 30, 500 + d)
+    sets[4] = (np.zeros((0, d), dtype=np.float32), sets[4][1])
+    sets[17] = (sets[17][0][:1], sets[17][1])
+    want = _oracle_each(sets, member_cutoff=2)
+    for _ in range(2):  # the worker contexts are reused by the second call
+        got, gs, ls = _run_batch(sets, "f32_band", member_cutoff=2)
+        for b, (g, w) in enumerate(zip(got, want)):
+            assert (g == w).all(), f"dataset {b} (n={len(w)})"
+        assert ls["n_clusters"] == sum(int(w.max(initial=0)) for w in want)
