@@ -93,30 +93,22 @@ static void fit_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int3
     CNN_HIP(hipStreamSynchronize(s));
 }
 
-// Datasets fitted independently, several at once: host threads take datasets
-// (largest first) from a shared counter, each on a worker context with its
-// own stream and arena, so the device overlaps the kernels, copies and host
-// round trips of different datasets.  Work already queued on the caller's
-// stream is finished first; every dataset's work is finished on return.
-static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d, int32_t dtype,
-                     const double *r2, const int32_t *c_sim, const int32_t *m_core, int32_t precision,
-                     int64_t member_cutoff, int64_t max_clusters, int64_t *labels, cnn_graph_stats &gs,
-                     cnn_label_stats &ls) {
-    if (n_sets == 0) return;
+// Runs item(wc, i) for i = 0 .. count-1 on up to kBatchWorkers worker contexts
+// (own stream and arena each) from as many host threads, items taken in the
+// given order from a shared counter.  Work already queued on the caller's
+// stream is finished first; each item finishes its own work before the next
+// one starts on that worker, so all of it is done on return.  The first
+// exception of any item is rethrown after every thread has stopped.
+template <typename F>
+static void run_on_workers(cnn_ctx *ctx, int32_t count, const std::vector<int32_t> &order, F &&item) {
+    if (count <= 0) return;
     CNN_HIP(hipStreamSynchronize(ctx->stream));
-    const int n_workers = std::min<int>(kBatchWorkers, n_sets);
+    const int n_workers = std::min<int>(kBatchWorkers, count);
     while ((int)ctx->workers.size() < n_workers) {
         cnn_ctx *w = nullptr;
-        const int rc = cnn_ctx_create(ctx->device, &w);
-        if (rc != CNN_OK) throw HipFailure{hipErrorUnknown, "worker context creation failed"};
+        if (cnn_ctx_create(ctx->device, &w) != CNN_OK) throw HipFailure{hipErrorUnknown, "worker context creation failed"};
         ctx->workers.push_back(w);
     }
-    std::vector<int32_t> order(n_sets);
-    for (int32_t b = 0; b < n_sets; ++b) order[b] = b;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
-    });
-    const size_t esize = dtype == CNN_F64 ? sizeof(double) : sizeof(float);
     std::atomic<int32_t> next{0};
     std::atomic<bool> failed{false};
     std::mutex mu;
@@ -125,22 +117,10 @@ static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_
         try {
             CNN_HIP(hipSetDevice(wc->device));
             for (;;) {
-                const int32_t i = next.fetch_add(1);
-                if (i >= n_sets || failed.load()) break;
-                const int32_t b = order[i];
-                const int64_t n = offsets[b + 1] - offsets[b];
-                const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
-                cnn_graph *g = graph_build(wc, xb, n, d, dtype, r2[b], precision, nullptr, true);
-                std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
-                LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
-                               true, true, member_cutoff, max_clusters, labels + offsets[b],
-                               g->tp_ptr, g->tp_col, g->n_tiles};
-                cnn_label_stats one{};
-                labels_run(wc, r, &one);
+                const int32_t k = next.fetch_add(1);
+                if (k >= count || failed.load()) break;
+                item(wc, order[k], mu);
                 CNN_HIP(hipStreamSynchronize(wc->stream));
-                std::lock_guard<std::mutex> lock(mu);
-                add_graph_stats(gs, g->stats);
-                add_label_stats(ls, one);
             }
         } catch (...) {
             failed.store(true);
@@ -153,6 +133,35 @@ static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_
     work(ctx->workers[0]);
     for (auto &t : threads) t.join();
     if (first_error) std::rethrow_exception(first_error);
+}
+
+// Datasets fitted independently, several at once (largest first), so the
+// device overlaps the kernels, copies and host round trips of different
+// datasets.
+static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d, int32_t dtype,
+                     const double *r2, const int32_t *c_sim, const int32_t *m_core, int32_t precision,
+                     int64_t member_cutoff, int64_t max_clusters, int64_t *labels, cnn_graph_stats &gs,
+                     cnn_label_stats &ls) {
+    std::vector<int32_t> order(std::max(n_sets, 0));
+    for (int32_t b = 0; b < n_sets; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+    });
+    const size_t esize = dtype == CNN_F64 ? sizeof(double) : sizeof(float);
+    run_on_workers(ctx, n_sets, order, [&](cnn_ctx *wc, int32_t b, std::mutex &mu) {
+        const int64_t n = offsets[b + 1] - offsets[b];
+        const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
+        cnn_graph *g = graph_build(wc, xb, n, d, dtype, r2[b], precision, nullptr, true);
+        std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
+                       true, true, member_cutoff, max_clusters, labels + offsets[b], g->tp_ptr, g->tp_col,
+                       g->n_tiles};
+        cnn_label_stats one{};
+        labels_run(wc, r, &one);
+        std::lock_guard<std::mutex> lock(mu);
+        add_graph_stats(gs, g->stats);
+        add_label_stats(ls, one);
+    });
 }
 
 extern "C" {
@@ -249,14 +258,17 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
                     col = own_col;
                     cur_r2 = r2[ri];
                 }
-                for (int32_t ci = 0; ci < n_c; ++ci) {
+                // the cutoffs of one radius read the same lists: labelled concurrently
+                std::vector<int32_t> cis(n_c);
+                for (int32_t ci = 0; ci < n_c; ++ci) cis[ci] = ci;
+                run_on_workers(ctx, n_c, cis, [&](cnn_ctx *wc, int32_t ci, std::mutex &) {
                     LabelRequest req{rp, col, n, g->perm, c_sim[ci], m_core[ci], nullptr, 1, CNN_SIM_AUTO, true, true,
                                      member_cutoff, max_clusters, labels + ((int64_t)ri * n_c + ci) * n,
                                      g->tp_ptr, g->tp_col, g->n_tiles};
                     cnn_label_stats one{};
-                    labels_run(ctx, req, &one);
+                    labels_run(wc, req, &one);
                     if (lstats) lstats[(int64_t)ri * n_c + ci] = one;
-                }
+                });
             }
         } catch (...) {
             release();
