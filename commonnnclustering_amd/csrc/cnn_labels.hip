@@ -229,26 +229,41 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
-    for (int64_t t0 = b; t0 < e; t0 += 64) {
-        const int64_t t = t0 + lane;
-        const int j = t < e ? a.col[t] : -1;
-        uint64_t rj = 0ull;
-        if (t < e) {
-            const uint64_t tr = a.tile_rank ? a.tile_rank[j >> 6] : ~0ull;
-            rj = tr != ~0ull ? tr : a.rank[j];
+    // four 64-entry windows per step: their loads and rank lookups are issued
+    // together, so a row's scan waits on one memory round trip per 256 entries
+    constexpr int kWin = 4;
+    for (int64_t t0 = b; t0 < e; t0 += 64 * kWin) {
+        int j[kWin];
+        uint64_t rj[kWin];
+#pragma unroll
+        for (int u = 0; u < kWin; ++u) {
+            const int64_t t = t0 + 64 * u + lane;
+            j[u] = t < e ? a.col[t] : -1;
         }
-        const int kj = (int)(uint32_t)rj;
-        const bool cand = rj > rw && kj != resolved;
-        uint64_t bits = __ballot(cand);
-        while (bits) {
-            const int src = __ffsll((unsigned long long)bits) - 1;
-            bits &= bits - 1;
-            const int jj = __shfl(j, src, 64);
-            const int kjj = __shfl(kj, src, 64);
-            if (test_and_link(a, (int)w, jj, b, e, c, tests, passed)) {
-                // w has joined snapshot component kjj: its other entries are joined too
-                resolved = kjj;
-                bits &= ~__ballot(kj == kjj);
+#pragma unroll
+        for (int u = 0; u < kWin; ++u) {
+            uint64_t tr = 0ull;
+            if (j[u] >= 0) tr = a.tile_rank ? a.tile_rank[j[u] >> 6] : ~0ull;
+            rj[u] = tr;
+        }
+#pragma unroll
+        for (int u = 0; u < kWin; ++u)
+            if (rj[u] == ~0ull) rj[u] = a.rank[j[u]];
+#pragma unroll
+        for (int u = 0; u < kWin; ++u) {
+            const int kj = (int)(uint32_t)rj[u];
+            const bool cand = rj[u] > rw && kj != resolved;
+            uint64_t bits = __ballot(cand);
+            while (bits) {
+                const int src = __ffsll((unsigned long long)bits) - 1;
+                bits &= bits - 1;
+                const int jj = __shfl(j[u], src, 64);
+                const int kjj = __shfl(kj, src, 64);
+                if (test_and_link(a, (int)w, jj, b, e, c, tests, passed)) {
+                    // w has joined snapshot component kjj: its other entries are joined too
+                    resolved = kjj;
+                    bits &= ~__ballot(kj == kjj);
+                }
             }
         }
     }
