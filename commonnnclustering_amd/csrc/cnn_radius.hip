@@ -1471,6 +1471,29 @@ __global__ void k_pair_info(const int64_t *__restrict__ tp_ptr, const int32_t *_
 // broadcast to the wave, lane b holds bit b, and the set bits go out as one
 // contiguous store, each at its rank among the word's set bits (mbcnt) after
 // the word's prefix.  Rows come out sorted: candidate tiles ascend, then bits.
+// Position of the t-th (from 0) set bit of w; t < popcount(w).
+__device__ __forceinline__ int select_bit(uint64_t w, int t) {
+    int pos = 0;
+    uint32_t x = (uint32_t)w;
+    int c = __popc(x);
+    if (t >= c) {
+        t -= c;
+        pos = 32;
+        x = (uint32_t)(w >> 32);
+    }
+#pragma unroll
+    for (int width = 16; width >= 2; width >>= 1) {
+        c = __popc(x & ((1u << width) - 1u));
+        if (t >= c) {
+            t -= c;
+            pos += width;
+            x >>= width;
+        }
+    }
+    return pos + (t >= (int)(x & 1u) ? 1 : 0);
+}
+
+template <bool OUTPUT_PARALLEL>
 __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
                                                    int32_t *__restrict__ col) {
     __shared__ uint64_t sm[64 * 65];
@@ -1528,6 +1551,27 @@ __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__
             if (total == 0) continue;
             const int64_t base = __shfl(mypos, rr, 64);
             const int excl = incl - pc;
+            if constexpr (OUTPUT_PARALLEL) {
+                // output-parallel: lane l writes entry E + l; its word is the first
+                // with inclusive count > E + l, its bit the rank within that word
+                for (int E = 0; E < total; E += 64) {
+                    const int e = E + lane;
+                    int lo = 0, hi = 63;
+#pragma unroll
+                    for (int step = 0; step < 6; ++step) {
+                        const int mid = (lo + hi) >> 1;
+                        if (__shfl(incl, mid, 64) > e) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const int q = min(lo, 63);
+                    const int t = e - __shfl(excl, q, 64);
+                    const uint64_t wq = __shfl(v, q, 64);
+                    const int cq = __shfl(cbase, q, 64);
+                    if (e < total) col[base + e] = cq + select_bit(wq, t);
+                }
+                if (lane == rr) mypos += total;
+                continue;
+            }
             uint64_t nz = __ballot(v != 0ull);
             while (nz) {
                 const int q = __ffsll((unsigned long long)nz) - 1;
@@ -2035,7 +2079,14 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
             g->col = col;
         }
         CNN_HIP(hipEventRecord(ctx->ev[4], s));
-        if (rows > 0) k_emit_rows<<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
+        // word-serial emission for the spatial layout (words hold many bits each),
+        // output-parallel for the dense layout (few bits per word)
+        if (rows > 0) {
+            if (g->layout == 1)
+                k_emit_rows<false><<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
+            else
+                k_emit_rows<true><<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
+        }
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
         CNN_HIP(hipEventSynchronize(ctx->ev[5]));
