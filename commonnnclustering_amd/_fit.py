@@ -10,11 +10,14 @@ GPU neighbour search + similarity test + union-find.
 
 from __future__ import annotations
 
+import weakref
 from abc import ABC, abstractmethod
+from collections.abc import Iterable
 
 import numpy as np
 
 from . import _lib
+from ._primitive_types import P_AINDEX
 from ._types import (
     ClusterParameters,
     InputDataComponents,
@@ -187,7 +190,292 @@ class FitterGPU(Fitter):
         return lab.cpu().numpy()
 
 
+    def fit_steps(self, input_data, params_list, sort_by_size: bool = True, member_cutoff=None,
+                  max_clusters=None):
+        """Fresh labels (host int64 arrays) for a list of ClusterParameters.
+
+        Each entry equals fit() on zero labels with every point considered
+        and current_start 1, followed by Labels.sort_by_size(member_cutoff,
+        max_clusters) when ``sort_by_size``.  Steps of one radius share one
+        graph; smaller radii are cut from the graph of the largest by exact
+        FP64 re-thresholding (not for the periodic metric, whose graphs are
+        built per radius).
+        """
+        import torch
+        from . import engine
+
+        results = [None] * len(params_list)
+        if not params_list:
+            return results
+        if isinstance(input_data, InputDataNeighbourhoods) or isinstance(self._neighbours_getter,
+                                                                        NeighboursGetterLookup) \
+                or self._neighbours_getter.precomputed:
+            csr = {}
+            for k, p in enumerate(params_list):
+                key = None if isinstance(input_data, InputDataNeighbourhoods) or isinstance(
+                    self._neighbours_getter, NeighboursGetterLookup) else p.radius_cutoff
+                if key not in csr:
+                    csr[key] = (input_data.device_csr() if key is None
+                                else self._neighbours_getter.get_precomputed_csr(input_data, p))
+                row_ptr, col = csr[key]
+                lab = torch.zeros(input_data.n_points, dtype=torch.int64, device=row_ptr.device)
+                engine.labels_from_csr(row_ptr, col, p.similarity_cutoff, p.n_member_cutoff, current_start=1,
+                                       labels=lab, sim_mode=self._sim_mode())
+                if sort_by_size:
+                    engine.sort_by_size(lab, member_cutoff, max_clusters)
+                results[k] = lab.cpu().numpy()
+            return results
+        ng = self._neighbours_getter
+        by_thr = {}
+        for k, p in enumerate(params_list):
+            by_thr.setdefault(ng.squared_threshold(p), []).append(k)
+        periodic = ng.periodic_bounds is not None
+        base = None
+        try:
+            for thr in sorted(by_thr, reverse=True):
+                steps = by_thr[thr]
+                if base is None or periodic:
+                    g = ng.get_graph(input_data, params_list[steps[0]])
+                    if base is None:
+                        base = g
+                else:
+                    g = base.rethreshold(thr)
+                try:
+                    for k in steps:
+                        p = params_list[k]
+                        lab, _ = g.fit(p.similarity_cutoff, p.n_member_cutoff, sort=sort_by_size,
+                                       member_cutoff=member_cutoff, max_clusters=max_clusters)
+                        results[k] = lab.cpu().numpy()
+                finally:
+                    if g is not base:
+                        g.release()
+        finally:
+            if base is not None:
+                base.release()
+        return results
+
+    def neighbour_csr(self, input_data, cluster_params: ClusterParameters):
+        """Neighbour lists at the radius of ``cluster_params``: (row_ptr int64[n+1],
+        col int32[nnz]) on the device, caller order, rows ascending."""
+        return neighbour_csr(self._neighbours_getter, input_data, cluster_params)
+
+
+def neighbour_csr(neighbours_getter, input_data, cluster_params: ClusterParameters):
+    """Device CSR of the neighbourhoods NeighboursGetter.get gives for every
+    point (caller order, rows ascending, self included for coordinates)."""
+    if isinstance(input_data, InputDataNeighbourhoods) or isinstance(neighbours_getter, NeighboursGetterLookup):
+        return input_data.device_csr()
+    if neighbours_getter.precomputed:
+        return neighbours_getter.get_precomputed_csr(input_data, cluster_params)
+    graph = neighbours_getter.get_graph(input_data, cluster_params)
+    try:
+        return graph.to_csr()
+    finally:
+        graph.release()
+
+
 FitterExtBFS = FitterGPU
+
+
+# ---------------------------------------------------------------- hierarchical fitters
+
+class HierarchicalFitter(ABC):
+    """Hierarchical fitter interface (_fit.pyx:88-126).
+
+    ``make_parameters`` looks up ``self._neighbours_getter`` as the fitter's
+    does; a hierarchical fitter without one (HierarchicalFitterRepeat) gets
+    the radius and cnn_cutoff unchanged, as in the reference.
+    """
+
+    @abstractmethod
+    def fit(self, bundle, **kwargs):
+        """Generic hierarchical clustering"""
+
+    make_parameters = Fitter.make_parameters
+
+
+def _as_list(v, cast):
+    if not isinstance(v, Iterable) or isinstance(v, (str, bytes)):
+        v = [v]
+    return [cast(x) for x in v]
+
+
+class HierarchicalFitterRepeat(HierarchicalFitter):
+    """Hierarchy from repeated fits (HierarchicalFitterRepeat, _fit.pyx:1014-1187).
+
+    Step k fits all points at (radius_cutoff[k], cnn_cutoff[k]) (a scalar is
+    repeated to the other list's length) on fresh labels, then sorts them by
+    size.  The clusters of step k become children of the step-(k-1) cluster
+    holding their points: each terminal bundle of the previous step gets as
+    labels its points' step-k labels and is isolated (without input data);
+    the new children are the next terminal bundles.  The step fits run on
+    the GPU with one graph per radius.
+
+    ``make_parameters`` passes the radius and cnn_cutoff unchanged (this
+    fitter has no neighbours getter; the reference's does the same), so the
+    step fits compare reduced distances with ``radius_cutoff`` itself and use
+    ``cnn_cutoff`` as similarity and member cutoff.
+
+    The reference visits the parents of a step in std::unordered_map order;
+    here they are visited in ascending label.  The order matters only when
+    one cluster has points under several parents (possible when a later step
+    merges clusters, e.g. a growing radius): the parent visited last holds
+    the terminal bundle of that cluster, and the next step's isolate of it
+    fails with an IndexError, as the reference's does.  Steps that only split
+    clusters (radius not growing, cnn_cutoff not falling) never do this.
+    """
+
+    def __init__(self, fitter=None):
+        self._fitter = FitterGPU() if fitter is None else fitter
+
+    def __str__(self):
+        return f"{type(self).__name__}(fitter={self._fitter})"
+
+    @classmethod
+    def get_builder_kwargs(cls):
+        return [("fitter", None)]
+
+    def fit(self, bundle, **kwargs):
+        self._fit(bundle, kwargs["radius_cutoff"], kwargs["cnn_cutoff"], kwargs.get("member_cutoff"),
+                  kwargs.get("max_clusters"), kwargs.get("cnn_offset"), kwargs.get("sort_by_size", True),
+                  kwargs.get("info", True), kwargs.get("v", True))
+
+    def _step_labels(self, input_data, params_list, sort_by_size, member_cutoff, max_clusters):
+        if hasattr(self._fitter, "fit_steps"):
+            return self._fitter.fit_steps(input_data, params_list, sort_by_size, member_cutoff, max_clusters)
+        out = []
+        for p in params_list:
+            lab = Labels(np.zeros(input_data.n_points, order="C", dtype=P_AINDEX))
+            self._fitter.fit(input_data, lab, p)
+            if sort_by_size:
+                lab.sort_by_size(member_cutoff, max_clusters)
+            out.append(lab.labels.copy())
+        return out
+
+    def _fit(self, bundle, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None, cnn_offset=None,
+             sort_by_size=True, info=True, v=True):
+        radius_cutoff = _as_list(radius_cutoff, float)
+        cnn_cutoff = _as_list(cnn_cutoff, int)
+        if len(radius_cutoff) == 1:
+            radius_cutoff *= len(cnn_cutoff)
+        if len(cnn_cutoff) == 1:
+            cnn_cutoff *= len(radius_cutoff)
+        n_steps = len(radius_cutoff)
+        assert n_steps == len(cnn_cutoff)
+        if cnn_offset is None:
+            cnn_offset = 0
+        n = bundle._input_data.n_points
+        params_list = [self.make_parameters(radius_cutoff[k], cnn_cutoff[k] - cnn_offset, 1)
+                       for k in range(n_steps)]
+        step_labels = self._step_labels(bundle._input_data, params_list, sort_by_size, member_cutoff,
+                                        max_clusters)
+
+        previous = np.ones(n, order="C", dtype=P_AINDEX)
+        terminal = {1: bundle}
+        for step in range(n_steps):
+            if v:
+                print(f"Running step {step:<5} (r = {radius_cutoff[step]}, c = {cnn_cutoff[step]})")
+            current = np.ascontiguousarray(step_labels[step], dtype=P_AINDEX)
+            new_terminal = {}
+            points = np.flatnonzero(previous)
+            order = points[np.argsort(previous[points], kind="stable")]
+            plabs = previous[order]
+            starts = np.flatnonzero(np.r_[True, plabs[1:] != plabs[:-1]]) if len(order) else np.zeros(0, int)
+            ends = np.r_[starts[1:], len(order)]
+            for s, e in zip(starts.tolist(), ends.tolist()):
+                parent = terminal[int(plabs[s])]
+                parent._labels = Labels.from_sequence(current[order[s:e]])
+                if info:
+                    params = {k: (radius_cutoff[step], cnn_cutoff[step]) for k in parent._labels.to_set() if k != 0}
+                    parent._labels.meta.update({"params": params, "reference": weakref.proxy(parent),
+                                                "origin": "fit"})
+                parent.isolate(isolate_input_data=False)
+                for c_label, child in parent._children.items():
+                    if c_label == 0:
+                        continue
+                    new_terminal[c_label] = child
+            terminal = new_terminal
+            previous = current
+
+
+class HierarchicalFitterMSTPrim(HierarchicalFitter):
+    """MST cluster hierarchy (HierarchicalFitterMSTPrim, _fit.pyx:733-1011).
+
+    The neighbour graph at ``radius_cutoff`` is weighted by common-neighbour
+    counts and its maximum spanning forest is grown by Prim's procedure with
+    the reference's queue order (weights and forest in the native library:
+    cnn_similarity_mst).  The tree edges are then merged in order of
+    decreasing weight into a hierarchy of bundles whose ``_graph`` holds the
+    merged edges and whose ``_lambda`` is the weight of the merging edge;
+    children are pruned and folded by ``check_children`` with
+    ``member_cutoff`` (default 10).  Labels of the root are all 1 with every
+    consider flag 0, as the reference leaves them.
+
+    Args:
+        neighbours_getter: ``NeighboursGetterGPU`` (coordinates, precomputed
+            distances) or ``NeighboursGetterLookup``.
+        neighbours, neighbour_neighbours, similarity_checker, priority_queue,
+        priority_queue_tree: accepted for recipe compatibility
+            (_fit.pyx:782-790); the queues' order is built into the native
+            forest.
+    """
+
+    def __init__(self, neighbours_getter=None, neighbours=None, neighbour_neighbours=None,
+                 similarity_checker=None, priority_queue=None, priority_queue_tree=None):
+        self._neighbours_getter = NeighboursGetterGPU() if neighbours_getter is None else neighbours_getter
+        self._neighbours = neighbours
+        self._neighbour_neighbours = neighbour_neighbours
+        self._similarity_checker = SimilarityCheckerGPU() if similarity_checker is None else similarity_checker
+        self._priority_queue = priority_queue
+        self._priority_queue_tree = priority_queue_tree
+        self.last_tree = None
+
+    def __str__(self):
+        return (f"{type(self).__name__}(ngetter={self._neighbours_getter}, na={self._neighbours}, "
+                f"nb={self._neighbour_neighbours}, checker={self._similarity_checker}, "
+                f"prioq={self._priority_queue}, prioq (tree)={self._priority_queue_tree})")
+
+    @classmethod
+    def get_builder_kwargs(cls):
+        return [
+            ("neighbours_getter", None),
+            ("neighbours", None),
+            ("neighbour_neighbours", "neighbours"),
+            ("similarity_checker", None),
+            ("priority_queue", None),
+            ("priority_queue_tree", "priority_queue"),
+        ]
+
+    def fit(self, bundle, **kwargs):
+        self._fit(bundle, kwargs["radius_cutoff"], kwargs.get("member_cutoff"), kwargs.get("max_clusters"),
+                  kwargs.get("cnn_offset"), kwargs.get("sort_by_size", True), kwargs.get("info", True),
+                  kwargs.get("v", True))
+
+    def spanning_forest(self, input_data, radius_cutoff: float):
+        """Tree edges (a, b, weight) as host int64 arrays sorted by (-weight, a, b)."""
+        from . import engine
+
+        params = self.make_parameters(radius_cutoff, 0, 1)
+        row_ptr, col = neighbour_csr(self._neighbours_getter, input_data, params)
+        return engine.similarity_mst(row_ptr, col)
+
+    def _fit(self, bundle, radius_cutoff, member_cutoff=None, max_clusters=None, cnn_offset=None,
+             sort_by_size=True, info=True, v=True):
+        try:
+            import networkx  # noqa: F401
+        except ModuleNotFoundError:  # pragma: no cover
+            raise ModuleNotFoundError("No module named 'networkx'") from None
+        from ._bundle import build_mst_hierarchy
+
+        n = bundle._input_data.n_points
+        labels = Labels(np.ones(n, order="C", dtype=P_AINDEX))
+        bundle._labels = labels
+        if member_cutoff is None:
+            member_cutoff = 10
+        a, b, w = self.spanning_forest(bundle._input_data, radius_cutoff)
+        labels._consider[:] = 0
+        self.last_tree = (a, b, w)
+        build_mst_hierarchy(bundle, n, a.tolist(), b.tolist(), w.tolist(), member_cutoff)
 
 
 class Predictor(ABC):

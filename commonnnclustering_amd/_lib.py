@@ -44,7 +44,7 @@ EXPORTED = [
     "cnn_graph_bytes", "cnn_graph_export_csr", "cnn_radius_csr", "cnn_graph_rethreshold",
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
     "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_graph_build_periodic",
-    "cnn_version",
+    "cnn_csr_edge_similarity", "cnn_similarity_mst", "cnn_version",
 ]
 
 
@@ -143,6 +143,8 @@ def _declare(L):
         "cnn_precomputed_csr": (c, [_vp, _vp, _i64, _i32, ctypes.c_double, _vp, _vp, _i64,
                                     ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
         "cnn_graph_build_periodic": (c, [_vp, _vp, _i64, _i32, _i32, ctypes.c_double, _vp, ctypes.POINTER(_vp)]),
+        "cnn_csr_edge_similarity": (c, [_vp, _vp, _vp, _i64, _vp]),
+        "cnn_similarity_mst": (c, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64)]),
         "cnn_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

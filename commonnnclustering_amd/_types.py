@@ -19,6 +19,7 @@ and for code written against the reference's Python-level API.
 
 from __future__ import annotations
 
+import heapq
 from abc import ABC, abstractmethod
 from collections import defaultdict
 from typing import Optional, Sequence
@@ -174,6 +175,25 @@ class Labels:
         self._labels[:] = new
 
 
+class ReferenceIndices:
+    """Root and parent indices relating a child clustering to its parent
+    (_types.pyx:289-302)."""
+
+    def __init__(self, root_indices, parent_indices):
+        if root_indices is None or parent_indices is None:
+            raise TypeError("root_indices and parent_indices must not be None")
+        self._root = root_indices
+        self._parent = parent_indices
+
+    @property
+    def root(self):
+        return np.asarray(self._root)
+
+    @property
+    def parent(self):
+        return np.asarray(self._parent)
+
+
 # ---------------------------------------------------------------- input data
 
 class InputData(ABC):
@@ -251,7 +271,19 @@ class InputDataComponents(InputData):
         return float(self.to_components_array()[point, dimension])
 
     def get_subset(self, indices: Sequence) -> "InputDataComponents":
-        return type(self)(self.to_components_array()[indices])
+        """Rows ``indices`` of the coordinates, or the sub-matrix ``[indices][:, indices]``
+        of a distance matrix (meta["access_dists"]) (_types.pyx:1064-1086)."""
+        has_coordinates = self.meta.get("access_coords", False)
+        has_distances = self.meta.get("access_dists", False)
+        if has_distances and has_coordinates:
+            raise RuntimeError(f"{type(self)} can not store coordinates and distances at the same time. "
+                               f"Set 'access_coords' or 'access_dists' in {type(self)}.meta accordingly to proceed.")
+        if has_distances:
+            return type(self)(np.array(self.to_components_array()[indices][:, indices], order="C"))
+        if has_coordinates:
+            return type(self)(self.to_components_array()[indices])
+        raise RuntimeError(f"No information whether coordinates or distances or stored. "
+                           f"Set 'access_coords' or 'access_dists' in {type(self)}.meta accordingly to proceed.")
 
     def by_parts(self):
         """Yield data by parts (meta['edges']) as InputDataExtComponentsMemoryview.by_parts."""
@@ -303,6 +335,25 @@ class InputDataNeighbourhoods(InputData):
 
     def get_neighbour(self, point: int, member: int) -> int:
         return int(self._data[point, member])
+
+    def get_subset(self, indices: Sequence) -> "InputDataNeighbourhoods":
+        """Neighbourhoods of the points ``indices``, each cut to members in
+        ``indices``; member indices keep the numbering of the full set, as
+        InputDataExtNeighbourhoodsMemoryview.get_subset (_types.pyx:1199-1217).
+        The reference filters the padded rows and constructs the subset
+        without ``n_neighbours`` (which its constructor requires); here only
+        the first ``n_neighbours[i]`` entries are filtered and the new lengths
+        are passed."""
+        keep = np.zeros(self.n_points, dtype=bool)
+        idx = np.asarray(indices, dtype=np.intp)
+        keep[idx] = True
+        rows = [[int(m) for m in self._data[i, :self._n_neighbours[i]] if keep[m]] for i in idx.tolist()]
+        lengths = [len(a) for a in rows]
+        width = max(lengths) if lengths else 0
+        padded = np.zeros((len(rows), width), dtype=P_AINDEX)
+        for i, a in enumerate(rows):
+            padded[i, :len(a)] = a
+        return type(self)(padded, np.asarray(lengths, dtype=P_AINDEX))
 
     def device_csr(self, device=None):
         """(row_ptr int64[n+1], col int32[nnz]) on the device, rows sorted ascending."""
@@ -596,6 +647,34 @@ class SimilarityCheckerGPU(SimilarityChecker):
 
     def __str__(self):
         return f"{type(self).__name__}(mode={self.mode})"
+
+
+class PriorityQueueMaxHeap:
+    """Max-heap of (a, b, weight) entries ordered by (-weight, a, b)
+    (PriorityQueueMaxHeap, _types.pyx:2825-2851).  The MST fitter's native
+    forest uses this same order; the class is kept for recipes and callers."""
+
+    def __init__(self):
+        self.reset()
+
+    def reset(self) -> None:
+        self._queue = []
+
+    def push(self, a, b, weight) -> None:
+        heapq.heappush(self._queue, (-weight, (a, b)))
+
+    def pop(self):
+        weight, (a, b) = heapq.heappop(self._queue)
+        return a, b, -weight
+
+    def is_empty(self) -> bool:
+        return not self._queue
+
+    def size(self):
+        return len(self._queue)
+
+    def __str__(self):
+        return f"{type(self).__name__}"
 
 
 class QueueUnused:

@@ -5,7 +5,9 @@ through the recipe builder; ``fit(radius_cutoff, cnn_cutoff, member_cutoff,
 ...)`` keeps the reference's contract: cnn_offset subtraction and the
 ``cnn_cutoff >= 0`` assertion, fresh or frozen label initialisation,
 ``make_parameters``, timed fitter call, label meta information,
-``sort_by_size`` and the ``Record`` appended to the summary.
+``sort_by_size`` and the ``Record`` appended to the summary.  The hierarchy
+methods (``fit_hierarchical``, ``isolate``, ``reel``, the trims and
+``to_nx_DiGraph``) work on the bundle tree (cluster.pyx:394-558, 1127-1164).
 """
 
 from __future__ import annotations
@@ -15,41 +17,14 @@ from collections import Counter
 
 import numpy as np
 
+from typing import Optional
+
 from ._primitive_types import P_AINDEX
 from ._types import InputData, Labels
-from ._fit import Fitter, FitterGPU, Predictor
-from .report import Record, Summary, timed
+from ._bundle import Bundle
+from ._fit import Fitter, FitterGPU, HierarchicalFitter, Predictor
+from .report import Record, timed
 from . import recipes
-
-
-class Bundle:
-    """Holder of input data, labels and summary (the container role of _bundle.pyx:20-272)."""
-
-    def __init__(self, input_data=None, labels=None, *, summary=None, meta=None):
-        self._input_data = input_data
-        self._labels = labels
-        self._children = None
-        self._summary = Summary() if summary is None else summary
-        self.meta = {} if meta is None else meta
-
-    @property
-    def input_data(self):
-        return None if self._input_data is None else self._input_data.data
-
-    @property
-    def labels(self):
-        return None if self._labels is None else self._labels.labels
-
-    @labels.setter
-    def labels(self, value):
-        if isinstance(value, Labels):
-            self._labels = value
-        else:
-            self._labels = Labels.from_sequence(value)
-
-    @property
-    def summary(self):
-        return self._summary
 
 
 class Clustering:
@@ -88,10 +63,14 @@ class Clustering:
         else:
             f = builder.make_component("fitter")
             self._fitter = None if f is object else f
-        if hierarchical_fitter is not None or "hierarchical_fitter" in builder.recipe:
-            raise NotImplementedError("hierarchical fitters are not part of this engine; "
-                                      "use commonnnclustering_amd.engine.fit_scan for (r, c) scans")
-        self._hierarchical_fitter = None
+        if hierarchical_fitter is not None:
+            if not isinstance(hierarchical_fitter, HierarchicalFitter):
+                raise TypeError(f"Can't use object of type {type(hierarchical_fitter).__name__} as hierarchical "
+                                f"fitter. Expected type {HierarchicalFitter.__name__}.")
+            self._hierarchical_fitter = hierarchical_fitter
+        else:
+            h = builder.make_component("hierarchical_fitter")
+            self._hierarchical_fitter = None if h is object else h
         if predictor is not None:
             if not isinstance(predictor, Predictor):
                 raise TypeError(f"Can't use object of type {type(predictor).__name__} as predictor. "
@@ -135,8 +114,20 @@ class Clustering:
         self._fitter = value
 
     @property
+    def children(self):
+        """Mapping of child cluster labels to the child bundles of the root."""
+        return self._bundle._children
+
+    @property
+    def hierarchical_fitter(self):
+        return self._hierarchical_fitter
+
+    @property
     def summary(self):
         return self._bundle._summary
+
+    def __getitem__(self, value):
+        return self._bundle.get_child(value)
 
     def __str__(self):
         return (f"{type(self).__name__}(input_data={None if self._bundle is None else self._bundle._input_data}, "
@@ -217,6 +208,129 @@ class Clustering:
             if v:
                 print(rec)
             bundle._summary.append(rec)
+
+    def fit_hierarchical(self, purge: bool = True, bundle=None, **fitter_kwargs) -> None:
+        """Run the hierarchical fitter on a bundle (cluster.pyx:394-411); keyword
+        arguments go to its ``fit`` (e.g. radius_cutoff, cnn_cutoff,
+        member_cutoff, max_clusters, cnn_offset, sort_by_size, info, v)."""
+        if bundle is None:
+            bundle = self._bundle
+        if self._hierarchical_fitter is None:
+            raise AttributeError("this clustering has no hierarchical fitter; pass hierarchical_fitter= "
+                                 "or a recipe with one")
+        if purge or bundle._children is None:
+            bundle._children = {}
+        self._hierarchical_fitter.fit(bundle, **fitter_kwargs)
+
+    def isolate(self, purge: bool = True, isolate_input_data: bool = True, bundle=None) -> None:
+        """Child clusterings from the cluster labels (cluster.pyx:413-433)."""
+        if bundle is None:
+            bundle = self._bundle
+        bundle.isolate(purge, isolate_input_data)
+
+    def trim_shrinking_leafs(self, bundle=None) -> None:
+        """Drop the labels and children of hierarchy nodes below the last split
+        on each path (cluster.pyx:435-470): a node is kept if it is new (its
+        parent split), splits itself or has a splitting descendant."""
+        if bundle is None:
+            bundle = self._bundle
+
+        def _trim_shrinking(node, new=True):
+            if not node._children:
+                splits = will_split = False
+            else:
+                label_set = node._labels.set
+                label_set.discard(0)
+                splits = len(label_set) > 1
+                will_split = any([_trim_shrinking(child, new=splits) for child in node._children.values()])
+            keep = new or will_split or splits
+            if not keep:
+                node._labels = None
+                node._children = {}
+            return keep
+
+        _trim_shrinking(bundle)
+
+    def trim_trivial_leafs(self, bundle=None) -> None:
+        """Reset labels and children of nodes whose labels are all noise
+        (cluster.pyx:472-500)."""
+        if bundle is None:
+            bundle = self._bundle
+
+        def _trim_trivial(node):
+            if node._labels is None:
+                return
+            if node._labels.set == {0}:
+                node._labels = None
+                node._children = {}
+                return
+            if not node._children:
+                return
+            for child in node._children.values():
+                _trim_trivial(child)
+
+        _trim_trivial(bundle)
+
+    def reel(self, depth: Optional[int] = None, bundle=None) -> None:
+        """Fold the labels of lower hierarchy levels into their parents
+        (cluster.pyx:502-558): for each labelled child (recursively, up to
+        ``depth`` levels), its points get the child's labels shifted past the
+        parent's current maximum label (noise stays 0), and the child's
+        ``params`` move to the parent under the shifted labels."""
+        if bundle is None:
+            bundle = self._bundle
+
+        def _reel(parent, depth):
+            if not parent._children:
+                return
+            if depth is not None:
+                depth -= 1
+            parent._labels.meta["origin"] = "reel"
+            parent_labels = parent._labels.labels
+            for label, child in parent._children.items():
+                if child._labels is None:
+                    continue
+                if depth is None or depth > 0:
+                    _reel(child, depth)
+                n_clusters = max(parent_labels)
+                child_labels = child._labels.labels
+                parent_labels[child.parent_indices] = np.where(child_labels == 0, 0, child_labels + n_clusters)
+                try:
+                    _ = parent._labels.meta["params"].pop(label)
+                except KeyError:
+                    pass
+                params = child._labels.meta.get("params", {})
+                for old_label, p in params.items():
+                    parent._labels.meta["params"][old_label + n_clusters] = p
+
+        if depth is not None:
+            assert depth > 0
+        _reel(bundle, depth)
+
+    def to_nx_DiGraph(self, ignore=None, bundle=None):
+        """The cluster hierarchy as a networkx DiGraph (cluster.pyx:1127-1164):
+        nodes are dotted label paths from "1" (the root), each with the bundle
+        as attribute ``object``; children in ascending label order."""
+        import networkx as nx
+
+        if bundle is None:
+            bundle = self._bundle
+        ignore = set() if ignore is None else set(ignore)
+
+        def add_children(node_label, node, graph):
+            for child_label, child in sorted(node._children.items()):
+                if child_label in ignore:
+                    continue
+                padded = ".".join([node_label, str(child_label)])
+                graph.add_node(padded, object=child)
+                graph.add_edge(node_label, padded)
+                if child._children:
+                    add_children(padded, child, graph)
+
+        graph = nx.DiGraph()
+        graph.add_node("1", object=bundle)
+        add_children("1", bundle, graph)
+        return graph
 
     def predict(self, other, radius_cutoff: float, cnn_cutoff: int, clusters=None, cnn_offset: int = None,
                 info: bool = True, record: bool = True, record_time: bool = True, v: bool = True,

@@ -178,6 +178,9 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
                        bool transient = false);
 void graph_free(cnn_graph *g);
 void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col);
+// Throws InvalidArg unless row_ptr (n+1, from 0) is non-decreasing and every
+// row of col is strictly ascending within [0, n).
+void validate_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n);
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);
 int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int64_t *rp, const int32_t *col,
                    double r2, int64_t *out_rp, int32_t **out_col);
@@ -344,6 +347,41 @@ __device__ inline bool wave_sim(const int32_t *__restrict__ A, int64_t na, const
         if ((int64_t)count + min(na - ia, nb - ib) < (int64_t)c) return false;
     }
     return false;
+}
+
+// |A ∩ B| of two strictly ascending lists (the count SimilarityChecker._get
+// returns, _types.pyx:2612-2634): the merge of wave_sim without its early exits.
+// Every lane of the wave must call it with the same arguments.
+__device__ inline int wave_common(const int32_t *__restrict__ A, int64_t na, const int32_t *__restrict__ B,
+                                  int64_t nb) {
+    const int lane = __lane_id();
+    int64_t ia = 0, ib = 0;
+    int count = 0;
+    while (ia < na && ib < nb) {
+        const int va = (int)min<int64_t>(64, na - ia);
+        const int vb = (int)min<int64_t>(64, nb - ib);
+        const int a = lane < va ? A[ia + lane] : INT_MAX;
+        const int b = lane < vb ? B[ib + lane] : INT_MAX;
+        const int alast = __shfl(a, va - 1, 64);
+        const int blast = __shfl(b, vb - 1, 64);
+        const int m = min(alast, blast);
+        int lo = 0, hi = va;
+#pragma unroll
+        for (int step = 0; step < 7; ++step) {
+            const int mid = (lo + hi) >> 1;
+            const int am = __shfl(a, min(mid, 63), 64);
+            if (lo < hi) {
+                if (am < b) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        const int found = __shfl(a, min(lo, 63), 64);
+        const bool hit = (lane < vb) && (b <= m) && (lo < va) && (found == b);
+        count += __popcll(__ballot(hit));
+        ia += __popcll(__ballot(lane < va && a <= m));
+        ib += __popcll(__ballot(lane < vb && b <= m));
+    }
+    return count;
 }
 
 }  // namespace cnn

@@ -639,7 +639,7 @@ __global__ void k_validate_csr(const int64_t *__restrict__ row_ptr, const int32_
     }
 }
 
-static void validate_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n) {
+void validate_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n) {
     if (n == 0) return;
     hipStream_t s = ctx->stream;
     const int64_t first = read_scalar(ctx, row_ptr);

@@ -378,6 +378,52 @@ def precomputed_csr(D: torch.Tensor, r: float):
     return row_ptr, col, int(asym.value)
 
 
+def _check_csr(row_ptr: torch.Tensor, col: torch.Tensor):
+    if not (row_ptr.is_cuda and col.is_cuda):
+        raise ValueError("row_ptr and col must be CUDA tensors")
+    if row_ptr.dtype != torch.int64 or col.dtype != torch.int32:
+        raise ValueError("row_ptr must be int64 and col int32")
+    if not (row_ptr.is_contiguous() and col.is_contiguous()) or row_ptr.dim() != 1 or col.dim() != 1:
+        raise ValueError("row_ptr and col must be contiguous 1-D tensors")
+    n = row_ptr.shape[0] - 1
+    if n < 0:
+        raise ValueError("row_ptr must hold n + 1 offsets")
+    return n
+
+
+def edge_similarity(row_ptr: torch.Tensor, col: torch.Tensor) -> torch.Tensor:
+    """weight[t] = |N(a) ∩ N(b)| for the entry t of row a holding b
+    (SimilarityChecker._get, _types.pyx:2612-2634): int32[nnz] on the device.
+    Rows must be strictly ascending (checked on the device)."""
+    n = _check_csr(row_ptr, col)
+    ctx = get_context(row_ptr.device)
+    weight = torch.empty(max(col.shape[0], 1), dtype=torch.int32, device=col.device)
+    if n > 0 and int(row_ptr[-1].item()) > col.shape[0]:
+        raise ValueError("col holds fewer entries than row_ptr[n]")
+    ctx.check(_lib.lib().cnn_csr_edge_similarity(ctx.handle, ptr(row_ptr), ptr(col), n, ptr(weight)),
+              "cnn_csr_edge_similarity")
+    return weight[:col.shape[0]]
+
+
+def similarity_mst(row_ptr: torch.Tensor, col: torch.Tensor):
+    """Maximum spanning forest of the similarity-weighted lists in the order
+    of HierarchicalFitterMSTPrim's Prim loop (_fit.pyx:832-915): returns host
+    int64 arrays (a, b, weight), sorted by (-weight, a, b)."""
+    n = _check_csr(row_ptr, col)
+    if n > 0 and int(row_ptr[-1].item()) > col.shape[0]:
+        raise ValueError("col holds fewer entries than row_ptr[n]")
+    ctx = get_context(row_ptr.device)
+    cap = max(n - 1, 0)
+    a = np.empty(max(cap, 1), dtype=np.int64)
+    b = np.empty_like(a)
+    w = np.empty_like(a)
+    k = ctypes.c_int64(0)
+    ctx.check(_lib.lib().cnn_similarity_mst(ctx.handle, ptr(row_ptr), ptr(col), n, a.ctypes.data, b.ctypes.data,
+                                            w.ctypes.data, cap, ctypes.byref(k)), "cnn_similarity_mst")
+    m = k.value
+    return a[:m], b[:m], w[:m]
+
+
 def sqrt_decision_threshold(r: float) -> float:
     """Largest double s with fl(sqrt(s)) <= r, for IEEE (correctly rounded)
     sqrt.  The non-reduced Euclidean metric decides sqrt(s_ij) <= r

@@ -339,12 +339,17 @@ COMPONENT_NAME_TYPE_MAP = {
         "bfs": _fit.FitterGPU,
         "gpu": _fit.FitterGPU,
     },
-    "hierarchical_fitter": {},
+    "hierarchical_fitter": {
+        "repeat": _fit.HierarchicalFitterRepeat,
+        "mst": _fit.HierarchicalFitterMSTPrim,
+    },
     "predictor": {
         "firstmatch": _fit.PredictorGPU,
         "gpu": _fit.PredictorGPU,
     },
-    "priority_queue": {},
+    "priority_queue": {
+        "maxheap": _types.PriorityQueueMaxHeap,
+    },
 }
 
 # Reference components without a GPU implementation in this engine, with the reason.
@@ -354,9 +359,4 @@ UNSUPPORTED_COMPONENTS = {
     },
     "distance_getter": {"lookup": "DistanceGetterExtLookup (precomputed distances) is not implemented"},
     "fitter": {"bfs_debug": "FitterExtBFSDebug (step-by-step debug generator) is not implemented"},
-    "hierarchical_fitter": {
-        "repeat": "HierarchicalFitterRepeat is not implemented; use commonnnclustering_amd.engine.fit_scan",
-        "mst": "HierarchicalFitterMSTPrim is not implemented",
-    },
-    "priority_queue": {"maxheap": "priority queues (MST hierarchy) are not implemented"},
 }

@@ -259,6 +259,31 @@ int cnn_precomputed_csr(cnn_ctx *ctx, const void *D, int64_t n, int32_t dtype, d
 int cnn_graph_build_periodic(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype,
                              double r2, const double *bounds, cnn_graph **out);
 
+/* ------------------------------------------------------------------ MST hierarchy */
+/* Similarity weight of every neighbour-list entry: weight[t] = |N(a) ∩ N(b)|
+ * for the entry t of row a holding b, the count SimilarityChecker._get returns
+ * (_types.pyx:2612-2634), which HierarchicalFitterMSTPrim._fit uses as edge
+ * weight (_fit.pyx:851-856, 888-893).  row_ptr: device int64[n+1] from 0;
+ * col: device int32[nnz], every row strictly ascending within [0, n) (checked:
+ * CNN_ERR_INVALID otherwise); weight: device int32[nnz]. */
+int cnn_csr_edge_similarity(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n,
+                            int32_t *weight);
+
+/* Maximum spanning forest of the similarity-weighted lists, replacing the
+ * Prim loop of HierarchicalFitterMSTPrim._fit (_fit.pyx:832-915) with its
+ * PriorityQueueMaxHeap queues (_types.pyx:2825-2851): weights as
+ * cnn_csr_edge_similarity on the device, then, on the host, every unvisited
+ * point in ascending index starts a tree and the queued edge first in
+ * (-weight, a, b) order with b unvisited joins it.  The tree edges (a, b,
+ * weight) are written sorted in that order, the order in which the reference
+ * pops its tree queue (_fit.pyx:937-939).  row_ptr / col: device, as
+ * cnn_csr_edge_similarity; tree_a / tree_b / tree_w: HOST int64[cap]
+ * (cap >= n - 1 always suffices); *n_edges_out: the tree edge count
+ * (CNN_ERR_CAPACITY with outputs untouched when it exceeds cap). */
+int cnn_similarity_mst(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n,
+                       int64_t *tree_a, int64_t *tree_b, int64_t *tree_w, int64_t cap,
+                       int64_t *n_edges_out);
+
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
 

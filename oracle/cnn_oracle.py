@@ -25,6 +25,10 @@ default ``"coordinates"`` recipe:
              :2296-2317 (``MetricExtEuclideanPeriodicReduced._calc_distance``)
   predict    _fit.pyx:1241-1291 (``PredictorFirstmatch.predict``), _types.pyx:1712-1731
              (``_get_other``), cluster.pyx:560-653 (``Clustering.predict``)
+  hierarchy  _fit.pyx:1014-1187 (``HierarchicalFitterRepeat``), _bundle.pyx:275-352
+             (``isolate``), _fit.pyx:733-1011 (``HierarchicalFitterMSTPrim``),
+             _bundle.pyx:355-395 (``check_children``), _types.pyx:2825-2851
+             (``PriorityQueueMaxHeap``)
 
 How each part is pinned (tests/test_oracle.py):
   * metric: bit-identical to a literal scalar Python loop (IEEE double, the
@@ -50,6 +54,16 @@ How each part is pinned (tests/test_oracle.py):
     (test/integration/test_procedures.py:246-277: half of a two-moons set
     labelled by the generator predicts the whole set's generator labels) and
     hand-built cases whose first match is fixed by construction.
+  * hierarchy: ``isolate`` and ``reel`` against test/unit/test_clustering.py's
+    regression reports and expected labels (the product's host code, which the
+    Repeat restatement's ``_isolate_nodes`` must agree with); every Repeat
+    node's labels equal a plain fit at that step restricted to the node's
+    points; the Prim forest's total weight equals scipy's maximum spanning
+    forest and, with distinct weights, its edge set; every MST hierarchy node
+    (distinct weights) is a single-linkage cluster: the component holding its
+    merge edge among the tree edges of weight >= its lambda.  The folding of
+    tied weights in ``_check_children`` has no independent pin ("parity
+    unpinned" for ties): it is checked only against the product.
 """
 
 from __future__ import annotations
@@ -539,3 +553,226 @@ def exact_reduced_distance(a, b):
     return sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(a, b))
 
 
+
+
+# ---------------------------------------------------------------- hierarchy
+
+_sort_labels = sort_by_size  # the Repeat restatement's keyword argument shadows the name
+#
+# Hierarchy nodes are plain dicts:
+#   Repeat:  {"labels": list | None, "root": ndarray | None, "parent": ndarray | None,
+#             "children": {label: node}}
+#   MST:     {"nodes": set, "edges": {(u, v): weight} with u < v, "lambda": weight,
+#             "children": {label: node}}
+
+def _isolate_nodes(node: dict) -> None:
+    """``isolate`` without input data (_bundle.pyx:297-319): one child per
+    label except 0, in order of first occurrence, with parent indices into
+    the node's points and root indices through the node's own."""
+    node["children"] = {}
+    mapping = {}
+    for index, label in enumerate(node["labels"]):
+        mapping.setdefault(int(label), []).append(index)
+    for label, indices in mapping.items():
+        if label == 0:
+            continue
+        parent = np.array(indices, dtype=np.intp)
+        root = parent if node["root"] is None else node["root"][parent]
+        node["children"][label] = {"labels": None, "root": root, "parent": parent, "children": {}}
+
+
+def hierarchical_repeat(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
+                        cnn_offset=None, sort_by_size=True) -> dict:
+    """Restates ``HierarchicalFitterRepeat._fit`` (_fit.pyx:1055-1187).
+
+    ``HierarchicalFitter.make_parameters`` (_fit.pyx:95-126) reads
+    ``self._neighbours_getter``, which the Repeat fitter does not have, so the
+    radius and cnn_cutoff pass unchanged: N(i) = {j : s_ij <= radius_cutoff},
+    similarity cutoff c and member cutoff c (no self-counting shift).  Each
+    step is a literal BFS fit (``fit_bfs``) on fresh labels followed by
+    ``sort_by_size``.  The parents of a step are visited in ascending label;
+    the reference's std::unordered_map order is unspecified and matters only
+    when a cluster has points under several parents.  Returns the root node.
+    """
+    X = prepare_points(data)
+    radii = [float(x) for x in (radius_cutoff if np.iterable(radius_cutoff) else [radius_cutoff])]
+    cuts = [int(x) for x in (cnn_cutoff if np.iterable(cnn_cutoff) else [cnn_cutoff])]
+    if len(radii) == 1:
+        radii *= len(cuts)
+    if len(cuts) == 1:
+        cuts *= len(radii)
+    assert len(radii) == len(cuts)
+    offset = 0 if cnn_offset is None else cnn_offset
+    n = X.shape[0]
+    D = reduced_distances(X, X)
+    root = {"labels": None, "root": None, "parent": None, "children": {}}
+    previous = np.ones(n, dtype=np.int64)
+    terminal = {1: root}
+    for r, c in zip(radii, cuts):
+        c_used = c - offset
+        neigh = [np.nonzero(row <= r)[0] for row in D]
+        params = {"similarity_cutoff": c_used, "n_member_cutoff": c_used, "current_start": 1}
+        labels = np.zeros(n, dtype=np.int64)
+        fit_bfs(neigh, params, labels, np.ones(n, dtype=np.uint8))
+        if sort_by_size:
+            _sort_labels(labels, member_cutoff, max_clusters)
+        parent_map = {}
+        for i in range(n):
+            if previous[i] != 0:
+                parent_map.setdefault(int(previous[i]), []).append(int(labels[i]))
+        new_terminal = {}
+        for p in sorted(parent_map):
+            node = terminal[p]
+            node["labels"] = parent_map[p]
+            _isolate_nodes(node)
+            new_terminal.update(node["children"])
+        terminal = new_terminal
+        previous = labels
+    return root
+
+
+def similarity_weights(neigh: list) -> list:
+    """Per list entry b of row a: |N(a) ∩ N(b)| (``SimilarityChecker._get``,
+    _types.pyx:2612-2634), the MST edge weight of _fit.pyx:851-856, 888-893."""
+    return [[similarity_count(neigh[a], neigh[b]) for b in neigh[a]] for a in range(len(neigh))]
+
+
+def mst_prim(neigh: list, weights: list = None) -> list:
+    """Restates the Prim loop of ``HierarchicalFitterMSTPrim._fit``
+    (_fit.pyx:832-915) with ``PriorityQueueMaxHeap`` (_types.pyx:2825-2851,
+    heapq on (-weight, (a, b))).  Returns the tree edges (a, b, weight) in the
+    order the tree queue pops them (_fit.pyx:937-939)."""
+    import heapq
+
+    if weights is None:
+        weights = similarity_weights(neigh)
+    n = len(neigh)
+    consider = [1] * n
+    queue, tree = [], []
+
+    def push_members(point):
+        for t, member in enumerate(neigh[point]):
+            if consider[member] == 0:
+                continue
+            heapq.heappush(queue, (-weights[point][t], (point, int(member))))
+
+    for point in range(n):
+        if consider[point] == 0:
+            continue
+        consider[point] = 0
+        push_members(point)
+        while queue:
+            weight, (a, b) = heapq.heappop(queue)
+            if consider[b] == 0:
+                continue
+            consider[b] = 0
+            heapq.heappush(tree, (weight, (a, b)))
+            push_members(b)
+    out = []
+    while tree:
+        weight, (a, b) = heapq.heappop(tree)
+        out.append((a, b, -weight))
+    return out
+
+
+def _graph_size(node: dict) -> int:
+    return len(node["nodes"])
+
+
+def _check_children(node: dict, member_cutoff: int, needs_folding: bool) -> None:
+    """Restates ``check_children`` (_bundle.pyx:355-395) on dict nodes."""
+    from collections import deque
+
+    if needs_folding:
+        leafs, queue = [], deque()
+        for child in node["children"].values():
+            if child["lambda"] == node["lambda"]:
+                queue.append(child)
+            else:
+                leafs.append(child)
+        while queue:
+            candidate = queue.popleft()
+            for child in candidate["children"].values():
+                if child["lambda"] == node["lambda"]:
+                    queue.append(child)
+                else:
+                    leafs.append(child)
+        node["children"] = {}
+        for k, child in enumerate(leafs, 1):
+            node["children"][k] = child
+    kept = {}
+    for k, v in enumerate(node["children"].values(), 1):
+        if _graph_size(v) >= member_cutoff:
+            kept[k] = v
+    node["children"] = kept
+    if len(node["children"]) == 1:
+        _, child = node["children"].popitem()
+        for label, grandchild in enumerate(child["children"].values(), 1):
+            node["children"][label] = grandchild
+            node["lambda"] = child["lambda"]
+
+
+def mst_hierarchy(n: int, tree_edges: list, member_cutoff: int = 10) -> dict:
+    """Restates the hierarchy construction of ``HierarchicalFitterMSTPrim._fit``
+    (_fit.pyx:918-1011): merges the tree edges in order with a union-find
+    (parent pointers, no compression, the union points the root of a at the
+    root of b), one top node per edge holding the edge plus the graphs of the
+    top nodes it joins; pruning by ``_check_children`` before each drop of the
+    weight.  Returns the root node."""
+    parent = list(range(n))
+    label_of = list(range(n))
+    tops = {}
+    checked = [False] * max(n - 1, 0)
+
+    def get_root(p):
+        q = parent[p]
+        while q != p:
+            p = q
+            q = parent[p]
+        return p
+
+    current = np.inf
+    folding = False
+    i = -1
+    for i, (a, b, weight) in enumerate(tree_edges):
+        ra, rb = get_root(a), get_root(b)
+        la, lb = label_of[ra], label_of[rb]
+        if weight < current:
+            for index, top in tops.items():
+                if checked[index]:
+                    continue
+                _check_children(top, member_cutoff, folding)
+                checked[index] = True
+            folding = False
+        else:
+            folding = True
+        top = {"nodes": {a, b}, "edges": {(min(a, b), max(a, b)): weight}, "lambda": weight, "children": {}}
+        tops[i] = top
+        label = 1
+        for lx in (la, lb):
+            if lx >= n:
+                sub = tops.pop(lx - n)
+                top["nodes"] |= sub["nodes"]
+                top["edges"].update(sub["edges"])
+                top["children"][label] = sub
+                label += 1
+        parent[ra] = rb
+        label_of[rb] = i + n
+        current = weight
+    for index, top in tops.items():
+        if not checked[index]:
+            _check_children(top, member_cutoff, folding)
+    root = {"nodes": set(), "edges": {}, "lambda": -np.inf, "children": {}}
+    if i < n - 2:
+        count_ = 1
+        for top in tops.values():
+            if _graph_size(top) >= member_cutoff:
+                root["nodes"] |= top["nodes"]
+                root["edges"].update(top["edges"])
+                root["children"][count_] = top
+                count_ += 1
+    else:
+        top = tops[n - 2]
+        root["nodes"], root["edges"], root["lambda"] = top["nodes"], top["edges"], top["lambda"]
+        root["top"] = top  # the reference does not attach the tree's hierarchy to the root
+    return root

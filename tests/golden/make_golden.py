@@ -137,6 +137,60 @@ def metric_points():
             "data": data[0], "other": data[1], "significant": 12}
 
 
+def _literal(node):
+    """A literal node, or the first argument of an np.array(...) call, as data."""
+    if isinstance(node, ast.Call):
+        return ast.literal_eval(node.args[0])
+    return ast.literal_eval(node)
+
+
+def _parametrize_cases(tree, func_name):
+    for node in ast.walk(tree):
+        if isinstance(node, ast.FunctionDef) and node.name == func_name:
+            for dec in node.decorator_list:
+                if isinstance(dec, ast.Call) and getattr(dec.func, "attr", "") == "parametrize":
+                    return dec.args[1].elts, node.lineno
+    raise KeyError(func_name)
+
+
+def isolate_cases():
+    """test_isolate (test/unit/test_clustering.py) with its regression reports."""
+    rel = "test/unit/test_clustering.py"
+    tree = ast.parse(read(rel))
+    cases, lineno = _parametrize_cases(tree, "test_isolate")
+    reports = sorted(os.listdir(os.path.join(REF, "test/unit/test_clustering")))
+    out = []
+    for k, case in enumerate(cases):
+        _, data, meta, labels, root, parent = case.elts
+        name = next(r for r in reports if f"_data{k}_" in r)
+        out.append({"data": _literal(data), "meta": _literal(meta), "labels": _literal(labels),
+                    "root_indices": _literal(root), "parent_indices": _literal(parent),
+                    "report": read(f"test/unit/test_clustering/{name}"),
+                    "report_source": f"test/unit/test_clustering/{name}"})
+    return {"source": f"{rel}:{lineno}", "cases": out}
+
+
+def reel_cases():
+    """test_reel (test/unit/test_clustering.py) and its fixture hierarchy
+    (test/conftest.py make_hierarchical_clustering_a)."""
+    rel = "test/unit/test_clustering.py"
+    cases, lineno = _parametrize_cases(ast.parse(read(rel)), "test_reel")
+    conf = read("test/conftest.py")
+    fixture = next(n for n in ast.walk(ast.parse(conf))
+                   if isinstance(n, ast.FunctionDef) and n.name == "make_hierarchical_clustering_a")
+    calls = [n for n in ast.walk(fixture) if isinstance(n, ast.Call) and getattr(n.func, "attr", "") == "array"]
+    arrays = [_literal(n) for n in sorted(calls, key=lambda n: (n.lineno, n.col_offset))]
+    # fixture order: root labels; child 1 labels, root, parent; child 1.2 labels, root, parent;
+    # children 0, 1, 2 of the root and of child 1 are added empty
+    root_labels, c1_labels, c1_root, c1_parent, c12_labels, c12_root, c12_parent = arrays
+    return {"source": f"{rel}:{lineno}; test/conftest.py:{fixture.lineno}",
+            "root_labels": root_labels,
+            "empty_children": [[0], [1], [2], [1, 0], [1, 1], [1, 2]],
+            "nodes": [{"path": [1], "labels": c1_labels, "root": c1_root, "parent": c1_parent},
+                      {"path": [1, 2], "labels": c12_labels, "root": c12_root, "parent": c12_parent}],
+            "cases": [{"depth": d, "expected": e} for _, d, e in (ast.literal_eval(c) for c in cases)]}
+
+
 def main():
     kats = {
         "quickstart": quickstart(),
@@ -146,6 +200,8 @@ def main():
         "sort_by_size": sort_cases(),
         "params_repr": params_repr(),
         "metric": metric_points(),
+        "isolate": isolate_cases(),
+        "reel": reel_cases(),
     }
     with open(OUT, "w") as fh:
         json.dump(kats, fh, indent=1)
