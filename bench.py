@@ -1,7 +1,8 @@
 """Benchmark: CommonNN fit throughput on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
-                [--workload md_basins_3d|features_64d|blobs_2d|serving|scan_2d|many_small]
+                [--workload md_basins_3d|features_64d|blobs_2d|serving|scan_2d|many_small|
+                            hierarchy_repeat|hierarchy_mst]
 
 A step is one full fit (radius graph, similarity test, union-find,
 sort_by_size) of the whole synthetic dataset; the coordinates are already
@@ -344,6 +345,96 @@ def batch_bench(args):
                  {"one_call_each_ms": t_seq * 1e3, "batch_vs_one_call_each": t_seq / t})
 
 
+def hierarchy_repeat_bench(args):
+    """Repeat hierarchy (HierarchicalFitterRepeat) of the md_basins_3d points
+    (n=1e6, d=3): four steps at one radius (reduced distance 0.0625, the fit's
+    graph) with cnn_cutoff 5, 10, 20, 40, member_cutoff 50, FP32-banded graph.
+    A step is one fit_hierarchical call: step fits on one resident graph plus
+    the host bundle tree (isolate per parent).  Beside it: the CPU restatement's
+    fit rate (as the md_basins_3d baseline) divided by the number of steps."""
+    import torch
+    from commonnnclustering_amd import Clustering, synthetic
+    from commonnnclustering_amd._fit import FitterGPU, HierarchicalFitterRepeat
+    from commonnnclustering_amd._types import NeighboursGetterGPU
+
+    X, _ = synthetic.md_basins_3d()
+    n = X.shape[0]
+    Xd = torch.from_numpy(X).cuda()
+    cuts = [5, 10, 20, 40]
+    fitter = FitterGPU(neighbours_getter=NeighboursGetterGPU(precision="f32_band"))
+    hf = HierarchicalFitterRepeat(fitter)
+    cl = Clustering(Xd, hierarchical_fitter=hf)
+
+    def run():
+        cl.fit_hierarchical(radius_cutoff=0.0625, cnn_cutoff=cuts, member_cutoff=50, v=False)
+
+    t = _timed(run, args.steps, args.warmup)
+    params = [hf.make_parameters(0.0625, c, 1) for c in cuts]
+    t_steps = _timed(lambda: fitter.fit_steps(cl._bundle._input_data, params, True, 50, None), args.steps, 1)
+
+    def count(b):
+        return 1 + sum(count(c) for c in b.children.values())
+
+    nodes = count(cl._bundle)
+    cpu = None
+    if not args.no_cpu:
+        # the reference runs one full fit per step: the C fit restatement's rate over the steps
+        one = cpu_baseline(X, {"radius_cutoff": 0.25, "cnn_cutoff": 10, "member_cutoff": 50},
+                           target_s=args.cpu_seconds)
+        cpu = dict(one, value=one["value"] / len(cuts),
+                   sample=f"one fit per step ({len(cuts)} steps) at the C restatement's fit rate: " + one["sample"])
+    log(f"hierarchy_repeat: {t * 1e3:.1f} ms per hierarchy ({len(cuts)} steps, {nodes} bundles); "
+        f"step fits {t_steps * 1e3:.1f} ms")
+    return _line(n / t, t * 1e3, args.steps, args.warmup,
+                 {"workload": "hierarchy_repeat", "n": n, "d": 3, "radius_cutoff": 0.0625, "cnn_cutoff": cuts,
+                  "member_cutoff": 50},
+                 {"step_fits_ms": t_steps * 1e3, "host_tree_ms": (t - t_steps) * 1e3, "bundles": nodes,
+                  "cpu_baseline": cpu})
+
+
+def hierarchy_mst_bench(args):
+    """MST hierarchy (HierarchicalFitterMSTPrim) of 1e5 seeded points of the
+    md_basins_3d data (d=3) at radius_cutoff 0.25, member_cutoff 50.  A step is
+    one fit_hierarchical call: radius graph, per-entry common-neighbour
+    counts (k_edge_common), the host Prim forest and the bundle hierarchy.
+    The weights call (CSR check + kernel) is also timed alone on the host clock."""
+    import torch
+    from commonnnclustering_amd import Clustering, engine, synthetic
+    from commonnnclustering_amd._fit import HierarchicalFitterMSTPrim
+    from commonnnclustering_amd._types import NeighboursGetterGPU
+
+    X, _ = synthetic.md_basins_3d()
+    rng = np.random.default_rng(1)
+    m = 100_000
+    Xs = np.ascontiguousarray(X[rng.choice(X.shape[0], m, replace=False)])
+    Xd = torch.from_numpy(Xs).cuda()
+    hf = HierarchicalFitterMSTPrim(neighbours_getter=NeighboursGetterGPU(precision="f32_band"))
+    cl = Clustering(Xd, hierarchical_fitter=hf)
+
+    def run():
+        cl.fit_hierarchical(radius_cutoff=0.25, member_cutoff=50)
+
+    t = _timed(run, args.steps, args.warmup)
+    g = engine.build_graph(Xd, 0.0625, "f32_band")
+    row_ptr, col = g.to_csr()
+    g.release()
+    nnz = int(col.shape[0])
+    engine.edge_similarity(row_ptr, col)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        engine.edge_similarity(row_ptr, col)  # synchronous: returns after the kernel
+    t_w = (time.perf_counter() - t0) / 3 * 1e3
+    t0 = time.perf_counter()
+    engine.similarity_mst(row_ptr, col)
+    t_mst = time.perf_counter() - t0
+    log(f"hierarchy_mst: {t * 1e3:.1f} ms per hierarchy; weights {t_w:.2f} ms for {nnz} entries; "
+        f"weights + Prim {t_mst * 1e3:.1f} ms")
+    return _line(m / t, t * 1e3, args.steps, args.warmup,
+                 {"workload": "hierarchy_mst", "n": m, "d": 3, "radius_cutoff": 0.25, "member_cutoff": 50},
+                 {"nnz": nnz, "weights_kernel_ms": t_w, "weights_entries_per_s": nnz / (t_w * 1e-3),
+                  "weights_plus_prim_ms": t_mst * 1e3, "tree_edges": len(cl.hierarchical_fitter.last_tree[0])})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +474,12 @@ def main():
         return
     if args.workload == "scan_2d":
         print(json.dumps(scan_bench(args)))
+        return
+    if args.workload == "hierarchy_repeat":
+        print(json.dumps(hierarchy_repeat_bench(args)))
+        return
+    if args.workload == "hierarchy_mst":
+        print(json.dumps(hierarchy_mst_bench(args)))
         return
     if args.workload == "many_small":
         print(json.dumps(batch_bench(args)))

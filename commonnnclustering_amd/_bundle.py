@@ -18,18 +18,26 @@ from ._types import InputData, Labels, ReferenceIndices
 from .report import Summary
 
 
-def label_groups(labels: np.ndarray):
+def label_groups(labels: np.ndarray, order_by: str = "first"):
     """[(label, ascending point indices)] in order of each label's first
-    occurrence: the items of Labels.to_mapping (_types.pyx:187-198), grouped
-    with one stable sort instead of a loop over the points."""
+    occurrence (``order_by="first"``: the items of Labels.to_mapping,
+    _types.pyx:187-198) or of ascending label (``"label"``), grouped with one
+    stable sort instead of a loop over the points (a radix sort when the
+    labels span fewer than 2^16 values)."""
     labels = np.asarray(labels)
-    if labels.shape[0] == 0:
+    n = labels.shape[0]
+    if n == 0:
         return []
-    uniq, first, inverse = np.unique(labels, return_index=True, return_inverse=True)
-    order = np.argsort(inverse, kind="stable")
-    bounds = np.cumsum(np.bincount(inverse, minlength=len(uniq)))
-    groups = np.split(order, bounds[:-1])
-    return [(int(uniq[g]), groups[g]) for g in np.argsort(first, kind="stable")]
+    lo, hi = int(labels.min()), int(labels.max())
+    key = (labels - lo).astype(np.uint16) if hi - lo < 65536 else labels
+    order = np.argsort(key, kind="stable")
+    sk = key[order]
+    starts = np.flatnonzero(np.r_[True, sk[1:] != sk[:-1]])
+    ends = np.r_[starts[1:], n]
+    groups = [(int(labels[order[a]]), order[a:b]) for a, b in zip(starts.tolist(), ends.tolist())]
+    if order_by == "label":
+        return groups
+    return [groups[k] for k in np.argsort(order[starts], kind="stable")]
 
 
 class Bundle:
@@ -50,6 +58,8 @@ class Bundle:
     def __init__(self, input_data=None, graph=None, labels=None, reference_indices=None, children=None,
                  alias=None, parent=None, meta=None, summary=None, hierarchy_level=0):
         self.input_data = input_data
+        self._graph_build = None
+        self._graph_n = None
         self._graph = graph
         self.labels = labels
         self._reference_indices = reference_indices
@@ -64,6 +74,34 @@ class Bundle:
         self.meta = {} if meta is None else meta
         self.summary = summary
         self._lambda = -np.inf
+
+    # -- hierarchy graph: a value, or built on first access
+    @property
+    def _graph(self):
+        if self._graph_build is not None:
+            build, self._graph_build = self._graph_build, None
+            self._graph_value = build()
+        return self._graph_value
+
+    @_graph.setter
+    def _graph(self, value):
+        self._graph_build = None
+        self._graph_n = None
+        self._graph_value = value
+
+    def _set_graph_builder(self, build, n_nodes=None):
+        """Defer the hierarchy graph: ``build()`` makes it on first access;
+        ``n_nodes`` answers its node count until then."""
+        self._graph_value = None
+        self._graph_build = build
+        self._graph_n = n_nodes
+
+    def _graph_len(self) -> int:
+        """len(self._graph), without building a deferred graph whose node count is known."""
+        if self._graph_build is not None and self._graph_n is not None:
+            return self._graph_n
+        g = self._graph
+        return 0 if g is None else len(g)
 
     # -- properties (_bundle.pyx:54-140)
     @property
@@ -104,7 +142,7 @@ class Bundle:
     def children(self, value):
         if value is None:
             value = {}
-        if not isinstance(value, MutableMapping):
+        elif type(value) is not dict and not isinstance(value, MutableMapping):
             raise TypeError("Expected a mutable mapping")
         self._children = value
 
@@ -125,7 +163,7 @@ class Bundle:
     def summary(self, value):
         if value is None:
             value = Summary()
-        if not isinstance(value, MutableSequence):
+        elif not isinstance(value, MutableSequence):
             raise TypeError("Expected a mutable sequence")
         self._summary = value
 
@@ -266,7 +304,7 @@ def check_children(bundle: Bundle, member_cutoff: int, needs_folding: bool = Fal
                 (queue if child._lambda == bundle._lambda else leafs).append(child)
         bundle._children = {k: child for k, child in enumerate(leafs, 1)}
 
-    bundle._children = {k: v for k, v in enumerate(bundle._children.values(), 1) if len(v._graph) >= member_cutoff}
+    bundle._children = {k: v for k, v in enumerate(bundle._children.values(), 1) if v._graph_len() >= member_cutoff}
 
     if len(bundle._children) == 1:
         child = bundle._children.popitem()[1]
@@ -294,22 +332,44 @@ def build_mst_hierarchy(bundle: Bundle, n_points: int, tree_a, tree_b, tree_w, m
     as the second half of HierarchicalFitterMSTPrim._fit (_fit.pyx:918-1011).
 
     The edges are merged in order into growing point sets (union-find): each
-    merge makes a top bundle whose graph is the merging edge plus the graphs of
-    the (up to two) top bundles it joins, which become its children (1 and 2),
-    and whose ``_lambda`` is the edge weight.  Before the first edge of a lower
-    weight, every top bundle not yet checked is pruned by ``check_children``
-    (folding children of equal lambda when the previous edges tied).  A forest
-    leaves the root with an empty-lambda graph of the top bundles having at
-    least ``member_cutoff`` nodes, as its children; a single tree gives the
-    root the graph and lambda of the last top bundle (its children are left
-    as they were, as in the reference).
+    merge makes a top bundle holding the merging edge plus everything the (up
+    to two) top bundles it joins hold, which become its children (1 and 2);
+    its ``_lambda`` is the edge weight.  Before the first edge of a lower
+    weight, every live top bundle not yet checked is pruned by
+    ``check_children`` (folding children of equal lambda when the previous
+    edges tied).  A forest leaves the root with lambda -inf and the live top
+    bundles of at least ``member_cutoff`` nodes as children; a single tree
+    gives the root the graph and lambda of the last top bundle (its children
+    are left as they were, as in the reference).
+
+    The reference copies every merged subtree's graph at each merge (time
+    and memory of the order of points x depth).  Here a top bundle's graph
+    (networkx, edge attribute ``weight``) is built on first access from the
+    merge records, and pruning uses node counts kept with the union-find;
+    the graphs hold the same nodes and weighted edges as the reference's
+    (their insertion order can differ).
     """
+    import gc
+
+    # one bundle per edge: the cyclic collector's passes over them would dominate
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        _build_mst_hierarchy(bundle, n_points, tree_a, tree_b, tree_w, member_cutoff)
+    finally:
+        if enabled:
+            gc.enable()
+
+
+def _build_mst_hierarchy(bundle, n_points, tree_a, tree_b, tree_w, member_cutoff):
     import networkx as nx
 
     parent = list(range(n_points))
-    set_label = list(range(n_points))  # per set root: n_points + top bundle index, or the point itself
+    size = [1] * n_points
+    set_label = list(range(n_points))  # per set root: n_points + top index, or the point itself
+    rec_a, rec_b, rec_w, rec_l, rec_r = [], [], [], [], []
     top_bundles = {}
-    checked = set()
+    pending = []  # tops created since the last check, in creation order
 
     def find(p):
         root = p
@@ -319,6 +379,27 @@ def build_mst_hierarchy(bundle: Bundle, n_points: int, tree_a, tree_b, tree_w, m
             parent[p], p = root, parent[p]
         return root
 
+    def subtree_edges(tops):
+        out, stack = [], list(reversed(tops))
+        while stack:
+            k = stack.pop()
+            out.append((rec_a[k], rec_b[k], {"weight": rec_w[k]}))
+            if rec_r[k] >= 0:
+                stack.append(rec_r[k])
+            if rec_l[k] >= 0:
+                stack.append(rec_l[k])
+        return out
+
+    def graph_of(tops):
+        return lambda: nx.Graph(subtree_edges(tops))
+
+    def check_pending(folding):
+        for index in pending:
+            top = top_bundles.get(index)
+            if top is not None:
+                check_children(top, member_cutoff, folding)
+        pending.clear()
+
     current_weight = np.inf
     needs_folding = False
     i = -1
@@ -326,48 +407,47 @@ def build_mst_hierarchy(bundle: Bundle, n_points: int, tree_a, tree_b, tree_w, m
         ra, rb = find(a), find(b)
         la, lb = set_label[ra], set_label[rb]
         if weight < current_weight:
-            for index, top in top_bundles.items():
-                if index in checked:
-                    continue
-                check_children(top, member_cutoff, needs_folding)
-                checked.add(index)
+            check_pending(needs_folding)
             needs_folding = False
         else:
             needs_folding = True
 
-        top = top_bundles[i] = Bundle(graph=nx.Graph([(a, b, {"weight": weight})]))
+        top = Bundle()
         top._lambda = weight
+        top_bundles[i] = top
+        pending.append(i)
+        rec_a.append(a), rec_b.append(b), rec_w.append(weight)
         label = 1
+        left = right = -1
         if la >= n_points:
-            left = top_bundles.pop(la - n_points)
-            top._graph.add_edges_from(left._graph.edges(data=True))
-            top._children[label] = left
+            left = la - n_points
+            top._children[label] = top_bundles.pop(left)
             label += 1
         if lb >= n_points:
-            right = top_bundles.pop(lb - n_points)
-            top._graph.add_edges_from(right._graph.edges(data=True))
-            top._children[label] = right
+            right = lb - n_points
+            top._children[label] = top_bundles.pop(right)
+        rec_l.append(left), rec_r.append(right)
         parent[ra] = rb
+        size[rb] += size[ra]
+        top._set_graph_builder(graph_of([i]), size[rb])
         set_label[rb] = i + n_points
         current_weight = weight
 
-    for index, top in top_bundles.items():
-        if index in checked:
-            continue
-        check_children(top, member_cutoff, needs_folding)
+    check_pending(needs_folding)
 
     if i < n_points - 2:
         bundle._lambda = -np.inf
-        bundle._graph = nx.Graph()
         count = 1
-        for top in top_bundles.values():
-            if len(top._graph) >= member_cutoff:
-                bundle._graph.add_edges_from(top._graph.edges(data=True))
+        kept = []
+        for index, top in top_bundles.items():
+            if top._graph_len() >= member_cutoff:
+                kept.append(index)
                 bundle._children[count] = top
                 count += 1
+        bundle._set_graph_builder(graph_of(kept))
         for child in bundle._children.values():
             child._parent = weakref.proxy(bundle)
     else:
         top = top_bundles[n_points - 2]
-        bundle._graph = top._graph
+        bundle._set_graph_builder(lambda: top._graph, top._graph_len())
         bundle._lambda = top._lambda

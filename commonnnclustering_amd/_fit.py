@@ -354,6 +354,8 @@ class HierarchicalFitterRepeat(HierarchicalFitter):
 
     def _fit(self, bundle, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None, cnn_offset=None,
              sort_by_size=True, info=True, v=True):
+        from ._bundle import label_groups
+
         radius_cutoff = _as_list(radius_cutoff, float)
         cnn_cutoff = _as_list(cnn_cutoff, int)
         if len(radius_cutoff) == 1:
@@ -377,14 +379,11 @@ class HierarchicalFitterRepeat(HierarchicalFitter):
                 print(f"Running step {step:<5} (r = {radius_cutoff[step]}, c = {cnn_cutoff[step]})")
             current = np.ascontiguousarray(step_labels[step], dtype=P_AINDEX)
             new_terminal = {}
-            points = np.flatnonzero(previous)
-            order = points[np.argsort(previous[points], kind="stable")]
-            plabs = previous[order]
-            starts = np.flatnonzero(np.r_[True, plabs[1:] != plabs[:-1]]) if len(order) else np.zeros(0, int)
-            ends = np.r_[starts[1:], len(order)]
-            for s, e in zip(starts.tolist(), ends.tolist()):
-                parent = terminal[int(plabs[s])]
-                parent._labels = Labels.from_sequence(current[order[s:e]])
+            for p_label, points in label_groups(previous, order_by="label"):
+                if p_label == 0:
+                    continue
+                parent = terminal[p_label]
+                parent._labels = Labels.from_sequence(current[points])
                 if info:
                     params = {k: (radius_cutoff[step], cnn_cutoff[step]) for k in parent._labels.to_set() if k != 0}
                     parent._labels.meta.update({"params": params, "reference": weakref.proxy(parent),

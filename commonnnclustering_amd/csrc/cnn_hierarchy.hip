@@ -13,11 +13,12 @@
 // queued, and the queue's top edge (a, b) whose b is unvisited joins the tree.
 // The reference queue (PriorityQueueMaxHeap, _types.pyx:2825-2851) orders
 // entries by (-weight, a, b); each (a, b) is queued at most once (when a is
-// visited), so the order is total and any heap with the same comparison pops
-// the same sequence.  The tree edges are returned sorted by that same order,
-// the order in which the reference pops its tree queue (_fit.pyx:937-939).
+// visited), so the order is total and any queue with the same order pops the
+// same sequence (here: buckets by weight, a heap on (a, b) in each).  The
+// tree edges are returned sorted by that same order, the order in which the
+// reference pops its tree queue (_fit.pyx:937-939).
 #include <algorithm>
-#include <queue>
+#include <functional>
 #include <vector>
 
 #include "cnn_internal.hpp"
@@ -50,13 +51,41 @@ struct QueuedEdge {
     int32_t w, a, b;
 };
 
-// Heap order of (-w, a, b) ascending: true when x pops after y.
-struct PopsLater {
-    bool operator()(const QueuedEdge &x, const QueuedEdge &y) const {
-        if (x.w != y.w) return x.w < y.w;
-        if (x.a != y.a) return x.a > y.a;
-        return x.b > y.b;
+// (-w, a, b) ascending: true when x pops before y.
+static bool pops_before(const QueuedEdge &x, const QueuedEdge &y) {
+    if (x.w != y.w) return x.w > y.w;
+    if (x.a != y.a) return x.a < y.a;
+    return x.b < y.b;
+}
+
+// Max-weight queue of edges as buckets by weight (the weights are counts no
+// larger than a list length), each bucket a binary min-heap of a << 32 | b:
+// pops in (-w, a, b) order, as a single heap with that comparison would.
+class BucketQueue {
+  public:
+    bool empty() const { return count_ == 0; }
+    void push(int32_t w, int32_t a, int32_t b) {
+        if ((size_t)w >= buckets_.size()) buckets_.resize((size_t)w + 1);
+        std::vector<uint64_t> &h = buckets_[(size_t)w];
+        h.push_back(((uint64_t)(uint32_t)a << 32) | (uint32_t)b);
+        std::push_heap(h.begin(), h.end(), std::greater<uint64_t>());
+        if (w > top_) top_ = w;
+        ++count_;
     }
+    QueuedEdge pop() {
+        while (buckets_[(size_t)top_].empty()) --top_;
+        std::vector<uint64_t> &h = buckets_[(size_t)top_];
+        std::pop_heap(h.begin(), h.end(), std::greater<uint64_t>());
+        const uint64_t key = h.back();
+        h.pop_back();
+        --count_;
+        return QueuedEdge{top_, (int32_t)(key >> 32), (int32_t)(uint32_t)key};
+    }
+
+  private:
+    std::vector<std::vector<uint64_t>> buckets_;
+    int32_t top_ = -1;
+    size_t count_ = 0;
 };
 
 // Prim's spanning forest over host lists; returns tree edges sorted in pop order.
@@ -65,27 +94,26 @@ static std::vector<QueuedEdge> prim_forest(const std::vector<int64_t> &rp, const
     std::vector<uint8_t> visited((size_t)n, 0);
     std::vector<QueuedEdge> tree;
     tree.reserve(n > 0 ? (size_t)(n - 1) : 0);
-    std::priority_queue<QueuedEdge, std::vector<QueuedEdge>, PopsLater> heap;
+    BucketQueue queue;
     auto push_from = [&](int32_t a) {
         for (int64_t t = rp[a]; t < rp[a + 1]; ++t) {
             const int32_t b = col[t];
-            if (!visited[b]) heap.push(QueuedEdge{w[t], a, b});
+            if (!visited[b]) queue.push(w[t], a, b);
         }
     };
     for (int64_t p = 0; p < n; ++p) {
         if (visited[p]) continue;
         visited[p] = 1;
         push_from((int32_t)p);
-        while (!heap.empty()) {
-            const QueuedEdge e = heap.top();
-            heap.pop();
+        while (!queue.empty()) {
+            const QueuedEdge e = queue.pop();
             if (visited[e.b]) continue;
             visited[e.b] = 1;
             tree.push_back(e);
             push_from(e.b);
         }
     }
-    std::sort(tree.begin(), tree.end(), [](const QueuedEdge &x, const QueuedEdge &y) { return PopsLater{}(y, x); });
+    std::sort(tree.begin(), tree.end(), pops_before);
     return tree;
 }
 
