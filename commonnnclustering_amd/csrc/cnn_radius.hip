@@ -1493,9 +1493,18 @@ __device__ __forceinline__ int select_bit(uint64_t w, int t) {
     return pos + (t >= (int)(x & 1u) ? 1 : 0);
 }
 
+#ifndef CNN_EMIT_WAVES
+#define CNN_EMIT_WAVES 8
+#endif
+// Waves per emit workgroup: the 33 KB of staged words per workgroup bound the
+// resident workgroups per CU, so more waves per workgroup means more waves in
+// flight to cover the latency of the per-word chains.
+constexpr int kEmitWaves = CNN_EMIT_WAVES;
+
 template <bool OUTPUT_PARALLEL>
-__global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
-                                                   int32_t *__restrict__ col) {
+__global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
+                                                               int32_t *__restrict__ col) {
+    constexpr int kRows = 64 / kEmitWaves;  // rows per wave
     __shared__ uint64_t sm[64 * 65];
     __shared__ int64_t s_k[64];
     __shared__ int s_nk;
@@ -1503,9 +1512,9 @@ __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__
     const int64_t R = a.R0 + blockIdx.x;
     int64_t kb, ke;
     tile_range(a, R, kb, ke);
-    // lane rr < 16 holds the write position of row wave * 16 + rr
-    const int64_t wl = R * 64 + wave * 16 + lane;
-    int64_t mypos = (lane < 16 && wl < a.n) ? row_ptr[wl] : 0;
+    // lane rr < kRows holds the write position of row wave * kRows + rr
+    const int64_t wl = R * 64 + wave * kRows + lane;
+    int64_t mypos = (lane < kRows && wl < a.n) ? row_ptr[wl] : 0;
     int64_t next = kb;  // gathering cursor (wave 0)
     for (;;) {
         __syncthreads();
@@ -1535,15 +1544,15 @@ __global__ __launch_bounds__(256) void k_emit_rows(DistArgs a, const int64_t *__
         const int nk = s_nk;
         if (nk == 0) break;
 #pragma unroll 4
-        for (int st = 0; st < 16; ++st) {
-            const int idx = (int)threadIdx.x + 256 * st;
+        for (int st = 0; st < 64 / kEmitWaves; ++st) {
+            const int idx = (int)threadIdx.x + 64 * kEmitWaves * st;
             const int p = idx >> 6, r = idx & 63;
             sm[p * 65 + r] = p < nk ? a.mask[(s_k[p] - a.k_base) * 64 + r] : 0ull;
         }
         __syncthreads();
         const int cbase = lane < nk ? (int)tile_col(a, s_k[lane], kb) * 64 : 0;
-        for (int rr = 0; rr < 16; ++rr) {
-            const int r = wave * 16 + rr;
+        for (int rr = 0; rr < kRows; ++rr) {
+            const int r = wave * kRows + rr;
             const uint64_t v = sm[lane * 65 + r];
             const int pc = __popcll(v);
             const int incl = wave_incl_scan(pc);
@@ -2083,9 +2092,9 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         // output-parallel for the dense layout (few bits per word)
         if (rows > 0) {
             if (g->layout == 1)
-                k_emit_rows<false><<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
+                k_emit_rows<false><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col);
             else
-                k_emit_rows<true><<<(unsigned)(b.second - b.first), 256, 0, s>>>(a, g->row_ptr, col);
+                k_emit_rows<true><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col);
         }
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
