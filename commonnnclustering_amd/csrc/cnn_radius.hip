@@ -190,8 +190,20 @@ __global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ 
     }
 }
 
+#ifndef CNN_ROW_GROUPS
+#define CNN_ROW_GROUPS 8
+#endif
+// Row groups of a tile for the culling of candidate tile pairs: kRowGroups
+// groups of kGroupRows consecutive rows (a Morton-ordered tile's groups are
+// sub-boxes of its box); per candidate pair a bit mask of the groups of R
+// within reach of tile C.  Finer groups skip more rows of a pair.
+constexpr int kRowGroups = CNN_ROW_GROUPS;
+constexpr int kGroupRows = 64 / kRowGroups;
+static_assert(kRowGroups >= 1 && kRowGroups <= 16 && 64 % kRowGroups == 0, "row groups: 1..16, dividing 64");
+using GroupMask = uint16_t;
+
 // Per-tile bounding boxes over all d dims (finite coordinates of valid rows).
-// Also the boxes of the tile's four row groups (rows 16g .. 16g+15), used to
+// Also the boxes of the tile's row groups (rows kGroupRows*g ..), used to
 // skip row groups of a candidate tile pair (glo/ghi nullable).
 __global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, int64_t n_tiles,
                              double *__restrict__ blo, double *__restrict__ bhi, double *__restrict__ glo,
@@ -205,16 +217,16 @@ __global__ void k_tile_boxes(const double *__restrict__ xw64, int64_t n, int d, 
         double lo = isfinite(v) ? v : INFINITY;
         double hi = isfinite(v) ? v : -INFINITY;
 #pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) {
+        for (int off = 1; off < kGroupRows; off <<= 1) {
             lo = fmin(lo, __shfl_xor(lo, off, 64));
             hi = fmax(hi, __shfl_xor(hi, off, 64));
         }
-        if (glo && (lane & 15) == 0) {
-            glo[(t * 4 + (lane >> 4)) * d + k] = lo;
-            ghi[(t * 4 + (lane >> 4)) * d + k] = hi;
+        if (glo && (lane % kGroupRows) == 0) {
+            glo[(t * kRowGroups + lane / kGroupRows) * d + k] = lo;
+            ghi[(t * kRowGroups + lane / kGroupRows) * d + k] = hi;
         }
 #pragma unroll
-        for (int off = 32; off >= 16; off >>= 1) {
+        for (int off = kGroupRows; off < 64; off <<= 1) {
             lo = fmin(lo, __shfl_xor(lo, off, 64));
             hi = fmax(hi, __shfl_xor(hi, off, 64));
         }
@@ -337,7 +349,7 @@ struct DistArgs {
     unsigned long long *redo_cnt;
     int64_t *redo;              // (k, R) per entry
     uint8_t *pair_nz;           // per pair of the batch: 1 if its words hold any bit (nullable)
-    const uint8_t *pair_groups; // per pair (global index): row groups of R (16 rows each) in reach of C
+    const GroupMask *pair_groups; // per pair (global index): row groups of R in reach of C
     const double *bounds;       // periodic metric: period per dimension (<= 0: not periodic); nullptr: none
 };
 
@@ -689,11 +701,11 @@ __global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
         for (int q = 0; q < D; ++q) xc[q] = a.xw64[(C * 64 + lane) * D + q];
         uint32_t clo = 0, chi = 0;
         const double *rows = srow + opaque_zero();
-        const int groups = a.pair_groups ? a.pair_groups[k] : 0xf;  // row groups within reach of C
-        static_for<0, 4>([&](auto gc) {
+        const int groups = a.pair_groups ? a.pair_groups[k] : (1 << kRowGroups) - 1;  // row groups in reach of C
+        static_for<0, kRowGroups>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             if (groups & (1 << g)) {
-                static_for<16 * g, 16 * g + 16>([&](auto ic) {
+                static_for<kGroupRows * g, kGroupRows * g + kGroupRows>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
                     double t = rows[i * DP] - xc[0];
                     double s = t * t;  // 0 + t*t == t*t exactly (t*t is never -0)
@@ -750,11 +762,11 @@ __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
         uint32_t clo = 0, chi = 0;
         uint64_t band_cols = 0;  // columns with an FP32 value inside the guard band (any row)
         const float *rows = srow + opaque_zero();
-        const int groups = a.pair_groups ? a.pair_groups[k] : 0xf;  // row groups within reach of C
-        static_for<0, 4>([&](auto gc) {
+        const int groups = a.pair_groups ? a.pair_groups[k] : (1 << kRowGroups) - 1;  // row groups in reach of C
+        static_for<0, kRowGroups>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             if (groups & (1 << g)) {
-                static_for<16 * g, 16 * g + 16>([&](auto ic) {
+                static_for<kGroupRows * g, kGroupRows * g + kGroupRows>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
                     const float s = f32_d2_row<D>(rows, i, xc);
                     const bool in = s < lo;
@@ -768,7 +780,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
             // rare: those columns recompute their rows and settle the band pairs in FP64
             const bool col_band = ((band_cols & cmask) >> lane) & 1ull;
             for (int i = 0; i < 64; ++i) {
-                if (!(groups & (1 << (i >> 4)))) continue;
+                if (!(groups & (1 << (i / kGroupRows)))) continue;
                 const float s = f32_d2_row<D>(rows, i, xc);
                 const int64_t gi = R * 64 + i;
                 if (col_band && !(s < lo) && !(s > hi) && row_real(a, gi)) {
@@ -1433,15 +1445,25 @@ __global__ void k_pad_rows(const double *__restrict__ xw64, int64_t n_pad, int d
 
 // Mirror index of every candidate pair (R, C): position of R in C's list.
 // Per candidate pair k = (R, C), lane-parallel over the pairs of a row tile:
-// mirror[k] = index of (C, R) in C's list, and gmask[k] = the 16-row groups of
-// R whose boxes are within reach of tile C (bit g for group g).
-__global__ void k_pair_info(const int64_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col, int64_t n_tiles,
-                            int64_t *__restrict__ mirror, const double *__restrict__ glo,
-                            const double *__restrict__ ghi, const double *__restrict__ blo,
-                            const double *__restrict__ bhi, int d, double thr_all,
-                            const double *__restrict__ tile_thr, uint8_t *__restrict__ gmask) {
-    const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+// mirror[k] = index of (C, R) in C's list, and gmask[k] = the row groups of R
+// whose boxes are within reach of tile C (bit g for group g).
+__global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ tp_ptr,
+                                                   const int32_t *__restrict__ tp_col, int64_t n_tiles,
+                                                   int64_t *__restrict__ mirror, const double *__restrict__ glo,
+                                                   const double *__restrict__ ghi, const double *__restrict__ blo,
+                                                   const double *__restrict__ bhi, int d, double thr_all,
+                                                   const double *__restrict__ tile_thr,
+                                                   GroupMask *__restrict__ gmask) {
+    // the group boxes of R, shared by all of R's pairs: staged once per wave
+    __shared__ double sgl[4][kRowGroups * kSpatialMaxD], sgh[4][kRowGroups * kSpatialMaxD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t R = (int64_t)blockIdx.x * 4 + wave;
+    if (R < n_tiles)
+        for (int t = lane; t < kRowGroups * d; t += 64) {
+            sgl[wave][t] = glo[R * kRowGroups * d + t];
+            sgh[wave][t] = ghi[R * kRowGroups * d + t];
+        }
+    __syncthreads();
     if (R >= n_tiles) return;
     const double thr = tile_thr ? tile_thr[R] : thr_all;
     for (int64_t k = tp_ptr[R] + lane; k < tp_ptr[R + 1]; k += 64) {
@@ -1453,11 +1475,27 @@ __global__ void k_pair_info(const int64_t *__restrict__ tp_ptr, const int32_t *_
             else hi = mid;
         }
         mirror[k] = lo;
-        uint8_t gm = 0;
-        for (int gi = 0; gi < 4; ++gi)
-            gm |= (uint8_t)(boxes_close(glo + (R * 4 + gi) * d, ghi + (R * 4 + gi) * d, blo + C * d, bhi + C * d, d,
-                                        thr) << gi);
-        gmask[k] = gm;
+        double cl[kSpatialMaxD], ch[kSpatialMaxD];  // box of C, in registers
+#pragma unroll
+        for (int q = 0; q < kSpatialMaxD; ++q) {
+            cl[q] = q < d ? blo[C * d + q] : 0.0;
+            ch[q] = q < d ? bhi[C * d + q] : 0.0;
+        }
+        uint32_t gm = 0;
+        for (int gi = 0; gi < kRowGroups; ++gi) {
+            // boxes_close(group gi of R, C)
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < kSpatialMaxD; ++q) {
+                if (q < d) {
+                    double g = fmax(cl[q] - sgh[wave][gi * d + q], sgl[wave][gi * d + q] - ch[q]);
+                    g = fmax(g, 0.0);
+                    sum += g * g;
+                }
+            }
+            gm |= (uint32_t)(sum <= thr) << gi;
+        }
+        gmask[k] = (GroupMask)gm;
     }
 }
 
@@ -1935,12 +1973,12 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int64_t total_pairs = T * T;
     const double thr = g->r2 * (1.0 + 1e-6);
     std::unique_ptr<Scratch<double>> glo, ghi;
-    std::unique_ptr<Scratch<uint8_t>> gmask;
+    std::unique_ptr<Scratch<GroupMask>> gmask;
     std::unique_ptr<Scratch<int64_t>> mirror;
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
-        glo.reset(new Scratch<double>(ctx, T * 4 * d));
-        ghi.reset(new Scratch<double>(ctx, T * 4 * d));
+        glo.reset(new Scratch<double>(ctx, T * kRowGroups * d));
+        ghi.reset(new Scratch<double>(ctx, T * kRowGroups * d));
         k_tile_boxes<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(g->xw64, n, d, T, blo.p, bhi.p, glo->p, ghi->p);
         CNN_HIP(hipGetLastError());
         const int64_t NS = ceil_div(T, 64);
@@ -1963,7 +2001,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
                                                                     sl, sh, nullptr, tp_ptr, tp_col);
         CNN_HIP(hipGetLastError());
-        gmask.reset(new Scratch<uint8_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        gmask.reset(new Scratch<GroupMask>(ctx, std::max<int64_t>(total_pairs, 1)));
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         k_pair_info<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
                                                              bhi.p, d, thr, tthr, gmask->p);
