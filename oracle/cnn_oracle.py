@@ -582,7 +582,7 @@ def _isolate_nodes(node: dict) -> None:
 
 
 def hierarchical_repeat(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
-                        cnn_offset=None, sort_by_size=True) -> dict:
+                        cnn_offset=None, sort_by_size=True, bounds=None) -> dict:
     """Restates ``HierarchicalFitterRepeat._fit`` (_fit.pyx:1055-1187).
 
     ``HierarchicalFitter.make_parameters`` (_fit.pyx:95-126) reads
@@ -592,7 +592,8 @@ def hierarchical_repeat(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max
     step is a literal BFS fit (``fit_bfs``) on fresh labels followed by
     ``sort_by_size``.  The parents of a step are visited in ascending label;
     the reference's std::unordered_map order is unspecified and matters only
-    when a cluster has points under several parents.  Returns the root node.
+    when a cluster has points under several parents.  With ``bounds`` the
+    periodic reduced metric replaces the reduced one.  Returns the root node.
     """
     X = prepare_points(data)
     radii = [float(x) for x in (radius_cutoff if np.iterable(radius_cutoff) else [radius_cutoff])]
@@ -604,7 +605,7 @@ def hierarchical_repeat(data, radius_cutoff, cnn_cutoff, member_cutoff=None, max
     assert len(radii) == len(cuts)
     offset = 0 if cnn_offset is None else cnn_offset
     n = X.shape[0]
-    D = reduced_distances(X, X)
+    D = reduced_distances(X, X) if bounds is None else periodic_reduced_distances(X, X, bounds)
     root = {"labels": None, "root": None, "parent": None, "children": {}}
     previous = np.ones(n, dtype=np.int64)
     terminal = {1: root}

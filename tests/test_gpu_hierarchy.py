@@ -143,3 +143,62 @@ def test_repeat_then_reel_and_trim(cuda):
     # reeling relabels points of clusters with children; noise of the first step stays noise
     assert ((before == 0) <= (after == 0)).all()
     assert cl._bundle._labels.meta["origin"] == "reel"
+
+
+def _clusters_2d(seed, n=300):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-3, 3, size=(4, 2))
+    return np.ascontiguousarray(centres[rng.integers(0, 4, n)] + rng.normal(0, 0.5, size=(n, 2)))
+
+
+def test_repeat_hierarchy_precomputed_distances(cuda):
+    # a matrix of reduced distances: D[i, j] <= r is the oracle's s_ij <= r
+    P = _clusters_2d(21)
+    D = O.reduced_distances(P, P)
+    radii, cuts = [0.3, 0.2], [3, 6]
+    expect = O.hierarchical_repeat(P, radii, cuts, member_cutoff=4)
+    fitter = Clustering(D, recipe="distances").fitter
+    cl = Clustering(D, recipe="distances", hierarchical_fitter=HierarchicalFitterRepeat(fitter))
+    cl.fit_hierarchical(radius_cutoff=radii, cnn_cutoff=cuts, member_cutoff=4, v=False)
+    assert _compare_repeat(cl._bundle, expect) > 3
+
+
+def test_repeat_hierarchy_neighbourhood_input(cuda):
+    P = _clusters_2d(22)
+    r = 0.25
+    lists = [np.nonzero(row <= r)[0] for row in O.reduced_distances(P, P)]
+    expect = O.hierarchical_repeat(P, [r], [2, 5, 9], member_cutoff=3)
+    fitter = Clustering(lists, recipe="neighbourhoods").fitter
+    cl = Clustering(lists, recipe="neighbourhoods", hierarchical_fitter=HierarchicalFitterRepeat(fitter))
+    cl.fit_hierarchical(radius_cutoff=r, cnn_cutoff=[2, 5, 9], member_cutoff=3, v=False)
+    assert _compare_repeat(cl._bundle, expect) > 3
+
+
+def test_repeat_hierarchy_periodic_metric(cuda):
+    import os
+    from commonnnclustering_amd._types import MetricExtEuclideanPeriodicReduced
+
+    data = np.load(os.path.join(os.path.dirname(__file__), "golden", "backbone_dihedrals.npy"))[::3]
+    data = np.ascontiguousarray(data)
+    metric = (MetricExtEuclideanPeriodicReduced, (np.array([360.0, 360.0]),), {})
+    fitter = Clustering(data, fitter__getter__dgetter__metric=metric).fitter
+    cl = Clustering(data, hierarchical_fitter=HierarchicalFitterRepeat(fitter))
+    # squared-distance thresholds (the Repeat fitter passes r unchanged), shrinking: graphs rebuilt per radius
+    radii, cuts = [400.0, 150.0], [5, 10]
+    cl.fit_hierarchical(radius_cutoff=radii, cnn_cutoff=cuts, member_cutoff=10, v=False)
+    expect = O.hierarchical_repeat(data, radii, cuts, member_cutoff=10, bounds=[360.0, 360.0])
+    assert _compare_repeat(cl._bundle, expect) > 2
+
+
+def test_mst_hierarchy_neighbourhood_input(cuda):
+    from commonnnclustering_amd._types import NeighboursGetterLookup
+
+    P = _clusters_2d(23, 250)
+    lists = [np.nonzero(row <= 0.2)[0] for row in O.reduced_distances(P, P)]
+    tree = O.mst_prim(lists)
+    expect = O.mst_hierarchy(len(P), tree, 5)
+    expect.pop("top", None)
+    cl = Clustering(lists, recipe="neighbourhoods",
+                    hierarchical_fitter=HierarchicalFitterMSTPrim(neighbours_getter=NeighboursGetterLookup()))
+    cl.fit_hierarchical(radius_cutoff=0.0, member_cutoff=5)
+    assert canon_bundle(cl._bundle) == _canon_oracle(expect)
