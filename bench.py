@@ -116,12 +116,13 @@ def emit_roofline(gs, n, workload_name):
 def distance_roofline(gs, d, precision, workload_name):
     """Algorithmic work of the distance kernel per launch: 3 d flops (subtract,
     multiply, add) per evaluated point pair for the direct forms, 2 d for the
-    Gram form on MFMA; pairs = computed 64x64 tile pairs x 4096 (symmetric
-    evaluation computes each unordered tile pair once)."""
+    Gram form on MFMA; pairs = point pairs the kernel computed (64x64 tile
+    pairs, less the row groups of a pair out of reach in the spatial layout;
+    symmetric evaluation computes each unordered tile pair once)."""
     launches = max(mean(gs, "distance_launches"), 1.0)
     # the main kernel alone (the distance phase also holds guard-band settling)
     ms = mean(gs, "ms_distance_kernel") / launches
-    pairs = mean(gs, "tile_pairs_computed") * 4096.0 / launches
+    pairs = mean(gs, "pairs_evaluated") / launches
     gram = precision == "f32_band" and d >= 16
     # direct forms: subtract, multiply, add per dimension; Gram form: one FMA (2 flops) per dimension
     flops = (2.0 if gram else 3.0) * d * pairs

@@ -358,10 +358,63 @@ __device__ __forceinline__ void wave_add_by_key(int32_t *cnt, int32_t *mins, boo
     }
 }
 
-__global__ void k_root_count(const uint8_t *core, const int *parent, int64_t n, int32_t *cnt) {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = w < n && core[w];
-    wave_add_by_key(cnt, nullptr, active, active ? parent[w] : 0, 0);
+// Per-key sums over consecutive points: a wave walks kRunSteps steps of 64
+// points and keeps one run (key, count, minimum) while every active lane of a
+// step has the run's key, so a large component costs one atomic per run, not
+// one per wave (same-address atomics serialize).
+constexpr int kRunSteps = 16;
+
+struct KeyRun {
+    int key = -1;
+    int cnt = 0;
+    int mn = INT_MAX;
+};
+
+__device__ __forceinline__ void run_flush(KeyRun &run, int32_t *cnt, int32_t *mins) {
+    if (run.key >= 0 && __lane_id() == 0) {
+        atomicAdd(&cnt[run.key], run.cnt);
+        if (mins) atomicMin(&mins[run.key], run.mn);
+    }
+    run.key = -1;
+    run.cnt = 0;
+    run.mn = INT_MAX;
+}
+
+__device__ __forceinline__ void run_add(KeyRun &run, int32_t *cnt, int32_t *mins, bool active, int key, int val) {
+    const uint64_t act = __ballot(active);
+    if (!act) return;
+    const int k = __shfl(key, __ffsll((unsigned long long)act) - 1, 64);
+    if (__ballot(active && key != k) != 0ull) {  // mixed keys: one atomic per key
+        wave_add_by_key(cnt, mins, active, key, val);
+        return;
+    }
+    if (k != run.key) {
+        run_flush(run, cnt, mins);
+        run.key = k;
+    }
+    run.cnt += __popcll(act);
+    if (mins) {
+        int v = active ? val : INT_MAX;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+        run.mn = min(run.mn, v);
+    }
+}
+
+__device__ __forceinline__ int64_t run_base() {
+    return ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * kRunSteps);
+}
+
+__global__ __launch_bounds__(256) void k_root_count(const uint8_t *core, const int *parent, int64_t n,
+                                                    int32_t *cnt) {
+    const int64_t base = run_base();
+    KeyRun run;
+    for (int it = 0; it < kRunSteps; ++it) {
+        const int64_t w = base + it * 64 + __lane_id();
+        const bool active = w < n && core[w];
+        run_add(run, cnt, nullptr, active, active ? parent[w] : 0, 0);
+    }
+    run_flush(run, cnt, nullptr);
 }
 
 __global__ void k_count_roots(const int32_t *cnt, int64_t n, unsigned long long *roots) {
@@ -372,22 +425,30 @@ __global__ void k_count_roots(const int32_t *cnt, int64_t n, unsigned long long 
 
 // ------------------------------------------------------------------ labels
 
-__global__ void k_comp_stats(const uint8_t *core, const int *parent, const int32_t *perm, int64_t n, int32_t *cmin,
-                             int32_t *csize, unsigned long long *counters) {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool is_core = false, is_root = false;
-    int r = 0, orig = 0;
-    if (w < n && core[w]) {
-        is_core = true;
-        r = parent[w];
-        is_root = r == (int)w;
-        orig = perm ? perm[w] : (int32_t)w;
+__global__ __launch_bounds__(256) void k_comp_stats(const uint8_t *core, const int *parent, const int32_t *perm,
+                                                    int64_t n, int32_t *cmin, int32_t *csize,
+                                                    unsigned long long *counters) {
+    const int64_t base = run_base();
+    KeyRun run;
+    unsigned long long nc = 0, nr = 0;
+    for (int it = 0; it < kRunSteps; ++it) {
+        const int64_t w = base + it * 64 + __lane_id();
+        bool is_core = false, is_root = false;
+        int r = 0, orig = 0;
+        if (w < n && core[w]) {
+            is_core = true;
+            r = parent[w];
+            is_root = r == (int)w;
+            orig = perm ? perm[w] : (int32_t)w;
+        }
+        run_add(run, csize, cmin, is_core, r, orig);
+        nc += __popcll(__ballot(is_core));
+        nr += __popcll(__ballot(is_root));
     }
-    wave_add_by_key(csize, cmin, is_core, r, orig);
-    const uint64_t bc = __ballot(is_core), br = __ballot(is_root);
-    if ((threadIdx.x & 63) == 0) {
-        if (bc) atomicAdd(&counters[0], (unsigned long long)__popcll(bc));
-        if (br) atomicAdd(&counters[1], (unsigned long long)__popcll(br));
+    run_flush(run, csize, cmin);
+    if (__lane_id() == 0) {
+        if (nc) atomicAdd(&counters[0], nc);
+        if (nr) atomicAdd(&counters[1], nr);
     }
 }
 
@@ -518,7 +579,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     {
         Scratch<int32_t> cnt(ctx, n);
         fill_i32(ctx, cnt.p, 0, n);
-        k_root_count<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, n, cnt.p);
+        k_root_count<<<blocks_for(n, 256 * kRunSteps), 256, 0, s>>>(core.p, parent.p, n, cnt.p);
         k_count_roots<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters.p + 6);
         k_snapshot<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cnt.p, n, rank.p);
         CNN_HIP(hipGetLastError());
@@ -544,7 +605,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     Scratch<int32_t> cmin(ctx, n), csize(ctx, n);
     fill_i32(ctx, cmin.p, INT_MAX, n);
     fill_i32(ctx, csize.p, 0, n);
-    k_comp_stats<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, req.perm, n, cmin.p, csize.p, counters.p + 2);
+    k_comp_stats<<<blocks_for(n, 256 * kRunSteps), 256, 0, s>>>(core.p, parent.p, req.perm, n, cmin.p, csize.p,
+                                                                counters.p + 2);
     CNN_HIP(hipGetLastError());
     if (req.sort) {
         Scratch<uint64_t> keys(ctx, n);

@@ -1453,7 +1453,10 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
                                                    const double *__restrict__ ghi, const double *__restrict__ blo,
                                                    const double *__restrict__ bhi, int d, double thr_all,
                                                    const double *__restrict__ tile_thr,
-                                                   GroupMask *__restrict__ gmask) {
+                                                   GroupMask *__restrict__ gmask,
+                                                   unsigned long long *__restrict__ rows_evaluated) {
+    // rows_evaluated[2R]: rows of R evaluated against C over the pairs with
+    // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs
     // the group boxes of R, shared by all of R's pairs: staged once per wave
     __shared__ double sgl[4][kRowGroups * kSpatialMaxD], sgh[4][kRowGroups * kSpatialMaxD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1466,6 +1469,7 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     __syncthreads();
     if (R >= n_tiles) return;
     const double thr = tile_thr ? tile_thr[R] : thr_all;
+    unsigned long long rows = 0, rows_all = 0;
     for (int64_t k = tp_ptr[R] + lane; k < tp_ptr[R + 1]; k += 64) {
         const int64_t C = tp_col[k];
         int64_t lo = tp_ptr[C], hi = tp_ptr[C + 1];
@@ -1496,6 +1500,37 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
             gm |= (uint32_t)(sum <= thr) << gi;
         }
         gmask[k] = (GroupMask)gm;
+        const unsigned long long r = (unsigned long long)(__popc(gm) * kGroupRows);
+        rows_all += r;
+        if (C >= R) rows += r;
+    }
+    rows = (unsigned long long)wave_sum64((long long)rows);
+    rows_all = (unsigned long long)wave_sum64((long long)rows_all);
+    if (lane == 0) {
+        rows_evaluated[2 * R] = rows;
+        rows_evaluated[2 * R + 1] = rows_all;
+    }
+}
+
+// One workgroup: out[0] = sum of v[2i], out[1] = sum of v[2i + 1], i < n.
+__global__ __launch_bounds__(256) void k_sum_pairs_u64(const unsigned long long *__restrict__ v, int64_t n,
+                                                       unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long s0[4], s1[4];
+    unsigned long long a = 0, b = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        a += v[2 * i];
+        b += v[2 * i + 1];
+    }
+    a = (unsigned long long)wave_sum64((long long)a);
+    b = (unsigned long long)wave_sum64((long long)b);
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = a;
+        s1[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = s0[0] + s0[1] + s0[2] + s0[3];
+        out[1] = s1[0] + s1[1] + s1[2] + s1[3];
     }
 }
 
@@ -1974,6 +2009,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     const double thr = g->r2 * (1.0 + 1e-6);
     std::unique_ptr<Scratch<double>> glo, ghi;
     std::unique_ptr<Scratch<GroupMask>> gmask;
+    unsigned long long rows_evaluated[2] = {0, 0};  // spatial layout: rows evaluated (C >= R, all pairs)
     std::unique_ptr<Scratch<int64_t>> mirror;
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
@@ -2003,9 +2039,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipGetLastError());
         gmask.reset(new Scratch<GroupMask>(ctx, std::max<int64_t>(total_pairs, 1)));
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        Scratch<unsigned long long> rows_part(ctx, 2 * T), rows_eval(ctx, 2);
         k_pair_info<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
-                                                             bhi.p, d, thr, tthr, gmask->p);
+                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p);
         CNN_HIP(hipGetLastError());
+        k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval.p);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipMemcpyAsync(rows_evaluated, rows_eval.p, sizeof(rows_evaluated), hipMemcpyDeviceToHost, s));
         CNN_HIP(hipStreamSynchronize(s));
     }
     std::vector<int64_t> h_tp_ptr;
@@ -2156,7 +2196,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     g->n_tile_pairs = total_pairs;
     cnn_graph_stats &st = g->stats;
     st.tile_pairs = total_pairs;
-    st.pairs_evaluated = total_pairs * 64 * 64;
+    // lists are symmetric and hold the diagonal: the symmetric kernels compute C >= R only
+    st.tile_pairs_computed = sym ? (total_pairs + T) / 2 : total_pairs;
+    // point pairs computed: row groups out of reach are skipped in the spatial layout
+    st.pairs_evaluated = g->layout == 1 ? (int64_t)rows_evaluated[sym ? 0 : 1] * 64
+                                        : st.tile_pairs_computed * 64 * 64;
     st.band_hits = (int64_t)hb[0];
     st.band_flips = (int64_t)hb[1];
     st.max_abs_err_band = __builtin_bit_cast(double, (uint64_t)hb[2]);
@@ -2167,8 +2211,6 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     st.ms_distance_kernel = ms_kernel;
     st.distance_launches = (int32_t)batches.size();
     st.layout = g->layout;
-    // lists are symmetric and hold the diagonal: the symmetric kernels compute C >= R only
-    st.tile_pairs_computed = sym ? (total_pairs + T) / 2 : total_pairs;
 }
 
 static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype, int32_t precision) {

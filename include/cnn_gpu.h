@@ -77,7 +77,8 @@ typedef struct cnn_ctx cnn_ctx;
 typedef struct cnn_graph cnn_graph;
 
 typedef struct {
-    int64_t pairs_evaluated;   /* point pairs whose distance was computed (tile culling applied) */
+    int64_t pairs_evaluated;   /* point pairs whose distance was computed (tile and row-group culling
+                                  applied; a symmetric launch computes each unordered pair once) */
     int64_t band_hits;         /* FP32 pairs recomputed in FP64 */
     int64_t band_flips;        /* band pairs whose FP64 decision differed from the FP32 one */
     double max_abs_err_band;   /* max |d2_fp32 - d2_fp64| over band pairs */

@@ -22,6 +22,9 @@ def assert_same(X, r, precision):
     assert rp.shape == rp0.shape
     assert (rp == rp0).all(), "degrees differ"
     assert (col == col0).all(), "neighbour lists differ"
+    # every unordered pair within r was computed; culling never adds work
+    n = len(X)
+    assert (len(col0) + n) // 2 <= st["pairs_evaluated"] <= st["tile_pairs_computed"] * 4096
     return st
 
 
