@@ -1588,6 +1588,7 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
     // lane rr < kRows holds the write position of row wave * kRows + rr
     const int64_t wl = R * 64 + wave * kRows + lane;
     int64_t mypos = (lane < kRows && wl < a.n) ? row_ptr[wl] : 0;
+    const uint64_t lanebit = 1ull << lane;
     int64_t next = kb;  // gathering cursor (wave 0)
     for (;;) {
         __syncthreads();
@@ -1631,7 +1632,12 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
             const int incl = wave_incl_scan(pc);
             const int total = __shfl(incl, 63, 64);
             if (total == 0) continue;
-            const int64_t base = __shfl(mypos, rr, 64);
+            // the row's first entry as a wave-uniform (scalar) pointer: stores
+            // then take a scalar base plus a 32-bit lane offset
+            const int64_t base_v = __shfl(mypos, rr, 64);
+            const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base_v);
+            const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)base_v >> 32));
+            int32_t *__restrict__ rowp = col + (int64_t)(((uint64_t)bhi << 32) | blo);
             const int excl = incl - pc;
             if constexpr (OUTPUT_PARALLEL) {
                 // output-parallel: lane l writes entry E + l; its word is the first
@@ -1649,7 +1655,7 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                     const int t = e - __shfl(excl, q, 64);
                     const uint64_t wq = __shfl(v, q, 64);
                     const int cq = __shfl(cbase, q, 64);
-                    if (e < total) col[base + e] = cq + select_bit(wq, t);
+                    if (e < total) rowp[(uint32_t)e] = cq + select_bit(wq, t);
                 }
                 if (lane == rr) mypos += total;
                 continue;
@@ -1663,9 +1669,10 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                 const int eq = __builtin_amdgcn_readlane(excl, q);
                 const int cq = __builtin_amdgcn_readlane(cbase, q);
                 const uint64_t w = ((uint64_t)whi << 32) | wlo;
-                if ((w >> lane) & 1ull) {
-                    const int rank = (int)__builtin_amdgcn_mbcnt_hi(whi, __builtin_amdgcn_mbcnt_lo(wlo, 0u));
-                    col[base + eq + rank] = cq + lane;
+                if (w & lanebit) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(whi, __builtin_amdgcn_mbcnt_lo(wlo, 0u));
+                    int32_t *__restrict__ wp = rowp + eq;  // uniform
+                    wp[rank] = cq + lane;
                 }
             }
             if (lane == rr) mypos += total;
