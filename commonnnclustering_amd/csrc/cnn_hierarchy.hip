@@ -14,9 +14,9 @@
 // The reference queue (PriorityQueueMaxHeap, _types.pyx:2825-2851) orders
 // entries by (-weight, a, b); each (a, b) is queued at most once (when a is
 // visited), so the order is total and any queue with the same order pops the
-// same sequence (here: buckets by weight, a heap on (a, b) in each).  The
-// tree edges are returned sorted by that same order, the order in which the
-// reference pops its tree queue (_fit.pyx:937-939).
+// same sequence (here: the equivalent eager form, one heap entry per reached
+// point).  The tree edges are returned sorted by that same order, the order in
+// which the reference pops its tree queue (_fit.pyx:937-939).
 #include <algorithm>
 #include <functional>
 #include <vector>
@@ -58,59 +58,111 @@ static bool pops_before(const QueuedEdge &x, const QueuedEdge &y) {
     return x.b < y.b;
 }
 
-// Max-weight queue of edges as buckets by weight (the weights are counts no
-// larger than a list length), each bucket a binary min-heap of a << 32 | b:
-// pops in (-w, a, b) order, as a single heap with that comparison would.
-class BucketQueue {
+// Prim's spanning forest over host lists; returns tree edges sorted in pop order.
+//
+// The reference pops queued edges lazily in (-w, a, b) order, skipping edges
+// whose b is already visited.  Its next tree edge is therefore, over all
+// unvisited b, the smallest (-w(a, b), a, b) with a visited -- so keeping, per
+// unvisited b, only its best (-w, a) and taking the smallest (-w, a, b) gives
+// the same sequence ("eager" Prim).  An indexed binary heap holds one entry
+// per reached, unvisited point: at most n entries instead of up to nnz.
+class VertexHeap {
   public:
-    bool empty() const { return count_ == 0; }
-    void push(int32_t w, int32_t a, int32_t b) {
-        if ((size_t)w >= buckets_.size()) buckets_.resize((size_t)w + 1);
-        std::vector<uint64_t> &h = buckets_[(size_t)w];
-        h.push_back(((uint64_t)(uint32_t)a << 32) | (uint32_t)b);
-        std::push_heap(h.begin(), h.end(), std::greater<uint64_t>());
-        if (w > top_) top_ = w;
-        ++count_;
+    explicit VertexHeap(int64_t n) : pos_((size_t)n, -1), key_((size_t)n, 0), from_((size_t)n, 0) {}
+    bool empty() const { return heap_.empty(); }
+    // offer edge (a, b) of weight w; keeps b's best (-w, a)
+    void offer(int32_t b, int32_t w, int32_t a) {
+        const uint64_t k = pack(w, a);
+        if (pos_[(size_t)b] < 0) {
+            key_[(size_t)b] = k;
+            from_[(size_t)b] = a;
+            pos_[(size_t)b] = (int64_t)heap_.size();
+            heap_.push_back(b);
+            up(heap_.size() - 1);
+        } else if (k < key_[(size_t)b]) {
+            key_[(size_t)b] = k;
+            from_[(size_t)b] = a;
+            up((size_t)pos_[(size_t)b]);
+        }
     }
     QueuedEdge pop() {
-        while (buckets_[(size_t)top_].empty()) --top_;
-        std::vector<uint64_t> &h = buckets_[(size_t)top_];
-        std::pop_heap(h.begin(), h.end(), std::greater<uint64_t>());
-        const uint64_t key = h.back();
-        h.pop_back();
-        --count_;
-        return QueuedEdge{top_, (int32_t)(key >> 32), (int32_t)(uint32_t)key};
+        const int32_t b = heap_[0];
+        const QueuedEdge e{(int32_t)(0x7fffffff - (int32_t)(key_[(size_t)b] >> 32)), from_[(size_t)b], b};
+        const int32_t last = heap_.back();
+        heap_.pop_back();
+        pos_[(size_t)b] = -2;  // visited: never offered again
+        if (!heap_.empty()) {
+            heap_[0] = last;
+            pos_[(size_t)last] = 0;
+            down(0);
+        }
+        return e;
     }
+    bool reached_or_visited(int32_t b) const { return pos_[(size_t)b] != -1; }
+    bool visited(int32_t b) const { return pos_[(size_t)b] == -2; }
 
   private:
-    std::vector<std::vector<uint64_t>> buckets_;
-    int32_t top_ = -1;
-    size_t count_ = 0;
+    static uint64_t pack(int32_t w, int32_t a) {
+        return ((uint64_t)(uint32_t)(0x7fffffff - w) << 32) | (uint32_t)a;  // ascending = (-w, a)
+    }
+    bool less(int32_t x, int32_t y) const {
+        const uint64_t kx = key_[(size_t)x], ky = key_[(size_t)y];
+        return kx != ky ? kx < ky : x < y;  // ties: smaller b
+    }
+    void place(size_t i, int32_t v) {
+        heap_[i] = v;
+        pos_[(size_t)v] = (int64_t)i;
+    }
+    void up(size_t i) {
+        const int32_t v = heap_[i];
+        while (i > 0) {
+            const size_t p = (i - 1) / 2;
+            if (!less(v, heap_[p])) break;
+            place(i, heap_[p]);
+            i = p;
+        }
+        place(i, v);
+    }
+    void down(size_t i) {
+        const int32_t v = heap_[i];
+        const size_t n = heap_.size();
+        for (;;) {
+            size_t c = 2 * i + 1;
+            if (c >= n) break;
+            if (c + 1 < n && less(heap_[c + 1], heap_[c])) ++c;
+            if (!less(heap_[c], v)) break;
+            place(i, heap_[c]);
+            i = c;
+        }
+        place(i, v);
+    }
+    std::vector<int64_t> pos_;    // heap slot; -1 not reached, -2 visited
+    std::vector<uint64_t> key_;   // (-w, a) of the best offer
+    std::vector<int32_t> from_;   // a of the best offer
+    std::vector<int32_t> heap_;
 };
 
-// Prim's spanning forest over host lists; returns tree edges sorted in pop order.
 static std::vector<QueuedEdge> prim_forest(const std::vector<int64_t> &rp, const std::vector<int32_t> &col,
                                            const std::vector<int32_t> &w, int64_t n) {
-    std::vector<uint8_t> visited((size_t)n, 0);
     std::vector<QueuedEdge> tree;
     tree.reserve(n > 0 ? (size_t)(n - 1) : 0);
-    BucketQueue queue;
-    auto push_from = [&](int32_t a) {
+    VertexHeap heap(n);
+    auto offer_from = [&](int32_t a) {
         for (int64_t t = rp[a]; t < rp[a + 1]; ++t) {
             const int32_t b = col[t];
-            if (!visited[b]) queue.push(w[t], a, b);
+            if (!heap.visited(b)) heap.offer(b, w[t], a);
         }
     };
     for (int64_t p = 0; p < n; ++p) {
-        if (visited[p]) continue;
-        visited[p] = 1;
-        push_from((int32_t)p);
-        while (!queue.empty()) {
-            const QueuedEdge e = queue.pop();
-            if (visited[e.b]) continue;
-            visited[e.b] = 1;
+        if (heap.reached_or_visited((int32_t)p)) continue;
+        // a new tree: p is visited without an edge
+        heap.offer((int32_t)p, 0, 0);
+        heap.pop();
+        offer_from((int32_t)p);
+        while (!heap.empty()) {
+            const QueuedEdge e = heap.pop();
             tree.push_back(e);
-            push_from(e.b);
+            offer_from(e.b);
         }
     }
     std::sort(tree.begin(), tree.end(), pops_before);
