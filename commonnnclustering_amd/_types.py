@@ -69,6 +69,21 @@ class ClusterParameters:
 
 # ---------------------------------------------------------------- labels
 
+def label_counts(labels: np.ndarray):
+    """(distinct labels ascending, their counts), by one bincount when the
+    labels span fewer than 2^22 values (a sort otherwise)."""
+    lab = np.asarray(labels)
+    if lab.size == 0:
+        return np.zeros(0, dtype=P_AINDEX), np.zeros(0, dtype=np.int64)
+    lo, hi = int(lab.min()), int(lab.max())
+    if hi - lo < (1 << 22):
+        cnt = np.bincount(lab - lo)
+        present = np.flatnonzero(cnt)
+        return (present + lo).astype(P_AINDEX), cnt[present]
+    uniq, cnt = np.unique(lab, return_counts=True)
+    return uniq.astype(P_AINDEX), cnt
+
+
 class Labels:
     """Cluster label assignments (_types.pyx:88-286).
 
@@ -140,7 +155,7 @@ class Labels:
         return mapping
 
     def to_set(self):
-        return set(np.unique(self._labels).tolist())
+        return set(label_counts(self._labels)[0].tolist())
 
     def sort_by_size(self, member_cutoff: Optional[int] = None, max_clusters: Optional[int] = None,
                      bundle=None):

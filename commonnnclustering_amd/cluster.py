@@ -13,14 +13,13 @@ methods (``fit_hierarchical``, ``isolate``, ``reel``, the trims and
 from __future__ import annotations
 
 import weakref
-from collections import Counter
 
 import numpy as np
 
 from typing import Optional
 
 from ._primitive_types import P_AINDEX
-from ._types import InputData, Labels
+from ._types import InputData, Labels, label_counts
 from ._bundle import Bundle
 from ._fit import Fitter, FitterGPU, HierarchicalFitter, Predictor
 from .report import Record, timed
@@ -199,11 +198,14 @@ class Clustering:
             bundle._labels.sort_by_size(member_cutoff, max_clusters)
 
         if record:
-            frequencies = Counter(bundle._labels.labels.tolist())
-            n_noise = frequencies.pop(0, 0)
-            n_largest = frequencies.most_common(1)[0][1] if frequencies else 0
+            # the Record fields of cluster.pyx:366-390 (Counter of the labels),
+            # from one bincount
+            uniq, counts = label_counts(bundle._labels.labels)
+            noise = uniq == 0
+            n_noise = int(counts[noise].sum())
+            n_largest = int(counts[~noise].max()) if (~noise).any() else 0
             rec = Record(n, radius_cutoff, cnn_cutoff, member_cutoff, max_clusters,
-                         len(bundle._labels.to_set() - {0}), n_largest / n if n else 0.0,
+                         int((~noise).sum()), n_largest / n if n else 0.0,
                          n_noise / n if n else 0.0, execution_time)
             if v:
                 print(rec)
