@@ -516,6 +516,23 @@ def main():
     t_h2d = _timed(h2d_fit, max(1, args.steps // 2), 1)
     log(f"f32_band from host memory (H2D included): {t_h2d * 1e3:.2f} ms/fit")
 
+    # the drop-in API: Clustering(points).fit(r, c, member_cutoff) with host input
+    # (coordinates cached on the device after the first fit; labels copied back,
+    # label meta data and the Record built on the host)
+    from commonnnclustering_amd import Clustering
+
+    api = {}
+    for recipe in ("coordinates", "coordinates_f32"):
+        cl = Clustering(X, recipe=recipe)
+
+        def api_fit():
+            cl.fit(p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"], v=False)
+
+        api[recipe] = _timed(api_fit, max(1, args.steps // 2), 1) * 1e3
+        if not np.array_equal(cl.labels, lab_band):
+            raise SystemExit(f"Clustering.fit ({recipe}) labels differ from the engine's")
+    log(f"Clustering.fit: " + ", ".join(f"{k} {v:.2f} ms" for k, v in api.items()))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(X, p, args.cpu_seconds)
@@ -552,6 +569,7 @@ def main():
         "max_fp32_abs_err_band": float(gs_band[-1]["max_abs_err_band"]),
         "with_h2d": {"points_per_s": n / t_h2d, "ms_per_step": t_h2d * 1e3,
                      "note": "X copied from pinned host memory inside the timed region; not value"},
+        "clustering_fit_ms": api,
         "phase_ms": phases(gs_band, ls_band),
         "phase_ms_plain": ph_plain,
         "label_stats": ls_band[-1],

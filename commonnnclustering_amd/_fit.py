@@ -57,6 +57,22 @@ class Fitter(ABC):
         return ClusterParameters(radius_cutoff, cnn_cutoff, cnn_cutoff, n_member_cutoff, current_start)
 
 
+def _to_host(t, owner):
+    """Copy a device int64 vector to a new host array through a pinned
+    staging buffer cached on ``owner`` (a pageable copy runs at a fraction of
+    the link rate)."""
+    import torch
+
+    n = t.numel()
+    buf = getattr(owner, "_pinned", None)
+    if buf is None or buf.numel() < n:
+        buf = torch.empty(max(n, 1), dtype=t.dtype, pin_memory=True)
+        owner._pinned = buf
+    view = buf[:n]
+    view.copy_(t)
+    return view.numpy().copy()
+
+
 class FitterGPU(Fitter):
     """Common-nearest-neighbours fit on the GPU.
 
@@ -161,7 +177,7 @@ class FitterGPU(Fitter):
                                            sim_mode=self._sim_mode())
             engine.sort_by_size(lab, member_cutoff, max_clusters)
             self.last_graph_stats, self.last_label_stats = None, st
-            return lab.cpu().numpy()
+            return _to_host(lab, self)
         if self._neighbours_getter.precomputed:
             import torch
 
@@ -172,7 +188,7 @@ class FitterGPU(Fitter):
                                            sim_mode=self._sim_mode())
             engine.sort_by_size(lab, member_cutoff, max_clusters)
             self.last_graph_stats, self.last_label_stats = None, st
-            return lab.cpu().numpy()
+            return _to_host(lab, self)
         if self._neighbours_getter.periodic_bounds is not None:
             graph = self._neighbours_getter.get_graph(input_data, cluster_params)
             try:
@@ -181,13 +197,13 @@ class FitterGPU(Fitter):
                 self.last_graph_stats, self.last_label_stats = graph.stats, ls
             finally:
                 graph.release()
-            return lab.cpu().numpy()
+            return _to_host(lab, self)
         lab, _, gs, ls = engine.fit(input_data.device_tensor(), self._neighbours_getter.squared_threshold(cluster_params),
                                     cluster_params.similarity_cutoff, cluster_params.n_member_cutoff,
                                     precision=self._neighbours_getter.precision, sort=True,
                                     member_cutoff=member_cutoff, max_clusters=max_clusters)
         self.last_graph_stats, self.last_label_stats = gs, ls
-        return lab.cpu().numpy()
+        return _to_host(lab, self)
 
 
     def fit_steps(self, input_data, params_list, sort_by_size: bool = True, member_cutoff=None,

@@ -185,8 +185,9 @@ class Clustering:
                 execution_time = None
             unsorted_params = True
 
+        counted = label_counts(bundle._labels.labels) if (info or record) else None
         if info:
-            new_label_set = bundle._labels.to_set()
+            new_label_set = set(counted[0].tolist())
             params = {k: (radius_cutoff, cnn_cutoff) for k in new_label_set - old_label_set if k != 0}
             meta = {"params": params, "reference": weakref.proxy(bundle), "origin": "fit"}
             old_params = bundle._labels.meta.get("params", {})
@@ -198,9 +199,9 @@ class Clustering:
             bundle._labels.sort_by_size(member_cutoff, max_clusters)
 
         if record:
-            # the Record fields of cluster.pyx:366-390 (Counter of the labels),
-            # from one bincount
-            uniq, counts = label_counts(bundle._labels.labels)
+            # the Record fields of cluster.pyx:366-390 (a Counter of the labels) from
+            # label counts; counted again when sort_by_size ran after the meta step
+            uniq, counts = label_counts(bundle._labels.labels) if unsorted_params and sort_by_size else counted
             noise = uniq == 0
             n_noise = int(counts[noise].sum())
             n_largest = int(counts[~noise].max()) if (~noise).any() else 0
