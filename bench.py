@@ -430,10 +430,32 @@ def hierarchy_mst_bench(args):
     t_mst = time.perf_counter() - t0
     log(f"hierarchy_mst: {t * 1e3:.1f} ms per hierarchy; weights {t_w:.2f} ms for {nnz} entries; "
         f"weights + Prim {t_mst * 1e3:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        # the reference's procedure (set-intersection weights, heapq Prim, merge
+        # hierarchy) as restated by the oracle, on the 3000 points nearest to a
+        # seeded point of the same sample: the same density, so the same list
+        # lengths; its cost grows about linearly in n at fixed density
+        from oracle import cnn_oracle as O
+
+        centre = Xs[np.random.default_rng(2).integers(0, m)]
+        near = np.argsort(((Xs.astype(np.float64) - centre) ** 2).sum(1))[:3000]
+        S = np.ascontiguousarray(Xs[np.sort(near)], dtype=np.float64)
+        t0 = time.perf_counter()
+        neigh = O.neighbourhoods(S, 0.25)
+        O.mst_hierarchy(len(S), O.mst_prim(neigh), 50)
+        ts = time.perf_counter() - t0
+        deg = float(np.mean([len(x) for x in neigh]))
+        cpu = {"value": len(S) / ts, "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"3000 nearest points of a seeded centre (mean list length {deg:.0f}, the full "
+                         f"sample's {nnz / m:.0f}), oracle restatement of the reference's weights, heapq Prim "
+                         f"and merge hierarchy in {ts:.1f} s"}
+        log(f"cpu sample: {cpu['sample']}")
     return _line(m / t, t * 1e3, args.steps, args.warmup,
                  {"workload": "hierarchy_mst", "n": m, "d": 3, "radius_cutoff": 0.25, "member_cutoff": 50},
                  {"nnz": nnz, "weights_kernel_ms": t_w, "weights_entries_per_s": nnz / (t_w * 1e-3),
-                  "weights_plus_prim_ms": t_mst * 1e3, "tree_edges": len(cl.hierarchical_fitter.last_tree[0])})
+                  "weights_plus_prim_ms": t_mst * 1e3, "tree_edges": len(cl.hierarchical_fitter.last_tree[0]),
+                  "cpu_baseline": cpu})
 
 
 def main():
