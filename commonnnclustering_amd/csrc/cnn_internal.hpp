@@ -83,6 +83,7 @@ struct cnn_graph {
     int32_t *col = nullptr;
     int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout only)
     int32_t *tp_col = nullptr;
+    int32_t *selfpos = nullptr;  // per working row: position of the row's own index in its list
     int64_t n_tile_pairs = 0;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
@@ -215,6 +216,7 @@ struct LabelRequest {
     const int64_t *tp_ptr = nullptr;  // candidate column tiles per row tile
     const int32_t *tp_col = nullptr;
     int64_t n_tiles = 0;              // 0: no tile structure (caller CSR)
+    const int32_t *selfpos = nullptr; // per row: position of the row's own index in its list (nullable)
     // packed batches: per-tile cutoffs and the per-dataset cluster ordering
     const int32_t *tile_csim = nullptr;
     const int32_t *tile_mcore = nullptr;

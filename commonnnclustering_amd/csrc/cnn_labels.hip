@@ -74,6 +74,7 @@ struct LabelArgs {
     // per tile: the rank every point of the tile has (0: no core point; all
     // ones: ranks differ), so list entries in such tiles need no rank gather
     const uint64_t *tile_rank;
+    const int32_t *selfpos;     // per row: position of its own index in its list (nullable)
     const int32_t *tile_csim;   // packed batches: c_sim / m_core per row tile (nullable)
     const int32_t *tile_mcore;
 };
@@ -133,9 +134,10 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
-    // first entry >= w: a 64-way search (each step probes 64 evenly spaced
-    // entries at once and keeps the gap that holds the boundary)
-    int64_t lo = b, hi = e;
+    // first entry >= w: the row's own entry, recorded when the lists were
+    // emitted, or a 64-way search (each step probes 64 evenly spaced entries at
+    // once and keeps the gap that holds the boundary)
+    int64_t lo = a.selfpos ? min(b + (int64_t)a.selfpos[w], e) : b, hi = a.selfpos ? lo : e;
     while (lo < hi) {
         const int64_t step = (hi - lo + 63) >> 6;
         const int64_t pos = lo + lane * step;
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
             const int src = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
             const int jj = __shfl(j, src, 64);
+            if (jj >= (int)w) continue;  // parents point to smaller indices only: the hooks stay a forest
             const int64_t bj = a.row_ptr[jj], ej = a.row_ptr[jj + 1];
             ++attempts;
             ++tests;
@@ -561,7 +564,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
     CNN_HIP(hipMemsetAsync(tstats.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
     LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr,
-                nullptr, req.tile_csim, req.tile_mcore};
+                nullptr, req.selfpos, req.tile_csim, req.tile_mcore};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
@@ -843,7 +846,7 @@ int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, consider, current_start, sim_mode,
-                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles};
+                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
         labels_run(ctx, r, stats_out);
         return (int)CNN_OK;
     });
@@ -869,7 +872,7 @@ int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_cor
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
         labels_run(ctx, r, lstats);
         return (int)CNN_OK;
     });
@@ -887,7 +890,7 @@ int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, do
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
         labels_run(ctx, r, lstats);
         if (graph_out) *graph_out = hold.release();
         return (int)CNN_OK;

@@ -1576,7 +1576,8 @@ constexpr int kEmitWaves = CNN_EMIT_WAVES;
 
 template <bool OUTPUT_PARALLEL>
 __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
-                                                               int32_t *__restrict__ col) {
+                                                               int32_t *__restrict__ col,
+                                                               int32_t *__restrict__ selfpos) {
     constexpr int kRows = 64 / kEmitWaves;  // rows per wave
     __shared__ uint64_t sm[64 * 65];
     __shared__ int64_t s_k[64];
@@ -1588,6 +1589,7 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
     // lane rr < kRows holds the write position of row wave * kRows + rr
     const int64_t wl = R * 64 + wave * kRows + lane;
     int64_t mypos = (lane < kRows && wl < a.n) ? row_ptr[wl] : 0;
+    const int64_t myrow0 = mypos;  // the row's first entry
     const uint64_t lanebit = 1ull << lane;
     int64_t next = kb;  // gathering cursor (wave 0)
     for (;;) {
@@ -1639,6 +1641,9 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
             const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)base_v >> 32));
             int32_t *__restrict__ rowp = col + (int64_t)(((uint64_t)bhi << 32) | blo);
             const int excl = incl - pc;
+            const int64_t row = R * 64 + r;  // also the row's own column index
+            // entries of the row written by earlier chunks: self positions count from the row start
+            const int done = __builtin_amdgcn_readfirstlane((int)(base_v - __shfl(myrow0, rr, 64)));
             if constexpr (OUTPUT_PARALLEL) {
                 // output-parallel: lane l writes entry E + l; its word is the first
                 // with inclusive count > E + l, its bit the rank within that word
@@ -1655,7 +1660,11 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                     const int t = e - __shfl(excl, q, 64);
                     const uint64_t wq = __shfl(v, q, 64);
                     const int cq = __shfl(cbase, q, 64);
-                    if (e < total) rowp[(uint32_t)e] = cq + select_bit(wq, t);
+                    if (e < total) {
+                        const int v = cq + select_bit(wq, t);
+                        rowp[(uint32_t)e] = v;
+                        if (v == (int)row) selfpos[row] = done + e;
+                    }
                 }
                 if (lane == rr) mypos += total;
                 continue;
@@ -1673,6 +1682,7 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(whi, __builtin_amdgcn_mbcnt_lo(wlo, 0u));
                     int32_t *__restrict__ wp = rowp + eq;  // uniform
                     wp[rank] = cq + lane;
+                    if (cq + lane == (int)row) selfpos[row] = done + eq + (int)rank;
                 }
             }
             if (lane == rr) mypos += total;
@@ -1915,6 +1925,7 @@ void graph_free(cnn_graph *g) {
     graph_mem_free(g, g->col);
     graph_mem_free(g, g->tp_ptr);
     graph_mem_free(g, g->tp_col);
+    graph_mem_free(g, g->selfpos);
     graph_mem_free(g, g->bounds);
     delete g;
 }
@@ -2100,6 +2111,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
+    // every row with a finite point holds itself; rows without one (degree 0) are never core
+    g->selfpos = dev_alloc<int32_t>(g, std::max<int64_t>(n, 1));
     for (auto &b : batches) {
         DistArgs a{};
         a.xw64 = g->xw64;
@@ -2177,9 +2190,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         // output-parallel for the dense layout (few bits per word)
         if (rows > 0) {
             if (g->layout == 1)
-                k_emit_rows<false><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col);
+                k_emit_rows<false><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col,
+                                                                                             g->selfpos);
             else
-                k_emit_rows<true><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col);
+                k_emit_rows<true><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col,
+                                                                                            g->selfpos);
         }
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
