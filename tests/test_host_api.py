@@ -35,3 +35,20 @@ def test_summary_dataframe_types():
     assert df["n_points"].dtype == pd.Int64Dtype() and df["radius_cutoff"].dtype == np.float64
     assert df["member_cutoff"].isna().tolist() == [False, True]
     assert df["n_clusters"].tolist() == [3, 1]
+
+
+def test_label_counts_match_counter():
+    """to_set and the Record statistics come from label_counts: the same
+    values as the Counter / set over the labels that the reference uses
+    (cluster.pyx:366-390, _types.pyx:200-210)."""
+    from collections import Counter
+
+    from commonnnclustering_amd._types import Labels, label_counts
+
+    rng = np.random.default_rng(3)
+    for lab in (rng.integers(0, 40, 5000), rng.integers(-3, 3, 100), np.array([7]), np.zeros(0, dtype=np.int64),
+                rng.integers(0, 2 ** 40, 300)):
+        uniq, counts = label_counts(lab)
+        c = Counter(lab.tolist())
+        assert dict(zip(uniq.tolist(), counts.tolist())) == dict(c)
+        assert Labels(lab).to_set() == set(c)
