@@ -1670,12 +1670,12 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                 continue;
             }
             uint64_t nz = __ballot(v != 0ull);
+            int eq = 0;  // entries of this chunk's earlier words: a running scalar sum
             while (nz) {
                 const int q = __ffsll((unsigned long long)nz) - 1;
                 nz &= nz - 1;
                 const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)v, q);
                 const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), q);
-                const int eq = __builtin_amdgcn_readlane(excl, q);
                 const int cq = __builtin_amdgcn_readlane(cbase, q);
                 const uint64_t w = ((uint64_t)whi << 32) | wlo;
                 if (w & lanebit) {
@@ -1684,6 +1684,7 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
                     wp[rank] = cq + lane;
                     if (cq + lane == (int)row) selfpos[row] = done + eq + (int)rank;
                 }
+                eq += __popcll(w);
             }
             if (lane == rr) mypos += total;
         }
