@@ -337,8 +337,8 @@ def build_mst_hierarchy(bundle: Bundle, n_points: int, tree_a, tree_b, tree_w, m
     its ``_lambda`` is the edge weight.  Before the first edge of a lower
     weight, every live top bundle not yet checked is pruned by
     ``check_children`` (folding children of equal lambda when the previous
-    edges tied).  A forest leaves the root with lambda -inf and the live top
-    bundles of at least ``member_cutoff`` nodes as children; a single tree
+    edges tied).  A forest leaves the root with lambda -inf, an empty graph
+    and the live top bundles of at least ``member_cutoff`` nodes as children; a single tree
     gives the root the graph and lambda of the last top bundle (its children
     are left as they were, as in the reference).
 
@@ -438,13 +438,13 @@ def _build_mst_hierarchy(bundle, n_points, tree_a, tree_b, tree_w, member_cutoff
     if i < n_points - 2:
         bundle._lambda = -np.inf
         count = 1
-        kept = []
-        for index, top in top_bundles.items():
+        for top in top_bundles.values():
             if top._graph_len() >= member_cutoff:
-                kept.append(index)
                 bundle._children[count] = top
                 count += 1
-        bundle._set_graph_builder(graph_of(kept))
+        # the reference adds the root's own, empty, edge set to itself
+        # (_fit.pyx:1001): a forest's root keeps an empty graph
+        bundle._set_graph_builder(nx.Graph, 0)
         for child in bundle._children.values():
             child._parent = weakref.proxy(bundle)
     else:

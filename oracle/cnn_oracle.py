@@ -768,8 +768,8 @@ def mst_hierarchy(n: int, tree_edges: list, member_cutoff: int = 10) -> dict:
         count_ = 1
         for top in tops.values():
             if _graph_size(top) >= member_cutoff:
-                root["nodes"] |= top["nodes"]
-                root["edges"].update(top["edges"])
+                # _fit.pyx:1001 adds the root's own (empty) edge set to itself:
+                # a forest's root keeps an empty graph
                 root["children"][count_] = top
                 count_ += 1
     else:

@@ -315,6 +315,49 @@ def test_mst_hierarchy_builder_matches_oracle(distinct, member_cutoff):
         assert got == _canon_oracle(expect)
 
 
+def _leaf(lam, nodes, edges):
+    return {"lambda": lam, "nodes": nodes, "edges": edges, "children": {}}
+
+
+# Hand-worked runs of the merge loop of _fit.pyx:945-1011 with check_children
+# (_bundle.pyx:354-398); point n-1 stays isolated, so the root is a forest root:
+# lambda -inf, an empty graph (the no-op at _fit.pyx:1001), its children the
+# live top bundles of at least member_cutoff nodes.
+MST_KATS = [
+    # Weights 5, 5, 4, 3, 3: the last top bundle (lambda 3) has two children, the
+    # (0..3) bundle of the same lambda and the lambda-4 pair (4, 5).  Folding (the
+    # previous edge tied) replaces the same-lambda child by its children: the
+    # lambda-4 pair first (a direct child), then the two lambda-5 pairs.
+    (7, [(0, 1, 5), (2, 3, 5), (4, 5, 4), (1, 2, 3), (3, 4, 3)], 2,
+     {1: {"lambda": 3, "nodes": [0, 1, 2, 3, 4, 5],
+          "edges": [(0, 1, 5), (1, 2, 3), (2, 3, 5), (3, 4, 3), (4, 5, 4)],
+          "children": {1: _leaf(4, [4, 5], [(4, 5, 4)]), 2: _leaf(5, [0, 1], [(0, 1, 5)]),
+                       3: _leaf(5, [2, 3], [(2, 3, 5)])}}}),
+    # Distinct weights 6, 5, 4, 3: no folding; the last top bundle keeps one child
+    # (the lambda-4 bundle), whose two children are lifted and whose lambda (4)
+    # the parent takes over.
+    (6, [(0, 1, 6), (2, 3, 5), (1, 2, 4), (3, 4, 3)], 2,
+     {1: {"lambda": 4, "nodes": [0, 1, 2, 3, 4], "edges": [(0, 1, 6), (1, 2, 4), (2, 3, 5), (3, 4, 3)],
+          "children": {1: _leaf(6, [0, 1], [(0, 1, 6)]), 2: _leaf(5, [2, 3], [(2, 3, 5)])}}}),
+    # Weights 5, 4, 3 in a chain, member_cutoff 3: the lambda-4 bundle loses its
+    # 2-node child; the last bundle keeps that bundle alone, which has no children
+    # to lift, so its lambda stays 3 and it ends childless.
+    (5, [(0, 1, 5), (1, 2, 4), (2, 3, 3)], 3,
+     {1: _leaf(3, [0, 1, 2, 3], [(0, 1, 5), (1, 2, 4), (2, 3, 3)])}),
+]
+
+
+@pytest.mark.parametrize("n,edges,member_cutoff,children", MST_KATS)
+def test_mst_hierarchy_hand_worked(n, edges, member_cutoff, children):
+    expect = {"lambda": -np.inf, "nodes": [], "edges": [], "children": children}
+    assert _canon_oracle(O.mst_hierarchy(n, edges, member_cutoff)) == expect
+    root = Bundle()
+    build_mst_hierarchy(root, n, [e[0] for e in edges], [e[1] for e in edges], [e[2] for e in edges],
+                        member_cutoff)
+    assert canon_bundle(root) == expect
+
+
+
 def test_mst_hierarchy_oracle_nodes_are_single_linkage_clusters():
     from scipy.sparse import coo_matrix
     from scipy.sparse.csgraph import connected_components
