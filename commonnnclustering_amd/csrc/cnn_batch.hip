@@ -84,7 +84,7 @@ static void fit_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int3
     PackedSort ps{n_sets, 2 * size_bits, size_bits, (uint32_t)(((uint64_t)1 << size_bits) - 1), ds_bits + 2 * size_bits,
                   d_ds.p, d_off.p};
     LabelRequest r{g->row_ptr, g->col, g->n, g->perm, 0, 0, nullptr, 1, CNN_SIM_AUTO, true, true,
-                   member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
+                   member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
     r.tile_csim = d_c.p;
     r.tile_mcore = d_m.p;
     r.packed = &ps;
@@ -155,7 +155,7 @@ static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
                        true, true, member_cutoff, max_clusters, labels + offsets[b], g->tp_ptr, g->tp_col,
-                       g->n_tiles, g->selfpos};
+                       g->n_tiles, g->selfpos, g->mask, g->pair_nz};
         cnn_label_stats one{};
         labels_run(wc, r, &one);
         std::lock_guard<std::mutex> lock(mu);
@@ -262,10 +262,13 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
                 std::vector<int32_t> cis(n_c);
                 for (int32_t ci = 0; ci < n_c; ++ci) cis[ci] = ci;
                 run_on_workers(ctx, n_c, cis, [&](cnn_ctx *wc, int32_t ci, std::mutex &) {
-                    // self positions describe the graph's own lists, not lists filtered to a smaller r
+                    // self positions and row words describe the graph's own lists, not lists
+                    // filtered to a smaller r
+                    const bool own = rp == g->row_ptr;
                     LabelRequest req{rp, col, n, g->perm, c_sim[ci], m_core[ci], nullptr, 1, CNN_SIM_AUTO, true, true,
                                      member_cutoff, max_clusters, labels + ((int64_t)ri * n_c + ci) * n,
-                                     g->tp_ptr, g->tp_col, g->n_tiles, rp == g->row_ptr ? g->selfpos : nullptr};
+                                     g->tp_ptr, g->tp_col, g->n_tiles, own ? g->selfpos : nullptr,
+                                     own ? g->mask : nullptr, own ? g->pair_nz : nullptr};
                     cnn_label_stats one{};
                     labels_run(wc, req, &one);
                     if (lstats) lstats[(int64_t)ri * n_c + ci] = one;

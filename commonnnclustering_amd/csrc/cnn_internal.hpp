@@ -84,6 +84,11 @@ struct cnn_graph {
     int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout only)
     int32_t *tp_col = nullptr;
     int32_t *selfpos = nullptr;  // per working row: position of the row's own index in its list
+    // single-batch spatial builds: per candidate tile pair k (rows of tile R = the
+    // pairs tp_ptr[R] .. tp_ptr[R+1]), mask[64k + r] holds row 64R+r's bits over
+    // the 64 points of column tile tp_col[k], valid where pair_nz[k] != 0
+    uint64_t *mask = nullptr;
+    uint8_t *pair_nz = nullptr;
     int64_t n_tile_pairs = 0;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
@@ -217,6 +222,8 @@ struct LabelRequest {
     const int32_t *tp_col = nullptr;
     int64_t n_tiles = 0;              // 0: no tile structure (caller CSR)
     const int32_t *selfpos = nullptr; // per row: position of the row's own index in its list (nullable)
+    const uint64_t *mask = nullptr;   // the lists as row words per candidate tile pair (nullable)
+    const uint8_t *pair_nz = nullptr;
     // packed batches: per-tile cutoffs and the per-dataset cluster ordering
     const int32_t *tile_csim = nullptr;
     const int32_t *tile_mcore = nullptr;

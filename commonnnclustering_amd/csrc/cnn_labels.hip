@@ -77,6 +77,11 @@ struct LabelArgs {
     const int32_t *selfpos;     // per row: position of its own index in its list (nullable)
     const int32_t *tile_csim;   // packed batches: c_sim / m_core per row tile (nullable)
     const int32_t *tile_mcore;
+    // the lists as row words per candidate tile pair (nullable; see k_round_b)
+    const uint64_t *mask;
+    const uint8_t *pair_nz;
+    const int64_t *tp_ptr;
+    const int32_t *tp_col;
 };
 
 __device__ __forceinline__ int csim_of(const LabelArgs &a, int64_t w) {
@@ -232,6 +237,54 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
+    if (a.mask) {
+        // The row's entries as words, one per candidate column tile (64 tiles a
+        // step, ascending column tile, so entries come in list order): tiles
+        // whose points share a rank that is not above w's, or that w has joined,
+        // are skipped without reading their entries.
+        const int64_t R = w >> 6;
+        const int r = (int)(w & 63);
+        const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
+        for (int64_t k0 = kb; k0 < ke; k0 += 64) {
+            const int64_t k = k0 + lane;
+            uint64_t v = 0ull, tr = 0ull;
+            int C = 0;
+            if (k < ke && a.pair_nz[k]) v = a.mask[k * 64 + r];
+            if (v) {
+                C = a.tp_col[k];
+                tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
+            }
+            uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
+            while (tiles) {
+                const int src = __ffsll((unsigned long long)tiles) - 1;
+                tiles &= tiles - 1;
+                const uint32_t tlo = __builtin_amdgcn_readlane((uint32_t)tr, src);
+                const uint32_t thi = __builtin_amdgcn_readlane((uint32_t)(tr >> 32), src);
+                const uint64_t trs = ((uint64_t)thi << 32) | tlo;
+                if (trs != ~0ull && (int)tlo == resolved) continue;
+                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+                const int j = __builtin_amdgcn_readlane(C, src) * 64 + lane;
+                uint64_t rj = 0ull;
+                if (bit) rj = trs == ~0ull ? a.rank[j] : trs;
+                const int kj = (int)(uint32_t)rj;
+                uint64_t bits = __ballot(bit && rj > rw && kj != resolved);
+                while (bits) {
+                    const int jsrc = __ffsll((unsigned long long)bits) - 1;
+                    bits &= bits - 1;
+                    const int jj = __builtin_amdgcn_readlane(j, jsrc);
+                    const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
+                    if (test_and_link(a, (int)w, jj, b, e, c, tests, passed)) {
+                        resolved = kjj;
+                        bits &= ~__ballot(kj == kjj);
+                    }
+                }
+            }
+        }
+        add_stats(a, tests, passed, 1);
+        return;
+    }
     // four 64-entry windows per step: their loads and rank lookups are issued
     // together, so a row's scan waits on one memory round trip per 256 entries
     constexpr int kWin = 4;
@@ -599,6 +652,12 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         CNN_HIP(hipGetLastError());
         a.cand_max = cand_max->p;
         a.tile_rank = tile_common->p;
+        if (req.mask && req.pair_nz && req.tp_ptr) {
+            a.mask = req.mask;
+            a.pair_nz = req.pair_nz;
+            a.tp_ptr = req.tp_ptr;
+            a.tp_col = req.tp_col;
+        }
     }
     k_round_b<<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipGetLastError());
@@ -846,7 +905,7 @@ int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, consider, current_start, sim_mode,
-                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
+                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
         labels_run(ctx, r, stats_out);
         return (int)CNN_OK;
     });
@@ -872,7 +931,7 @@ int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_cor
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
         labels_run(ctx, r, lstats);
         return (int)CNN_OK;
     });
@@ -890,7 +949,7 @@ int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, do
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
         LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos};
+                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
         labels_run(ctx, r, lstats);
         if (graph_out) *graph_out = hold.release();
         return (int)CNN_OK;

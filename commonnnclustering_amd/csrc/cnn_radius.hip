@@ -1927,6 +1927,8 @@ void graph_free(cnn_graph *g) {
     graph_mem_free(g, g->tp_ptr);
     graph_mem_free(g, g->tp_col);
     graph_mem_free(g, g->selfpos);
+    graph_mem_free(g, g->mask);
+    graph_mem_free(g, g->pair_nz);
     graph_mem_free(g, g->bounds);
     delete g;
 }
@@ -2103,11 +2105,23 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     }
     int64_t max_batch_pairs = 0;
     for (auto &b : batches) max_batch_pairs = std::max(max_batch_pairs, pair_begin(b.second) - pair_begin(b.first));
-    Scratch<uint64_t> mask(ctx, std::max<int64_t>(max_batch_pairs, 1) * 64);
-    Scratch<uint8_t> pair_nz(ctx, std::max<int64_t>(max_batch_pairs, 1));
-
     // symmetric evaluation needs every mask of the graph in one buffer
     const int sym = batches.size() == 1 ? 1 : 0;
+    // a single-batch spatial build keeps its words: the label pass walks a row's
+    // lists tile by tile from them
+    uint64_t *mask_p = nullptr;
+    uint8_t *pair_nz_p = nullptr;
+    std::unique_ptr<Scratch<uint64_t>> mask_s;
+    std::unique_ptr<Scratch<uint8_t>> pair_nz_s;
+    if (sym && g->layout == 1) {
+        mask_p = g->mask = dev_alloc<uint64_t>(g, std::max<int64_t>(max_batch_pairs, 1) * 64);
+        pair_nz_p = g->pair_nz = dev_alloc<uint8_t>(g, std::max<int64_t>(max_batch_pairs, 1));
+    } else {
+        mask_s.reset(new Scratch<uint64_t>(ctx, std::max<int64_t>(max_batch_pairs, 1) * 64));
+        pair_nz_s.reset(new Scratch<uint8_t>(ctx, std::max<int64_t>(max_batch_pairs, 1)));
+        mask_p = mask_s->p;
+        pair_nz_p = pair_nz_s->p;
+    }
     float ms_dist = 0.0f, ms_emit = 0.0f, ms_kernel = 0.0f;
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
@@ -2125,7 +2139,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.tp_ptr = tp_ptr;
         a.tp_col = tp_col;
         a.k_base = pair_begin(b.first);
-        a.mask = mask.p;
+        a.mask = mask_p;
         a.deg = deg.p;
         a.bp = bp;
         a.band_stats = bstats.p;
@@ -2137,7 +2151,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.xca = xca ? xca->p : nullptr;
         a.emarg = emarg ? emarg->p : nullptr;
         a.col_chunk = 32;
-        a.pair_nz = pair_nz.p;
+        a.pair_nz = pair_nz_p;
         a.pair_groups = gmask ? gmask->p : nullptr;
         a.bounds = g->bounds;
         if (pk) {
