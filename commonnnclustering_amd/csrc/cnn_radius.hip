@@ -1590,7 +1590,6 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
     const int64_t wl = R * 64 + wave * kRows + lane;
     int64_t mypos = (lane < kRows && wl < a.n) ? row_ptr[wl] : 0;
     const int64_t myrow0 = mypos;  // the row's first entry
-    const uint64_t lanebit = 1ull << lane;
     int64_t next = kb;  // gathering cursor (wave 0)
     for (;;) {
         __syncthreads();
@@ -1673,15 +1672,17 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
             int eq = 0;  // entries of this chunk's earlier words: a running scalar sum
             while (nz) {
                 const int q = __ffsll((unsigned long long)nz) - 1;
-                nz &= nz - 1;
+                nz ^= 1ull << q;
                 const uint32_t wlo = __builtin_amdgcn_readlane((uint32_t)v, q);
                 const uint32_t whi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), q);
                 const int cq = __builtin_amdgcn_readlane(cbase, q);
                 const uint64_t w = ((uint64_t)whi << 32) | wlo;
-                if (w & lanebit) {
+                // lanes whose bit is set: w (uniform) becomes the exec mask directly
+                if (__builtin_amdgcn_inverse_ballot_w64(w)) {
                     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(whi, __builtin_amdgcn_mbcnt_lo(wlo, 0u));
-                    int32_t *__restrict__ wp = rowp + eq;  // uniform
-                    wp[rank] = cq + lane;
+                    // 32-bit byte offset from the scalar row base (a chunk holds < 2^12 entries)
+                    const uint32_t boff = (uint32_t)(eq + (int)rank) << 2;
+                    *(int32_t *)((char *)rowp + boff) = cq + lane;
                     if (cq + lane == (int)row) selfpos[row] = done + eq + (int)rank;
                 }
                 eq += __popcll(w);
