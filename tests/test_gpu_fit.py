@@ -175,3 +175,24 @@ def test_neighbourhood_recipes(cuda):
         cl = Clustering(shuffled if recipe == "neighbourhoods" else lists, recipe=recipe)
         cl.fit(r, c, v=False)
         assert (cl.labels == want).all(), recipe
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_labels_word_walk_and_list_walk(cuda, monkeypatch, d):
+    """Spatial graphs built in one batch keep their row words and the label pass
+    walks rows tile by tile from them; a mask budget that splits the build into
+    row-tile batches drops the words and the pass reads the lists.  Both give
+    the oracle's labels."""
+    rng = np.random.default_rng(40 + d)
+    n = 5000 + 37  # a partial last tile
+    centres = rng.uniform(-4, 4, size=(6, d))
+    X = (centres[rng.integers(0, 6, size=n)] + rng.normal(size=(n, d)) * 0.7).astype(np.float32)
+    r, c, mc = 0.45, 6, 5
+    want = C.fit(X, r, c, member_cutoff=mc, use_grid=True)
+    got, gs, ls = gpu_fit(X, r, c, member_cutoff=mc, precision="f32_band")
+    assert gs["distance_launches"] == 1 and gs["layout"] == 1
+    assert (got == want).all()
+    monkeypatch.setenv("CNN_HIP_MASK_BUDGET", str(512 * 200 * 3))
+    got_b, gs_b, ls_b = gpu_fit(X, r, c, member_cutoff=mc, precision="f32_band")
+    assert gs_b["distance_launches"] > 1
+    assert (got_b == want).all()
