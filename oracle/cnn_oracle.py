@@ -61,9 +61,11 @@ How each part is pinned (tests/test_oracle.py):
     points; the Prim forest's total weight equals scipy's maximum spanning
     forest and, with distinct weights, its edge set; every MST hierarchy node
     (distinct weights) is a single-linkage cluster: the component holding its
-    merge edge among the tree edges of weight >= its lambda.  The folding of
-    tied weights in ``_check_children`` has no independent pin ("parity
-    unpinned" for ties): it is checked only against the product.
+    merge edge among the tree edges of weight >= its lambda.  Folding of tied
+    weights, lifting of a single child's children (with its lambda) and
+    pruning below member_cutoff in ``_check_children`` are pinned by three
+    merge-loop runs worked out by hand from _fit.pyx:945-1011 and
+    _bundle.pyx:354-398 (tests/test_hierarchy.py, MST_KATS).
 """
 
 from __future__ import annotations
