@@ -42,6 +42,9 @@ def lib():
         L.oracle_sort_by_size.argtypes = [_i64p, _i64, _i64, _i64, ctypes.POINTER(_i64)]
         L.oracle_check_similarity.argtypes = [_i32p, _i64, _i32p, _i64, _i64]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_neighbours_bf.argtypes = [_f64p, _i64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, _i64,
+                                           _i64p, ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -118,3 +121,36 @@ def fit(X, radius_cutoff, cnn_cutoff, member_cutoff=None, max_clusters=None,
     if sort:
         sort_by_size(labels, member_cutoff, max_clusters)
     return labels
+
+
+def neighbours_bf(X, radius_cutoff: float = None, rows=None, r2: float = None):
+    """Brute-force N(i) (rows ascending) for every point, or for the listed rows:
+    (row_ptr int64[len(rows)+1], col int32[nnz]).  Exact early exit (see
+    oracle_neighbours_bf); the squared radius is r*r unless r2 is given."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    if r2 is None:
+        r = float(radius_cutoff)
+        r2 = r * r
+    L = lib()
+    if rows is None:
+        rows_arr, n_rows, rows_p = None, n, None
+    else:
+        rows_arr = np.ascontiguousarray(rows, dtype=np.int64)
+        if rows_arr.size and (rows_arr.min() < 0 or rows_arr.max() >= n):
+            raise IndexError("row index out of range")
+        n_rows = rows_arr.size
+        rows_p = rows_arr.ctypes.data_as(ctypes.c_void_p)
+    row_ptr = np.zeros(n_rows + 1, dtype=np.int64)
+    out = ctypes.c_void_p()
+    rc = L.oracle_neighbours_bf(X, n, d, float(r2), rows_p, n_rows, row_ptr, ctypes.byref(out))
+    if rc != 0:
+        raise MemoryError("oracle_neighbours_bf failed")
+    nnz = int(row_ptr[-1])
+    try:
+        col = np.empty(nnz, dtype=np.int32)
+        if nnz:
+            ctypes.memmove(col.ctypes.data, out.value, nnz * 4)
+    finally:
+        L.oracle_free(out)
+    return row_ptr, col

@@ -220,6 +220,122 @@ int oracle_neighbours_fill(const double *X, i64 n, int d, double r2, int use_gri
     return 0;
 }
 
+/* ------------------------------------------------------------ brute force, exact early exit */
+
+/* Brute-force neighbour rows for large inputs (the same sets as
+ * oracle_neighbours_fill without the grid).  Two exact shortcuts, neither of
+ * which changes a single decision:
+ *   - two columns per SSE2 vector: the vector subtract, multiply and add are
+ *     the same correctly rounded IEEE operations as the scalar ones (built
+ *     with -ffp-contract=off, so no FMA), applied to each lane in ascending k;
+ *   - early exit: every term fl(t*t) is >= +0 and round-to-nearest addition of
+ *     a non-negative term never yields less than the running sum (t + x >= t
+ *     and rounding is monotone), so partial sums never decrease.  Once the
+ *     partial sums of all columns of a block exceed r2, their full sums do
+ *     too and none of them is a neighbour.  A NaN partial sum never compares
+ *     greater and keeps its block going; its full sum is NaN, which fails
+ *     s <= r2 as in the reference.
+ * rows: the row indices to compute (NULL: all n, in order); row_ptr receives
+ * n_rows + 1 offsets and *col_out a malloc'ed array of the entries, every row
+ * ascending (free it with oracle_free). */
+typedef double v2d __attribute__((vector_size(16)));
+
+#define BF_COLS 8      /* columns per block: 4 vectors of 2 */
+#define BF_CHUNK 8     /* dimensions between early-exit checks */
+
+static i64 bf_row(const double *XT, i64 n, i64 n_pad, int d, double r2, const double *xi, i32 *out) {
+    i64 cnt = 0;
+    for (i64 j0 = 0; j0 < n; j0 += BF_COLS) {
+        v2d s0 = {0.0, 0.0}, s1 = s0, s2 = s0, s3 = s0;
+        int k = 0;
+        for (;;) {
+            const int kend = k + BF_CHUNK < d ? k + BF_CHUNK : d;
+            for (; k < kend; ++k) {
+                const double *col = XT + (i64)k * n_pad + j0;
+                const v2d a = {xi[k], xi[k]};
+                v2d t0 = a - *(const v2d *)(col + 0);
+                v2d t1 = a - *(const v2d *)(col + 2);
+                v2d t2 = a - *(const v2d *)(col + 4);
+                v2d t3 = a - *(const v2d *)(col + 6);
+                t0 = t0 * t0; t1 = t1 * t1; t2 = t2 * t2; t3 = t3 * t3;
+                s0 = s0 + t0; s1 = s1 + t1; s2 = s2 + t2; s3 = s3 + t3;
+            }
+            if (k >= d) break;
+            if (s0[0] > r2 && s0[1] > r2 && s1[0] > r2 && s1[1] > r2 &&
+                s2[0] > r2 && s2[1] > r2 && s3[0] > r2 && s3[1] > r2)
+                break;
+        }
+        if (k < d) continue;  /* every column of the block is beyond r2 */
+        const double s[BF_COLS] = {s0[0], s0[1], s1[0], s1[1], s2[0], s2[1], s3[0], s3[1]};
+        for (int c = 0; c < BF_COLS; ++c) {
+            const i64 j = j0 + c;
+            if (j < n && s[c] <= r2) {
+                if (out) out[cnt] = (i32)j;
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+int oracle_neighbours_bf(const double *X, i64 n, int d, double r2, const i64 *rows, i64 n_rows,
+                         i64 *row_ptr, i32 **col_out) {
+    *col_out = NULL;
+    const i64 n_pad = (n + BF_COLS - 1) / BF_COLS * BF_COLS;
+    double *XT = (double *)aligned_alloc(64, sizeof(double) * (size_t)(n_pad > 0 ? n_pad : 1) * (d > 0 ? d : 1));
+    i32 **lists = (i32 **)calloc((size_t)(n_rows > 0 ? n_rows : 1), sizeof(i32 *));
+    i64 *deg = (i64 *)calloc((size_t)(n_rows > 0 ? n_rows : 1), sizeof(i64));
+    if (!XT || !lists || !deg) { free(XT); free(lists); free(deg); return -2; }
+    /* columns padded with +inf: (x - inf)^2 = inf, never a neighbour */
+    #pragma omp parallel for schedule(static)
+    for (i64 j = 0; j < n_pad; ++j)
+        for (int k = 0; k < d; ++k) XT[(i64)k * n_pad + j] = j < n ? X[j * d + k] : INFINITY;
+    int failed = 0;
+    #pragma omp parallel
+    {
+        i32 *buf = (i32 *)malloc(sizeof(i32) * (size_t)(n > 0 ? n : 1));
+        if (!buf) {
+            #pragma omp atomic write
+            failed = 1;
+        }
+        #pragma omp for schedule(dynamic, 16)
+        for (i64 t = 0; t < n_rows; ++t) {
+            if (!buf) continue;
+            const i64 i = rows ? rows[t] : t;
+            const i64 cnt = bf_row(XT, n, n_pad, d, r2, X + i * d, buf);
+            lists[t] = (i32 *)malloc(sizeof(i32) * (size_t)(cnt > 0 ? cnt : 1));
+            if (!lists[t]) {
+                #pragma omp atomic write
+                failed = 1;
+                continue;
+            }
+            memcpy(lists[t], buf, sizeof(i32) * (size_t)cnt);
+            deg[t] = cnt;
+        }
+        free(buf);
+    }
+    free(XT);
+    i32 *col = NULL;
+    if (!failed) {
+        row_ptr[0] = 0;
+        for (i64 t = 0; t < n_rows; ++t) row_ptr[t + 1] = row_ptr[t] + deg[t];
+        col = (i32 *)malloc(sizeof(i32) * (size_t)(row_ptr[n_rows] > 0 ? row_ptr[n_rows] : 1));
+        if (!col) failed = 1;
+    }
+    if (!failed) {
+        #pragma omp parallel for schedule(dynamic, 64)
+        for (i64 t = 0; t < n_rows; ++t) memcpy(col + row_ptr[t], lists[t], sizeof(i32) * (size_t)deg[t]);
+    }
+    for (i64 t = 0; t < n_rows; ++t) free(lists[t]);
+    free(lists);
+    free(deg);
+    if (failed) { free(col); return -2; }
+    *col_out = col;
+    return 0;
+}
+
+void oracle_free(void *p) { free(p); }
+
 /* ------------------------------------------------------------ similarity */
 
 /* Two-pointer merge with early exit at c (_types.pyx:2654-2706).  *la / *lb

@@ -225,6 +225,35 @@ def test_c_oracle_neighbourhoods_match_numpy(d, use_grid):
     assert (rp == rp2).all() and (col == col2).all()
 
 
+@pytest.mark.parametrize("d", [1, 3, 7, 8, 9, 16, 64, 65])
+def test_c_oracle_brute_force_early_exit_matches_numpy(d):
+    """The SIMD brute force with its early exit (used at full workload sizes)
+    gives the literal restatement's sets, including non-finite and huge
+    coordinates, duplicates, pairs at exactly r and a row subset."""
+    rng = np.random.default_rng(100 + d)
+    X = rng.normal(size=(403, d))
+    X[5] = X[6]
+    X[9, 0] = np.nan
+    X[11, d // 2] = np.inf
+    X[13] = 1e200
+    X[20] = X[21]
+    X[20, 0] = X[21, 0] + 0.75           # exactly r apart along one axis
+    X[22] = X[21]
+    X[22, 0] = np.nextafter(X[21, 0] + 0.75, np.inf)
+    r = 0.75 if d == 1 else 1.1 * np.sqrt(d)
+    if d > 1:
+        X[20, 1:] = X[21, 1:]
+        X[22, 1:] = X[21, 1:]
+        r = 0.75 if d < 9 else r
+    rp, col = O.neighbourhoods_csr(X, r)
+    rp2, col2 = C.neighbours_bf(X, r)
+    assert (rp == rp2).all() and (col == col2).all()
+    rows = rng.integers(0, len(X), size=37)
+    rp3, col3 = C.neighbours_bf(X, r, rows=rows)
+    for t, i in enumerate(rows):
+        assert (col3[rp3[t]:rp3[t + 1]] == col[rp[i]:rp[i + 1]]).all()
+
+
 # ------------------------------------------------------------------ predict
 
 def test_predict_moons_reference_case():
