@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_objs")
 LIB = os.path.join(HERE, "libcnn_hip.so")
 SOURCES = ["cnn_util.hip", "cnn_radius.hip", "cnn_labels.hip", "cnn_batch.hip", "cnn_predict.hip", "cnn_metrics.hip",
-           "cnn_hierarchy.hip"]
+           "cnn_hierarchy.hip", "cnn_noalloc.hip"]
 
 # -ffp-contract=off: FP64 distance code must keep sub/mul/add separately
 # rounded (reference order); FP32 code requests FMA explicitly.

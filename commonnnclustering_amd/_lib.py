@@ -44,7 +44,7 @@ EXPORTED = [
     "cnn_graph_bytes", "cnn_graph_export_csr", "cnn_radius_csr", "cnn_graph_rethreshold",
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
     "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_graph_build_periodic",
-    "cnn_csr_edge_similarity", "cnn_similarity_mst", "cnn_version",
+    "cnn_csr_edge_similarity", "cnn_similarity_mst", "cnn_version", "cnn_labels_from_csr_noalloc",
 ]
 
 
@@ -130,6 +130,8 @@ def _declare(L):
         "cnn_graph_labels": (c, [_vp, _vp, _i32, _i32, _vp, _i64, _i32, _vp, ctypes.POINTER(LabelStats)]),
         "cnn_labels_from_csr": (c, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _vp,
                                     ctypes.POINTER(LabelStats)]),
+        "cnn_labels_from_csr_noalloc": (c, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _vp,
+                                            ctypes.POINTER(LabelStats)]),
         "cnn_sort_by_size": (c, [_vp, _vp, _i64, _i64, _i64, _i64p]),
         "cnn_fit": (c, [_vp, _vp, _i64, _i32, _i32, _dbl, _i32, _i32, _i32, _i32, _i64, _i64, _vp,
                         ctypes.POINTER(_vp), ctypes.POINTER(GraphStats), ctypes.POINTER(LabelStats)]),

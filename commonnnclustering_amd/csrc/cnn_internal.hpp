@@ -45,6 +45,16 @@ private:
     int64_t total_ = 0;
 };
 
+// A fixed device buffer handed out front to back (entry points that may not
+// allocate).  Blocks are 256-byte aligned; releasing the most recent block
+// makes its bytes available again (Scratch buffers die in reverse order).
+struct FixedWorkspace {
+    char *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    size_t peak = 0;
+};
+
 }  // namespace cnn
 
 struct cnn_ctx {
@@ -58,6 +68,9 @@ struct cnn_ctx {
     // contexts (own stream and arena each) that fit independent datasets of a
     // batch concurrently; created on first use, destroyed with this context
     std::vector<cnn_ctx *> workers;
+    // when set, every Scratch buffer comes from this fixed buffer instead of
+    // the arena (no device allocation)
+    cnn::FixedWorkspace *fixed = nullptr;
 };
 
 struct cnn_graph {
@@ -96,16 +109,45 @@ struct cnn_graph {
 
 namespace cnn {
 
-// RAII scratch buffer from the context arena.
+struct InvalidArg { std::string msg; };
+struct OutOfMemory { std::string msg; };
+struct Capacity { std::string msg; };
+
+[[noreturn]] inline void fixed_exhausted(size_t bytes, const FixedWorkspace &f) {
+    throw Capacity{"fixed workspace exhausted: " + std::to_string(bytes) + " bytes requested, " +
+                   std::to_string(f.size - f.used) + " of " + std::to_string(f.size) + " free"};
+}
+
+// RAII scratch buffer from the context arena (or from the context's fixed
+// workspace when one is set).
 template <typename T>
 struct Scratch {
     cnn_ctx *ctx;
     T *p = nullptr;
+    size_t fixed_from = 0, fixed_to = 0;  // block within ctx->fixed (fixed_to == 0: arena)
     Scratch(cnn_ctx *c, size_t count) : ctx(c) {
         if (count == 0) count = 1;
+        if (ctx->fixed) {
+            FixedWorkspace &f = *ctx->fixed;
+            const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+            if (bytes > f.size - f.used) fixed_exhausted(bytes, f);
+            fixed_from = f.used;
+            fixed_to = f.used + bytes;
+            f.used = fixed_to;
+            if (f.used > f.peak) f.peak = f.used;
+            p = reinterpret_cast<T *>(f.base + fixed_from);
+            return;
+        }
         p = static_cast<T *>(ctx->arena->get(count * sizeof(T)));
     }
-    ~Scratch() { if (p) ctx->arena->put(p); }
+    ~Scratch() {
+        if (!p) return;
+        if (fixed_to) {
+            if (ctx->fixed && ctx->fixed->used == fixed_to) ctx->fixed->used = fixed_from;
+            return;
+        }
+        ctx->arena->put(p);
+    }
     Scratch(const Scratch &) = delete;
     Scratch &operator=(const Scratch &) = delete;
 };
@@ -121,9 +163,6 @@ inline void hip_check(hipError_t e, const char *what) {
 
 #define CNN_HIP(call) ::cnn::hip_check((call), #call)
 
-struct InvalidArg { std::string msg; };
-struct OutOfMemory { std::string msg; };
-struct Capacity { std::string msg; };
 
 // Convert exceptions raised inside an entry point into a status code.
 template <typename F>
@@ -242,6 +281,8 @@ struct PackedSort {
     const int64_t *ds_offset;   // device, per dataset (nullable)
 };
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st);
+// Looks up this device's copy of the no-allocation workspace (device current).
+void noalloc_setup(int device);
 int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,
                              int64_t max_clusters);
 

@@ -718,9 +718,9 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     if (req.consider) fill_u8(ctx, req.consider, 0, n);
     CNN_HIP(hipEventRecord(ctx->ev[2], s));
     unsigned long long h[8];
-    std::vector<unsigned long long> ht(4 * kStatSlots);
+    unsigned long long ht[4 * kStatSlots];
     CNN_HIP(hipMemcpyAsync(h, counters.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipMemcpyAsync(ht.data(), tstats.p, ht.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    CNN_HIP(hipMemcpyAsync(ht, tstats.p, sizeof(ht), hipMemcpyDeviceToHost, s));
     CNN_HIP(hipStreamSynchronize(s));
     h[0] = h[1] = 0;
     unsigned long long rows = 0;

@@ -183,6 +183,7 @@ int cnn_ctx_create(int device, cnn_ctx **out) {
         CNN_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
         ctx->arena = new Arena(device);
+        noalloc_setup(device);
         return (int)CNN_OK;
     });
     if (rc != CNN_OK) {

@@ -162,6 +162,20 @@ int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_
 int cnn_labels_from_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n,
                         int32_t c_sim, int32_t m_core, uint8_t *consider, int64_t current_start,
                         int32_t sim_mode, int64_t *labels, cnn_label_stats *stats_out);
+/* cnn_labels_from_csr without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, serving loops with a fixed memory plan).  Same arguments,
+ * semantics and results as cnn_labels_from_csr; every temporary comes from a
+ * fixed device workspace that the library holds per device (32 MiB, from the
+ * first context created on the device until the process ends), so
+ * n <= CNN_NOALLOC_MAX_POINTS (larger n: CNN_ERR_INVALID) and similarities are
+ * tested by the sorted-list merge (CNN_SIM_BITSET: CNN_ERR_INVALID).  All calls
+ * on one device share the workspace: calls that overlap on one device (other
+ * host threads, other streams) race and must be serialised by the caller. */
+#define CNN_NOALLOC_MAX_POINTS (1 << 20)
+int cnn_labels_from_csr_noalloc(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n,
+                                int32_t c_sim, int32_t m_core, uint8_t *consider, int64_t current_start,
+                                int32_t sim_mode, int64_t *labels, cnn_label_stats *stats_out);
 /* Labels.sort_by_size in place on device int64 labels: clusters ranked by size
  * descending, ties by first occurrence; ranks beyond max_clusters (< 0: no
  * limit) and clusters smaller than member_cutoff become 0; survivors are
