@@ -26,6 +26,10 @@ CNN_ERR_UNSUPPORTED = -5
 CNN_F32 = 0
 CNN_F64 = 1
 
+CNN_SIM_FORM_MERGE = 1
+CNN_SIM_FORM_TILE_WORDS = 2
+CNN_SIM_FORM_ROW_WORDS = 3
+
 CNN_PLAIN_F64 = 0
 CNN_F32_BAND = 1
 
@@ -92,6 +96,7 @@ class LabelStats(ctypes.Structure):
         ("rows_scanned", ctypes.c_int64),
         ("ms_similarity", ctypes.c_float),
         ("ms_labels", ctypes.c_float),
+        ("sim_form", ctypes.c_int32),
     ]
 
     def to_dict(self) -> dict:

@@ -55,6 +55,7 @@ static void add_label_stats(cnn_label_stats &ls, const cnn_label_stats &o) {
     ls.rows_scanned += o.rows_scanned;
     ls.ms_similarity += o.ms_similarity;
     ls.ms_labels += o.ms_labels;
+    ls.sim_form = std::max(ls.sim_form, o.sim_form);
 }
 
 // All datasets in one working order (1 <= d <= 8): one graph build, one
@@ -83,8 +84,12 @@ static void fit_packed(cnn_ctx *ctx, const void *X, const int64_t *offsets, int3
     CNN_HIP(hipMemcpyAsync(d_off.p, offsets, (n_sets + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     PackedSort ps{n_sets, 2 * size_bits, size_bits, (uint32_t)(((uint64_t)1 << size_bits) - 1), ds_bits + 2 * size_bits,
                   d_ds.p, d_off.p};
-    LabelRequest r{g->row_ptr, g->col, g->n, g->perm, 0, 0, nullptr, 1, CNN_SIM_AUTO, true, true,
-                   member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
+    LabelRequest r = graph_label_request(ctx, g, CNN_SIM_AUTO);
+    r.sort = true;
+    r.write_all = true;
+    r.member_cutoff = member_cutoff;
+    r.max_clusters = max_clusters;
+    r.labels = labels;
     r.tile_csim = d_c.p;
     r.tile_mcore = d_m.p;
     r.packed = &ps;
@@ -153,9 +158,14 @@ static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_
         const char *xb = (const char *)X + (size_t)offsets[b] * d * esize;
         cnn_graph *g = graph_build(wc, xb, n, d, dtype, r2[b], precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
-        LabelRequest r{g->row_ptr, g->col, n, g->perm, c_sim[b], m_core[b], nullptr, 1, CNN_SIM_AUTO,
-                       true, true, member_cutoff, max_clusters, labels + offsets[b], g->tp_ptr, g->tp_col,
-                       g->n_tiles, g->selfpos, g->mask, g->pair_nz};
+        LabelRequest r = graph_label_request(wc, g, CNN_SIM_AUTO);
+        r.c_sim = c_sim[b];
+        r.m_core = m_core[b];
+        r.sort = true;
+        r.write_all = true;
+        r.member_cutoff = member_cutoff;
+        r.max_clusters = max_clusters;
+        r.labels = labels + offsets[b];
         cnn_label_stats one{};
         labels_run(wc, r, &one);
         std::lock_guard<std::mutex> lock(mu);
@@ -222,6 +232,10 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
         CNN_HIP(hipSetDevice(ctx->device));
         cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2[order[0]], precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        // smaller radii filter the lists of the largest
+        bool smaller = false;
+        for (int32_t i = 0; i < n_r; ++i) smaller = smaller || r2[i] < r2[order[0]];
+        if (smaller && n > 0) graph_lists(ctx, g);
         if (gstats) *gstats = g->stats;
         // lists of the radius being labelled; the graph's own for the largest
         const int64_t *rp = g->row_ptr;
@@ -265,10 +279,21 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
                     // self positions and row words describe the graph's own lists, not lists
                     // filtered to a smaller r
                     const bool own = rp == g->row_ptr;
-                    LabelRequest req{rp, col, n, g->perm, c_sim[ci], m_core[ci], nullptr, 1, CNN_SIM_AUTO, true, true,
-                                     member_cutoff, max_clusters, labels + ((int64_t)ri * n_c + ci) * n,
-                                     g->tp_ptr, g->tp_col, g->n_tiles, own ? g->selfpos : nullptr,
-                                     own ? g->mask : nullptr, own ? g->pair_nz : nullptr};
+                    LabelRequest req = graph_label_request(wc, g, CNN_SIM_AUTO);
+                    if (!own) {
+                        req.row_ptr = rp;
+                        req.col = col;
+                        req.selfpos = nullptr;
+                        req.mask = nullptr;
+                        req.pair_nz = nullptr;
+                    }
+                    req.c_sim = c_sim[ci];
+                    req.m_core = m_core[ci];
+                    req.sort = true;
+                    req.write_all = true;
+                    req.member_cutoff = member_cutoff;
+                    req.max_clusters = max_clusters;
+                    req.labels = labels + ((int64_t)ri * n_c + ci) * n;
                     cnn_label_stats one{};
                     labels_run(wc, req, &one);
                     if (lstats) lstats[(int64_t)ri * n_c + ci] = one;

@@ -105,6 +105,9 @@ struct cnn_graph {
     int64_t n_tile_pairs = 0;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
+    // a graph that keeps its words emits its lists (col, selfpos) only when a
+    // caller needs them; this lock makes that happen once
+    std::mutex lists_mu;
 };
 
 namespace cnn {
@@ -227,6 +230,10 @@ void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *c
 // row of col is strictly ascending within [0, n).
 void validate_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col, int64_t n);
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_t precision);
+// Emits the neighbour lists (col, selfpos) of a graph that holds only its row
+// words; no-op when they exist.  Safe to call from several contexts at once;
+// the lists are complete when it returns.
+void graph_lists(cnn_ctx *ctx, const cnn_graph *g);
 int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int64_t *rp, const int32_t *col,
                    double r2, int64_t *out_rp, int32_t **out_col);
 
@@ -263,6 +270,12 @@ struct LabelRequest {
     const int32_t *selfpos = nullptr; // per row: position of the row's own index in its list (nullable)
     const uint64_t *mask = nullptr;   // the lists as row words per candidate tile pair (nullable)
     const uint8_t *pair_nz = nullptr;
+    // lists as per-row word lists (nullable): row w's words are wword[wptr[w] ..
+    // wptr[w+1]), word t holding the row's neighbours among the 64 points of
+    // column tile wtile[t], tiles ascending
+    const int64_t *wptr = nullptr;
+    const int32_t *wtile = nullptr;
+    const uint64_t *wword = nullptr;
     // packed batches: per-tile cutoffs and the per-dataset cluster ordering
     const int32_t *tile_csim = nullptr;
     const int32_t *tile_mcore = nullptr;
@@ -281,6 +294,10 @@ struct PackedSort {
     const int64_t *ds_offset;   // device, per dataset (nullable)
 };
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st);
+// Request over a resident graph (lists and/or words, tile structure, order);
+// emits the lists first when the similarity form needs them (sim_mode MERGE,
+// or a graph without words).  The caller fills in cutoffs and outputs.
+LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_mode);
 // Looks up this device's copy of the no-allocation workspace (device current).
 void noalloc_setup(int device);
 int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,
@@ -432,6 +449,78 @@ __device__ inline int wave_common(const int32_t *__restrict__ A, int64_t na, con
         ib += __popcll(__ballot(lane < vb && b <= m));
     }
     return count;
+}
+
+// A row's neighbour set as a list of 64-bit words over column tiles: word t
+// holds the row's neighbours among points 64*tile[t] .. 64*tile[t] + 63 (bit b
+// = point 64*tile[t] + b), tiles strictly ascending.  Words are strided
+// (stride 64 for the per-tile-pair masks of a spatial graph, 1 for per-row
+// lists); nz (nullable) flags the words that were stored -- an unflagged word
+// is empty.
+struct WordRow {
+    const int32_t *tile;
+    const uint64_t *word;
+    const uint8_t *nz;
+    int64_t len;
+    int stride;
+};
+
+__device__ __forceinline__ uint64_t word_at(const WordRow &r, int64_t t) {
+    if (r.nz && r.nz[t] == 0) return 0ull;
+    return r.word[t * (int64_t)r.stride];
+}
+
+// |A & B| >= c from two rows' word lists, by the whole wave: the words of
+// column tiles present in both lists are and-ed and their bits counted.
+// 64-wide windows of both tile lists advance like the two-pointer merge of
+// wave_sim, each lane looking its B tile up in the A window by a shuffle binary
+// search; rows sharing one tile list (the rows of one row tile of a spatial
+// graph) pair their words lane by lane.  Early exit once c common neighbours
+// are counted.  Wave-uniform result; every lane must call it.
+__device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c) {
+    if (c <= 0) return true;
+    const int lane = __lane_id();
+    int count = 0;
+    if (A.tile == B.tile && A.len == B.len) {
+        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+            const int64_t t = t0 + lane;
+            int cnt = 0;
+            if (t < A.len && (!A.nz || A.nz[t] != 0))
+                cnt = __popcll(A.word[t * (int64_t)A.stride] & B.word[t * (int64_t)B.stride]);
+            count += wave_sum(cnt);
+            if (count >= c) return true;
+        }
+        return false;
+    }
+    int64_t ia = 0, ib = 0;
+    while (ia < A.len && ib < B.len) {
+        const int va = (int)min<int64_t>(64, A.len - ia);
+        const int vb = (int)min<int64_t>(64, B.len - ib);
+        const int ta = lane < va ? A.tile[ia + lane] : INT_MAX;
+        const int tb = lane < vb ? B.tile[ib + lane] : INT_MAX;
+        const int alast = __shfl(ta, va - 1, 64);
+        const int blast = __shfl(tb, vb - 1, 64);
+        const int m = min(alast, blast);
+        int lo = 0, hi = va;
+#pragma unroll
+        for (int step = 0; step < 7; ++step) {
+            const int mid = (lo + hi) >> 1;
+            const int am = __shfl(ta, min(mid, 63), 64);
+            if (lo < hi) {
+                if (am < tb) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        const int found = __shfl(ta, min(lo, 63), 64);
+        int cnt = 0;
+        if (lane < vb && tb <= m && lo < va && found == tb)
+            cnt = __popcll(word_at(A, ia + lo) & word_at(B, ib + lane));
+        count += wave_sum(cnt);
+        if (count >= c) return true;
+        ia += __popcll(__ballot(lane < va && ta <= m));
+        ib += __popcll(__ballot(lane < vb && tb <= m));
+    }
+    return false;
 }
 
 }  // namespace cnn

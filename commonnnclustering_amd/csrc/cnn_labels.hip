@@ -77,12 +77,39 @@ struct LabelArgs {
     const int32_t *selfpos;     // per row: position of its own index in its list (nullable)
     const int32_t *tile_csim;   // packed batches: c_sim / m_core per row tile (nullable)
     const int32_t *tile_mcore;
-    // the lists as row words per candidate tile pair (nullable; see k_round_b)
+    // the lists as row words: per candidate tile pair of a spatial graph
+    // (mask / pair_nz over tp_ptr / tp_col), or per row (wptr / wtile / wword)
     const uint64_t *mask;
     const uint8_t *pair_nz;
     const int64_t *tp_ptr;
     const int32_t *tp_col;
+    const int64_t *wptr;
+    const int32_t *wtile;
+    const uint64_t *wword;
 };
+
+// Row w's words: its row of the candidate tile pairs of its row tile (spatial
+// graph masks), or its own word list.
+__device__ __forceinline__ WordRow word_row(const LabelArgs &a, int64_t w) {
+    WordRow r;
+    if (a.mask) {
+        const int64_t R = w >> 6;
+        const int64_t kb = a.tp_ptr[R];
+        r.tile = a.tp_col + kb;
+        r.word = a.mask + kb * 64 + (w & 63);
+        r.nz = a.pair_nz + kb;
+        r.len = a.tp_ptr[R + 1] - kb;
+        r.stride = 64;
+    } else {
+        const int64_t b = a.wptr[w];
+        r.tile = a.wtile + b;
+        r.word = a.wword + b;
+        r.nz = nullptr;
+        r.len = a.wptr[w + 1] - b;
+        r.stride = 1;
+    }
+    return r;
+}
 
 __device__ __forceinline__ int csim_of(const LabelArgs &a, int64_t w) {
     return a.tile_csim ? a.tile_csim[w >> 6] : a.c_sim;
@@ -128,17 +155,90 @@ __device__ __forceinline__ bool test_and_link(const LabelArgs &a, int w, int j, 
     return pass;
 }
 
+// The same on row words (bitset similarity); A is w's word row, dw its degree.
+__device__ __forceinline__ bool test_and_link_words(const LabelArgs &a, int w, int j, const WordRow &A, int64_t dw,
+                                                    int c, unsigned &tests, unsigned &passed) {
+    const int rw = uf_find(a.parent, w);
+    const int rj = uf_find(a.parent, j);
+    if (rw == rj) return true;
+    const int64_t dj = a.row_ptr[j + 1] - a.row_ptr[j];
+    const bool pass = dw >= c && dj >= c && wave_sim_words(A, word_row(a, j), c);
+    ++tests;
+    if (pass) {
+        ++passed;
+        if (__lane_id() == 0) uf_union(a.parent, w, j);
+    }
+    return pass;
+}
+
+__device__ __forceinline__ uint64_t lane_bit_mask(int lane) { return 1ull << lane; }
+
 // Sampling round (hooking): each core row tests its nearest smaller-index core
 // neighbours in list order (the previous points of the working order, which
 // are spatially closest) and points its parent at the first one that passes.
 // Every row writes only its own parent entry and parents have smaller
 // indices, so the hooks form a forest whose trees are the components of the
 // passing edges tested here -- no atomics, no contention.
+template <bool WORDS>
 __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    if constexpr (WORDS) {
+        // the row's words from its own tile down, 64 tiles a step (lane l holds
+        // the l-th tile below the window's top), bits from the highest down:
+        // smaller indices, nearest first
+        const WordRow A = word_row(a, w);
+        const int Rw = (int)(w >> 6);
+        int64_t top = 0;  // tiles up to and including the row's own tile
+        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+            const int64_t t = t0 + lane;
+            const uint64_t le = __ballot(t < A.len && A.tile[t] <= Rw);
+            top += __popcll(le);
+            if (__popcll(le) < 64) break;
+        }
+        const int c = csim_of(a, w);
+        unsigned tests = 0, passed = 0;
+        int attempts = 0;
+        bool done = false;
+        for (int64_t t1 = top; t1 > 0 && attempts < max_tests && !done; t1 -= 64) {
+            const int64_t t = t1 - 1 - lane;
+            uint64_t v = 0ull;
+            int C = 0;
+            if (t >= 0) {
+                C = A.tile[t];
+                v = word_at(A, t);
+                if (C == Rw) v &= lane_bit_mask((int)(w & 63)) - 1ull;  // smaller indices only
+            }
+            uint64_t tiles = __ballot(v != 0ull);
+            while (tiles && attempts < max_tests && !done) {
+                const int src = __ffsll((unsigned long long)tiles) - 1;
+                tiles &= tiles - 1;
+                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+                const int Cs = __builtin_amdgcn_readlane(C, src);
+                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+                uint64_t bits = __ballot(bit && a.core[Cs * 64 + lane]);
+                while (bits && attempts < max_tests) {
+                    const int hb = 63 - __clzll((long long)bits);
+                    bits &= ~lane_bit_mask(hb);
+                    const int jj = Cs * 64 + hb;
+                    ++attempts;
+                    ++tests;
+                    const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
+                    if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
+                        ++passed;
+                        if (lane == 0) a.parent[w] = jj;
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        }
+        add_stats(a, tests, passed);
+        return;
+    }
     // first entry >= w: the row's own entry, recorded when the lists were
     // emitted, or a 64-way search (each step probes 64 evenly spaced entries at
     // once and keeps the gap that holds the boundary)
@@ -185,6 +285,7 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
 // current root is smaller than itself (either side of it in the list) and
 // re-points itself at that root on the first pass.  Only roots write their own
 // entry and only to smaller values, so the parents stay a forest.
+template <bool WORDS>
 __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -192,6 +293,50 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     unsigned tests = 0, passed = 0;
     int attempts = 0;
+    if constexpr (WORDS) {
+        const WordRow A = word_row(a, w);
+        const int c = csim_of(a, w);
+        bool done = false;
+        for (int64_t t0 = 0; t0 < A.len && attempts < max_tests && !done; t0 += 64) {
+            const int64_t t = t0 + lane;
+            uint64_t v = 0ull;
+            int C = 0;
+            if (t < A.len) {
+                v = word_at(A, t);
+                C = A.tile[t];
+            }
+            uint64_t tiles = __ballot(v != 0ull);
+            while (tiles && attempts < max_tests && !done) {
+                const int src = __ffsll((unsigned long long)tiles) - 1;
+                tiles &= tiles - 1;
+                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+                const int Cs = __builtin_amdgcn_readlane(C, src);
+                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+                const int j = Cs * 64 + lane;
+                const int rj = (bit && a.core[j]) ? a.parent[j] : INT_MAX;  // parents are flattened
+                uint64_t bits = __ballot(rj < (int)w);
+                while (bits && attempts < max_tests) {
+                    const int src2 = __ffsll((unsigned long long)bits) - 1;
+                    bits &= bits - 1;
+                    const int jj = Cs * 64 + src2;
+                    const int rr = __shfl(rj, src2, 64);
+                    ++attempts;
+                    ++tests;
+                    const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
+                    if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
+                        ++passed;
+                        if (lane == 0) a.parent[w] = rr;
+                        done = true;
+                        break;
+                    }
+                    bits &= ~__ballot(rj == rr);  // one test per neighbouring component
+                }
+            }
+        }
+        add_stats(a, tests, passed);
+        return;
+    }
     for (int64_t t0 = b; t0 < e && attempts < max_tests; t0 += 64) {
         const int64_t t = t0 + lane;
         const int j = t < e ? a.col[t] : -1;
@@ -226,6 +371,7 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
 // reading its list.  Candidates are filtered with the read-only snapshot (one
 // 8-byte gather per entry); a stale snapshot only yields redundant candidates,
 // which test_and_link's find() check discards.
+template <bool WORDS>
 __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -237,21 +383,19 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
-    if (a.mask) {
-        // The row's entries as words, one per candidate column tile (64 tiles a
-        // step, ascending column tile, so entries come in list order): tiles
-        // whose points share a rank that is not above w's, or that w has joined,
-        // are skipped without reading their entries.
-        const int64_t R = w >> 6;
-        const int r = (int)(w & 63);
-        const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
-        for (int64_t k0 = kb; k0 < ke; k0 += 64) {
-            const int64_t k = k0 + lane;
+    if constexpr (WORDS) {
+        // The row's words, 64 tiles a step (ascending column tile, so entries
+        // come in list order): tiles whose points share a rank that is not
+        // above w's, or that w has joined, are skipped without reading their
+        // bits; tests are bitset intersections of the two rows' words.
+        const WordRow A = word_row(a, w);
+        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+            const int64_t t = t0 + lane;
             uint64_t v = 0ull, tr = 0ull;
             int C = 0;
-            if (k < ke && a.pair_nz[k]) v = a.mask[k * 64 + r];
+            if (t < A.len) v = word_at(A, t);
             if (v) {
-                C = a.tp_col[k];
+                C = A.tile[t];
                 tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
             }
             uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
@@ -275,7 +419,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
                     bits &= bits - 1;
                     const int jj = __builtin_amdgcn_readlane(j, jsrc);
                     const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
-                    if (test_and_link(a, (int)w, jj, b, e, c, tests, passed)) {
+                    if (test_and_link_words(a, (int)w, jj, A, e - b, c, tests, passed)) {
                         resolved = kjj;
                         bits &= ~__ballot(kj == kjj);
                     }
@@ -597,7 +741,82 @@ __global__ void k_write_unsorted(const uint8_t *core, const int *parent, const i
     else if (write_all) labels[o] = 0;
 }
 
+// ------------------------------------------------------------------ per-row word lists from lists
+
+// Rows of a CSR (strictly ascending) as word lists: one word per distinct
+// column tile (index >> 6) of a row.  Count per row, one wave per row.
+__global__ __launch_bounds__(256) void k_row_word_count(const int64_t *__restrict__ row_ptr,
+                                                        const int32_t *__restrict__ col, int64_t n,
+                                                        int32_t *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    int total = 0;
+    for (int64_t t0 = b; t0 < e; t0 += 64) {
+        const int64_t t = t0 + lane;
+        bool head = false;
+        if (t < e) head = t == b || (col[t - 1] >> 6) != (col[t] >> 6);
+        total += __popcll(__ballot(head));
+    }
+    if (lane == 0) cnt[w] = total;
+}
+
+// Fill: the entries of one word are consecutive in the row, so each window of
+// 64 entries ors its bits per segment (a segmented scan over the lanes, the
+// open segment of the previous window carried in) and the segment's last
+// entry writes the word and its tile.
+__global__ __launch_bounds__(256) void k_row_word_fill(const int64_t *__restrict__ row_ptr,
+                                                       const int32_t *__restrict__ col, int64_t n,
+                                                       const int64_t *__restrict__ wptr, int32_t *__restrict__ wtile,
+                                                       uint64_t *__restrict__ wword) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    const int64_t base = wptr[w];
+    int64_t run = 0;          // words started in earlier windows
+    uint64_t carry = 0ull;    // bits of the word still open at the end of the previous window
+    const uint64_t upto_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    for (int64_t t0 = b; t0 < e; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const bool valid = t < e;
+        int tile = 0;
+        uint64_t x = 0ull;
+        bool head = false, tail = false;
+        if (valid) {
+            const int v = col[t];
+            tile = v >> 6;
+            x = 1ull << (v & 63);
+            head = t == b || (col[t - 1] >> 6) != tile;
+            tail = t + 1 == e || (col[t + 1] >> 6) != tile;
+        }
+        const uint64_t hm = __ballot(head);
+        const uint64_t upto = hm & upto_mask;
+        const int start = upto ? 63 - __clzll((long long)upto) : -1;  // this lane's word starts here
+        if (start < 0) x |= carry;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t y = __shfl_up(x, off, 64);
+            if (lane >= off && lane - off >= start) x |= y;
+        }
+        const int64_t idx = base + run + __popcll(upto) - 1;
+        if (valid && tail) {
+            wtile[idx] = tile;
+            wword[idx] = x;
+        }
+        const bool open = __shfl((int)(valid && !tail), 63, 64) != 0;
+        const uint64_t last = __shfl(x, 63, 64);
+        carry = open ? last : 0ull;
+        run += __popcll(hm);
+    }
+}
+
 static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, ceil_div(n, per)); }
+
+// Entries per word from which the automatic choice tests similarities on
+// per-row word lists built from the lists (below it, the sorted-list merge).
+constexpr int64_t kWordsMinFill = 8;
 
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     hipStream_t s = ctx->stream;
@@ -620,15 +839,61 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
                 nullptr, req.selfpos, req.tile_csim, req.tile_mcore};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    // similarity form: bitset intersections of row words (the spatial graph's
+    // tile-pair masks, or per-row word lists built here from the lists), or the
+    // sorted-list merge
+    const bool tile_words = req.mask && req.pair_nz && req.tp_ptr;
+    bool words = false;
+    std::unique_ptr<Scratch<int64_t>> wptr;
+    std::unique_ptr<Scratch<int32_t>> wtile;
+    std::unique_ptr<Scratch<uint64_t>> wword;
+    if (req.sim_mode != CNN_SIM_MERGE && tile_words) {
+        words = true;
+        a.mask = req.mask;
+        a.pair_nz = req.pair_nz;
+        a.tp_ptr = req.tp_ptr;
+        a.tp_col = req.tp_col;
+    } else if (req.sim_mode != CNN_SIM_MERGE && req.col) {
+        wptr.reset(new Scratch<int64_t>(ctx, n + 1));
+        {
+            Scratch<int32_t> wcnt(ctx, n);
+            k_row_word_count<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wcnt.p);
+            CNN_HIP(hipGetLastError());
+            exclusive_scan_i32_to_i64(ctx, wcnt.p, wptr->p, n);
+            const int64_t last_excl = read_scalar(ctx, wptr->p + (n - 1));
+            const int32_t last_cnt = read_scalar(ctx, wcnt.p + (n - 1));
+            const int64_t n_words = last_excl + last_cnt;
+            const int64_t nnz = read_scalar(ctx, req.row_ptr + n) - read_scalar(ctx, req.row_ptr);
+            if (req.sim_mode == CNN_SIM_BITSET || nnz >= kWordsMinFill * n_words) {
+                CNN_HIP(hipMemcpyAsync(wptr->p + n, &n_words, sizeof(int64_t), hipMemcpyHostToDevice, s));
+                wtile.reset(new Scratch<int32_t>(ctx, n_words));
+                wword.reset(new Scratch<uint64_t>(ctx, n_words));
+                k_row_word_fill<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wptr->p, wtile->p,
+                                                                 wword->p);
+                CNN_HIP(hipGetLastError());
+                // the host value copied above must outlive the copy
+                CNN_HIP(hipStreamSynchronize(s));
+                words = true;
+                a.wptr = wptr->p;
+                a.wtile = wtile->p;
+                a.wword = wword->p;
+            }
+        }
+        if (!words) wptr.reset();
+    }
+    if (!words && !req.col) throw InvalidArg{"the sorted-list merge needs the neighbour lists"};
+
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
     CNN_HIP(hipGetLastError());
-    k_hook<<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
+    if (words) k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
+    else k_hook<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     CNN_HIP(hipGetLastError());
     // flatten (every parent becomes its root), then let roots re-point to smaller
     // neighbouring roots twice, flattening after each pass
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     for (int pass = 0; pass < 2; ++pass) {
-        k_hook_roots<<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
+        if (words) k_hook_roots<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
+        else k_hook_roots<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
         k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     }
     CNN_HIP(hipGetLastError());
@@ -652,14 +917,9 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         CNN_HIP(hipGetLastError());
         a.cand_max = cand_max->p;
         a.tile_rank = tile_common->p;
-        if (req.mask && req.pair_nz && req.tp_ptr) {
-            a.mask = req.mask;
-            a.pair_nz = req.pair_nz;
-            a.tp_ptr = req.tp_ptr;
-            a.tp_col = req.tp_col;
-        }
     }
-    k_round_b<<<blocks_for(n, 4), 256, 0, s>>>(a);
+    if (words) k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
+    else k_round_b<false><<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipGetLastError());
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     CNN_HIP(hipEventRecord(ctx->ev[1], s));
@@ -741,7 +1001,28 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     if (!req.sort) local.n_clusters = local.n_components;
     local.ms_similarity = t1;
     local.ms_labels = t2;
+    local.sim_form = words ? (tile_words ? CNN_SIM_FORM_TILE_WORDS : CNN_SIM_FORM_ROW_WORDS) : CNN_SIM_FORM_MERGE;
     if (st) *st = local;
+}
+
+LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_mode) {
+    if (g->n > 0 && !g->col && (sim_mode == CNN_SIM_MERGE || !g->mask)) graph_lists(ctx, g);
+    LabelRequest r{};
+    r.row_ptr = g->row_ptr;
+    r.col = g->col;
+    r.n = g->n;
+    r.perm = g->perm;
+    r.sim_mode = sim_mode;
+    r.current_start = 1;
+    r.member_cutoff = 2;
+    r.max_clusters = -1;
+    r.tp_ptr = g->tp_ptr;
+    r.tp_col = g->tp_col;
+    r.n_tiles = g->n_tiles;
+    r.selfpos = g->selfpos;
+    r.mask = g->mask;
+    r.pair_nz = g->pair_nz;
+    return r;
 }
 
 // ------------------------------------------------------------------ caller CSR validation
@@ -904,8 +1185,14 @@ int cnn_graph_labels(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_
     return guarded(ctx, [&]() {
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
-        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, consider, current_start, sim_mode,
-                       false, false, 2, -1, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
+        if (sim_mode != CNN_SIM_AUTO && sim_mode != CNN_SIM_MERGE && sim_mode != CNN_SIM_BITSET)
+            throw InvalidArg{"unknown sim_mode"};
+        LabelRequest r = graph_label_request(ctx, g, sim_mode);
+        r.c_sim = c_sim;
+        r.m_core = m_core;
+        r.consider = consider;
+        r.current_start = current_start;
+        r.labels = labels;
         labels_run(ctx, r, stats_out);
         return (int)CNN_OK;
     });
@@ -930,8 +1217,14 @@ int cnn_graph_fit(cnn_ctx *ctx, const cnn_graph *g, int32_t c_sim, int32_t m_cor
     return guarded(ctx, [&]() {
         if (g->n > 0 && !labels) throw InvalidArg{"null labels"};
         CNN_HIP(hipSetDevice(ctx->device));
-        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
+        LabelRequest r = graph_label_request(ctx, g, CNN_SIM_AUTO);
+        r.c_sim = c_sim;
+        r.m_core = m_core;
+        r.sort = sort != 0;
+        r.write_all = true;
+        r.member_cutoff = member_cutoff;
+        r.max_clusters = max_clusters;
+        r.labels = labels;
         labels_run(ctx, r, lstats);
         return (int)CNN_OK;
     });
@@ -948,8 +1241,14 @@ int cnn_fit(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtype, do
         cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2, precision, nullptr, graph_out == nullptr);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
         if (gstats) *gstats = g->stats;
-        LabelRequest r{g->row_ptr, g->col, g->n, g->perm, c_sim, m_core, nullptr, 1, CNN_SIM_AUTO,
-                       sort != 0, true, member_cutoff, max_clusters, labels, g->tp_ptr, g->tp_col, g->n_tiles, g->selfpos, g->mask, g->pair_nz};
+        LabelRequest r = graph_label_request(ctx, g, CNN_SIM_AUTO);
+        r.c_sim = c_sim;
+        r.m_core = m_core;
+        r.sort = sort != 0;
+        r.write_all = true;
+        r.member_cutoff = member_cutoff;
+        r.max_clusters = max_clusters;
+        r.labels = labels;
         labels_run(ctx, r, lstats);
         if (graph_out) *graph_out = hold.release();
         return (int)CNN_OK;

@@ -159,6 +159,7 @@ int cnn_predict(cnn_ctx *ctx, const void *X, int64_t n_x, const void *Y, int64_t
         CNN_HIP(hipMemcpyAsync(Z.p + (size_t)n_x * d * es, Y, (size_t)n_y * d * es, hipMemcpyDeviceToDevice, s));
         cnn_graph *g = graph_build(ctx, Z.p, n, d, dtype, r2, precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        graph_lists(ctx, g);
         const unsigned wblocks = (unsigned)ceil_div(n, 4);
         Scratch<int32_t> cnt(ctx, n);
         Scratch<int64_t> rp(ctx, n + 1);

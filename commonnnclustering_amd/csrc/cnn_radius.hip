@@ -2127,8 +2127,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
-    // every row with a finite point holds itself; rows without one (degree 0) are never core
-    g->selfpos = dev_alloc<int32_t>(g, std::max<int64_t>(n, 1));
+    // A single-batch spatial graph whose words take no more memory than its
+    // lists would (512 bytes per candidate tile pair against 4 per entry) keeps
+    // only the words: similarities are tested on them and the lists are
+    // emitted when a caller needs them (graph_lists).
+    bool words_only = false;
     for (auto &b : batches) {
         DistArgs a{};
         a.xw64 = g->xw64;
@@ -2182,6 +2185,20 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
             CNN_HIP(hipStreamSynchronize(s));
         }
         const int64_t need = nnz_total + batch_total;
+        words_only = sym && g->layout == 1 && !pk && g->mask && (int64_t)512 * total_pairs <= (int64_t)4 * need;
+        if (words_only) {
+            CNN_HIP(hipEventRecord(ctx->ev[5], s));
+            CNN_HIP(hipEventSynchronize(ctx->ev[5]));
+            float t1 = 0, t3 = 0;
+            CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
+            CNN_HIP(hipEventElapsedTime(&t3, ctx->ev[6], ctx->ev[7]));
+            ms_dist += t1;
+            ms_kernel += t3;
+            nnz_total = need;
+            continue;
+        }
+        // every row with a finite point holds itself; rows without one (degree 0) are never core
+        if (!g->selfpos) g->selfpos = dev_alloc<int32_t>(g, std::max<int64_t>(n, 1));
         if (need > col_cap) {
             const int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
             int32_t *nc = nullptr;
@@ -2223,6 +2240,15 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         ms_emit += t2;
         ms_kernel += t3;
         nnz_total = need;
+    }
+    if (!words_only && g->mask) {
+        // the lists are the smaller form: drop the words
+        graph_mem_free(g, g->mask);
+        graph_mem_free(g, g->pair_nz);
+        g->bytes -= std::max<int64_t>(max_batch_pairs, 1) * 64 * (int64_t)sizeof(uint64_t) +
+                    std::max<int64_t>(max_batch_pairs, 1);
+        g->mask = nullptr;
+        g->pair_nz = nullptr;
     }
     CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &nnz_total, sizeof(int64_t), hipMemcpyHostToDevice, s));
     unsigned long long hb[4];
@@ -2516,9 +2542,47 @@ cnn_graph *graph_build_packed(cnn_ctx *ctx, const void *X, const int64_t *offset
     return std::exchange(guard.g, nullptr);
 }
 
+void graph_lists(cnn_ctx *ctx, const cnn_graph *gc) {
+    cnn_graph *g = const_cast<cnn_graph *>(gc);
+    std::lock_guard<std::mutex> lock(g->lists_mu);
+    if (g->col) return;
+    if (!g->mask || !g->pair_nz || !g->tp_ptr) throw InvalidArg{"graph holds neither neighbour lists nor row words"};
+    CNN_HIP(hipSetDevice(g->device));
+    hipStream_t s = ctx->stream;
+    const int64_t n = g->n, T = g->n_tiles;
+    int32_t *col = dev_alloc<int32_t>(g, std::max<int64_t>(g->nnz, 1));
+    int32_t *selfpos = nullptr;
+    try {
+        selfpos = dev_alloc<int32_t>(g, std::max<int64_t>(n, 1));
+    } catch (...) {
+        graph_mem_free(g, col);
+        throw;
+    }
+    DistArgs a{};
+    a.n = n;
+    a.d = g->d;
+    a.n_tiles = T;
+    a.tp_ptr = g->tp_ptr;
+    a.tp_col = g->tp_col;
+    a.mask = g->mask;
+    a.pair_nz = g->pair_nz;
+    CNN_HIP(hipEventRecord(ctx->ev[4], s));
+    if (T > 0) k_emit_rows<false><<<(unsigned)T, 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col, selfpos);
+    CNN_HIP(hipGetLastError());
+    CNN_HIP(hipEventRecord(ctx->ev[5], s));
+    // complete before any stream (of any context) reads them
+    CNN_HIP(hipEventSynchronize(ctx->ev[5]));
+    float ms = 0.0f;
+    CNN_HIP(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    g->stats.ms_emit += ms;
+    g->selfpos = selfpos;
+    g->col = col;
+}
+
 void graph_export(cnn_ctx *ctx, const cnn_graph *g, int64_t *row_ptr, int32_t *col) {
     hipStream_t s = ctx->stream;
     const int64_t n = g->n;
+    if (n > 0) graph_lists(ctx, g);
     if (n == 0) {
         fill_i64(ctx, row_ptr, 0, 1);
         return;
@@ -2576,6 +2640,7 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
     if (!(r2 <= src->r2)) throw InvalidArg{"re-threshold radius must not exceed the graph radius"};
     CNN_HIP(hipSetDevice(ctx->device));
+    if (src->n > 0) graph_lists(ctx, src);
     hipStream_t s = ctx->stream;
     GraphGuard guard{new cnn_graph()};
     cnn_graph *g = guard.g;

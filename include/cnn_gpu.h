@@ -70,7 +70,13 @@ enum {
                            are recomputed in FP64 in reference order */
 };
 
-/* similarity kernel selection */
+/* similarity kernel selection.  A similarity test |N(i) & N(j)| >= c_sim is
+ * either a merge of the two sorted lists or an AND + popcount of the rows'
+ * 64-bit words over the column tiles both rows reach.  CNN_SIM_AUTO takes the
+ * words when the graph keeps them (spatial graphs whose words take no more
+ * memory than their lists: at least two neighbours per word on average) or,
+ * for lists, when they average at least 8 entries per word (dense rows);
+ * otherwise the merge. */
 enum { CNN_SIM_AUTO = 0, CNN_SIM_MERGE = 1, CNN_SIM_BITSET = 2 };
 
 typedef struct cnn_ctx cnn_ctx;
@@ -103,7 +109,16 @@ typedef struct {
     int64_t rows_scanned;      /* rows whose candidate edges the full round scanned */
     float ms_similarity;       /* similarity + union-find kernels */
     float ms_labels;           /* compress, label assignment, sort_by_size */
+    int32_t sim_form;          /* how similarities were tested: CNN_SIM_FORM_* */
 } cnn_label_stats;
+
+/* cnn_label_stats.sim_form */
+enum {
+    CNN_SIM_FORM_MERGE = 1,       /* wave-wide merge of the two sorted neighbour lists */
+    CNN_SIM_FORM_TILE_WORDS = 2,  /* AND + popcount of the rows' words in the spatial graph's
+                                     tile-pair masks (no neighbour lists needed) */
+    CNN_SIM_FORM_ROW_WORDS = 3    /* AND + popcount of per-row word lists built from the lists */
+};
 
 /* ------------------------------------------------------------------ context */
 /* Create a context on HIP device `device`.  The context owns a non-blocking

@@ -191,8 +191,10 @@ def test_labels_word_walk_and_list_walk(cuda, monkeypatch, d):
     want = C.fit(X, r, c, member_cutoff=mc, use_grid=True)
     got, gs, ls = gpu_fit(X, r, c, member_cutoff=mc, precision="f32_band")
     assert gs["distance_launches"] == 1 and gs["layout"] == 1
+    assert ls["sim_form"] in (2, 3, 1)
     assert (got == want).all()
     monkeypatch.setenv("CNN_HIP_MASK_BUDGET", str(512 * 200 * 3))
     got_b, gs_b, ls_b = gpu_fit(X, r, c, member_cutoff=mc, precision="f32_band")
     assert gs_b["distance_launches"] > 1
+    assert ls_b["sim_form"] != 2  # no tile-pair words kept across batches
     assert (got_b == want).all()

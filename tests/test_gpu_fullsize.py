@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 import torch
 
-from commonnnclustering_amd import engine
+from commonnnclustering_amd import _lib, engine
 from commonnnclustering_amd.synthetic import features_64d, md_basins_3d
 from oracle import c_oracle as C
 
@@ -63,6 +63,13 @@ def _fit_and_compare(X, p, rp, col, want, precision):
     assert mism == 0
     assert ls["n_clusters"] == len(np.unique(want[want > 0]))
     try:
+        # the same graph labelled with the other similarity form: lists merged
+        # (emitted on demand when the graph kept only its words)
+        lab_m, st_m = g.labels(c + 2, c + 1, sim_mode=_lib.CNN_SIM_MERGE)
+        assert st_m["sim_form"] == _lib.CNN_SIM_FORM_MERGE
+        engine.sort_by_size(lab_m, m)
+        assert np.array_equal(lab_m.cpu().numpy(), want), "merge form differs"
+        del lab_m
         g_rp, g_col = g.to_csr()
         g_rp = g_rp.cpu().numpy()
         assert np.array_equal(g_rp, rp), "row offsets differ"
@@ -85,9 +92,29 @@ def _fit_and_compare(X, p, rp, col, want, precision):
 def test_md_basins_3d_full_size(cuda, md3d, precision):
     X, p, rp, col, want = md3d
     gs, ls = _fit_and_compare(X, p, rp, col, want, precision)
-    assert gs["layout"] == 1
+    assert gs["layout"] == 1 and gs["distance_launches"] == 1
+    # one batch keeps the tile-pair words: similarities on words, no list emission in the fit
+    assert ls["sim_form"] == _lib.CNN_SIM_FORM_TILE_WORDS
+    assert gs["ms_emit"] == 0.0
     if precision == "f32_band":
         assert gs["band_hits"] > 0  # the guard band is exercised at this size
+
+
+@pytest.mark.timeout(900)
+def test_md_basins_3d_row_tile_batches(cuda, md3d, monkeypatch):
+    """A mask budget that splits the build into row-tile batches: lists are
+    emitted batch by batch, the words dropped, and similarities tested on
+    per-row word lists built from the lists -- the same labels."""
+    X, p, rp, col, want = md3d
+    monkeypatch.setenv("CNN_HIP_MASK_BUDGET", str(200 << 20))
+    r, c, m = p["radius_cutoff"], p["cnn_cutoff"], p["member_cutoff"]
+    lab, _, gs, ls = engine.fit(torch.from_numpy(X).cuda(), r * r, c + 2, c + 1, precision="f32_band",
+                                member_cutoff=m)
+    print(f"[fullsize] md_basins_3d batches={gs['distance_launches']} sim_form={ls['sim_form']} "
+          f"mismatches={int((lab.cpu().numpy() != want).sum())}", flush=True)
+    assert gs["distance_launches"] > 1
+    assert ls["sim_form"] in (_lib.CNN_SIM_FORM_ROW_WORDS, _lib.CNN_SIM_FORM_MERGE)
+    assert np.array_equal(lab.cpu().numpy(), want)
 
 
 @pytest.mark.timeout(900)
