@@ -650,11 +650,14 @@ class SimilarityCheckerGPU(SimilarityChecker):
 
     Counts |N(i) & N(j)| with early exit at the similarity cutoff, the
     semantics of SimilarityCheckerExtSwitchContains._check / Screensorted._check
-    (_types.pyx:2580-2610, 2654-2706).
+    (_types.pyx:2580-2610, 2654-2706).  ``mode``: "merge" (wave-wide merge of
+    the sorted lists), "bitset" (AND + popcount of the rows' 64-bit words over
+    common column tiles) or "auto" (words when the graph keeps them or the
+    lists are dense enough, else the merge).
     """
 
     def __init__(self, mode: str = "auto"):
-        modes = {"auto": _lib.CNN_SIM_AUTO, "merge": _lib.CNN_SIM_MERGE}
+        modes = {"auto": _lib.CNN_SIM_AUTO, "merge": _lib.CNN_SIM_MERGE, "bitset": _lib.CNN_SIM_BITSET}
         if mode not in modes:
             raise ValueError(f"unknown similarity mode {mode!r}")
         self.mode = mode

@@ -822,8 +822,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     hipStream_t s = ctx->stream;
     const int64_t n = req.n;
     if (req.c_sim < 0 || req.m_core < 0) throw InvalidArg{"c_sim and m_core must be >= 0"};
-    if (req.sim_mode == CNN_SIM_BITSET) throw InvalidArg{"CNN_SIM_BITSET is not available; use CNN_SIM_AUTO or CNN_SIM_MERGE"};
-    if (req.sim_mode != CNN_SIM_AUTO && req.sim_mode != CNN_SIM_MERGE) throw InvalidArg{"unknown sim_mode"};
+    if (req.sim_mode != CNN_SIM_AUTO && req.sim_mode != CNN_SIM_MERGE && req.sim_mode != CNN_SIM_BITSET)
+        throw InvalidArg{"unknown sim_mode"};
     cnn_label_stats local{};
     if (n == 0) {
         if (st) *st = local;

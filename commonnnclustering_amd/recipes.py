@@ -331,6 +331,7 @@ COMPONENT_NAME_TYPE_MAP = {
         "screen": _types.SimilarityCheckerGPU,
         "gpu": _types.SimilarityCheckerGPU,
         "merge": (lambda: _types.SimilarityCheckerGPU("merge")),
+        "bitset": (lambda: _types.SimilarityCheckerGPU("bitset")),
     },
     "queue": {
         "fifo": _types.QueueUnused,
