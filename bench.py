@@ -10,14 +10,21 @@ resident in HBM when the timed region starts.  The timed K steps use the
 FP32-banded form (``value``); the plain FP64 form is timed the same way and
 reported beside it, and the two label vectors must be identical.
 
-``roofline`` describes the longer of the two largest single kernels of the
-fit, the distance kernel (VALU-bound) and the CSR emit kernel (HBM-bound);
-both are in ``kernel_rooflines``.  Durations come from HIP events the library
-records around each launch on the stream the kernel runs on, averaged over the
-timed fits.  ``cpu_baseline`` times the C
-restatement of the reference fit (brute-force FP64 neighbours, components,
-sort_by_size) on all host threads over a seeded subsample; the time is
-extrapolated to the full workload by the brute force's n^2 cost.
+``roofline`` describes the longest single kernel of the fit (the distance
+kernel, VALU- or MFMA-bound; the CSR emit kernel when a fit writes lists; the
+label pass's full round, HBM-bound); ``kernel_rooflines`` and
+``similarity_rooflines`` hold the others.  Durations come from HIP events the
+library records around each launch on the stream the kernel runs on, averaged
+over the timed fits.  ``cpu_baseline`` times the C restatement of the
+reference fit (brute-force FP64 neighbours, components, sort_by_size) on all
+host threads over a seeded subsample, extrapolated to the full workload by the
+brute force's n^2 cost, and calibrated against the reference's published
+single-thread tutorial timings.
+
+The default line (md_basins_3d) also carries ``features_64d`` (n=2e5, d=64:
+FP32-banded MFMA form against plain FP64, their ratio and the MFMA roofline)
+and ``served`` (the serving workload: capacity and p50/p99 latency at Poisson
+arrival rates of 0.5, 0.7 and 0.9 of capacity).
 
 One JSON line is printed on stdout; progress goes to stderr.
 """
@@ -47,6 +54,10 @@ TRAFFIC_FILES = {
     ("k_emit_rows", "md_basins_3d"): "profiles/k_emit_rows_md_basins_3d_traffic.json",
     ("k_dist_mfma_lds", "features_64d"): "profiles/k_dist_mfma_lds_features_64d_traffic.json",
     ("k_emit_rows", "features_64d"): "profiles/k_emit_rows_features_64d_traffic.json",
+    ("k_round_b", "md_basins_3d"): "profiles/k_round_b_md_basins_3d_traffic.json",
+    ("k_hook", "md_basins_3d"): "profiles/k_hook_md_basins_3d_traffic.json",
+    ("k_round_b", "features_64d"): "profiles/k_round_b_features_64d_traffic.json",
+    ("k_hook", "features_64d"): "profiles/k_hook_features_64d_traffic.json",
 }
 
 
@@ -146,31 +157,83 @@ def distance_roofline(gs, d, precision, workload_name):
             "pairs_per_launch": pairs, "timing": "hipEvent pair around the launch on the library stream"}
 
 
+# Published single-thread fit times of the reference (cnnclustering 0.5.0,
+# Cython, brute-force neighbours) for the six n=2000 datasets of its
+# scikit-learn tutorial (docsrc/tutorial/scikit_learn_datasets.ipynb:384-414).
+TUTORIAL_FITS = [  # dataset, radius_cutoff, cnn_cutoff, member_cutoff, published seconds
+    ("circles", 0.5, 20, 100, 0.173), ("moons", 0.5, 20, 2, 0.193), ("varied", 0.28, 20, 20, 0.254),
+    ("aniso", 0.29, 30, 5, 0.160), ("blobs", 0.4, 20, 2, 0.247), ("None", 0.5, 20, 1, 0.124),
+]
+
+
 def cpu_baseline(X, p, target_s=15.0, seed=0):
     """C restatement of the reference fit on all host threads, brute-force FP64
     neighbours as NeighboursGetterExtBruteForce (cost ~ n^2): a seeded
     subsample sized for about target_s seconds; points/s is extrapolated to the
-    full workload by the n^2 cost."""
+    full workload by the n^2 cost.
+
+    Calibration against the reference itself: the restatement is timed on one
+    thread on the six tutorial datasets whose single-thread reference fit times
+    are published (another machine), and on one thread on the first sample,
+    which gives its multi-thread speed-up; the reference's rate on this host is
+    estimated as value / speed-up / (published time / restatement time)."""
+    from commonnnclustering_amd import synthetic
     from oracle import c_oracle
 
     n = X.shape[0]
     rng = np.random.default_rng(seed)
+    cores = c_oracle.num_threads()
 
-    def run(m):
+    def sample(m):
         idx = np.sort(rng.choice(n, size=min(m, n), replace=False))
-        Xs = np.ascontiguousarray(X[idx], dtype=np.float64)
+        return np.ascontiguousarray(X[idx], dtype=np.float64)
+
+    def run(Xs):
         t0 = time.perf_counter()
         c_oracle.fit(Xs, p["radius_cutoff"], p["cnn_cutoff"], member_cutoff=p["member_cutoff"], use_grid=False)
-        return len(idx), time.perf_counter() - t0
+        return time.perf_counter() - t0
 
-    m0, t0 = run(min(8000, n))
+    X0 = sample(8000)
+    m0, t0 = len(X0), run(X0)
     m = min(n, int(m0 * np.sqrt(target_s / max(t0, 1e-3))))
-    ms, ts = run(m) if m > m0 else (m0, t0)
+    if m > m0:
+        Xm = sample(m)
+        ms, ts = len(Xm), run(Xm)
+    else:
+        ms, ts = m0, t0
     t_full = ts * (n / ms) ** 2
-    return {"value": n / t_full, "unit": "points/s", "cores": c_oracle.num_threads(), "kind": "port",
+    value = n / t_full
+    # calibration: one thread
+    tut = synthetic.sklearn_tutorial()
+    c_oracle.set_num_threads(1)
+    try:
+        t1_sample = run(X0)
+        ratios, port_s = [], []
+        for name, r, c, mc, pub in TUTORIAL_FITS:
+            Xt = np.ascontiguousarray(tut[name], dtype=np.float64)
+            t_a = time.perf_counter()
+            for _ in range(3):
+                c_oracle.fit(Xt, r, c, member_cutoff=mc, use_grid=False)
+            t_port = (time.perf_counter() - t_a) / 3
+            port_s.append(t_port)
+            ratios.append(pub / t_port)
+    finally:
+        c_oracle.set_num_threads(cores)
+    speedup = t1_sample / max(t0, 1e-9)
+    factor = float(np.median(ratios))
+    return {"value": value, "unit": "points/s", "cores": cores, "kind": "port",
             "sample": f"{ms} of {n} points (seed {seed}), brute-force FP64 neighbour lists + components + "
                       f"sort_by_size in {ts:.2f} s ({ms / ts:.0f} points/s on the sample); full-size time "
-                      f"extrapolated by (n/{ms})^2 to {t_full:.0f} s"}
+                      f"extrapolated by (n/{ms})^2 to {t_full:.0f} s",
+            "calibration": {
+                "published_reference_s": [f[4] for f in TUTORIAL_FITS],
+                "port_one_thread_s": port_s,
+                "reference_over_port": factor,
+                "port_speedup_on_cores": speedup,
+                "reference_estimate_points_per_s": value / speedup / factor,
+                "note": "six n=2000 tutorial fits (scikit_learn_datasets.ipynb:384-414): published single-thread "
+                        "reference times (another machine) over the restatement's one-thread times on this "
+                        "host; the reference estimate is value / speed-up / that factor"}}
 
 
 def serving_requests(count, seed=5000, refit_frac=0.3):
@@ -201,18 +264,55 @@ def serving_requests(count, seed=5000, refit_frac=0.3):
     return reqs
 
 
-def serving_bench(args):
-    """Closed loop (all requests queued at once) for capacity, then Poisson
-    arrivals at --rate-frac of that capacity for latency."""
+def su_rooflines(ls, workload_name):
+    """Similarity + union-find kernels.  achieved = HBM bytes per launch
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated by a copy kernel) over the
+    kernel's average duration (HIP events on the library stream).  Context: the
+    phase's canonical algorithmic bytes B_S (every candidate edge merged with
+    early exit at c + 2) + B_U, computed once by oracle/similarity_work.py; the
+    union-find tests only a small share of the candidate edges, so B over the
+    phase time can exceed the HBM peak -- that effective rate is not a frac."""
+    out = {}
+    for kernel, key in (("k_round_b", "ms_round"), ("k_hook", "ms_hook")):
+        ms = mean(ls, key)
+        traffic = None
+        path = TRAFFIC_FILES.get((kernel, workload_name))
+        if path and os.path.exists(os.path.join(ROOT, path)):
+            with open(os.path.join(ROOT, path)) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        achieved = traffic / (ms * 1e-3) / 1e9 if (traffic and ms > 0) else None
+        out[kernel] = {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                       "ms_per_launch": ms,
+                       "achieved_from": "measured HBM bytes per launch / kernel time (hipEvent pair)"}
+    phase_ms = mean(ls, "ms_similarity")
+    path = os.path.join(ROOT, "profiles", f"similarity_work_{workload_name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            w = json.load(f)
+        b = w["B_S_bytes"] + w["B_U_bytes"]
+        out["phase"] = {"ms": phase_ms, "B_S_bytes": w["B_S_bytes"], "B_U_bytes": w["B_U_bytes"],
+                        "candidate_edges": w["candidate_edges"], "tests_executed": mean(ls, "tests"),
+                        "effective_GBs": b / (phase_ms * 1e-3) / 1e9,
+                        "note": "canonical work of the whole phase over its time; the union-find skips most "
+                                "candidate edges, so this is not bounded by the HBM peak"}
+    return out
+
+
+def served_block(args, count=None, fracs=(0.5, 0.7, 0.9)):
+    """Serving workload: capacity from a closed loop (every request queued at
+    once), then Poisson arrivals at each fraction of that capacity; every rate
+    replays the same requests on a fresh server (cold graph cache)."""
     from commonnnclustering_amd.serving import Server
 
-    reqs = serving_requests(args.requests)
-    n_pts = [r.X.shape[0] if r.X is not None else None for r in reqs]
+    count = count or args.requests
+    reqs = serving_requests(count)
+    n_pts = [r.X.shape[0] for r in reqs if r.X is not None]
     log(f"serving: {len(reqs)} requests, {sum(r.X is None for r in reqs)} re-fits, "
-        f"n in [{min(x for x in n_pts if x)}, {max(x for x in n_pts if x)}]")
-    # warm-up: one small request per d compiles nothing but pages in code and pools
+        f"n in [{min(n_pts)}, {max(n_pts)}]")
+    # warm-up: small requests page in code and pools
     with Server(slots=args.slots) as srv:
-        for r in reqs[:4]:
+        for r in reqs[:6]:
             if r.X is not None:
                 srv.submit(type(r)(r.radius_cutoff, r.cnn_cutoff, X=r.X[:2000])).result()
     with Server(slots=args.slots) as srv:
@@ -223,40 +323,107 @@ def serving_bench(args):
         t_closed = time.perf_counter() - t0
         closed = srv.latency_summary()
     cap = len(reqs) / t_closed
-    rate = args.rate_frac * cap
-    log(f"closed loop: {t_closed * 1e3:.1f} ms, capacity {cap:.1f} req/s; open loop at {rate:.1f} req/s")
-    rng = np.random.default_rng(1)
-    gaps = rng.exponential(1.0 / rate, size=len(reqs))
-    with Server(slots=args.slots) as srv:
-        futs = []
-        t_next = time.perf_counter()
-        for r, g in zip(reqs, gaps):
-            t_next += g
-            dt = t_next - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            futs.append(srv.submit(r))
-        for f in futs:
-            f.result()
-        opened = srv.latency_summary()
+    log(f"closed loop: {t_closed * 1e3:.1f} ms, capacity {cap:.1f} req/s, p50 {closed['p50_ms']:.1f} ms, "
+        f"p99 {closed['p99_ms']:.1f} ms")
+    sweep = []
+    for k, frac in enumerate(fracs):
+        rate = frac * cap
+        gaps = np.random.default_rng(1 + k).exponential(1.0 / rate, size=len(reqs))
+        with Server(slots=args.slots) as srv:
+            futs = []
+            t_next = time.perf_counter()
+            for r, g in zip(reqs, gaps):
+                t_next += g
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                futs.append(srv.submit(r))
+            for f in futs:
+                f.result()
+            summ = srv.latency_summary()
+        log(f"poisson {frac:.1f} x capacity ({rate:.1f} req/s): p50 {summ['p50_ms']:.1f} ms, "
+            f"p99 {summ['p99_ms']:.1f} ms, {summ['points_per_s'] / 1e6:.1f} M points/s")
+        sweep.append(dict(summ, rate_frac=frac, rate_req_per_s=rate))
+    return {"requests_per_rate": len(reqs), "slots": args.slots, "refit_frac": 0.3,
+            "n_range": [1000, 300000], "dims": [2, 3, 16, 64], "capacity_req_per_s": cap,
+            "closed_loop": closed, "poisson": sweep}
+
+
+def serving_bench(args):
+    served = served_block(args)
+    closed = served["closed_loop"]
     return {
         "metric": METRIC,
         "value": closed["points_per_s"],
         "unit": "points/s",
         "n_gpus": 1,
-        "steps": len(reqs),
+        "steps": served["requests_per_rate"],
         "warmup": 1,
-        "ms_per_step": t_closed * 1e3 / len(reqs),
+        "ms_per_step": 1e3 / served["capacity_req_per_s"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (seeded numpy generator)",
-        "config": {"workload": "serving", "requests": len(reqs), "slots": args.slots,
+        "config": {"workload": "serving", "requests": served["requests_per_rate"], "slots": args.slots,
                    "n_range": [1000, 300000], "dims": [2, 3, 16, 64], "refit_frac": 0.3},
-        "served": {"capacity_req_per_s": cap, "closed_loop": closed, "poisson_rate_req_per_s": rate,
-                   "open_loop": opened},
+        "served": served,
     }
+
+
+def neighbour_mismatches(Xd, r2):
+    """Rows whose neighbour sets differ between the FP32-banded and the plain
+    FP64 graph (the plain form is the reference arithmetic), entries compared."""
+    import torch
+    from commonnnclustering_amd import engine
+
+    out = []
+    for prec in ("f32_band", "f64"):
+        g = engine.build_graph(Xd, r2, prec)
+        try:
+            out.append(g.to_csr())
+        finally:
+            g.release()
+    (rp_a, col_a), (rp_b, col_b) = out
+    deg_a, deg_b = rp_a[1:] - rp_a[:-1], rp_b[1:] - rp_b[:-1]
+    bad = deg_a != deg_b
+    if not bool(bad.any()):
+        diff = (col_a != col_b).nonzero().flatten()
+        if diff.numel():
+            rows = torch.bucketize(diff, rp_a[1:], right=True)
+            bad[rows] = True
+    return int(bad.sum()), int(col_b.numel())
+
+
+def features_block(args):
+    """features_64d (n=2e5, d=64, float32): FP32-banded (MFMA Gram form with the
+    FP64 guard band) against the plain FP64 form, per fit."""
+    import torch
+
+    X, p = workload("features_64d")
+    n, d = X.shape
+    Xd = torch.from_numpy(X).cuda()
+    steps = max(2, args.steps)
+    t_band, lab_band, gs_band, ls_band = run_fits(Xd, p, "f32_band", steps, max(1, args.warmup))
+    t_plain, lab_plain, gs_plain, ls_plain = run_fits(Xd, p, "f64", max(2, steps // 2), 1)
+    rows_bad, entries = neighbour_mismatches(Xd, float(p["radius_cutoff"]) ** 2)
+    log(f"features_64d: f32_band {t_band * 1e3:.2f} ms/fit, plain f64 {t_plain * 1e3:.2f} ms/fit "
+        f"({t_plain / t_band:.2f}x); neighbour rows differing {rows_bad}")
+    return {"config": {"workload": "features_64d", "n": int(n), "d": int(d), "dtype_in": str(X.dtype),
+                       "radius_cutoff": float(p["radius_cutoff"]), "cnn_cutoff": int(p["cnn_cutoff"]),
+                       "member_cutoff": int(p["member_cutoff"])},
+            "f32_band": {"points_per_s": n / t_band, "ms_per_step": t_band * 1e3, "steps": steps},
+            "plain_f64": {"points_per_s": n / t_plain, "ms_per_step": t_plain * 1e3,
+                          "distance_roofline": distance_roofline(gs_plain, d, "f64", "features_64d")},
+            "band_vs_plain": t_plain / t_band,
+            "labels_identical": bool(np.array_equal(lab_band, lab_plain)),
+            "neighbour_rows_differing_from_plain_f64": rows_bad, "neighbour_entries": entries,
+            "band_hits": int(gs_band[-1]["band_hits"]), "band_flips": int(gs_band[-1]["band_flips"]),
+            "max_fp32_abs_err_band": float(gs_band[-1]["max_abs_err_band"]),
+            "phase_ms": phases(gs_band, ls_band), "phase_ms_plain": phases(gs_plain, ls_plain),
+            "label_stats": ls_band[-1],
+            "mfma_roofline": distance_roofline(gs_band, d, "f32_band", "features_64d"),
+            "similarity_rooflines": su_rooflines(ls_band, "features_64d")}
 
 
 def _timed(fn, steps, warmup):
@@ -468,10 +635,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 fits and the Clustering.fit timing (profiling runs)")
-    ap.add_argument("--requests", type=int, default=48, help="serving workload: number of requests")
+    ap.add_argument("--requests", type=int, default=300, help="serving workload: requests per arrival rate")
+    ap.add_argument("--served-requests", type=int, default=300,
+                    help="requests per arrival rate of the served block on the default line")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload: leave out the features_64d and served blocks (profiling runs)")
     ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
-    ap.add_argument("--rate-frac", type=float, default=0.7,
-                    help="serving workload: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device-to-device copy of MIB MiB three times and exit; its rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters")
@@ -562,10 +731,24 @@ def main():
         cpu = cpu_baseline(X, p, args.cpu_seconds)
         log(f"cpu baseline: {cpu['sample']}")
 
-    # the roofline line describes the longer of the two single-kernel phases
+    # the roofline line describes the longest single kernel of the fit: the
+    # distance kernel, the CSR emit (when the fit emits lists) or the label pass's
+    # full round
     dist_rf = distance_roofline(gs_band, d, "f32_band", args.workload)
-    emit_rf = emit_roofline(gs_band, n, args.workload)
-    dominant = dist_rf if dist_rf["ms_per_launch"] >= emit_rf["ms_per_launch"] else emit_rf
+    emit_rf = emit_roofline(gs_band, n, args.workload) if mean(gs_band, "ms_emit") > 0 else None
+    su_rf = su_rooflines(ls_band, args.workload)
+    dominant = dist_rf
+    if emit_rf and emit_rf["ms_per_launch"] > dominant["ms_per_launch"]:
+        dominant = emit_rf
+    rb = su_rf["k_round_b"]
+    if rb["ms_per_launch"] > dominant["ms_per_launch"] and rb["achieved"] is not None:
+        dominant = rb
+    rows_bad, entries = neighbour_mismatches(Xd, r2)
+    log(f"neighbour rows differing between f32_band and plain f64: {rows_bad} of {n}")
+    extra = {}
+    if not args.no_extra:
+        extra["features_64d"] = features_block(args)
+        extra["served"] = served_block(args, count=args.served_requests)
 
     out = {
         "metric": METRIC,
@@ -597,10 +780,14 @@ def main():
         "phase_ms": phases(gs_band, ls_band),
         "phase_ms_plain": ph_plain,
         "label_stats": ls_band[-1],
+        "neighbour_rows_differing_from_plain_f64": rows_bad,
+        "neighbour_entries": entries,
         "roofline": dominant,
         "kernel_rooflines": {"distance": dist_rf, "emit": emit_rf},
+        "similarity_rooflines": su_rf,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

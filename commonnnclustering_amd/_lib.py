@@ -97,6 +97,8 @@ class LabelStats(ctypes.Structure):
         ("ms_similarity", ctypes.c_float),
         ("ms_labels", ctypes.c_float),
         ("sim_form", ctypes.c_int32),
+        ("ms_hook", ctypes.c_float),
+        ("ms_round", ctypes.c_float),
     ]
 
     def to_dict(self) -> dict:

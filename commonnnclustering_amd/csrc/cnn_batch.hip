@@ -56,6 +56,8 @@ static void add_label_stats(cnn_label_stats &ls, const cnn_label_stats &o) {
     ls.ms_similarity += o.ms_similarity;
     ls.ms_labels += o.ms_labels;
     ls.sim_form = std::max(ls.sim_form, o.sim_form);
+    ls.ms_hook += o.ms_hook;
+    ls.ms_round += o.ms_round;
 }
 
 // All datasets in one working order (1 <= d <= 8): one graph build, one

@@ -63,7 +63,7 @@ struct cnn_ctx {
     hipStream_t stream = nullptr;
     cnn::Arena *arena = nullptr;
     std::string last_error;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
     int cu_count = 256;
     // contexts (own stream and arena each) that fit independent datasets of a
     // batch concurrently; created on first use, destroyed with this context

@@ -885,9 +885,11 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
 
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
     CNN_HIP(hipGetLastError());
+    CNN_HIP(hipEventRecord(ctx->ev[8], s));
     if (words) k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     else k_hook<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     CNN_HIP(hipGetLastError());
+    CNN_HIP(hipEventRecord(ctx->ev[9], s));
     // flatten (every parent becomes its root), then let roots re-point to smaller
     // neighbouring roots twice, flattening after each pass
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
@@ -918,8 +920,10 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.cand_max = cand_max->p;
         a.tile_rank = tile_common->p;
     }
+    CNN_HIP(hipEventRecord(ctx->ev[10], s));
     if (words) k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
     else k_round_b<false><<<blocks_for(n, 4), 256, 0, s>>>(a);
+    CNN_HIP(hipEventRecord(ctx->ev[11], s));
     CNN_HIP(hipGetLastError());
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     CNN_HIP(hipEventRecord(ctx->ev[1], s));
@@ -1001,6 +1005,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     if (!req.sort) local.n_clusters = local.n_components;
     local.ms_similarity = t1;
     local.ms_labels = t2;
+    CNN_HIP(hipEventElapsedTime(&local.ms_hook, ctx->ev[8], ctx->ev[9]));
+    CNN_HIP(hipEventElapsedTime(&local.ms_round, ctx->ev[10], ctx->ev[11]));
     local.sim_form = words ? (tile_words ? CNN_SIM_FORM_TILE_WORDS : CNN_SIM_FORM_ROW_WORDS) : CNN_SIM_FORM_MERGE;
     if (st) *st = local;
 }

@@ -17,6 +17,8 @@
                     percentile of 1e5 pair distances, c=5
   serving_request   one request of the serving workload (n log-uniform in
                     [1e3, 3e5], d in {2,3,16,64}, 2-30 blobs + 5% noise)
+  sklearn_tutorial  the six n=2000 datasets of the reference's scikit-learn
+                    tutorial (scikit_learn_datasets.ipynb:205-322)
 """
 
 from __future__ import annotations
@@ -116,3 +118,29 @@ def serving_request(seed: int, n_range=(1_000, 300_000), dims=(2, 3, 16, 64)):
     r = pair_distance_percentile(X, q, rng=rng)
     c = int(rng.integers(2, 51))
     return X, {"radius_cutoff": r, "cnn_cutoff": c, "member_cutoff": 2}
+
+
+def sklearn_tutorial():
+    """The six n=2000 datasets of scikit_learn_datasets.ipynb (cells 6-8, :205-322),
+    regenerated with the same global seed and generator calls, by name."""
+    from sklearn import datasets
+    from sklearn.preprocessing import StandardScaler
+
+    state = np.random.get_state()
+    try:
+        np.random.seed(0)
+        n = 2000
+        circles, _ = datasets.make_circles(n_samples=n, factor=.5, noise=.05)
+        circles = StandardScaler().fit_transform(circles)
+        moons, _ = datasets.make_moons(n_samples=n, noise=.05)
+        moons = StandardScaler().fit_transform(moons)
+        blobs, _ = datasets.make_blobs(n_samples=n, random_state=8)
+        blobs = StandardScaler().fit_transform(blobs)
+        none = StandardScaler().fit_transform(np.random.rand(n, 2))
+        X, _ = datasets.make_blobs(n_samples=n, random_state=170)
+        aniso = StandardScaler().fit_transform(np.dot(X, [[0.6, -0.6], [-0.4, 0.8]]))
+        varied, _ = datasets.make_blobs(n_samples=n, cluster_std=[1.0, 2.5, 0.5], random_state=170)
+        varied = StandardScaler().fit_transform(varied)
+    finally:
+        np.random.set_state(state)
+    return {"circles": circles, "moons": moons, "varied": varied, "aniso": aniso, "blobs": blobs, "None": none}

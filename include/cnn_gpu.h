@@ -110,6 +110,8 @@ typedef struct {
     float ms_similarity;       /* similarity + union-find kernels */
     float ms_labels;           /* compress, label assignment, sort_by_size */
     int32_t sim_form;          /* how similarities were tested: CNN_SIM_FORM_* */
+    float ms_hook;             /* the hooking kernel (k_hook) alone */
+    float ms_round;            /* the full-round kernel (k_round_b) alone */
 } cnn_label_stats;
 
 /* cnn_label_stats.sim_form */

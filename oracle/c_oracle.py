@@ -42,6 +42,7 @@ def lib():
         L.oracle_sort_by_size.argtypes = [_i64p, _i64, _i64, _i64, ctypes.POINTER(_i64)]
         L.oracle_check_similarity.argtypes = [_i32p, _i64, _i32p, _i64, _i64]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_num_threads.argtypes = [ctypes.c_int]
         L.oracle_neighbours_bf.argtypes = [_f64p, _i64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, _i64,
                                            _i64p, ctypes.POINTER(ctypes.c_void_p)]
         L.oracle_free.argtypes = [ctypes.c_void_p]
@@ -51,6 +52,11 @@ def lib():
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+def set_num_threads(k: int) -> None:
+    """Threads of the OpenMP loops from now on (all host threads by default)."""
+    lib().oracle_set_num_threads(int(k))
 
 
 def neighbours_csr(X, radius_cutoff: float, use_grid: bool = True):

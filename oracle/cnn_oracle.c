@@ -567,3 +567,5 @@ int oracle_sort_by_size(i64 *labels, i64 n, i64 member_cutoff, i64 max_clusters,
 }
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+void oracle_set_num_threads(int k) { if (k > 0) omp_set_num_threads(k); }

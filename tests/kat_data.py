@@ -16,29 +16,9 @@ def kats():
 def sklearn_tutorial_datasets():
     """The six n=2000 datasets of scikit_learn_datasets.ipynb (cells 6-8, :205-322),
     regenerated with the same global seed and generator calls, in record order."""
-    from sklearn import datasets
-    from sklearn.preprocessing import StandardScaler
+    from commonnnclustering_amd.synthetic import sklearn_tutorial
 
-    state = np.random.get_state()
-    try:
-        np.random.seed(0)
-        n = 2000
-        circles, _ = datasets.make_circles(n_samples=n, factor=.5, noise=.05)
-        circles = StandardScaler().fit_transform(circles)
-        moons, _ = datasets.make_moons(n_samples=n, noise=.05)
-        moons = StandardScaler().fit_transform(moons)
-        blobs, _ = datasets.make_blobs(n_samples=n, random_state=8)
-        blobs = StandardScaler().fit_transform(blobs)
-        none = StandardScaler().fit_transform(np.random.rand(n, 2))
-        X, _ = datasets.make_blobs(n_samples=n, random_state=170)
-        aniso = StandardScaler().fit_transform(np.dot(X, [[0.6, -0.6], [-0.4, 0.8]]))
-        varied, _ = datasets.make_blobs(n_samples=n, cluster_std=[1.0, 2.5, 0.5], random_state=170)
-        varied = StandardScaler().fit_transform(varied)
-    finally:
-        np.random.set_state(state)
-    by_name = {"circles": circles, "moons": moons, "varied": varied, "aniso": aniso, "blobs": blobs,
-               "None": none}
-    return by_name
+    return sklearn_tutorial()
 
 
 def equalise_labels(labels):
