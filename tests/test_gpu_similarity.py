@@ -120,19 +120,21 @@ def test_fit_keeps_words_and_skips_emission(cuda):
     """A dense spatial fit tests similarities on the tile-pair words and writes
     no lists; a kept graph emits them when exported or re-thresholded."""
     X = _data(21, 60000, 3, k=6, spread=3.0, scale=0.5)
-    r, c, m = 0.12, 10, 20
+    r, c, m = 0.3, 10, 20
     want = C.fit(X, r, c, member_cutoff=m)
     lab, g, gs, ls = engine.fit(to_dev(X), r * r, c + 2, c + 1, precision="f32_band", member_cutoff=m,
                                 keep_graph=True)
     try:
         assert (lab.cpu().numpy() == want).all()
+        # words are kept when they take no more memory than the lists would
+        assert 512 * gs["tile_pairs"] <= 4 * gs["nnz"], (gs["tile_pairs"], gs["nnz"])
         assert ls["sim_form"] == F_TILE
         assert gs["ms_emit"] == 0.0
         rp, col = C.neighbours_csr(X, r)
         g_rp, g_col = g.to_csr()
         assert (g_rp.cpu().numpy() == rp).all() and (g_col.cpu().numpy() == col).all()
         assert g.stats["ms_emit"] > 0.0
-        r2 = 0.09
+        r2 = 0.2
         g2 = g.rethreshold(r2 * r2)
         try:
             lab2, _ = g2.fit(c + 2, c + 1, member_cutoff=m)
