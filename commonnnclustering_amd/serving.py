@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import collections
 import concurrent.futures as cf
+import dataclasses
 import itertools
 import threading
 import time
@@ -129,6 +130,10 @@ class Server:
         self._closed = False
         self._cache: "collections.OrderedDict[str, _Resident]" = collections.OrderedDict()
         self._producing: dict = {}  # dataset_id -> fresh request that will make it resident
+        # dataset_id -> the caller's coordinates of its latest fresh request (a
+        # reference, no copy): a re-fit of a dataset whose graph was evicted runs
+        # as a fresh fit from them
+        self._sources: dict = {}
         self._ids = itertools.count()
         self.results: list[FitResult] = []
         self._workers = [threading.Thread(target=self._worker, name=f"cnn-slot-{k}", daemon=True)
@@ -148,8 +153,14 @@ class Server:
                 res = self._cache.get(req.dataset_id)
                 prod = self._producing.get(req.dataset_id)
                 if res is None and prod is None:
-                    raise KeyError(f"no resident dataset {req.dataset_id!r}; send X with the first request")
-                if res is not None and r2 <= res.r2:
+                    src = self._sources.get(req.dataset_id)
+                    if src is None:
+                        raise KeyError(f"no resident dataset {req.dataset_id!r}; send X with the first request")
+                    # evicted: a fresh fit from the dataset's coordinates makes it resident again
+                    req = dataclasses.replace(req, X=src)
+                    self._producing[req.dataset_id] = req
+                    est = _estimate_bytes(src.shape[0], src.shape[1], r2, src, None)
+                elif res is not None and r2 <= res.r2:
                     est = _estimate_bytes(res.graph.n, res.graph.d, r2, None, res)
                 else:
                     Xr = res.X if res is not None else prod.X
@@ -159,6 +170,7 @@ class Server:
                 est = _estimate_bytes(n, d, r2, req.X, None)
                 if req.dataset_id is not None:
                     self._producing[req.dataset_id] = req
+                    self._sources[req.dataset_id] = req.X
             p = _Pending(next(self._ids), req, est, fut, time.perf_counter())
             self._pending.append(p)
             self._cv.notify_all()
@@ -275,31 +287,43 @@ class Server:
                 res = self._cache.get(req.dataset_id)
                 if res is not None:
                     self._cache.move_to_end(req.dataset_id)
-            if res is None:
+                src = self._sources.get(req.dataset_id)
+            out = None if res is None else self._refit(res, req, r2, c)
+            if out is not None:
+                return out
+            # evicted since it was queued: a fresh fit from the dataset's coordinates
+            if src is None:
                 raise KeyError(f"resident dataset {req.dataset_id!r} was evicted; send X again")
-            with res.lock:
-                if res.graph._handle is None:
-                    raise KeyError(f"resident dataset {req.dataset_id!r} was evicted; send X again")
-                if r2 <= res.r2 and req.precision == res.precision:
-                    g = res.graph if r2 == res.r2 else res.graph.rethreshold(r2)
-                    try:
-                        lab, _ = g.fit(c + 2, c + 1, sort=True, member_cutoff=req.member_cutoff,
-                                       max_clusters=req.max_clusters)
-                        out = lab.cpu().numpy()
-                    finally:
-                        if g is not res.graph:
-                            g.release()
-                    return out, True, res.graph.n
-                # larger radius: a new graph replaces the resident one
-                lab, g, _, _ = engine.fit(res.X, r2, c + 2, c + 1, precision=req.precision,
-                                          member_cutoff=req.member_cutoff, max_clusters=req.max_clusters,
-                                          keep_graph=True)
-                out = lab.cpu().numpy()
-                old = res.graph
-                res.graph, res.r2, res.precision = g, r2, req.precision
-                old.release()
-                res.nbytes = g.nbytes + res.X.numel() * res.X.element_size()
-                return out, False, g.n
+            req = dataclasses.replace(req, X=src)
+        return self._fresh(req, r2, c)
+
+    def _refit(self, res: _Resident, req: FitRequest, r2: float, c: int):
+        """Re-fit of a resident dataset; None when its graph was released meanwhile."""
+        with res.lock:
+            if res.graph._handle is None:
+                return None
+            if r2 <= res.r2 and req.precision == res.precision:
+                g = res.graph if r2 == res.r2 else res.graph.rethreshold(r2)
+                try:
+                    lab, _ = g.fit(c + 2, c + 1, sort=True, member_cutoff=req.member_cutoff,
+                                   max_clusters=req.max_clusters)
+                    out = lab.cpu().numpy()
+                finally:
+                    if g is not res.graph:
+                        g.release()
+                return out, True, res.graph.n
+            # larger radius: a new graph replaces the resident one
+            lab, g, _, _ = engine.fit(res.X, r2, c + 2, c + 1, precision=req.precision,
+                                      member_cutoff=req.member_cutoff, max_clusters=req.max_clusters,
+                                      keep_graph=True)
+            out = lab.cpu().numpy()
+            old = res.graph
+            res.graph, res.r2, res.precision = g, r2, req.precision
+            old.release()
+            res.nbytes = g.nbytes + res.X.numel() * res.X.element_size()
+            return out, False, g.n
+
+    def _fresh(self, req: FitRequest, r2: float, c: int):
         X = req.X
         if not isinstance(X, torch.Tensor):
             X = torch.from_numpy(np.ascontiguousarray(X))

@@ -76,3 +76,20 @@ def test_unknown_dataset_is_rejected(cuda):
     with Server(slots=1) as srv:
         with pytest.raises(KeyError):
             srv.submit(FitRequest(0.5, 3, dataset_id="missing"))
+
+
+def test_evicted_dataset_refits_from_its_coordinates(cuda):
+    """A cache too small to keep any graph: every re-fit finds its dataset
+    evicted and runs as a fresh fit from the coordinates of the dataset's first
+    request -- the same labels as a fit alone."""
+    reqs, data = _requests()
+    with Server(slots=2, cache_budget=1) as srv:
+        first = [srv.submit(r) for r in reqs[:6]]
+        for f in first:
+            f.result(timeout=120)
+        later = [srv.submit(r) for r in reqs[6:]]
+        got = [f.result(timeout=120) for f in later]
+    for k, g in enumerate(got):
+        r = reqs[6 + k]
+        assert (g.labels == _alone(r, data)).all(), r.dataset_id
+        assert not g.graph_reused
