@@ -97,10 +97,9 @@ struct cnn_graph {
     int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout only)
     int32_t *tp_col = nullptr;
     int32_t *selfpos = nullptr;  // per working row: position of the row's own index in its list
-    // single-batch spatial builds: per candidate tile pair k of row tile R (the
-    // pairs kb = tp_ptr[R] .. tp_ptr[R+1], len of them), mask[64 kb + r len + (k - kb)]
-    // holds row 64R+r's bits over the 64 points of column tile tp_col[k], valid
-    // where pair_nz[k] != 0 (row-major per row tile)
+    // single-batch spatial builds: per candidate tile pair k (rows of tile R = the
+    // pairs tp_ptr[R] .. tp_ptr[R+1]), mask[64k + r] holds row 64R+r's bits over
+    // the 64 points of column tile tp_col[k], valid where pair_nz[k] != 0
     uint64_t *mask = nullptr;
     uint8_t *pair_nz = nullptr;
     int64_t n_tile_pairs = 0;
@@ -454,20 +453,21 @@ __device__ inline int wave_common(const int32_t *__restrict__ A, int64_t na, con
 
 // A row's neighbour set as a list of 64-bit words over column tiles: word t
 // holds the row's neighbours among points 64*tile[t] .. 64*tile[t] + 63 (bit b
-// = point 64*tile[t] + b), tiles strictly ascending, words contiguous (a
-// spatial graph's tile-pair masks are row-major per row tile, per-row word
-// lists are per row).  nz (nullable) flags the words that were stored -- an
-// unflagged word is empty.
+// = point 64*tile[t] + b), tiles strictly ascending.  Words are strided
+// (stride 64 for the per-tile-pair masks of a spatial graph, 1 for per-row
+// lists); nz (nullable) flags the words that were stored -- an unflagged word
+// is empty.
 struct WordRow {
     const int32_t *tile;
     const uint64_t *word;
     const uint8_t *nz;
     int64_t len;
+    int stride;
 };
 
 __device__ __forceinline__ uint64_t word_at(const WordRow &r, int64_t t) {
     if (r.nz && r.nz[t] == 0) return 0ull;
-    return r.word[t];
+    return r.word[t * (int64_t)r.stride];
 }
 
 // |A & B| >= c from two rows' word lists, by the whole wave: the words of
@@ -485,7 +485,8 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
         for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
             const int64_t t = t0 + lane;
             int cnt = 0;
-            if (t < A.len && (!A.nz || A.nz[t] != 0)) cnt = __popcll(A.word[t] & B.word[t]);
+            if (t < A.len && (!A.nz || A.nz[t] != 0))
+                cnt = __popcll(A.word[t * (int64_t)A.stride] & B.word[t * (int64_t)B.stride]);
             count += wave_sum(cnt);
             if (count >= c) return true;
         }

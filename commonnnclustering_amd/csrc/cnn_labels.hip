@@ -89,23 +89,24 @@ struct LabelArgs {
 };
 
 // Row w's words: its row of the candidate tile pairs of its row tile (spatial
-// graph masks, row-major within the row tile), or its own word list.
+// graph masks), or its own word list.
 __device__ __forceinline__ WordRow word_row(const LabelArgs &a, int64_t w) {
     WordRow r;
     if (a.mask) {
-        // the row tile's words are row-major: 64 rows x len words
         const int64_t R = w >> 6;
         const int64_t kb = a.tp_ptr[R];
         r.tile = a.tp_col + kb;
-        r.len = a.tp_ptr[R + 1] - kb;
-        r.word = a.mask + kb * 64 + (w & 63) * r.len;
+        r.word = a.mask + kb * 64 + (w & 63);
         r.nz = a.pair_nz + kb;
+        r.len = a.tp_ptr[R + 1] - kb;
+        r.stride = 64;
     } else {
         const int64_t b = a.wptr[w];
         r.tile = a.wtile + b;
         r.word = a.wword + b;
         r.nz = nullptr;
         r.len = a.wptr[w + 1] - b;
+        r.stride = 1;
     }
     return r;
 }

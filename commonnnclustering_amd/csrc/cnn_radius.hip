@@ -9,9 +9,8 @@
 //   xw32[n_pad][d]   FP32 coordinates (CNN_F32_BAND only)
 //   tp_ptr[R], tp_col candidate column tiles of row tile R (box-culled; implicit
 //                     "all tiles" for the dense layout)
-//   mask             one 64-bit word per (tile pair k, row): bit b set when
-//                     column C*64+b is a neighbour of the row; row-major per
-//                     row tile for explicit candidate lists (word_index)
+//   mask[k][64]      one 64-bit word per (tile pair k, row): bit b set when
+//                     column C*64+b is a neighbour of the row
 //   row_ptr[n+1] (int64), col[nnz] (int32): CSR in working order, rows ascending
 //
 // The distance decision is exact: plain FP64 kernels evaluate
@@ -370,19 +369,6 @@ __device__ __forceinline__ int64_t tile_col(const DistArgs &a, int64_t k, int64_
     return a.tp_ptr ? (int64_t)a.tp_col[k] : (k - kb);
 }
 
-// Position of row `row`'s word of tile pair k (a pair of row tile R) in the
-// mask buffer.  Explicit candidate lists (spatial layout): row-major per row
-// tile -- the row tile's pairs kb .. ke hold 64 rows x (ke - kb) words, each
-// row's words contiguous in candidate order, so a row's words read as one
-// contiguous run.  Dense layout: 64 row words per pair.
-__device__ __forceinline__ int64_t word_index(const DistArgs &a, int64_t k, int64_t R, int row) {
-    if (a.tp_ptr) {
-        const int64_t kb = a.tp_ptr[R];
-        return (kb - a.k_base) * 64 + (int64_t)row * (a.tp_ptr[R + 1] - kb) + (k - kb);
-    }
-    return (k - a.k_base) * 64 + row;
-}
-
 __device__ __forceinline__ uint64_t valid_cols(int64_t n, int64_t C) {
     int64_t v = n - C * 64;
     if (v >= 64) return ~0ull;
@@ -472,7 +458,7 @@ __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t 
         if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = any ? 1 : 0;
     }
     if (a.pair_nz && !any && !force) return 0;
-    a.mask[word_index(a, k, R, lane)] = m;
+    a.mask[(k - a.k_base) * 64 + lane] = m;
     if (a.sym && C != R) {
         if (!have_col) colword = transpose_bits(m);
         else {
@@ -480,7 +466,7 @@ __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t 
             if (!row_real(a, C * 64 + lane)) colword = 0;  // padding columns of C
         }
         const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
-        a.mask[word_index(a, km, C, lane)] = colword;
+        a.mask[(km - a.k_base) * 64 + lane] = colword;
         const int cd = __popcll(colword);
         if (cd) atomicAdd(&a.deg[C * 64 + lane], cd);  // column side of the symmetric pair
     }
@@ -1073,12 +1059,12 @@ __device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hit
     const bool any_new = __ballot(word != 0ull) != 0ull;
     if (a.pair_nz && lane == 0 && any_new) mark_pair_nz(a, k, R, C);
     if (word) {
-        atomicOr(&a.mask[word_index(a, k, R, lane)], (unsigned long long)word);
+        atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
         atomicAdd(&a.deg[R * 64 + lane], __popcll(word));
     }
     if (a.sym && C != R && colword) {
         const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
-        atomicOr(&a.mask[word_index(a, km, C, lane)], (unsigned long long)colword);
+        atomicOr(&a.mask[(km - a.k_base) * 64 + lane], (unsigned long long)colword);
         atomicAdd(&a.deg[C * 64 + lane], __popcll(colword));
     }
 }
@@ -1392,12 +1378,12 @@ __global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_red
     const bool any_new = __ballot(word != 0ull) != 0ull;
     if (a.pair_nz && lane == 0 && any_new) mark_pair_nz(a, k, R, C);
     if (word) {
-        atomicOr(&a.mask[word_index(a, k, R, lane)], (unsigned long long)word);
+        atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
         atomicAdd(&a.deg[i], __popcll(word));
     }
     if (a.sym && C != R && colword) {
         const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
-        atomicOr(&a.mask[word_index(a, km, C, lane)], (unsigned long long)colword);
+        atomicOr(&a.mask[(km - a.k_base) * 64 + lane], (unsigned long long)colword);
         atomicAdd(&a.deg[C * 64 + lane], __popcll(colword));
     }
     flush_band(a, hits, flips, maxerr);
@@ -1635,10 +1621,8 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
 #pragma unroll 4
         for (int st = 0; st < 64 / kEmitWaves; ++st) {
             const int idx = (int)threadIdx.x + 64 * kEmitWaves * st;
-            // explicit lists: lanes over pairs (a row's words are contiguous);
-            // dense layout: lanes over rows (a pair's words are contiguous)
-            const int p = a.tp_ptr ? (idx & 63) : (idx >> 6), r = a.tp_ptr ? (idx >> 6) : (idx & 63);
-            sm[p * 65 + r] = p < nk ? a.mask[word_index(a, s_k[p], R, r)] : 0ull;
+            const int p = idx >> 6, r = idx & 63;
+            sm[p * 65 + r] = p < nk ? a.mask[(s_k[p] - a.k_base) * 64 + r] : 0ull;
         }
         __syncthreads();
         const int cbase = lane < nk ? (int)tile_col(a, s_k[lane], kb) * 64 : 0;
