@@ -186,6 +186,10 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     if (w >= a.n || !a.core[w]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     if constexpr (WORDS) {
+        // rows hooked by k_hook_tile are done; the test against w - 1 in the
+        // same tile was made there
+        if (a.mask && a.parent[w] != (int)w) return;
+        const bool tested_prev = a.mask && (w & 63) != 0;
         // the row's words from its own tile down, 64 tiles a step (lane l holds
         // the l-th tile below the window's top), bits from the highest down:
         // smaller indices, nearest first
@@ -224,6 +228,7 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
                     const int hb = 63 - __clzll((long long)bits);
                     bits &= ~lane_bit_mask(hb);
                     const int jj = Cs * 64 + hb;
+                    if (tested_prev && jj == (int)w - 1) continue;
                     ++attempts;
                     ++tests;
                     const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
@@ -278,6 +283,45 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
         }
         if (done) break;
     }
+    add_stats(a, tests, passed);
+}
+
+// Hooking of adjacent rows on a spatial graph's tile-pair words: one wave per
+// row tile, lane r = row w = 64R + r.  Rows of one tile share the tile's
+// candidate list, and pair k's 64 row words are contiguous (lane r reads row
+// r's word), so one pass over the tile's non-empty pairs counts
+// |N(w) & N(w - 1)| for all 63 adjacent row pairs at once (w - 1's word comes
+// from the lane below).  A core row whose predecessor is a core neighbour with
+// at least c_sim common neighbours hooks to it.  The pass stops once every
+// row still in question has reached c_sim.  Writes only parent[w] of hooked
+// rows, to w - 1: a forest, as the per-row hooking.
+__global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    const int64_t w = R * 64 + lane;
+    const bool core_w = w < a.n && a.core[w] != 0;
+    const bool core_p = __shfl_up((int)core_w, 1, 64) != 0;
+    const int c = csim_of(a, R * 64);
+    // rows whose test is still open: core, predecessor core within the tile
+    bool open = lane > 0 && core_w && core_p;
+    if (__ballot(open) == 0ull) return;
+    const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
+    int cnt = 0;
+    bool nbr = false;  // w - 1 in N(w): bit lane-1 of the row's word in its own tile
+    for (int64_t k = kb; k < ke; ++k) {
+        const int C = a.tp_col[k];
+        if (a.pair_nz[k] == 0) continue;
+        const uint64_t v = a.mask[k * 64 + lane];
+        const uint64_t pv = __shfl_up(v, 1, 64);
+        cnt += __popcll(v & pv);
+        if (C == (int)R) nbr = lane > 0 && ((v >> (lane - 1)) & 1ull) != 0ull;
+        // past the row tile's own tile every row knows whether w - 1 is a neighbour
+        if (((k - kb) & 7) == 7 && C >= (int)R && __ballot(open && nbr && cnt < c) == 0ull) break;
+    }
+    if (open && nbr && cnt >= c) a.parent[w] = (int)(w - 1);
+    const unsigned tests = (unsigned)__popcll(__ballot(open && nbr));
+    const unsigned passed = (unsigned)__popcll(__ballot(open && nbr && cnt >= c));
     add_stats(a, tests, passed);
 }
 
@@ -886,6 +930,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
     CNN_HIP(hipGetLastError());
     CNN_HIP(hipEventRecord(ctx->ev[8], s));
+    if (words && a.mask) k_hook_tile<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(a, req.n_tiles);
     if (words) k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     else k_hook<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     CNN_HIP(hipGetLastError());
