@@ -179,6 +179,67 @@ __device__ __forceinline__ uint64_t lane_bit_mask(int lane) { return 1ull << lan
 // Every row writes only its own parent entry and parents have smaller
 // indices, so the hooks form a forest whose trees are the components of the
 // passing edges tested here -- no atomics, no contention.
+// Hooking of one core row on row words: its smaller-index neighbours in
+// descending order, at most max_tests tests; skip_prev: the test against w - 1
+// was already made (k_hook_tile).
+__device__ __forceinline__ void hook_row_words(const LabelArgs &a, int64_t w, int max_tests, bool skip_prev) {
+    const int lane = threadIdx.x & 63;
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    const bool tested_prev = skip_prev;
+    // the row's words from its own tile down, 64 tiles a step (lane l holds
+    // the l-th tile below the window's top), bits from the highest down:
+    // smaller indices, nearest first
+    const WordRow A = word_row(a, w);
+    const int Rw = (int)(w >> 6);
+    int64_t top = 0;  // tiles up to and including the row's own tile
+    for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const uint64_t le = __ballot(t < A.len && A.tile[t] <= Rw);
+        top += __popcll(le);
+        if (__popcll(le) < 64) break;
+    }
+    const int c = csim_of(a, w);
+    unsigned tests = 0, passed = 0;
+    int attempts = 0;
+    bool done = false;
+    for (int64_t t1 = top; t1 > 0 && attempts < max_tests && !done; t1 -= 64) {
+        const int64_t t = t1 - 1 - lane;
+        uint64_t v = 0ull;
+        int C = 0;
+        if (t >= 0) {
+            C = A.tile[t];
+            v = word_at(A, t);
+            if (C == Rw) v &= lane_bit_mask((int)(w & 63)) - 1ull;  // smaller indices only
+        }
+        uint64_t tiles = __ballot(v != 0ull);
+        while (tiles && attempts < max_tests && !done) {
+            const int src = __ffsll((unsigned long long)tiles) - 1;
+            tiles &= tiles - 1;
+            const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+            const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+            const int Cs = __builtin_amdgcn_readlane(C, src);
+            const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+            uint64_t bits = __ballot(bit && a.core[Cs * 64 + lane]);
+            while (bits && attempts < max_tests) {
+                const int hb = 63 - __clzll((long long)bits);
+                bits &= ~lane_bit_mask(hb);
+                const int jj = Cs * 64 + hb;
+                if (tested_prev && jj == (int)w - 1) continue;
+                ++attempts;
+                ++tests;
+                const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
+                if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
+                    ++passed;
+                    if (lane == 0) a.parent[w] = jj;
+                    done = true;
+                    break;
+                }
+            }
+        }
+    }
+    add_stats(a, tests, passed);
+}
+
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     const int lane = threadIdx.x & 63;
@@ -186,62 +247,7 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
     if (w >= a.n || !a.core[w]) return;
     const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     if constexpr (WORDS) {
-        // rows hooked by k_hook_tile are done; the test against w - 1 in the
-        // same tile was made there
-        if (a.mask && a.parent[w] != (int)w) return;
-        const bool tested_prev = a.mask && (w & 63) != 0;
-        // the row's words from its own tile down, 64 tiles a step (lane l holds
-        // the l-th tile below the window's top), bits from the highest down:
-        // smaller indices, nearest first
-        const WordRow A = word_row(a, w);
-        const int Rw = (int)(w >> 6);
-        int64_t top = 0;  // tiles up to and including the row's own tile
-        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
-            const int64_t t = t0 + lane;
-            const uint64_t le = __ballot(t < A.len && A.tile[t] <= Rw);
-            top += __popcll(le);
-            if (__popcll(le) < 64) break;
-        }
-        const int c = csim_of(a, w);
-        unsigned tests = 0, passed = 0;
-        int attempts = 0;
-        bool done = false;
-        for (int64_t t1 = top; t1 > 0 && attempts < max_tests && !done; t1 -= 64) {
-            const int64_t t = t1 - 1 - lane;
-            uint64_t v = 0ull;
-            int C = 0;
-            if (t >= 0) {
-                C = A.tile[t];
-                v = word_at(A, t);
-                if (C == Rw) v &= lane_bit_mask((int)(w & 63)) - 1ull;  // smaller indices only
-            }
-            uint64_t tiles = __ballot(v != 0ull);
-            while (tiles && attempts < max_tests && !done) {
-                const int src = __ffsll((unsigned long long)tiles) - 1;
-                tiles &= tiles - 1;
-                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
-                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
-                const int Cs = __builtin_amdgcn_readlane(C, src);
-                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
-                uint64_t bits = __ballot(bit && a.core[Cs * 64 + lane]);
-                while (bits && attempts < max_tests) {
-                    const int hb = 63 - __clzll((long long)bits);
-                    bits &= ~lane_bit_mask(hb);
-                    const int jj = Cs * 64 + hb;
-                    if (tested_prev && jj == (int)w - 1) continue;
-                    ++attempts;
-                    ++tests;
-                    const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
-                    if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
-                        ++passed;
-                        if (lane == 0) a.parent[w] = jj;
-                        done = true;
-                        break;
-                    }
-                }
-            }
-        }
-        add_stats(a, tests, passed);
+        hook_row_words(a, w, max_tests, false);
         return;
     }
     // first entry >= w: the row's own entry, recorded when the lists were
@@ -295,7 +301,8 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
 // at least c_sim common neighbours hooks to it.  The pass stops once every
 // row still in question has reached c_sim.  Writes only parent[w] of hooked
 // rows, to w - 1: a forest, as the per-row hooking.
-__global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles) {
+__global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles, int32_t *__restrict__ queue,
+                                                   unsigned int *__restrict__ queue_len) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -304,25 +311,71 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles)
     const bool core_p = __shfl_up((int)core_w, 1, 64) != 0;
     const int c = csim_of(a, R * 64);
     // rows whose test is still open: core, predecessor core within the tile
-    bool open = lane > 0 && core_w && core_p;
-    if (__ballot(open) == 0ull) return;
-    const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
-    int cnt = 0;
-    bool nbr = false;  // w - 1 in N(w): bit lane-1 of the row's word in its own tile
-    for (int64_t k = kb; k < ke; ++k) {
-        const int C = a.tp_col[k];
-        if (a.pair_nz[k] == 0) continue;
-        const uint64_t v = a.mask[k * 64 + lane];
-        const uint64_t pv = __shfl_up(v, 1, 64);
-        cnt += __popcll(v & pv);
-        if (C == (int)R) nbr = lane > 0 && ((v >> (lane - 1)) & 1ull) != 0ull;
-        // past the row tile's own tile every row knows whether w - 1 is a neighbour
-        if (((k - kb) & 7) == 7 && C >= (int)R && __ballot(open && nbr && cnt < c) == 0ull) break;
+    const bool open = lane > 0 && core_w && core_p;
+    bool hooked = false;
+    unsigned tests = 0, passed = 0;
+    if (__ballot(open) != 0ull) {
+        const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
+        int cnt = 0;
+        bool nbr = false;  // w - 1 in N(w): bit lane-1 of the row's word in its own tile
+        bool done = false;
+        for (int64_t k0 = kb; k0 < ke && !done; k0 += 64) {
+            // the window's non-empty pairs, then their words 8 at a time (loads in flight together)
+            const int64_t kk = k0 + lane;
+            const bool nzl = kk < ke && a.pair_nz[kk] != 0;
+            const int coll = kk < ke ? a.tp_col[kk] : INT_MAX;
+            uint64_t nzm = __ballot(nzl);
+            while (nzm && !done) {
+                int idx[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    idx[u] = nzm ? __ffsll((unsigned long long)nzm) - 1 : -1;
+                    nzm &= nzm - 1;
+                }
+                uint64_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = idx[u] >= 0 ? a.mask[(k0 + idx[u]) * 64 + lane] : 0ull;
+                int cmax = INT_MIN;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (idx[u] < 0) continue;
+                    const uint64_t pv = __shfl_up(v[u], 1, 64);
+                    cnt += __popcll(v[u] & pv);
+                    const int C = __shfl(coll, idx[u], 64);
+                    if (C == (int)R) nbr = lane > 0 && ((v[u] >> (lane - 1)) & 1ull) != 0ull;
+                    cmax = C;
+                }
+                // past the row tile's own tile every row knows whether w - 1 is a neighbour
+                if (cmax >= (int)R && __ballot(open && nbr && cnt < c) == 0ull) done = true;
+            }
+        }
+        hooked = open && nbr && cnt >= c;
+        if (hooked) a.parent[w] = (int)(w - 1);
+        tests = (unsigned)__popcll(__ballot(open && nbr));
+        passed = (unsigned)__popcll(__ballot(hooked));
     }
-    if (open && nbr && cnt >= c) a.parent[w] = (int)(w - 1);
-    const unsigned tests = (unsigned)__popcll(__ballot(open && nbr));
-    const unsigned passed = (unsigned)__popcll(__ballot(open && nbr && cnt >= c));
+    // core rows left to the per-row hooking
+    const bool left = core_w && !hooked;
+    const uint64_t lm = __ballot(left);
+    if (lm) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(queue_len, (unsigned)__popcll(lm));
+        base = __shfl(base, 0, 64);
+        if (left) queue[base + __popcll(lm & ((1ull << lane) - 1ull))] = (int32_t)w;
+    }
     add_stats(a, tests, passed);
+}
+
+// Per-row hooking of the rows k_hook_tile left (a queue; grid-stride, one wave
+// per row).
+__global__ __launch_bounds__(256) void k_hook_queue(LabelArgs a, const int32_t *__restrict__ queue,
+                                                    const unsigned int *__restrict__ queue_len, int max_tests) {
+    const unsigned nq = *queue_len;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); q < (int64_t)nq; q += waves) {
+        const int64_t w = queue[q];
+        hook_row_words(a, w, max_tests, (w & 63) != 0);
+    }
 }
 
 // Root hooking: a core point that is still a root tests core neighbours whose
@@ -930,9 +983,20 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     k_init<<<blocks_for(n), 256, 0, s>>>(a, req.consider, req.perm);
     CNN_HIP(hipGetLastError());
     CNN_HIP(hipEventRecord(ctx->ev[8], s));
-    if (words && a.mask) k_hook_tile<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(a, req.n_tiles);
-    if (words) k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
-    else k_hook<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
+    if (words && a.mask) {
+        // adjacent rows per row tile, then the rows left, one wave each
+        Scratch<int32_t> hq(ctx, n);
+        Scratch<unsigned int> hq_len(ctx, 1);
+        CNN_HIP(hipMemsetAsync(hq_len.p, 0, sizeof(unsigned int), s));
+        k_hook_tile<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(a, req.n_tiles, hq.p, hq_len.p);
+        CNN_HIP(hipGetLastError());
+        const unsigned qblocks = (unsigned)std::min<int64_t>(blocks_for(n, 4), 4096);
+        k_hook_queue<<<qblocks, 256, 0, s>>>(a, hq.p, hq_len.p, 2);
+    } else if (words) {
+        k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
+    } else {
+        k_hook<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
+    }
     CNN_HIP(hipGetLastError());
     CNN_HIP(hipEventRecord(ctx->ev[9], s));
     // flatten (every parent becomes its root), then let roots re-point to smaller
