@@ -399,6 +399,7 @@ def features_block(args):
     """features_64d (n=2e5, d=64, float32): FP32-banded (MFMA Gram form with the
     FP64 guard band) against the plain FP64 form, per fit."""
     import torch
+    from commonnnclustering_amd import autotune
 
     X, p = workload("features_64d")
     n, d = X.shape
@@ -422,6 +423,7 @@ def features_block(args):
             "max_fp32_abs_err_band": float(gs_band[-1]["max_abs_err_band"]),
             "phase_ms": phases(gs_band, ls_band), "phase_ms_plain": phases(gs_plain, ls_plain),
             "label_stats": ls_band[-1],
+            "tuning": {"f32_band": autotune.current(d, n, "f32_band"), "f64": autotune.current(d, n, "f64")},
             "mfma_roofline": distance_roofline(gs_band, d, "f32_band", "features_64d"),
             "similarity_rooflines": su_rooflines(ls_band, "features_64d")}
 
@@ -695,7 +697,7 @@ def main():
 
     # host-buffer form: X starts in (pinned) host memory, the copy to HBM is timed with the fit
     import torch
-    from commonnnclustering_amd import engine
+    from commonnnclustering_amd import autotune, engine
 
     Xh = torch.from_numpy(X).pin_memory()
     r2 = float(p["radius_cutoff"]) ** 2
@@ -780,6 +782,8 @@ def main():
         "phase_ms": phases(gs_band, ls_band),
         "phase_ms_plain": ph_plain,
         "label_stats": ls_band[-1],
+        "tuning": {"f32_band": autotune.current(d, n, "f32_band"), "f64": autotune.current(d, n, "f64"),
+                   "source": "commonnnclustering_amd/tuning_cache.json (python -m commonnnclustering_amd.autotune)"},
         "neighbour_rows_differing_from_plain_f64": rows_bad,
         "neighbour_entries": entries,
         "roofline": dominant,

@@ -49,6 +49,7 @@ EXPORTED = [
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
     "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_graph_build_periodic",
     "cnn_csr_edge_similarity", "cnn_similarity_mst", "cnn_version", "cnn_labels_from_csr_noalloc",
+    "cnn_tune_set", "cnn_tune_get", "cnn_tune_clear",
 ]
 
 
@@ -79,6 +80,21 @@ class GraphStats(ctypes.Structure):
         ("layout", ctypes.c_int32),
         ("tile_pairs_computed", ctypes.c_int64),
         ("ms_distance_kernel", ctypes.c_float),
+    ]
+
+    def to_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class TuneParams(ctypes.Structure):
+    """cnn_tune_params: launch shapes and switches of one problem class."""
+
+    _fields_ = [
+        ("mfma_col_chunk", ctypes.c_int32),
+        ("wide_col_chunk", ctypes.c_int32),
+        ("emit_waves", ctypes.c_int32),
+        ("root_passes", ctypes.c_int32),
+        ("words_min_fill", ctypes.c_int32),
     ]
 
     def to_dict(self) -> dict:
@@ -140,6 +156,9 @@ def _declare(L):
         "cnn_labels_from_csr_noalloc": (c, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _vp,
                                             ctypes.POINTER(LabelStats)]),
         "cnn_sort_by_size": (c, [_vp, _vp, _i64, _i64, _i64, _i64p]),
+        "cnn_tune_set": (c, [_i32, _i32, _i64, _i64, _i32, ctypes.POINTER(TuneParams)]),
+        "cnn_tune_get": (c, [_i32, _i64, _i32, ctypes.POINTER(TuneParams)]),
+        "cnn_tune_clear": (c, []),
         "cnn_fit": (c, [_vp, _vp, _i64, _i32, _i32, _dbl, _i32, _i32, _i32, _i32, _i64, _i64, _vp,
                         ctypes.POINTER(_vp), ctypes.POINTER(GraphStats), ctypes.POINTER(LabelStats)]),
         "cnn_graph_fit": (c, [_vp, _vp, _i32, _i32, _i32, _i64, _i64, _vp, ctypes.POINTER(LabelStats)]),
@@ -175,6 +194,10 @@ def lib():
                 L = ctypes.CDLL(LIB_PATH)
                 _declare(L)
                 _lib = L
+                # tuned launch shapes (commonnnclustering_amd/tuning_cache.json), when present
+                from . import autotune
+
+                autotune.load_cache()
     return _lib
 
 

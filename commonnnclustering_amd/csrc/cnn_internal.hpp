@@ -18,7 +18,6 @@ constexpr int kTile = 64;        // points per tile: one wave64 lane per row
 constexpr int kWave = 64;
 constexpr int kSpatialMaxD = 8;  // d <= 8: spatial (Morton) order + tile-box culling
 constexpr int kMfmaMinD = 16;    // d >= 16: FP32 Gram form on MFMA in CNN_F32_BAND
-constexpr int kMfmaColChunk = 64;   // column tiles per workgroup of the dense LDS MFMA kernel
 constexpr int kBatchWorkers = 6;    // datasets of a batch fitted at once (own stream each)
 constexpr int64_t kPivotMinN = 2048;  // d > 8: smallest n ordered by nearest-pivot cells
 
@@ -26,6 +25,17 @@ struct Error {
     int code;
     std::string msg;
 };
+
+// Tuned launch shapes of one problem class (cnn_tune_params); defaults when
+// the table holds no entry for the class.
+struct TuneParams {
+    int mfma_col_chunk = 64;
+    int wide_col_chunk = 32;
+    int emit_waves = 8;
+    int root_passes = 2;
+    int64_t words_min_fill = 8;
+};
+TuneParams tune_lookup(int d, int64_t n, int precision);
 
 // Device-memory arena with best-fit reuse.  Blocks stay allocated until trim();
 // all users enqueue on the context stream, so reuse is stream-ordered.
@@ -105,6 +115,7 @@ struct cnn_graph {
     int64_t n_tile_pairs = 0;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
+    cnn::TuneParams tune{};      // launch shapes of the graph's class
     // a graph that keeps its words emits its lists (col, selfpos) only when a
     // caller needs them; this lock makes that happen once
     std::mutex lists_mu;
@@ -280,6 +291,9 @@ struct LabelRequest {
     const int32_t *tile_csim = nullptr;
     const int32_t *tile_mcore = nullptr;
     const struct PackedSort *packed = nullptr;
+    // tuned switches (graph requests take the graph's; caller lists the d = 0 class)
+    int root_passes = 2;
+    int64_t words_min_fill = 8;
 };
 
 // Sort key layout of a packed batch: dataset << ds_shift | (size_max - size)

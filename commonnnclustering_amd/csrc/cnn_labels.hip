@@ -911,9 +911,6 @@ __global__ __launch_bounds__(256) void k_row_word_fill(const int64_t *__restrict
 
 static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, ceil_div(n, per)); }
 
-// Entries per word from which the automatic choice tests similarities on
-// per-row word lists built from the lists (below it, the sorted-list merge).
-constexpr int64_t kWordsMinFill = 8;
 
 void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     hipStream_t s = ctx->stream;
@@ -961,7 +958,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             const int32_t last_cnt = read_scalar(ctx, wcnt.p + (n - 1));
             const int64_t n_words = last_excl + last_cnt;
             const int64_t nnz = read_scalar(ctx, req.row_ptr + n) - read_scalar(ctx, req.row_ptr);
-            if (req.sim_mode == CNN_SIM_BITSET || nnz >= kWordsMinFill * n_words) {
+            if (req.sim_mode == CNN_SIM_BITSET || (req.words_min_fill > 0 && nnz >= req.words_min_fill * n_words)) {
                 CNN_HIP(hipMemcpyAsync(wptr->p + n, &n_words, sizeof(int64_t), hipMemcpyHostToDevice, s));
                 wtile.reset(new Scratch<int32_t>(ctx, n_words));
                 wword.reset(new Scratch<uint64_t>(ctx, n_words));
@@ -1002,7 +999,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     // flatten (every parent becomes its root), then let roots re-point to smaller
     // neighbouring roots twice, flattening after each pass
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < req.root_passes; ++pass) {
         if (words) k_hook_roots<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
         else k_hook_roots<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
         k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
@@ -1137,6 +1134,8 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
     r.selfpos = g->selfpos;
     r.mask = g->mask;
     r.pair_nz = g->pair_nz;
+    r.root_passes = g->tune.root_passes;
+    r.words_min_fill = g->tune.words_min_fill;
     return r;
 }
 
@@ -1289,6 +1288,9 @@ int cnn_labels_from_csr(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col
         validate_csr(ctx, row_ptr, col, n);
         LabelRequest r{row_ptr, col, n, nullptr, c_sim, m_core, consider, current_start, sim_mode,
                        false, false, 2, -1, labels};
+        const TuneParams t = tune_lookup(0, n, CNN_PLAIN_F64);
+        r.root_passes = t.root_passes;
+        r.words_min_fill = t.words_min_fill;
         labels_run(ctx, r, stats_out);
         return (int)CNN_OK;
     });

@@ -1566,15 +1566,11 @@ __device__ __forceinline__ int select_bit(uint64_t w, int t) {
     return pos + (t >= (int)(x & 1u) ? 1 : 0);
 }
 
-#ifndef CNN_EMIT_WAVES
-#define CNN_EMIT_WAVES 8
-#endif
-// Waves per emit workgroup: the 33 KB of staged words per workgroup bound the
-// resident workgroups per CU, so more waves per workgroup means more waves in
-// flight to cover the latency of the per-word chains.
-constexpr int kEmitWaves = CNN_EMIT_WAVES;
-
-template <bool OUTPUT_PARALLEL>
+// Waves per emit workgroup (a template parameter, 4 or 8, chosen per problem
+// class by the tuning table): the 33 KB of staged words per workgroup bound
+// the resident workgroups per CU, so more waves per workgroup means more
+// waves in flight to cover the latency of the per-word chains.
+template <bool OUTPUT_PARALLEL, int kEmitWaves>
 __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const int64_t *__restrict__ row_ptr,
                                                                int32_t *__restrict__ col,
                                                                int32_t *__restrict__ selfpos) {
@@ -1692,6 +1688,21 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
     }
 }
 
+// Emission of the row tiles R0 .. R0 + n_rt - 1 with `waves` waves per workgroup (4 or 8).
+static void launch_emit(hipStream_t s, const DistArgs &a, int64_t n_rt, bool output_parallel, int waves,
+                        const int64_t *row_ptr, int32_t *col, int32_t *selfpos) {
+    if (n_rt <= 0) return;
+    const unsigned g = (unsigned)n_rt;
+    if (waves == 4) {
+        if (output_parallel) k_emit_rows<true, 4><<<g, 256, 0, s>>>(a, row_ptr, col, selfpos);
+        else k_emit_rows<false, 4><<<g, 256, 0, s>>>(a, row_ptr, col, selfpos);
+    } else {
+        if (output_parallel) k_emit_rows<true, 8><<<g, 512, 0, s>>>(a, row_ptr, col, selfpos);
+        else k_emit_rows<false, 8><<<g, 512, 0, s>>>(a, row_ptr, col, selfpos);
+    }
+    CNN_HIP(hipGetLastError());
+}
+
 __global__ void k_deg_from_rowptr_perm(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ perm,
                                        int64_t n, int64_t *__restrict__ deg_o) {
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1782,7 +1793,8 @@ static int wide_kmax(const DistArgs &a) {
 }
 
 // Events 6 and 7 bracket the main distance kernel (without guard-band settling).
-static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band, int64_t batch_pairs) {
+static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band, int64_t batch_pairs,
+                        int mfma_col_chunk) {
     dim3 grid((unsigned)n_rt), block(256);
     hipStream_t s = ctx->stream;
     CNN_HIP(hipEventRecord(ctx->ev[6], s));
@@ -1810,7 +1822,7 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         const int G = a.dp / 8;
         if (!a.tp_ptr && !a.tile_valid && G >= 3 && G <= 10) {
             // dense pairs, row operands in registers, column tiles shared through LDS
-            b.col_chunk = kMfmaColChunk;
+            b.col_chunk = mfma_col_chunk;
             const int64_t n_rb = ceil_div(n_rt, 4), n_cc = ceil_div(a.n_tiles, b.col_chunk);
             int64_t n_items = 0;
             for (int64_t cc = 0; cc < n_cc; ++cc)
@@ -2154,7 +2166,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.xra = xra ? xra->p : nullptr;
         a.xca = xca ? xca->p : nullptr;
         a.emarg = emarg ? emarg->p : nullptr;
-        a.col_chunk = 32;
+        a.col_chunk = g->tune.wide_col_chunk;
         a.pair_nz = pair_nz_p;
         a.pair_groups = gmask ? gmask->p : nullptr;
         a.bounds = g->bounds;
@@ -2166,7 +2178,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         }
         fill_i32(ctx, deg.p, 0, n);
         CNN_HIP(hipEventRecord(ctx->ev[2], s));
-        launch_dist(ctx, a, b.second - b.first, band, pair_begin(b.second) - pair_begin(b.first));
+        launch_dist(ctx, a, b.second - b.first, band, pair_begin(b.second) - pair_begin(b.first),
+                    g->tune.mfma_col_chunk);
         CNN_HIP(hipEventRecord(ctx->ev[3], s));
         // row_ptr for this batch's rows: exclusive scan + running offset
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
@@ -2221,15 +2234,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipEventRecord(ctx->ev[4], s));
         // word-serial emission for the spatial layout (words hold many bits each),
         // output-parallel for the dense layout (few bits per word)
-        if (rows > 0) {
-            if (g->layout == 1)
-                k_emit_rows<false><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col,
-                                                                                             g->selfpos);
-            else
-                k_emit_rows<true><<<(unsigned)(b.second - b.first), 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col,
-                                                                                            g->selfpos);
-        }
-        CNN_HIP(hipGetLastError());
+        if (rows > 0)
+            launch_emit(s, a, b.second - b.first, g->layout != 1, g->tune.emit_waves, g->row_ptr, col, g->selfpos);
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
         CNN_HIP(hipEventSynchronize(ctx->ev[5]));
         float t1 = 0, t2 = 0, t3 = 0;
@@ -2303,6 +2309,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     g->r2 = r2;
     g->radius = std::sqrt(r2);
     g->n_tiles = ceil_div(n, 64);
+    g->tune = tune_lookup(d, n, precision);
     const int64_t n_pad = g->n_tiles * 64;
     const bool band = precision == CNN_F32_BAND;
 
@@ -2451,6 +2458,7 @@ cnn_graph *graph_build_packed(cnn_ctx *ctx, const void *X, const int64_t *offset
     g->d = d;
     g->dtype = dtype;
     g->precision = precision;
+    g->tune = tune_lookup(d, N, precision);
     g->r2 = r2max;
     g->radius = std::sqrt(r2max);
     g->n_tiles = T;
@@ -2567,8 +2575,7 @@ void graph_lists(cnn_ctx *ctx, const cnn_graph *gc) {
     a.mask = g->mask;
     a.pair_nz = g->pair_nz;
     CNN_HIP(hipEventRecord(ctx->ev[4], s));
-    if (T > 0) k_emit_rows<false><<<(unsigned)T, 64 * kEmitWaves, 0, s>>>(a, g->row_ptr, col, selfpos);
-    CNN_HIP(hipGetLastError());
+    launch_emit(s, a, T, false, g->tune.emit_waves, g->row_ptr, col, selfpos);
     CNN_HIP(hipEventRecord(ctx->ev[5], s));
     // complete before any stream (of any context) reads them
     CNN_HIP(hipEventSynchronize(ctx->ev[5]));
@@ -2653,6 +2660,7 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     g->radius = std::sqrt(r2);
     g->layout = src->layout;
     g->n_tiles = src->n_tiles;
+    g->tune = src->tune;
     const int64_t n = g->n, n_pad = g->n_tiles * 64;
     const int dd = std::max(g->d, 1);
     g->row_ptr = dev_alloc<int64_t>(g, n + 1);

@@ -159,6 +159,24 @@ void segmented_sort_i32(cnn_ctx *ctx, int32_t *keys, int64_t nnz, const int64_t 
     CNN_HIP(hipMemcpyAsync(keys, k2.p, nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
 }
 
+// ------------------------------------------------------------------ tuning table
+struct TuneEntry {
+    int32_t d_lo, d_hi;
+    int64_t n_lo, n_hi;
+    int32_t precision;
+    TuneParams p;
+};
+static std::mutex g_tune_mu;
+static std::vector<TuneEntry> g_tune;
+
+TuneParams tune_lookup(int d, int64_t n, int precision) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (auto it = g_tune.rbegin(); it != g_tune.rend(); ++it)
+        if (d >= it->d_lo && d <= it->d_hi && n >= it->n_lo && n < it->n_hi && precision == it->precision)
+            return it->p;
+    return TuneParams{};
+}
+
 }  // namespace cnn
 
 // ================================================================== C-ABI: context
@@ -245,6 +263,42 @@ int cnn_ctx_trim(cnn_ctx *ctx) {
         ctx->arena->trim();
         return (int)CNN_OK;
     });
+}
+
+int cnn_tune_set(int32_t d_lo, int32_t d_hi, int64_t n_lo, int64_t n_hi, int32_t precision,
+                 const cnn_tune_params *p) {
+    if (!p || d_lo < 0 || d_hi < d_lo || n_lo < 0 || n_hi <= n_lo) return CNN_ERR_INVALID;
+    if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) return CNN_ERR_INVALID;
+    if (p->mfma_col_chunk < 8 || p->mfma_col_chunk > 1024 || p->wide_col_chunk < 1 || p->wide_col_chunk > 1024 ||
+        (p->emit_waves != 4 && p->emit_waves != 8) || p->root_passes < 0 || p->root_passes > 8 ||
+        p->words_min_fill < 0 || p->words_min_fill > (1 << 30))
+        return CNN_ERR_INVALID;
+    TuneEntry e{d_lo, d_hi, n_lo, n_hi, precision, {}};
+    e.p.mfma_col_chunk = p->mfma_col_chunk;
+    e.p.wide_col_chunk = p->wide_col_chunk;
+    e.p.emit_waves = p->emit_waves;
+    e.p.root_passes = p->root_passes;
+    e.p.words_min_fill = p->words_min_fill;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.push_back(e);
+    return CNN_OK;
+}
+
+int cnn_tune_get(int32_t d, int64_t n, int32_t precision, cnn_tune_params *out) {
+    if (!out) return CNN_ERR_INVALID;
+    const TuneParams t = tune_lookup(d, n, precision);
+    out->mfma_col_chunk = t.mfma_col_chunk;
+    out->wide_col_chunk = t.wide_col_chunk;
+    out->emit_waves = t.emit_waves;
+    out->root_passes = t.root_passes;
+    out->words_min_fill = (int32_t)t.words_min_fill;
+    return CNN_OK;
+}
+
+int cnn_tune_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+    return CNN_OK;
 }
 
 const char *cnn_version(void) { return "commonnnclustering_amd 0.1.0 (gfx950)"; }

@@ -316,6 +316,30 @@ int cnn_similarity_mst(cnn_ctx *ctx, const int64_t *row_ptr, const int32_t *col,
                        int64_t *tree_a, int64_t *tree_b, int64_t *tree_w, int64_t cap,
                        int64_t *n_edges_out);
 
+/* ------------------------------------------------------------------ tuning */
+/* Launch shapes and algorithm switches chosen per problem class by the
+ * autotuner (commonnnclustering_amd/autotune.py).  No counterpart in the
+ * reference.  Every value is exact-result neutral: it changes speed only. */
+typedef struct {
+    int32_t mfma_col_chunk;  /* column tiles per workgroup of the dense MFMA distance kernel (8..1024; 64) */
+    int32_t wide_col_chunk;  /* column tiles per workgroup of the dense plain-FP64 kernel (1..1024; 32) */
+    int32_t emit_waves;      /* waves per workgroup of the neighbour-list emission (4 or 8; 8) */
+    int32_t root_passes;     /* root-hooking passes of the labelling before its full round (0..8; 2) */
+    int32_t words_min_fill;  /* CNN_SIM_AUTO on lists: per-row word lists when the lists hold at least
+                                this many entries per word, else the merge (0..2^30; 8) */
+} cnn_tune_params;
+
+/* Process-wide table of tuned shapes.  cnn_tune_set stores p for the class
+ * d_lo <= d <= d_hi, n_lo <= n < n_hi, precision (CNN_PLAIN_F64 / CNN_F32_BAND);
+ * later entries take precedence.  Graph builds and fits look up their class
+ * (lists given by the caller: d = 0, precision CNN_PLAIN_F64); without an
+ * entry the defaults above apply.  cnn_tune_get returns the shapes a problem
+ * of that class uses; cnn_tune_clear empties the table. */
+int cnn_tune_set(int32_t d_lo, int32_t d_hi, int64_t n_lo, int64_t n_hi, int32_t precision,
+                 const cnn_tune_params *p);
+int cnn_tune_get(int32_t d, int64_t n, int32_t precision, cnn_tune_params *out);
+int cnn_tune_clear(void);
+
 /* ------------------------------------------------------------------ info */
 const char *cnn_version(void);
 
