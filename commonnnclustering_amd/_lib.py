@@ -49,7 +49,8 @@ EXPORTED = [
     "cnn_graph_labels", "cnn_labels_from_csr", "cnn_sort_by_size", "cnn_fit", "cnn_graph_fit",
     "cnn_fit_batch", "cnn_fit_scan", "cnn_predict", "cnn_precomputed_csr", "cnn_graph_build_periodic",
     "cnn_csr_edge_similarity", "cnn_similarity_mst", "cnn_version", "cnn_labels_from_csr_noalloc",
-    "cnn_tune_set", "cnn_tune_get", "cnn_tune_clear",
+    "cnn_tune_set", "cnn_tune_get", "cnn_tune_clear", "cnn_graph_scan_create", "cnn_scan_launch",
+    "cnn_scan_destroy",
 ]
 
 
@@ -159,6 +160,9 @@ def _declare(L):
         "cnn_tune_set": (c, [_i32, _i32, _i64, _i64, _i32, ctypes.POINTER(TuneParams)]),
         "cnn_tune_get": (c, [_i32, _i64, _i32, ctypes.POINTER(TuneParams)]),
         "cnn_tune_clear": (c, []),
+        "cnn_graph_scan_create": (c, [_vp, _vp, _vp, _i32, _vp, _vp, _i32, _i64, _i64, _vp, ctypes.POINTER(_vp)]),
+        "cnn_scan_launch": (c, [_vp, _vp, _vp]),
+        "cnn_scan_destroy": (c, [_vp]),
         "cnn_fit": (c, [_vp, _vp, _i64, _i32, _i32, _dbl, _i32, _i32, _i32, _i32, _i64, _i64, _vp,
                         ctypes.POINTER(_vp), ctypes.POINTER(GraphStats), ctypes.POINTER(LabelStats)]),
         "cnn_graph_fit": (c, [_vp, _vp, _i32, _i32, _i32, _i64, _i64, _vp, ctypes.POINTER(LabelStats)]),

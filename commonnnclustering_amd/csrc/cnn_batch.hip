@@ -176,7 +176,228 @@ static void fit_each(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_
     });
 }
 
+// ------------------------------------------------------------------ captured scans
+
+// A parameter scan of a spatial graph that keeps its row words, recorded once
+// into a hipGraph: per radius (descending) the words are re-thresholded exactly
+// (k_word_filter, from the previous radius's words), then one launch per
+// labelling kernel covers every cutoff pair of that radius (launch y = pair
+// y).  Every buffer is allocated before the capture and no step reads a value
+// back to the host, so the graph replays the whole scan with one launch.
+struct cnn_scan {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {};
+    std::vector<void *> bufs;
+    int32_t n_r = 0, n_c = 0;
+    int64_t n = 0;
+    unsigned long long *counters = nullptr;  // n_r * n_c * 8
+    unsigned long long *tstats = nullptr;    // n_r * n_c * 4 * 1024
+};
+
+namespace {
+
+constexpr int kScanStatSlots = 4 * 1024;
+
+void scan_free(cnn_scan *sc) {
+    if (!sc) return;
+    (void)hipSetDevice(sc->device);
+    if (sc->exec) (void)hipGraphExecDestroy(sc->exec);
+    if (sc->graph) (void)hipGraphDestroy(sc->graph);
+    for (auto &e : sc->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : sc->bufs) (void)hipFree(p);
+    delete sc;
+}
+
+template <typename T>
+T *scan_alloc(cnn_scan *sc, int64_t count) {
+    void *p = nullptr;
+    const size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        throw OutOfMemory{"scan buffer of " + std::to_string(bytes) + " bytes"};
+    }
+    sc->bufs.push_back(p);
+    return (T *)p;
+}
+
+cnn_scan *scan_create(cnn_ctx *ctx, const cnn_graph *g, const double *r2, int32_t n_r, const int32_t *c_sim,
+                      const int32_t *m_core, int32_t n_c, int64_t member_cutoff, int64_t max_clusters,
+                      int64_t *labels) {
+    if (!g->mask || !g->pair_nz || !g->tp_ptr)
+        throw Capacity{"captured scans need a graph that keeps its row words"};
+    for (int32_t i = 0; i < n_r; ++i)
+        if (!(r2[i] <= g->r2)) throw InvalidArg{"scan radii must not exceed the graph radius"};
+    for (int32_t ci = 0; ci < n_c; ++ci)
+        if (c_sim[ci] < 0 || m_core[ci] < 0) throw InvalidArg{"c_sim and m_core must be >= 0"};
+    std::unique_ptr<cnn_scan, void (*)(cnn_scan *)> sc(new cnn_scan(), scan_free);
+    sc->device = ctx->device;
+    sc->n_r = n_r;
+    sc->n_c = n_c;
+    const int64_t n = g->n, T = g->n_tiles, P = g->n_tile_pairs;
+    sc->n = n;
+    for (auto &e : sc->ev) CNN_HIP(hipEventCreate(&e));
+    std::vector<int32_t> order(n_r);
+    for (int32_t i = 0; i < n_r; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return r2[x] > r2[y]; });
+    // buffers: words of two radii (rotating), degrees, the labelling slices of n_c pairs
+    bool smaller = false;
+    for (int32_t i = 0; i < n_r; ++i) smaller = smaller || r2[i] < g->r2;
+    uint64_t *wmask[2] = {nullptr, nullptr};
+    uint8_t *wnz[2] = {nullptr, nullptr};
+    int32_t *wdeg[2] = {nullptr, nullptr};
+    if (smaller)
+        for (int k = 0; k < 2; ++k) {
+            wmask[k] = scan_alloc<uint64_t>(sc.get(), P * 64);
+            wnz[k] = scan_alloc<uint8_t>(sc.get(), P);
+            wdeg[k] = scan_alloc<int32_t>(sc.get(), n);
+        }
+    int32_t *own_deg = scan_alloc<int32_t>(sc.get(), n);
+    MultiLabelSizes sz;
+    multi_labels_bytes(n, T, n_c, sz);
+    MultiLabelBufs b;
+    b.n_c = n_c;
+    b.n = n;
+    b.n_tiles = T;
+    const int64_t m = sz.per_point, mt = sz.per_tile;
+    b.core = scan_alloc<uint8_t>(sc.get(), m);
+    b.parent = scan_alloc<int>(sc.get(), m);
+    b.rank = scan_alloc<uint64_t>(sc.get(), m);
+    b.cnt = scan_alloc<int32_t>(sc.get(), m);
+    b.hq = scan_alloc<int32_t>(sc.get(), m);
+    b.hq_len = scan_alloc<unsigned int>(sc.get(), n_c);
+    b.tmax = scan_alloc<uint64_t>(sc.get(), mt);
+    b.cand_max = scan_alloc<uint64_t>(sc.get(), mt);
+    b.tile_common = scan_alloc<uint64_t>(sc.get(), mt);
+    b.cmin = scan_alloc<int32_t>(sc.get(), m);
+    b.csize = scan_alloc<int32_t>(sc.get(), m);
+    b.newlab = scan_alloc<int32_t>(sc.get(), m);
+    b.ckey = scan_alloc<uint64_t>(sc.get(), m);
+    b.cidx = scan_alloc<int32_t>(sc.get(), m);
+    b.ccount = scan_alloc<unsigned int>(sc.get(), n_c);
+    int32_t *d_csim = scan_alloc<int32_t>(sc.get(), n_c);
+    int32_t *d_mcore = scan_alloc<int32_t>(sc.get(), n_c);
+    CNN_HIP(hipMemcpy(d_csim, c_sim, n_c * sizeof(int32_t), hipMemcpyHostToDevice));
+    CNN_HIP(hipMemcpy(d_mcore, m_core, n_c * sizeof(int32_t), hipMemcpyHostToDevice));
+    b.csim = d_csim;
+    b.mcore = d_mcore;
+    sc->counters = scan_alloc<unsigned long long>(sc.get(), (int64_t)n_r * n_c * 8);
+    sc->tstats = scan_alloc<unsigned long long>(sc.get(), (int64_t)n_r * n_c * kScanStatSlots);
+
+    // record the scan on a private stream
+    hipStream_t cs = nullptr;
+    CNN_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipStream_t saved = ctx->stream;
+    ctx->stream = cs;
+    bool capturing = false;
+    try {
+        CNN_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        const uint64_t *cur_mask = g->mask;
+        const uint8_t *cur_nz = g->pair_nz;
+        const int32_t *cur_deg = own_deg;
+        words_degrees(ctx, g, own_deg);
+        double cur_r2 = g->r2;
+        int next = 0;
+        for (int32_t oi = 0; oi < n_r; ++oi) {
+            const int32_t ri = order[oi];
+            if (r2[ri] < cur_r2) {
+                words_filter(ctx, g, cur_mask, cur_nz, r2[ri], wmask[next], wnz[next], wdeg[next]);
+                cur_mask = wmask[next];
+                cur_nz = wnz[next];
+                cur_deg = wdeg[next];
+                cur_r2 = r2[ri];
+                next ^= 1;
+            }
+            MultiLabelSource src{cur_mask, cur_nz, g->tp_ptr, g->tp_col, cur_deg, g->perm};
+            labels_enqueue_multi(ctx, b, src, g->tune.root_passes, member_cutoff, max_clusters,
+                                 labels + (int64_t)ri * n_c * n, sc->counters + (int64_t)ri * n_c * 8,
+                                 sc->tstats + (int64_t)ri * n_c * kScanStatSlots);
+        }
+        capturing = false;
+        CNN_HIP(hipStreamEndCapture(cs, &sc->graph));
+        CNN_HIP(hipGraphInstantiate(&sc->exec, sc->graph, nullptr, nullptr, 0));
+    } catch (...) {
+        if (capturing) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(cs, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+        }
+        ctx->stream = saved;
+        (void)hipStreamDestroy(cs);
+        (void)hipGetLastError();
+        throw;
+    }
+    ctx->stream = saved;
+    CNN_HIP(hipStreamDestroy(cs));
+    return sc.release();
+}
+
+void scan_launch(cnn_ctx *ctx, cnn_scan *sc, cnn_label_stats *lstats, float *ms_out) {
+    hipStream_t s = ctx->stream;
+    CNN_HIP(hipEventRecord(sc->ev[0], s));
+    CNN_HIP(hipGraphLaunch(sc->exec, s));
+    CNN_HIP(hipEventRecord(sc->ev[1], s));
+    CNN_HIP(hipEventSynchronize(sc->ev[1]));
+    float ms = 0.0f;
+    CNN_HIP(hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]));
+    if (ms_out) *ms_out = ms;
+    if (!lstats) return;
+    const int64_t pairs = (int64_t)sc->n_r * sc->n_c;
+    std::vector<unsigned long long> hc(pairs * 8), ht(pairs * kScanStatSlots);
+    CNN_HIP(hipMemcpy(hc.data(), sc->counters, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    CNN_HIP(hipMemcpy(ht.data(), sc->tstats, ht.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < pairs; ++p) {
+        cnn_label_stats st{};
+        const unsigned long long *c = hc.data() + p * 8, *t = ht.data() + p * kScanStatSlots;
+        for (int i = 0; i < kScanStatSlots / 4; ++i) {
+            st.tests += (int64_t)t[4 * i];
+            st.tests_passed += (int64_t)t[4 * i + 1];
+            st.rows_scanned += (int64_t)t[4 * i + 2];
+        }
+        st.n_core = (int64_t)c[2];
+        st.n_components = (int64_t)c[3];
+        st.n_clusters = (int64_t)c[7];
+        st.ms_similarity = ms / (float)pairs;  // the graph's time, shared evenly
+        st.sim_form = CNN_SIM_FORM_TILE_WORDS;
+        lstats[p] = st;
+    }
+}
+
+}  // namespace
+
 extern "C" {
+
+int cnn_graph_scan_create(cnn_ctx *ctx, const cnn_graph *g, const double *r2, int32_t n_r, const int32_t *c_sim,
+                          const int32_t *m_core, int32_t n_c, int64_t member_cutoff, int64_t max_clusters,
+                          int64_t *labels, cnn_scan **out) {
+    if (!ctx || !g || !out || n_r <= 0 || n_c <= 0) return CNN_ERR_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&]() {
+        if (!r2 || !c_sim || !m_core) throw InvalidArg{"null parameter array"};
+        if (g->n > 0 && !labels) throw InvalidArg{"labels is null"};
+        CNN_HIP(hipSetDevice(ctx->device));
+        *out = scan_create(ctx, g, r2, n_r, c_sim, m_core, n_c, member_cutoff, max_clusters, labels);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_scan_launch(cnn_ctx *ctx, cnn_scan *scan, cnn_label_stats *lstats) {
+    if (!ctx || !scan) return CNN_ERR_INVALID;
+    return guarded(ctx, [&]() {
+        CNN_HIP(hipSetDevice(ctx->device));
+        scan_launch(ctx, scan, lstats, nullptr);
+        return (int)CNN_OK;
+    });
+}
+
+int cnn_scan_destroy(cnn_scan *scan) {
+    if (!scan) return CNN_ERR_INVALID;
+    scan_free(scan);
+    return CNN_OK;
+}
 
 int cnn_fit_batch(cnn_ctx *ctx, const void *X, const int64_t *offsets, int32_t n_sets, int32_t d, int32_t dtype,
                   const double *r2, const int32_t *c_sim, const int32_t *m_core, int32_t precision,
@@ -234,6 +455,15 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
         CNN_HIP(hipSetDevice(ctx->device));
         cnn_graph *g = graph_build(ctx, X, n, d, dtype, r2[order[0]], precision, nullptr, true);
         std::unique_ptr<cnn_graph, void (*)(cnn_graph *)> hold(g, graph_free);
+        if (gstats) *gstats = g->stats;
+        if (g->mask && n > 0) {
+            // the graph keeps its row words: the whole scan as one captured graph
+            std::unique_ptr<cnn_scan, int (*)(cnn_scan *)> sc(
+                scan_create(ctx, g, r2, n_r, c_sim, m_core, n_c, member_cutoff, max_clusters, labels),
+                cnn_scan_destroy);
+            scan_launch(ctx, sc.get(), lstats, nullptr);
+            return (int)CNN_OK;
+        }
         // smaller radii filter the lists of the largest
         bool smaller = false;
         for (int32_t i = 0; i < n_r; ++i) smaller = smaller || r2[i] < r2[order[0]];

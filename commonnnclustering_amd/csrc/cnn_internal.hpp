@@ -247,6 +247,13 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *g, double r2, int32_
 void graph_lists(cnn_ctx *ctx, const cnn_graph *g);
 int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int64_t *rp, const int32_t *col,
                    double r2, int64_t *out_rp, int32_t **out_col);
+// Row words of a spatial graph (tile pairs tp_ptr / tp_col, pair-major words)
+// re-thresholded exactly at r2 <= the words' radius: dst / dst_nz in the same
+// layout, per-row degrees in deg.  Enqueued only (no host synchronisation).
+void words_filter(cnn_ctx *ctx, const cnn_graph *g, const uint64_t *src, const uint8_t *src_nz, double r2,
+                  uint64_t *dst, uint8_t *dst_nz, int32_t *deg);
+// Per-row degrees of a graph (from its row offsets); enqueued only.
+void words_degrees(cnn_ctx *ctx, const cnn_graph *g, int32_t *deg);
 
 // Host-side description of a packed batch graph.
 struct PackedHost {
@@ -316,6 +323,46 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
 void noalloc_setup(int device);
 int64_t sort_by_size_generic(cnn_ctx *ctx, int64_t *labels, int64_t n, int64_t member_cutoff,
                              int64_t max_clusters);
+
+// Labelling of several cutoff pairs of one graph at once (launch y of every
+// kernel takes pair y), enqueued without host synchronisation or memset, so
+// a stream capture can record it.  Buffers hold n_c slices each.
+struct MultiLabelBufs {
+    int n_c = 0;
+    int64_t n = 0, n_tiles = 0;
+    uint8_t *core = nullptr;
+    int *parent = nullptr;
+    uint64_t *rank = nullptr;
+    int32_t *cnt = nullptr;
+    int32_t *hq = nullptr;
+    unsigned int *hq_len = nullptr;
+    uint64_t *tmax = nullptr, *cand_max = nullptr, *tile_common = nullptr;
+    int32_t *cmin = nullptr, *csize = nullptr, *newlab = nullptr;
+    // sort_by_size: the clusters of each pair (keys, roots) and their count
+    uint64_t *ckey = nullptr;
+    int32_t *cidx = nullptr;
+    unsigned int *ccount = nullptr;                    // n_c
+    const int32_t *csim = nullptr, *mcore = nullptr;  // device, n_c each
+};
+struct MultiLabelSizes {
+    int64_t per_point = 0, per_tile = 0;
+};
+// The row words the pass reads: a spatial graph's tile-pair words at one radius.
+struct MultiLabelSource {
+    const uint64_t *mask;
+    const uint8_t *pair_nz;
+    const int64_t *tp_ptr;
+    const int32_t *tp_col;
+    const int32_t *deg;
+    const int32_t *perm;
+};
+void multi_labels_bytes(int64_t n, int64_t n_tiles, int n_c, MultiLabelSizes &sz);
+// labels: n_c blocks of n (block y = pair y), sorted by size; counters: 8 per
+// pair ([2] core points, [3] components, [7] clusters); tstats: 4 * 1024 per pair.
+void labels_enqueue_multi(cnn_ctx *ctx, const MultiLabelBufs &b, const MultiLabelSource &src, int root_passes,
+                          int64_t member_cutoff, int64_t max_clusters, int64_t *labels, unsigned long long *counters,
+                          unsigned long long *tstats);
+void fill_u64(cnn_ctx *ctx, unsigned long long *p, unsigned long long v, int64_t n);
 
 }  // namespace cnn
 

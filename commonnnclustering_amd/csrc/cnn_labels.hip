@@ -86,7 +86,36 @@ struct LabelArgs {
     const int64_t *wptr;
     const int32_t *wtile;
     const uint64_t *wword;
+    // per-row degrees (nullable: row_ptr differences); word-only passes need no row_ptr
+    const int32_t *deg;
+    // several cutoff pairs in one launch, one per blockIdx.y (nullable: one
+    // pair, c_sim / m_core): launch y takes csim_y[y] / mcore_y[y] and its own
+    // slices of the per-point (stride n), per-tile (stride tiles) and
+    // statistics arrays (see slice)
+    const int32_t *csim_y;
+    const int32_t *mcore_y;
+    int64_t tiles;
 };
+
+// The cutoff pair of this launch's blockIdx.y and its slices of the
+// per-pair arrays; unchanged for a single pair.
+__device__ __forceinline__ LabelArgs slice(LabelArgs a) {
+    if (!a.csim_y) return a;
+    const int64_t y = blockIdx.y;
+    a.core += y * a.n;
+    a.parent += y * a.n;
+    a.rank += y * a.n;
+    a.stats += y * 4 * kStatSlots;
+    if (a.cand_max) a.cand_max += y * a.tiles;
+    if (a.tile_rank) a.tile_rank += y * a.tiles;
+    a.c_sim = a.csim_y[y];
+    a.m_core = a.mcore_y[y];
+    return a;
+}
+
+__device__ __forceinline__ int64_t deg_of(const LabelArgs &a, int64_t w) {
+    return a.deg ? (int64_t)a.deg[w] : a.row_ptr[w + 1] - a.row_ptr[w];
+}
 
 // Row w's words: its row of the candidate tile pairs of its row tile (spatial
 // graph masks), or its own word list.
@@ -127,9 +156,10 @@ __device__ __forceinline__ void add_stats(const LabelArgs &a, unsigned tests, un
 }
 
 __global__ void k_init(LabelArgs a, const uint8_t *__restrict__ consider, const int32_t *__restrict__ perm) {
+    a = slice(a);
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n) return;
-    const int64_t deg = a.row_ptr[w + 1] - a.row_ptr[w];
+    const int64_t deg = deg_of(a, w);
     const int m_core = a.tile_mcore ? a.tile_mcore[w >> 6] : a.m_core;
     const int32_t o = perm ? perm[w] : (int32_t)w;  // -1: padding row of a packed batch
     bool ok = deg > (int64_t)m_core && o >= 0;
@@ -161,7 +191,7 @@ __device__ __forceinline__ bool test_and_link_words(const LabelArgs &a, int w, i
     const int rw = uf_find(a.parent, w);
     const int rj = uf_find(a.parent, j);
     if (rw == rj) return true;
-    const int64_t dj = a.row_ptr[j + 1] - a.row_ptr[j];
+    const int64_t dj = deg_of(a, j);
     const bool pass = dw >= c && dj >= c && wave_sim_words(A, word_row(a, j), c);
     ++tests;
     if (pass) {
@@ -184,7 +214,7 @@ __device__ __forceinline__ uint64_t lane_bit_mask(int lane) { return 1ull << lan
 // was already made (k_hook_tile).
 __device__ __forceinline__ void hook_row_words(const LabelArgs &a, int64_t w, int max_tests, bool skip_prev) {
     const int lane = threadIdx.x & 63;
-    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
+    const int64_t dw = deg_of(a, w);
     const bool tested_prev = skip_prev;
     // the row's words from its own tile down, 64 tiles a step (lane l holds
     // the l-th tile below the window's top), bits from the highest down:
@@ -227,8 +257,8 @@ __device__ __forceinline__ void hook_row_words(const LabelArgs &a, int64_t w, in
                 if (tested_prev && jj == (int)w - 1) continue;
                 ++attempts;
                 ++tests;
-                const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
-                if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
+                const int64_t dj = deg_of(a, jj);
+                if (dw >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
                     ++passed;
                     if (lane == 0) a.parent[w] = jj;
                     done = true;
@@ -242,6 +272,7 @@ __device__ __forceinline__ void hook_row_words(const LabelArgs &a, int64_t w, in
 
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
+    a = slice(a);
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w]) return;
@@ -303,6 +334,9 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
 // rows, to w - 1: a forest, as the per-row hooking.
 __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles, int32_t *__restrict__ queue,
                                                    unsigned int *__restrict__ queue_len) {
+    a = slice(a);
+    queue += (int64_t)blockIdx.y * a.n;
+    queue_len += blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -370,6 +404,9 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
 // per row).
 __global__ __launch_bounds__(256) void k_hook_queue(LabelArgs a, const int32_t *__restrict__ queue,
                                                     const unsigned int *__restrict__ queue_len, int max_tests) {
+    a = slice(a);
+    queue += (int64_t)blockIdx.y * a.n;
+    queue_len += blockIdx.y;
     const unsigned nq = *queue_len;
     const int64_t waves = (int64_t)gridDim.x * 4;
     for (int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); q < (int64_t)nq; q += waves) {
@@ -384,13 +421,14 @@ __global__ __launch_bounds__(256) void k_hook_queue(LabelArgs a, const int32_t *
 // entry and only to smaller values, so the parents stay a forest.
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) {
+    a = slice(a);
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n || !a.core[w] || a.parent[w] != (int)w) return;
-    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     unsigned tests = 0, passed = 0;
     int attempts = 0;
     if constexpr (WORDS) {
+        const int64_t dw = deg_of(a, w);
         const WordRow A = word_row(a, w);
         const int c = csim_of(a, w);
         bool done = false;
@@ -420,8 +458,8 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
                     const int rr = __shfl(rj, src2, 64);
                     ++attempts;
                     ++tests;
-                    const int64_t dj = a.row_ptr[jj + 1] - a.row_ptr[jj];
-                    if (e - b >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
+                    const int64_t dj = deg_of(a, jj);
+                    if (dw >= c && dj >= c && wave_sim_words(A, word_row(a, jj), c)) {
                         ++passed;
                         if (lane == 0) a.parent[w] = rr;
                         done = true;
@@ -434,6 +472,7 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
         add_stats(a, tests, passed);
         return;
     }
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     for (int64_t t0 = b; t0 < e && attempts < max_tests; t0 += 64) {
         const int64_t t = t0 + lane;
         const int j = t < e ? a.col[t] : -1;
@@ -470,13 +509,13 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
 // which test_and_link's find() check discards.
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
+    a = slice(a);
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= a.n) return;
     const uint64_t rw = a.rank[w];
     if (rw == 0) return;
     if (a.cand_max && rw >= a.cand_max[w >> 6]) return;
-    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
@@ -516,7 +555,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
                     bits &= bits - 1;
                     const int jj = __builtin_amdgcn_readlane(j, jsrc);
                     const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
-                    if (test_and_link_words(a, (int)w, jj, A, e - b, c, tests, passed)) {
+                    if (test_and_link_words(a, (int)w, jj, A, deg_of(a, w), c, tests, passed)) {
                         resolved = kjj;
                         bits &= ~__ballot(kj == kjj);
                     }
@@ -528,6 +567,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     }
     // four 64-entry windows per step: their loads and rank lookups are issued
     // together, so a row's scan waits on one memory round trip per 256 entries
+    const int64_t b = a.row_ptr[w], e = a.row_ptr[w + 1];
     constexpr int kWin = 4;
     for (int64_t t0 = b; t0 < e; t0 += 64 * kWin) {
         int j[kWin];
@@ -572,6 +612,9 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
 // non-core point has rank 0), all ones when they differ.
 __global__ void k_tile_rank_max(const uint64_t *__restrict__ rank, int64_t n, int64_t n_tiles,
                                 uint64_t *__restrict__ tmax, uint64_t *__restrict__ tcommon) {
+    rank += (int64_t)blockIdx.y * n;  // cutoff pair blockIdx.y (one pair: 0)
+    tmax += (int64_t)blockIdx.y * n_tiles;
+    tcommon += (int64_t)blockIdx.y * n_tiles;
     const int lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= n_tiles) return;
@@ -594,6 +637,8 @@ __global__ void k_tile_rank_max(const uint64_t *__restrict__ rank, int64_t n, in
 // Largest rank over the candidate column tiles of each row tile.
 __global__ void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__restrict__ tp_ptr,
                            const int32_t *__restrict__ tp_col, int64_t n_tiles, uint64_t *__restrict__ cmax) {
+    tmax += (int64_t)blockIdx.y * n_tiles;
+    cmax += (int64_t)blockIdx.y * n_tiles;
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -613,6 +658,11 @@ __global__ void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__r
 }
 
 __global__ void k_snapshot(const uint8_t *core, const int *parent, const int32_t *cnt, int64_t n, uint64_t *rank) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    core += off;
+    parent += off;
+    cnt += off;
+    rank += off;
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     const int r = parent[w];
@@ -624,6 +674,7 @@ __global__ void k_snapshot(const uint8_t *core, const int *parent, const int32_t
 // roots) and each thread writes only its own entry, so no write can leave an
 // entry below its root.
 __global__ void k_compress(int *parent, int64_t n) {
+    parent += (int64_t)blockIdx.y * n;
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     int x = (int)w;
@@ -704,6 +755,10 @@ __device__ __forceinline__ int64_t run_base() {
 
 __global__ __launch_bounds__(256) void k_root_count(const uint8_t *core, const int *parent, int64_t n,
                                                     int32_t *cnt) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    core += off;
+    parent += off;
+    cnt += off;
     const int64_t base = run_base();
     KeyRun run;
     for (int it = 0; it < kRunSteps; ++it) {
@@ -725,6 +780,12 @@ __global__ void k_count_roots(const int32_t *cnt, int64_t n, unsigned long long 
 __global__ __launch_bounds__(256) void k_comp_stats(const uint8_t *core, const int *parent, const int32_t *perm,
                                                     int64_t n, int32_t *cmin, int32_t *csize,
                                                     unsigned long long *counters) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    core += off;
+    parent += off;
+    cmin += off;
+    csize += off;
+    counters += 8 * blockIdx.y;
     const int64_t base = run_base();
     KeyRun run;
     unsigned long long nc = 0, nr = 0;
@@ -814,6 +875,11 @@ __global__ void k_rank_clusters_packed(const uint64_t *keys, const int32_t *vals
 
 __global__ void k_write_sorted(const uint8_t *core, const int *parent, const int32_t *newlab, const int32_t *perm,
                                int64_t n, int64_t *labels) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    core += off;
+    parent += off;
+    newlab += off;
+    labels += off;
     int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
     const int64_t o = perm ? perm[w] : w;
@@ -1137,6 +1203,120 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
     r.root_passes = g->tune.root_passes;
     r.words_min_fill = g->tune.words_min_fill;
     return r;
+}
+
+// ------------------------------------------------------------------ several cutoff pairs, no host synchronisation
+
+// sort_by_size of every cutoff pair without a library sort (a captured graph
+// may hold no memset node, and library sorts record some): the clusters of
+// pair y (roots of components with at least member_cutoff points) are
+// collected with their keys (size descending, smallest original index
+// ascending; all distinct), and each cluster's rank is the number of keys
+// below its own, counted against LDS-staged chunks of the pair's keys --
+// K^2 / 256 comparisons per block for K clusters.
+__global__ void k_collect_multi(const uint8_t *core, const int *parent, const int32_t *cmin, const int32_t *csize,
+                                int64_t n, int64_t member_cutoff, uint64_t *ckey, int32_t *cidx,
+                                unsigned int *ccount) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t y = blockIdx.y, off = y * n;
+    const bool cluster = w < n && core[off + w] && parent[off + w] == (int)w &&
+                         (int64_t)csize[off + w] >= member_cutoff;
+    const uint64_t b = __ballot(cluster);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&ccount[y], (unsigned)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (cluster) {
+        const unsigned pos = base + (unsigned)__popcll(b & ((1ull << lane) - 1ull));
+        ckey[off + pos] = ((uint64_t)(uint32_t)(0x7fffffff - csize[off + w]) << 32) | (uint32_t)cmin[off + w];
+        cidx[off + pos] = (int32_t)w;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rank_count(const uint64_t *ckey, const int32_t *cidx,
+                                                    const unsigned int *ccount, int64_t n, int64_t max_clusters,
+                                                    int32_t *newlab, unsigned long long *counters) {
+    __shared__ uint64_t chunk[256];
+    const int64_t y = blockIdx.y, off = y * n;
+    const int64_t K = ccount[y];
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if ((int64_t)blockIdx.x * 256 >= K) return;  // whole block: uniform
+    const uint64_t mine = x < K ? ckey[off + x] : ~0ull;
+    int64_t rank = 0;
+    for (int64_t base = 0; base < K; base += 256) {
+        chunk[threadIdx.x] = base + threadIdx.x < K ? ckey[off + base + threadIdx.x] : ~0ull;
+        __syncthreads();
+        const int m = (int)min<int64_t>(256, K - base);
+        for (int t = 0; t < m; ++t) rank += chunk[t] < mine ? 1 : 0;
+        __syncthreads();
+    }
+    const bool keep = x < K && (max_clusters < 0 || rank < max_clusters);
+    if (keep) newlab[off + cidx[off + x]] = (int32_t)(rank + 1);
+    const uint64_t b = __ballot(keep);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&counters[8 * y + 7], (unsigned long long)__popcll(b));
+}
+
+void multi_labels_bytes(int64_t n, int64_t n_tiles, int n_c, MultiLabelSizes &sz) {
+    sz.per_point = n * n_c;
+    sz.per_tile = std::max<int64_t>(n_tiles, 1) * n_c;
+}
+
+void labels_enqueue_multi(cnn_ctx *ctx, const MultiLabelBufs &b, const MultiLabelSource &src, int root_passes,
+                          int64_t member_cutoff, int64_t max_clusters, int64_t *labels, unsigned long long *counters,
+                          unsigned long long *tstats) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = b.n, T = b.n_tiles;
+    const unsigned nc = (unsigned)b.n_c;
+    if (n == 0 || nc == 0) return;
+    fill_u64(ctx, counters, 0, 8 * (int64_t)nc);
+    fill_u64(ctx, tstats, 0, 4 * (int64_t)kStatSlots * nc);
+    fill_i32(ctx, (int32_t *)b.hq_len, 0, nc);
+    fill_i32(ctx, (int32_t *)b.ccount, 0, nc);
+    fill_i32(ctx, b.cnt, 0, n * nc);
+    fill_i32(ctx, b.cmin, INT_MAX, n * nc);
+    fill_i32(ctx, b.csize, 0, n * nc);
+    fill_i32(ctx, b.newlab, 0, n * nc);
+    LabelArgs a{};
+    a.n = n;
+    a.core = b.core;
+    a.parent = b.parent;
+    a.stats = tstats;
+    a.rank = b.rank;
+    a.mask = src.mask;
+    a.pair_nz = src.pair_nz;
+    a.tp_ptr = src.tp_ptr;
+    a.tp_col = src.tp_col;
+    a.deg = src.deg;
+    a.csim_y = b.csim;
+    a.mcore_y = b.mcore;
+    a.tiles = T;
+    auto g2 = [&](int64_t items, int per) { return dim3(blocks_for(items, per), nc); };
+    k_init<<<g2(n, 256), 256, 0, s>>>(a, nullptr, src.perm);
+    k_hook_tile<<<g2(T, 4), 256, 0, s>>>(a, T, b.hq, b.hq_len);
+    k_hook_queue<<<dim3((unsigned)std::min<int64_t>(blocks_for(n, 4), 4096), nc), 256, 0, s>>>(a, b.hq, b.hq_len,
+                                                                                                   2);
+    k_compress<<<g2(n, 256), 256, 0, s>>>(b.parent, n);
+    for (int pass = 0; pass < root_passes; ++pass) {
+        k_hook_roots<true><<<g2(n, 4), 256, 0, s>>>(a, 4);
+        k_compress<<<g2(n, 256), 256, 0, s>>>(b.parent, n);
+    }
+    CNN_HIP(hipGetLastError());
+    k_root_count<<<g2(n, 256 * kRunSteps), 256, 0, s>>>(b.core, b.parent, n, b.cnt);
+    k_snapshot<<<g2(n, 256), 256, 0, s>>>(b.core, b.parent, b.cnt, n, b.rank);
+    k_tile_rank_max<<<g2(T, 4), 256, 0, s>>>(b.rank, n, T, b.tmax, b.tile_common);
+    k_cand_max<<<g2(T, 4), 256, 0, s>>>(b.tmax, src.tp_ptr, src.tp_col, T, b.cand_max);
+    CNN_HIP(hipGetLastError());
+    a.cand_max = b.cand_max;
+    a.tile_rank = b.tile_common;
+    k_round_b<true><<<g2(n, 4), 256, 0, s>>>(a);
+    k_compress<<<g2(n, 256), 256, 0, s>>>(b.parent, n);
+    k_comp_stats<<<g2(n, 256 * kRunSteps), 256, 0, s>>>(b.core, b.parent, src.perm, n, b.cmin, b.csize, counters);
+    k_collect_multi<<<g2(n, 256), 256, 0, s>>>(b.core, b.parent, b.cmin, b.csize, n, member_cutoff, b.ckey,
+                                               b.cidx, b.ccount);
+    k_rank_count<<<g2(n, 256), 256, 0, s>>>(b.ckey, b.cidx, b.ccount, n, max_clusters, b.newlab, counters);
+    k_write_sorted<<<g2(n, 256), 256, 0, s>>>(b.core, b.parent, b.newlab, src.perm, n, labels);
+    CNN_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ caller CSR validation

@@ -2718,6 +2718,67 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     return std::exchange(guard.g, nullptr);
 }
 
+// Row words at a smaller radius from row words of the same spatial graph's
+// tile pairs (the graph's own, or an earlier filter's): every set bit is
+// re-evaluated exactly in FP64 in reference order, so the result is the graph
+// re-thresholded at r2 (N_r'(i) is a subset of N_r(i) for r' <= r) in the same
+// pair-major layout.  One wave per tile pair (its row tile found by a binary
+// search over tp_ptr), lane = row; degrees are added to deg (zeroed first).
+__global__ __launch_bounds__(256) void k_word_filter(const double *__restrict__ xw64, int d, int64_t n_tiles,
+                                                     int64_t n_pairs, const int64_t *__restrict__ tp_ptr,
+                                                     const int32_t *__restrict__ tp_col,
+                                                     const uint64_t *__restrict__ src, const uint8_t *__restrict__ src_nz,
+                                                     double r2, uint64_t *__restrict__ dst, uint8_t *__restrict__ dst_nz,
+                                                     int32_t *__restrict__ deg) {
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= n_pairs) return;
+    if (src_nz[k] == 0) {
+        if (lane == 0) dst_nz[k] = 0;
+        return;
+    }
+    int64_t lo = 0, hi = n_tiles;  // row tile R: tp_ptr[R] <= k < tp_ptr[R + 1]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (tp_ptr[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    const int64_t R = lo, C = tp_col[k];
+    const int64_t w = R * 64 + lane;
+    uint64_t v = src[k * 64 + lane], keep = 0ull;
+    while (v) {
+        const int b = __ffsll((unsigned long long)v) - 1;
+        v &= v - 1;
+        if (exact_d2(xw64, d, w, C * 64 + b) <= r2) keep |= 1ull << b;
+    }
+    const bool any = __ballot(keep != 0ull) != 0ull;
+    if (lane == 0) dst_nz[k] = any ? 1 : 0;
+    if (any) dst[k * 64 + lane] = keep;
+    const int c = __popcll(keep);
+    if (c) atomicAdd(&deg[w], c);
+}
+
+// Row degrees of a graph from its row offsets.
+__global__ void k_deg_of_rows(const int64_t *__restrict__ row_ptr, int64_t n, int32_t *__restrict__ deg) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < n) deg[w] = (int32_t)(row_ptr[w + 1] - row_ptr[w]);
+}
+
+void words_filter(cnn_ctx *ctx, const cnn_graph *g, const uint64_t *src, const uint8_t *src_nz, double r2,
+                  uint64_t *dst, uint8_t *dst_nz, int32_t *deg) {
+    hipStream_t s = ctx->stream;
+    fill_i32(ctx, deg, 0, g->n);
+    if (g->n_tile_pairs > 0)
+        k_word_filter<<<(unsigned)ceil_div(g->n_tile_pairs, 4), 256, 0, s>>>(
+            g->xw64, g->d, g->n_tiles, g->n_tile_pairs, g->tp_ptr, g->tp_col, src, src_nz, r2, dst, dst_nz, deg);
+    CNN_HIP(hipGetLastError());
+}
+
+void words_degrees(cnn_ctx *ctx, const cnn_graph *g, int32_t *deg) {
+    if (g->n > 0) k_deg_of_rows<<<(unsigned)ceil_div(g->n, 256), 256, 0, ctx->stream>>>(g->row_ptr, g->n, deg);
+    CNN_HIP(hipGetLastError());
+}
+
 // Neighbour lists at r2 from lists at a larger radius (rows of the working
 // order): each entry is re-evaluated exactly in FP64.  out_rp has n+1
 // entries; out_col is allocated here (hipMalloc) and owned by the caller.

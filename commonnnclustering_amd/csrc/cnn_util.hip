@@ -86,6 +86,7 @@ static void fill(cnn_ctx *ctx, T *p, T v, int64_t n) {
 void fill_i32(cnn_ctx *ctx, int32_t *p, int32_t v, int64_t n) { fill(ctx, p, v, n); }
 void fill_u8(cnn_ctx *ctx, uint8_t *p, uint8_t v, int64_t n) { fill(ctx, p, v, n); }
 void fill_i64(cnn_ctx *ctx, int64_t *p, int64_t v, int64_t n) { fill(ctx, p, v, n); }
+void fill_u64(cnn_ctx *ctx, unsigned long long *p, unsigned long long v, int64_t n) { fill(ctx, p, v, n); }
 
 // ------------------------------------------------------------------ scans / sorts
 void exclusive_scan_i32_to_i64(cnn_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {

@@ -18,6 +18,7 @@ from ._lib import GraphStats, LabelStats, get_context, ptr
 
 __all__ = [
     "NeighbourGraph",
+    "ScanGraph",
     "build_graph",
     "radius_csr",
     "labels_from_csr",
@@ -145,9 +146,68 @@ class NeighbourGraph:
                                            ptr(labels), ctypes.byref(st)), "cnn_graph_fit")
         return labels, st.to_dict()
 
+    def scan(self, radii, cnn_cutoffs, member_cutoff=None, max_clusters=None, labels=None) -> "ScanGraph":
+        """Records the (r, c) scan of this graph into a hipGraph (cnn_graph_scan_create):
+        radii (each <= the graph's radius) and cnn cutoffs c; see ScanGraph."""
+        return ScanGraph(self, radii, cnn_cutoffs, member_cutoff, max_clusters, labels)
+
     def release(self) -> None:
         if self._handle is not None:
             _lib.lib().cnn_graph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class ScanGraph:
+    """A parameter scan of a resident graph recorded once into a hipGraph.
+
+    ``launch()`` replays it: labels[ri, ci] are the labels of a fresh fit at
+    radius radii[ri] and cnn_cutoff cnn_cutoffs[ci] (c_sim = c + 2,
+    m_core = c + 1), sorted by size -- each radius's row words re-thresholded
+    exactly in FP64, every labelling kernel launched once per radius for all
+    cutoffs.  The graph must outlive the scan.  Needs a graph that keeps its
+    row words (CNNCapacityError otherwise)."""
+
+    def __init__(self, graph: "NeighbourGraph", radii, cnn_cutoffs, member_cutoff=None, max_clusters=None,
+                 labels=None):
+        self.graph = graph
+        ctx = get_context(graph.device)
+        self._ctx = ctx
+        r2 = np.ascontiguousarray([float(r) * float(r) for r in radii], dtype=np.float64)
+        cs = np.asarray(list(cnn_cutoffs), dtype=np.int64)
+        c_sim = np.ascontiguousarray(cs + 2, dtype=np.int32)
+        m_core = np.ascontiguousarray(cs + 1, dtype=np.int32)
+        self.shape = (len(r2), len(cs), graph.n)
+        if labels is None:
+            labels = torch.zeros(self.shape, dtype=torch.int64, device=graph.device)
+        elif tuple(labels.shape) != self.shape or labels.dtype != torch.int64 or not labels.is_contiguous():
+            raise ValueError(f"labels must be a contiguous int64 tensor of shape {self.shape}")
+        self.labels = labels
+        h = ctypes.c_void_p()
+        ctx.check(_lib.lib().cnn_graph_scan_create(ctx.handle, graph.handle, r2.ctypes.data, len(r2),
+                                                   c_sim.ctypes.data, m_core.ctypes.data, len(cs),
+                                                   _member_cutoff(member_cutoff), _max_clusters(max_clusters),
+                                                   ptr(labels), ctypes.byref(h)), "cnn_graph_scan_create")
+        self._handle = h
+
+    def launch(self):
+        """Replays the scan; returns (labels [n_r, n_c, n], label stats per pair)."""
+        if self._handle is None:
+            raise RuntimeError("scan was released")
+        n_pairs = self.shape[0] * self.shape[1]
+        ls = (LabelStats * n_pairs)()
+        self._ctx.check(_lib.lib().cnn_scan_launch(self._ctx.handle, self._handle, ctypes.cast(ls, ctypes.c_void_p)),
+                        "cnn_scan_launch")
+        return self.labels, [ls[i].to_dict() for i in range(n_pairs)]
+
+    def release(self) -> None:
+        if self._handle is not None:
+            _lib.lib().cnn_scan_destroy(self._handle)
             self._handle = None
 
     def __del__(self):

@@ -240,6 +240,25 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
                  int32_t n_c, int32_t precision, int64_t member_cutoff, int64_t max_clusters,
                  int64_t *labels, cnn_graph_stats *gstats, cnn_label_stats *lstats);
 
+/* A parameter scan of a resident graph recorded once into a hipGraph and
+ * replayed with one launch: for every r2[ri] (<= the graph's radius) and
+ * every cutoff pair ci, the labels of a fresh fit, sorted by size, in block
+ * (ri * n_c + ci) * n of labels (device int64).  Per radius the graph's row
+ * words are re-thresholded exactly in FP64; each labelling kernel runs once
+ * per radius for all cutoff pairs.  Needs a graph that keeps its row words
+ * (spatial graphs of dense enough lists: CNN_ERR_CAPACITY otherwise; use
+ * cnn_fit_scan, which falls back to lists).  The scan holds its buffers until
+ * cnn_scan_destroy; the graph and labels must outlive it.  lstats (nullable):
+ * HOST array of n_r * n_c entries.  Replaces, like cnn_fit_scan, a loop of
+ * Clustering.fit calls or HierarchicalFitterRepeat._fit steps
+ * (_fit.pyx:1058-1187). */
+typedef struct cnn_scan cnn_scan;
+int cnn_graph_scan_create(cnn_ctx *ctx, const cnn_graph *g, const double *r2, int32_t n_r, const int32_t *c_sim,
+                          const int32_t *m_core, int32_t n_c, int64_t member_cutoff, int64_t max_clusters,
+                          int64_t *labels, cnn_scan **out);
+int cnn_scan_launch(cnn_ctx *ctx, cnn_scan *scan, cnn_label_stats *lstats);
+int cnn_scan_destroy(cnn_scan *scan);
+
 /* ------------------------------------------------------------------ predict */
 /* Label prediction for a second point set, replacing PredictorFirstmatch.predict
  * (_fit.pyx:1241-1291) as called by Clustering.predict (cluster.pyx:560-653)
