@@ -470,10 +470,22 @@ def scan_bench(args):
                 engine.fit(Xd, float(r) * float(r), c + 2, c + 1, precision="f32_band")
 
     t_ind = _timed(independent, 1, 1)
-    log(f"scan: {t * 1e3:.2f} ms for {pairs} (r, c) pairs; independent fits {t_ind * 1e3:.2f} ms")
+    # the scan recorded once into a hipGraph on a resident graph, then replayed
+    g = engine.build_graph(Xd, float(max(radii)) ** 2, "f32_band")
+    t0 = time.perf_counter()
+    sc = g.scan(radii, cs)
+    t_capture = time.perf_counter() - t0
+    t_replay = _timed(sc.launch, args.steps, 1)
+    sc.release()
+    g.release()
+    log(f"scan: {t * 1e3:.2f} ms for {pairs} (r, c) pairs (graph build, capture, one launch); replay of the "
+        f"captured scan {t_replay * 1e3:.2f} ms (capture {t_capture * 1e3:.1f} ms); independent fits "
+        f"{t_ind * 1e3:.2f} ms")
     return _line(n * pairs / t, t * 1e3, args.steps, args.warmup,
                  {"workload": "scan_2d", "n": n, "d": 2, "radii": [float(r) for r in radii], "cnn_cutoffs": cs},
-                 {"independent_fits_ms": t_ind * 1e3, "scan_vs_independent": t_ind / t})
+                 {"independent_fits_ms": t_ind * 1e3, "scan_vs_independent": t_ind / t,
+                  "captured_replay_ms": t_replay * 1e3, "capture_ms": t_capture * 1e3,
+                  "replay_points_per_s": n * pairs / t_replay})
 
 
 def batch_bench(args):
