@@ -619,25 +619,20 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
 
 // The full round on a spatial graph's tile-pair words, in three steps.
 // k_round_tile: one wave per row tile, lane = row, so every pair's 64 row
-// words load as one coalesced 512-byte read (8 pairs' words in flight at a
-// time).  A row owns the edges to points of higher snapshot rank.  Rows of
-// one snapshot component are connected already, so for a candidate tile
-// holding points of a higher component K (one component at a time for tiles
-// whose points' ranks differ) the rows of the tile that belong to one lower
-// component and have a neighbour in K form one group, queued once as
-// (tile pair, K, rows); each row skips K in later tiles once queued.
-// k_round_test: one wave per entry tests the group's rows against their
-// neighbours of K in that tile, in order, until one edge joins the two
-// components (or they are found joined already).  When every test of a group
-// fails, or the queue is full, its rows go to a fallback list that
-// k_round_rows walks completely (round_row_words).  Every owned edge is thus
-// tested, or its endpoints are shown to be connected, as in k_round_b.
+// words load as one coalesced 512-byte read.  A row owns the edges to points
+// of higher snapshot rank; for each candidate tile holding such points (one
+// component at a time for tiles whose points' ranks differ) the row queues
+// (row, tile pair, component) once per run of that component.  k_round_test:
+// one wave per queued entry tests the row against that component's
+// neighbours in the tile until one joins them (or they are joined already).
+// Rows whose tests all fail, or that find the queue full, go to a fallback
+// list, and k_round_rows walks those rows completely (round_row_words).
+// Every owned edge is thus tested or shown redundant, as in k_round_b.
 struct RoundEntry {
-    int32_t R;      // row tile
-    int32_t k;      // tile pair (candidate column tile) of the row tile
-    int32_t comp;   // root of the higher snapshot component
+    int32_t w;
+    int32_t k;      // tile pair (candidate column tile) of the row's tile
+    int32_t comp;   // root of the snapshot component
     int32_t pad;
-    uint64_t rows;  // rows of R in the group (bit r = row 64 R + r)
 };
 
 __device__ __forceinline__ void round_fallback(int64_t w, int32_t *fb, unsigned int *fb_len, uint32_t *fb_flag) {
@@ -673,7 +668,7 @@ __global__ __launch_bounds__(256) void k_round_tile(LabelArgs a, RoundEntry *__r
         const uint64_t o = __shfl_xor(minr, off, 64);
         minr = o < minr ? o : minr;
     }
-    int resolved = -1;  // last higher component this row was queued for
+    int resolved = -1;
     const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
     for (int64_t k0 = kb; k0 < ke; k0 += 64) {
         const int64_t kk = k0 + lane;
@@ -682,60 +677,43 @@ __global__ __launch_bounds__(256) void k_round_tile(LabelArgs a, RoundEntry *__r
         const uint64_t trl = nzl ? a.tile_rank[Cl] : 0ull;
         uint64_t im = __ballot(nzl && (trl == ~0ull || trl > minr));
         while (im) {
-            // up to 8 interesting pairs: their words (and point ranks) loaded together
-            int src[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                src[u] = im ? __ffsll((unsigned long long)im) - 1 : -1;
-                im &= im - 1;
+            const int src = __ffsll((unsigned long long)im) - 1;
+            im &= im - 1;
+            const int64_t k = k0 + src;
+            const int C = __builtin_amdgcn_readlane(Cl, src);
+            const uint64_t tr = readlane64(trl, src);
+            const uint64_t v = a.mask[k * 64 + lane];
+            uint64_t pending, rC = 0ull;
+            if (tr != ~0ull) {
+                pending = 1ull;  // one component: the tile's rank
+            } else {
+                const int64_t j = (int64_t)C * 64 + lane;
+                rC = j < a.n ? a.rank[j] : 0ull;
+                pending = __ballot(rC != 0ull);
             }
-            uint64_t v[8], rC[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                v[u] = 0ull;
-                rC[u] = 0ull;
-                if (src[u] >= 0) {
-                    v[u] = a.mask[(k0 + src[u]) * 64 + lane];
-                    const int C = __builtin_amdgcn_readlane(Cl, src[u]);
-                    const int64_t j = (int64_t)C * 64 + lane;
-                    if (readlane64(trl, src[u]) == ~0ull && j < a.n) rC[u] = a.rank[j];
+            while (pending) {
+                uint64_t K, bm;
+                if (tr != ~0ull) {
+                    K = tr;
+                    bm = ~0ull;
+                    pending = 0ull;
+                } else {
+                    K = readlane64(rC, __ffsll((unsigned long long)pending) - 1);
+                    bm = __ballot(rC == K);
+                    pending &= ~bm;
                 }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (src[u] < 0) continue;
-                const uint64_t tr = readlane64(trl, src[u]);
-                uint64_t pending = tr != ~0ull ? 1ull : __ballot(rC[u] != 0ull);
-                while (pending) {
-                    uint64_t K, bm;
-                    if (tr != ~0ull) {
-                        K = tr;
-                        bm = ~0ull;
-                        pending = 0ull;
-                    } else {
-                        K = readlane64(rC[u], __ffsll((unsigned long long)pending) - 1);
-                        bm = __ballot(rC[u] == K);
-                        pending &= ~bm;
-                    }
-                    const int kc = (int)(uint32_t)K;
-                    const bool cand = active && (v[u] & bm) != 0ull && K > rw && kc != resolved;
-                    uint64_t cm = __ballot(cand);
-                    if (cand) resolved = kc;
-                    while (cm) {
-                        // one entry per lower component among the candidate rows
-                        const uint64_t rl = readlane64(rw, __ffsll((unsigned long long)cm) - 1);
-                        const uint64_t gm = __ballot(cand && rw == rl);
-                        cm &= ~gm;
-                        unsigned pos = 0;
-                        if (lane == 0) pos = atomicAdd(qlen, 1u);
-                        pos = __shfl(pos, 0, 64);
-                        if ((int64_t)pos < qcap) {
-                            if (lane == 0)
-                                queue[pos] = RoundEntry{(int32_t)R, (int32_t)(k0 + src[u]), kc, 0, gm};
-                        } else if ((gm >> lane) & 1ull) {
-                            round_fallback(w, fb, fb_len, fb_flag);
-                        }
-                    }
+                const int kc = (int)(uint32_t)K;
+                const bool cand = active && (v & bm) != 0ull && K > rw && kc != resolved;
+                const uint64_t em = __ballot(cand);
+                if (!em) continue;
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(qlen, (unsigned)__popcll(em));
+                base = __shfl(base, 0, 64);
+                if (cand) {
+                    const int64_t pos = (int64_t)base + __popcll(em & ((1ull << lane) - 1ull));
+                    if (pos < qcap) queue[pos] = RoundEntry{(int32_t)w, (int32_t)k, kc, 0};
+                    else round_fallback(w, fb, fb_len, fb_flag);
+                    resolved = kc;
                 }
             }
         }
@@ -758,32 +736,25 @@ __global__ __launch_bounds__(256) void k_round_test(LabelArgs a, const RoundEntr
     unsigned tests = 0, passed = 0;
     for (int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); e < nq; e += waves) {
         const RoundEntry en = queue[e];
-        const int64_t k = en.k;
+        const int64_t w = en.w, k = en.k;
         const int C = a.tp_col[k];
+        const uint64_t v = a.mask[k * 64 + (w & 63)];
         const int64_t j = (int64_t)C * 64 + lane;
         const uint64_t rC = j < a.n ? a.rank[j] : 0ull;
-        const uint64_t bm = __ballot(rC != 0ull && (int)(uint32_t)rC == en.comp);
-        const uint64_t vl = a.mask[k * 64 + lane];  // lane r: row 64 R + r's word
-        uint64_t rows = en.rows;
+        uint64_t bits = v & __ballot(rC != 0ull && (int)(uint32_t)rC == en.comp);
+        const WordRow A = word_row(a, w);
+        const int64_t dw = deg_of(a, w);
+        const int c = csim_of(a, w);
         bool joined = false;
-        while (rows && !joined) {
-            const int r = __ffsll((unsigned long long)rows) - 1;
-            rows &= rows - 1;
-            const int64_t w = (int64_t)en.R * 64 + r;
-            uint64_t bits = readlane64(vl, r) & bm;
-            const WordRow A = word_row(a, w);
-            const int64_t dw = deg_of(a, w);
-            const int c = csim_of(a, w);
-            while (bits) {
-                const int b = __ffsll((unsigned long long)bits) - 1;
-                bits &= bits - 1;
-                if (test_and_link_words(a, (int)w, C * 64 + b, A, dw, c, tests, passed)) {
-                    joined = true;
-                    break;
-                }
+        while (bits) {
+            const int b = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            if (test_and_link_words(a, (int)w, C * 64 + b, A, dw, c, tests, passed)) {
+                joined = true;
+                break;
             }
         }
-        if (!joined && ((en.rows >> lane) & 1ull)) round_fallback((int64_t)en.R * 64 + lane, fb, fb_len, fb_flag);
+        if (!joined && lane == 0) round_fallback(w, fb, fb_len, fb_flag);
     }
     add_stats(a, tests, passed);
 }
