@@ -274,11 +274,6 @@ cnn_scan *scan_create(cnn_ctx *ctx, const cnn_graph *g, const double *r2, int32_
     b.cmin = scan_alloc<int32_t>(sc.get(), m);
     b.csize = scan_alloc<int32_t>(sc.get(), m);
     b.newlab = scan_alloc<int32_t>(sc.get(), m);
-    b.rqueue = scan_alloc<int4>(sc.get(), m);
-    b.rqlen = scan_alloc<unsigned int>(sc.get(), n_c);
-    b.rfb_len = scan_alloc<unsigned int>(sc.get(), n_c);
-    b.rfb = scan_alloc<int32_t>(sc.get(), m);
-    b.rfb_flag = scan_alloc<uint32_t>(sc.get(), m);
     b.ckey = scan_alloc<uint64_t>(sc.get(), m);
     b.cidx = scan_alloc<int32_t>(sc.get(), m);
     b.ccount = scan_alloc<unsigned int>(sc.get(), n_c);

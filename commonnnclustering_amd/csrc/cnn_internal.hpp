@@ -342,11 +342,6 @@ struct MultiLabelBufs {
     uint64_t *ckey = nullptr;
     int32_t *cidx = nullptr;
     unsigned int *ccount = nullptr;                    // n_c
-    // the tile-wise full round: queue of 16-byte entries (n per pair), fallback rows and flags
-    void *rqueue = nullptr;
-    unsigned int *rqlen = nullptr, *rfb_len = nullptr;  // n_c each
-    int32_t *rfb = nullptr;
-    uint32_t *rfb_flag = nullptr;
     const int32_t *csim = nullptr, *mcore = nullptr;  // device, n_c each
 };
 struct MultiLabelSizes {

@@ -507,58 +507,6 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
 // reading its list.  Candidates are filtered with the read-only snapshot (one
 // 8-byte gather per entry); a stale snapshot only yields redundant candidates,
 // which test_and_link's find() check discards.
-// Full-round edges of one row on row words (rank rw, not the largest among
-// its candidate tiles): see k_round_b.
-__device__ __forceinline__ void round_row_words(const LabelArgs &a, int64_t w, uint64_t rw) {
-    const int lane = threadIdx.x & 63;
-    const int c = csim_of(a, w);
-    unsigned tests = 0, passed = 0;
-    int resolved = -1;  // last snapshot component found joined to w
-    // The row's words, 64 tiles a step (ascending column tile, so entries
-    // come in list order): tiles whose points share a rank that is not
-    // above w's, or that w has joined, are skipped without reading their
-    // bits; tests are bitset intersections of the two rows' words.
-    const WordRow A = word_row(a, w);
-    for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
-        const int64_t t = t0 + lane;
-        uint64_t v = 0ull, tr = 0ull;
-        int C = 0;
-        if (t < A.len) v = word_at(A, t);
-        if (v) {
-            C = A.tile[t];
-            tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
-        }
-        uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
-        while (tiles) {
-            const int src = __ffsll((unsigned long long)tiles) - 1;
-            tiles &= tiles - 1;
-            const uint32_t tlo = __builtin_amdgcn_readlane((uint32_t)tr, src);
-            const uint32_t thi = __builtin_amdgcn_readlane((uint32_t)(tr >> 32), src);
-            const uint64_t trs = ((uint64_t)thi << 32) | tlo;
-            if (trs != ~0ull && (int)tlo == resolved) continue;
-            const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
-            const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
-            const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
-            const int j = __builtin_amdgcn_readlane(C, src) * 64 + lane;
-            uint64_t rj = 0ull;
-            if (bit) rj = trs == ~0ull ? a.rank[j] : trs;
-            const int kj = (int)(uint32_t)rj;
-            uint64_t bits = __ballot(bit && rj > rw && kj != resolved);
-            while (bits) {
-                const int jsrc = __ffsll((unsigned long long)bits) - 1;
-                bits &= bits - 1;
-                const int jj = __builtin_amdgcn_readlane(j, jsrc);
-                const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
-                if (test_and_link_words(a, (int)w, jj, A, deg_of(a, w), c, tests, passed)) {
-                    resolved = kjj;
-                    bits &= ~__ballot(kj == kjj);
-                }
-            }
-        }
-    }
-    add_stats(a, tests, passed, 1);
-}
-
 template <bool WORDS>
 __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     a = slice(a);
@@ -572,7 +520,49 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
     if constexpr (WORDS) {
-        round_row_words(a, w, rw);
+        // The row's words, 64 tiles a step (ascending column tile, so entries
+        // come in list order): tiles whose points share a rank that is not
+        // above w's, or that w has joined, are skipped without reading their
+        // bits; tests are bitset intersections of the two rows' words.
+        const WordRow A = word_row(a, w);
+        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+            const int64_t t = t0 + lane;
+            uint64_t v = 0ull, tr = 0ull;
+            int C = 0;
+            if (t < A.len) v = word_at(A, t);
+            if (v) {
+                C = A.tile[t];
+                tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
+            }
+            uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
+            while (tiles) {
+                const int src = __ffsll((unsigned long long)tiles) - 1;
+                tiles &= tiles - 1;
+                const uint32_t tlo = __builtin_amdgcn_readlane((uint32_t)tr, src);
+                const uint32_t thi = __builtin_amdgcn_readlane((uint32_t)(tr >> 32), src);
+                const uint64_t trs = ((uint64_t)thi << 32) | tlo;
+                if (trs != ~0ull && (int)tlo == resolved) continue;
+                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+                const int j = __builtin_amdgcn_readlane(C, src) * 64 + lane;
+                uint64_t rj = 0ull;
+                if (bit) rj = trs == ~0ull ? a.rank[j] : trs;
+                const int kj = (int)(uint32_t)rj;
+                uint64_t bits = __ballot(bit && rj > rw && kj != resolved);
+                while (bits) {
+                    const int jsrc = __ffsll((unsigned long long)bits) - 1;
+                    bits &= bits - 1;
+                    const int jj = __builtin_amdgcn_readlane(j, jsrc);
+                    const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
+                    if (test_and_link_words(a, (int)w, jj, A, deg_of(a, w), c, tests, passed)) {
+                        resolved = kjj;
+                        bits &= ~__ballot(kj == kjj);
+                    }
+                }
+            }
+        }
+        add_stats(a, tests, passed, 1);
         return;
     }
     // four 64-entry windows per step: their loads and rank lookups are issued
@@ -615,161 +605,6 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
         }
     }
     add_stats(a, tests, passed, 1);
-}
-
-// The full round on a spatial graph's tile-pair words, in three steps.
-// k_round_tile: one wave per row tile, lane = row, so every pair's 64 row
-// words load as one coalesced 512-byte read.  A row owns the edges to points
-// of higher snapshot rank; for each candidate tile holding such points (one
-// component at a time for tiles whose points' ranks differ) the row queues
-// (row, tile pair, component) once per run of that component.  k_round_test:
-// one wave per queued entry tests the row against that component's
-// neighbours in the tile until one joins them (or they are joined already).
-// Rows whose tests all fail, or that find the queue full, go to a fallback
-// list, and k_round_rows walks those rows completely (round_row_words).
-// Every owned edge is thus tested or shown redundant, as in k_round_b.
-struct RoundEntry {
-    int32_t w;
-    int32_t k;      // tile pair (candidate column tile) of the row's tile
-    int32_t comp;   // root of the snapshot component
-    int32_t pad;
-};
-
-__device__ __forceinline__ void round_fallback(int64_t w, int32_t *fb, unsigned int *fb_len, uint32_t *fb_flag) {
-    if (atomicExch(&fb_flag[w], 1u) == 0u) fb[atomicAdd(fb_len, 1u)] = (int32_t)w;
-}
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__global__ __launch_bounds__(256) void k_round_tile(LabelArgs a, RoundEntry *__restrict__ queue,
-                                                    unsigned int *__restrict__ qlen, int64_t qcap,
-                                                    int32_t *__restrict__ fb, unsigned int *__restrict__ fb_len,
-                                                    uint32_t *__restrict__ fb_flag) {
-    a = slice(a);
-    queue += (int64_t)blockIdx.y * qcap;
-    qlen += blockIdx.y;
-    fb += (int64_t)blockIdx.y * a.n;
-    fb_len += blockIdx.y;
-    fb_flag += (int64_t)blockIdx.y * a.n;
-    const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (R >= a.tiles) return;
-    const int64_t w = R * 64 + lane;
-    const uint64_t rw = w < a.n ? a.rank[w] : 0ull;
-    const bool active = rw != 0ull && rw < a.cand_max[R];
-    if (__ballot(active) == 0ull) return;
-    uint64_t minr = active ? rw : ~0ull;  // smallest rank of an active row: tiles of rank <= it own nothing here
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const uint64_t o = __shfl_xor(minr, off, 64);
-        minr = o < minr ? o : minr;
-    }
-    int resolved = -1;
-    const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
-    for (int64_t k0 = kb; k0 < ke; k0 += 64) {
-        const int64_t kk = k0 + lane;
-        const bool nzl = kk < ke && a.pair_nz[kk] != 0;
-        const int Cl = nzl ? a.tp_col[kk] : 0;
-        const uint64_t trl = nzl ? a.tile_rank[Cl] : 0ull;
-        uint64_t im = __ballot(nzl && (trl == ~0ull || trl > minr));
-        while (im) {
-            const int src = __ffsll((unsigned long long)im) - 1;
-            im &= im - 1;
-            const int64_t k = k0 + src;
-            const int C = __builtin_amdgcn_readlane(Cl, src);
-            const uint64_t tr = readlane64(trl, src);
-            const uint64_t v = a.mask[k * 64 + lane];
-            uint64_t pending, rC = 0ull;
-            if (tr != ~0ull) {
-                pending = 1ull;  // one component: the tile's rank
-            } else {
-                const int64_t j = (int64_t)C * 64 + lane;
-                rC = j < a.n ? a.rank[j] : 0ull;
-                pending = __ballot(rC != 0ull);
-            }
-            while (pending) {
-                uint64_t K, bm;
-                if (tr != ~0ull) {
-                    K = tr;
-                    bm = ~0ull;
-                    pending = 0ull;
-                } else {
-                    K = readlane64(rC, __ffsll((unsigned long long)pending) - 1);
-                    bm = __ballot(rC == K);
-                    pending &= ~bm;
-                }
-                const int kc = (int)(uint32_t)K;
-                const bool cand = active && (v & bm) != 0ull && K > rw && kc != resolved;
-                const uint64_t em = __ballot(cand);
-                if (!em) continue;
-                unsigned base = 0;
-                if (lane == 0) base = atomicAdd(qlen, (unsigned)__popcll(em));
-                base = __shfl(base, 0, 64);
-                if (cand) {
-                    const int64_t pos = (int64_t)base + __popcll(em & ((1ull << lane) - 1ull));
-                    if (pos < qcap) queue[pos] = RoundEntry{(int32_t)w, (int32_t)k, kc, 0};
-                    else round_fallback(w, fb, fb_len, fb_flag);
-                    resolved = kc;
-                }
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_round_test(LabelArgs a, const RoundEntry *__restrict__ queue,
-                                                    const unsigned int *__restrict__ qlen, int64_t qcap,
-                                                    int32_t *__restrict__ fb, unsigned int *__restrict__ fb_len,
-                                                    uint32_t *__restrict__ fb_flag) {
-    a = slice(a);
-    queue += (int64_t)blockIdx.y * qcap;
-    qlen += blockIdx.y;
-    fb += (int64_t)blockIdx.y * a.n;
-    fb_len += blockIdx.y;
-    fb_flag += (int64_t)blockIdx.y * a.n;
-    const int lane = threadIdx.x & 63;
-    const int64_t nq = min<int64_t>((int64_t)*qlen, qcap);
-    const int64_t waves = (int64_t)gridDim.x * 4;
-    unsigned tests = 0, passed = 0;
-    for (int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); e < nq; e += waves) {
-        const RoundEntry en = queue[e];
-        const int64_t w = en.w, k = en.k;
-        const int C = a.tp_col[k];
-        const uint64_t v = a.mask[k * 64 + (w & 63)];
-        const int64_t j = (int64_t)C * 64 + lane;
-        const uint64_t rC = j < a.n ? a.rank[j] : 0ull;
-        uint64_t bits = v & __ballot(rC != 0ull && (int)(uint32_t)rC == en.comp);
-        const WordRow A = word_row(a, w);
-        const int64_t dw = deg_of(a, w);
-        const int c = csim_of(a, w);
-        bool joined = false;
-        while (bits) {
-            const int b = __ffsll((unsigned long long)bits) - 1;
-            bits &= bits - 1;
-            if (test_and_link_words(a, (int)w, C * 64 + b, A, dw, c, tests, passed)) {
-                joined = true;
-                break;
-            }
-        }
-        if (!joined && lane == 0) round_fallback(w, fb, fb_len, fb_flag);
-    }
-    add_stats(a, tests, passed);
-}
-
-__global__ __launch_bounds__(256) void k_round_rows(LabelArgs a, const int32_t *__restrict__ fb,
-                                                    const unsigned int *__restrict__ fb_len) {
-    a = slice(a);
-    fb += (int64_t)blockIdx.y * a.n;
-    fb_len += blockIdx.y;
-    const int64_t nf = *fb_len;
-    const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); q < nf; q += waves) {
-        const int64_t w = fb[q];
-        round_row_words(a, w, a.rank[w]);
-    }
 }
 
 // Largest snapshot rank among the core points of each tile (0: none), and the
@@ -1069,31 +904,6 @@ __global__ void k_write_unsorted(const uint8_t *core, const int *parent, const i
     else if (write_all) labels[o] = 0;
 }
 
-// The tile-wise full round (k_round_tile, k_round_test, k_round_rows) for nc
-// cutoff pairs: queue capacity n entries per pair; the counters and flags are
-// cleared with fill kernels (a stream capture may hold no memset).
-struct RoundBufs {
-    RoundEntry *queue;
-    unsigned int *qlen;
-    int32_t *fb;
-    unsigned int *fb_len;
-    uint32_t *fb_flag;
-};
-
-static void launch_round_tiles(cnn_ctx *ctx, const LabelArgs &a, int64_t n, int64_t n_tiles, unsigned nc,
-                               const RoundBufs &rb) {
-    hipStream_t s = ctx->stream;
-    fill_i32(ctx, (int32_t *)rb.qlen, 0, nc);
-    fill_i32(ctx, (int32_t *)rb.fb_len, 0, nc);
-    fill_i32(ctx, (int32_t *)rb.fb_flag, 0, n * nc);
-    const unsigned tb = (unsigned)std::max<int64_t>(1, ceil_div(n_tiles, 4));
-    const unsigned qb = (unsigned)std::min<int64_t>(std::max<int64_t>(1, ceil_div(n, 4)), 4096);
-    k_round_tile<<<dim3(tb, nc), 256, 0, s>>>(a, rb.queue, rb.qlen, n, rb.fb, rb.fb_len, rb.fb_flag);
-    k_round_test<<<dim3(qb, nc), 256, 0, s>>>(a, rb.queue, rb.qlen, n, rb.fb, rb.fb_len, rb.fb_flag);
-    k_round_rows<<<dim3(qb, nc), 256, 0, s>>>(a, rb.fb, rb.fb_len);
-    CNN_HIP(hipGetLastError());
-}
-
 // ------------------------------------------------------------------ per-row word lists from lists
 
 // Rows of a CSR (strictly ascending) as word lists: one word per distinct
@@ -1283,18 +1093,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.tile_rank = tile_common->p;
     }
     CNN_HIP(hipEventRecord(ctx->ev[10], s));
-    if (words && a.mask && a.cand_max) {
-        Scratch<RoundEntry> rq(ctx, n);
-        Scratch<unsigned int> rlen(ctx, 2);
-        Scratch<int32_t> rfb(ctx, n);
-        Scratch<uint32_t> rflag(ctx, n);
-        a.tiles = req.n_tiles;
-        launch_round_tiles(ctx, a, n, req.n_tiles, 1, RoundBufs{rq.p, rlen.p, rfb.p, rlen.p + 1, rflag.p});
-    } else if (words) {
-        k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
-    } else {
-        k_round_b<false><<<blocks_for(n, 4), 256, 0, s>>>(a);
-    }
+    if (words) k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
+    else k_round_b<false><<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipEventRecord(ctx->ev[11], s));
     CNN_HIP(hipGetLastError());
     k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
@@ -1509,7 +1309,7 @@ void labels_enqueue_multi(cnn_ctx *ctx, const MultiLabelBufs &b, const MultiLabe
     CNN_HIP(hipGetLastError());
     a.cand_max = b.cand_max;
     a.tile_rank = b.tile_common;
-    launch_round_tiles(ctx, a, n, T, nc, RoundBufs{(RoundEntry *)b.rqueue, b.rqlen, b.rfb, b.rfb_len, b.rfb_flag});
+    k_round_b<true><<<g2(n, 4), 256, 0, s>>>(a);
     k_compress<<<g2(n, 256), 256, 0, s>>>(b.parent, n);
     k_comp_stats<<<g2(n, 256 * kRunSteps), 256, 0, s>>>(b.core, b.parent, src.perm, n, b.cmin, b.csize, counters);
     k_collect_multi<<<g2(n, 256), 256, 0, s>>>(b.core, b.parent, b.cmin, b.csize, n, member_cutoff, b.ckey,
