@@ -925,6 +925,28 @@ __global__ __launch_bounds__(256) void k_row_word_count(const int64_t *__restric
     if (lane == 0) cnt[w] = total;
 }
 
+// Entries and words of every stride-th row, summed into sums[0] / sums[1]:
+// the automatic similarity choice estimates the lists' fill from them.
+__global__ __launch_bounds__(256) void k_row_word_sample(const int64_t *__restrict__ row_ptr,
+                                                         const int32_t *__restrict__ col, int64_t n, int64_t stride,
+                                                         unsigned long long *__restrict__ sums) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * stride;
+    if (w >= n) return;
+    const int64_t b = row_ptr[w], e = row_ptr[w + 1];
+    int total = 0;
+    for (int64_t t0 = b; t0 < e; t0 += 64) {
+        const int64_t t = t0 + lane;
+        bool head = false;
+        if (t < e) head = t == b || (col[t - 1] >> 6) != (col[t] >> 6);
+        total += __popcll(__ballot(head));
+    }
+    if (lane == 0 && total) {
+        atomicAdd(&sums[0], (unsigned long long)(e - b));
+        atomicAdd(&sums[1], (unsigned long long)total);
+    }
+}
+
 // Fill: the entries of one word are consecutive in the row, so each window of
 // 64 entries ors its bits per segment (a segmented scan over the lanes, the
 // open segment of the previous window carried in) and the segment's last
@@ -1014,8 +1036,22 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.tp_ptr = req.tp_ptr;
         a.tp_col = req.tp_col;
     } else if (req.sim_mode != CNN_SIM_MERGE && req.col) {
-        wptr.reset(new Scratch<int64_t>(ctx, n + 1));
-        {
+        bool build = req.sim_mode == CNN_SIM_BITSET;
+        if (!build && req.words_min_fill > 0 && req.words_min_fill <= (int64_t)1 << 24) {
+            // automatic choice: the fill of every 16th row decides
+            constexpr int64_t kSampleStride = 16;
+            Scratch<unsigned long long> sums(ctx, 2);
+            fill_u64(ctx, sums.p, 0, 2);
+            const int64_t rows = ceil_div(n, kSampleStride);
+            k_row_word_sample<<<blocks_for(rows, 4), 256, 0, s>>>(req.row_ptr, req.col, n, kSampleStride, sums.p);
+            CNN_HIP(hipGetLastError());
+            unsigned long long h[2] = {0, 0};
+            CNN_HIP(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            CNN_HIP(hipStreamSynchronize(s));
+            build = h[1] > 0 && h[0] >= (unsigned long long)req.words_min_fill * h[1];
+        }
+        if (build) {
+            wptr.reset(new Scratch<int64_t>(ctx, n + 1));
             Scratch<int32_t> wcnt(ctx, n);
             k_row_word_count<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wcnt.p);
             CNN_HIP(hipGetLastError());
@@ -1023,23 +1059,18 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             const int64_t last_excl = read_scalar(ctx, wptr->p + (n - 1));
             const int32_t last_cnt = read_scalar(ctx, wcnt.p + (n - 1));
             const int64_t n_words = last_excl + last_cnt;
-            const int64_t nnz = read_scalar(ctx, req.row_ptr + n) - read_scalar(ctx, req.row_ptr);
-            if (req.sim_mode == CNN_SIM_BITSET || (req.words_min_fill > 0 && nnz >= req.words_min_fill * n_words)) {
-                CNN_HIP(hipMemcpyAsync(wptr->p + n, &n_words, sizeof(int64_t), hipMemcpyHostToDevice, s));
-                wtile.reset(new Scratch<int32_t>(ctx, n_words));
-                wword.reset(new Scratch<uint64_t>(ctx, n_words));
-                k_row_word_fill<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wptr->p, wtile->p,
-                                                                 wword->p);
-                CNN_HIP(hipGetLastError());
-                // the host value copied above must outlive the copy
-                CNN_HIP(hipStreamSynchronize(s));
-                words = true;
-                a.wptr = wptr->p;
-                a.wtile = wtile->p;
-                a.wword = wword->p;
-            }
+            CNN_HIP(hipMemcpyAsync(wptr->p + n, &n_words, sizeof(int64_t), hipMemcpyHostToDevice, s));
+            wtile.reset(new Scratch<int32_t>(ctx, n_words));
+            wword.reset(new Scratch<uint64_t>(ctx, n_words));
+            k_row_word_fill<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wptr->p, wtile->p, wword->p);
+            CNN_HIP(hipGetLastError());
+            // the host value copied above must outlive the copy
+            CNN_HIP(hipStreamSynchronize(s));
+            words = true;
+            a.wptr = wptr->p;
+            a.wtile = wtile->p;
+            a.wword = wword->p;
         }
-        if (!words) wptr.reset();
     }
     if (!words && !req.col) throw InvalidArg{"the sorted-list merge needs the neighbour lists"};
 

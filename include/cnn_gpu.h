@@ -75,8 +75,9 @@ enum {
  * 64-bit words over the column tiles both rows reach.  CNN_SIM_AUTO takes the
  * words when the graph keeps them (spatial graphs whose words take no more
  * memory than their lists: at least two neighbours per word on average) or,
- * for lists, when they average at least 8 entries per word (dense rows);
- * otherwise the merge. */
+ * for lists, when a sample of their rows averages at least 8 entries per word
+ * (dense rows; the threshold is tunable, cnn_tune_params); otherwise the
+ * merge. */
 enum { CNN_SIM_AUTO = 0, CNN_SIM_MERGE = 1, CNN_SIM_BITSET = 2 };
 
 typedef struct cnn_ctx cnn_ctx;
@@ -345,7 +346,8 @@ typedef struct {
     int32_t emit_waves;      /* waves per workgroup of the neighbour-list emission (4 or 8; 8) */
     int32_t root_passes;     /* root-hooking passes of the labelling before its full round (0..8; 2) */
     int32_t words_min_fill;  /* CNN_SIM_AUTO on lists: per-row word lists when the lists hold at least
-                                this many entries per word, else the merge (0..2^30; 8) */
+                                this many entries per word (estimated on every 16th row), else the
+                                merge; 0 or more than 2^24: always the merge (0..2^30; 8) */
 } cnn_tune_params;
 
 /* Process-wide table of tuned shapes.  cnn_tune_set stores p for the class
