@@ -17,10 +17,9 @@
 
 using namespace cnn;
 
-// The packed path (all datasets in one working order) is switched off: its
-// first run on the device ended in a memory fault whose cause is not yet
-// identified.  Every batch goes through the per-dataset loop below.
-constexpr bool kPackedBatches = false;
+// Batches of low-dimensional datasets (1 <= d <= 8) are fitted in one working
+// order (fit_packed); the others go through the per-dataset loop below.
+constexpr bool kPackedBatches = true;
 
 // Bits needed to hold values 0 .. v.
 static int bits_for(int64_t v) {

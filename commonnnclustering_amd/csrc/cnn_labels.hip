@@ -1141,10 +1141,9 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         Scratch<uint64_t> keys(ctx, n);
         Scratch<int32_t> vals(ctx, n), newlab(ctx, n);
         fill_i32(ctx, newlab.p, 0, n);
-        k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n, req.member_cutoff,
-                                                         counters.p + 5, keys.p, vals.p);
-        CNN_HIP(hipGetLastError());
         if (req.packed) {
+            // (only the packed keys: their top bits are the dataset, which
+            // k_first_of_set and k_rank_clusters_packed use as an index)
             const PackedSort &ps = *req.packed;
             PackedKey pk{ps.ds_shift, ps.size_shift, ps.size_max, ps.tile_ds, ps.ds_offset};
             k_collect_clusters_packed<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
@@ -1164,6 +1163,9 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             CNN_HIP(hipGetLastError());
             local.n_clusters = (int64_t)read_scalar(ctx, counters.p + 7);
         } else {
+            k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
+                                                             req.member_cutoff, counters.p + 5, keys.p, vals.p);
+            CNN_HIP(hipGetLastError());
             const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
             sort_pairs_u64_i32(ctx, keys.p, vals.p, K, 63);
             if (K > 0) k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(vals.p, K, req.max_clusters, newlab.p);

@@ -11,12 +11,6 @@ from commonnnclustering_amd import engine, synthetic
 
 pytestmark = pytest.mark.gpu
 
-# The packed path (one working order for all datasets, 1 <= d <= 8) is switched
-# off in the library: its first device run ended in a GPU memory fault whose
-# cause has not been shown.  Its tests stay here, skipped, until it is fixed.
-PACKED_OFF = pytest.mark.skip(reason="packed batch path disabled: unexplained GPU memory fault on its first run")
-
-
 def _stack(sets):
     X = np.concatenate([x for x, _ in sets]).astype(np.float32)
     off = np.zeros(len(sets) + 1, dtype=np.int64)
@@ -56,7 +50,6 @@ def _small_sets(d, count, seed):
     return sets
 
 
-@PACKED_OFF
 @pytest.mark.parametrize("precision", ["f64", "f32_band"])
 @pytest.mark.parametrize("d", [2, 3, 8])
 def test_packed_batch_matches_single_fits(cuda, d, precision):
@@ -69,7 +62,6 @@ def test_packed_batch_matches_single_fits(cuda, d, precision):
     assert ls["n_clusters"] == sum(int(w.max(initial=0)) for w in want)
 
 
-@PACKED_OFF
 def test_packed_batch_edge_sizes(cuda):
     # empty datasets, single points, exactly one tile, one tile + 1
     rng = np.random.default_rng(7)
@@ -84,7 +76,6 @@ def test_packed_batch_edge_sizes(cuda):
         assert (g == w).all(), f"dataset {b} (n={len(w)})"
 
 
-@PACKED_OFF
 def test_packed_batch_max_clusters_and_identical_copies(cuda):
     x, p = synthetic.blobs_2d()
     sets = [(x, p), (x, p), (x, dict(p, radius_cutoff=0.7))]
