@@ -916,11 +916,21 @@ __global__ __launch_bounds__(256) void k_row_word_count(const int64_t *__restric
     if (w >= n) return;
     const int64_t b = row_ptr[w], e = row_ptr[w + 1];
     int total = 0;
-    for (int64_t t0 = b; t0 < e; t0 += 64) {
-        const int64_t t = t0 + lane;
-        bool head = false;
-        if (t < e) head = t == b || (col[t - 1] >> 6) != (col[t] >> 6);
-        total += __popcll(__ballot(head));
+    // four windows of 64 entries per step, their loads issued together
+    for (int64_t t0 = b; t0 < e; t0 += 256) {
+        int32_t cur[4], prev[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = t0 + 64 * u + lane;
+            cur[u] = t < e ? col[t] : 0;
+            prev[u] = (t < e && t > b) ? col[t - 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = t0 + 64 * u + lane;
+            const bool head = t < e && (t == b || (prev[u] >> 6) != (cur[u] >> 6));
+            total += __popcll(__ballot(head));
+        }
     }
     if (lane == 0) cnt[w] = total;
 }
@@ -1038,12 +1048,14 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     } else if (req.sim_mode != CNN_SIM_MERGE && req.col) {
         bool build = req.sim_mode == CNN_SIM_BITSET;
         if (!build && req.words_min_fill > 0 && req.words_min_fill <= (int64_t)1 << 24) {
-            // automatic choice: the fill of every 16th row decides
-            constexpr int64_t kSampleStride = 16;
+            // automatic choice: the fill of every stride-th row decides, the
+            // stride at least 16 and large enough to sample about 2^21 entries
+            const int64_t nnz = read_scalar(ctx, req.row_ptr + n);
+            const int64_t stride = std::max<int64_t>(16, ceil_div(nnz, (int64_t)1 << 21));
             Scratch<unsigned long long> sums(ctx, 2);
             fill_u64(ctx, sums.p, 0, 2);
-            const int64_t rows = ceil_div(n, kSampleStride);
-            k_row_word_sample<<<blocks_for(rows, 4), 256, 0, s>>>(req.row_ptr, req.col, n, kSampleStride, sums.p);
+            const int64_t rows = ceil_div(n, stride);
+            k_row_word_sample<<<blocks_for(rows, 4), 256, 0, s>>>(req.row_ptr, req.col, n, stride, sums.p);
             CNN_HIP(hipGetLastError());
             unsigned long long h[2] = {0, 0};
             CNN_HIP(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, s));

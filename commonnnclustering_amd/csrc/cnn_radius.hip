@@ -119,15 +119,55 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
 
 // Working order for d > 8 (all tile pairs are evaluated, in any order): points
 // grouped by their nearest of L pivots (the points at caller indices j*n/L),
-// measured in FP32 over the first min(d, 64) dimensions.  Only the order comes
-// from this; clustered data then fills tiles from one cluster each, so most
-// tile pairs lie entirely outside the radius and take the distance kernels'
-// early exits.  Key = pivot index; the sort is stable (caller order within a
-// cell).
+// measured in FP32 over the first min(d, 64) dimensions, the cells themselves
+// grouped by the nearest of S super-pivots (pivots s * L / S) so that the
+// cells of one cluster are adjacent.  Only the order comes from this;
+// clustered data then fills tiles from one cluster each, so most tile pairs
+// lie entirely outside the radius, and the exact pivot-bound test below
+// (k_dense_live) removes them before the distance kernels run.  Key = (super
+// cell, pivot index); the sort is stable (caller order within a cell).
 constexpr int kPivotDims = 64;
 constexpr int kPivotChunk = 128;
+constexpr int kSuperPivots = 16;
+
+// super_of[p] = nearest super-pivot of pivot p (one thread per pivot, L <= 256).
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_pivot_super(const Tin *__restrict__ X, int64_t n, int d, int L, int S,
+                                                     int32_t *__restrict__ super_of) {
+    __shared__ float ss[kSuperPivots][kPivotDims];
+    const int dd = min(d, kPivotDims);
+    for (int q = threadIdx.x; q < S * kPivotDims; q += 256) {
+        const int sp = q / kPivotDims, k = q % kPivotDims;
+        const int64_t row = ((int64_t)(sp * L / S)) * n / L;
+        ss[sp][k] = k < dd ? (float)X[row * d + k] : 0.0f;
+    }
+    __syncthreads();
+    const int p = threadIdx.x;
+    if (p >= L) return;
+    float x[kPivotDims];
+    const int64_t row = (int64_t)p * n / L;
+#pragma unroll
+    for (int k = 0; k < kPivotDims; ++k) x[k] = k < dd ? (float)X[row * d + k] : 0.0f;
+    float best = INFINITY;
+    int bs = 0;
+    for (int sp = 0; sp < S; ++sp) {
+        float acc = 0.0f;  // dimensions past dd are zero on both sides
+#pragma unroll
+        for (int k = 0; k < kPivotDims; ++k) {
+            const float t = x[k] - ss[sp][k];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+        if (acc < best) {
+            best = acc;
+            bs = sp;
+        }
+    }
+    super_of[p] = bs;
+}
+
 template <typename Tin>
 __global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, int64_t n, int d, int L,
+                                                     const int32_t *__restrict__ super_of,
                                                      uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
     __shared__ float4 sp[kPivotChunk * kPivotDims / 4];
     const int dd = min(d, kPivotDims);
@@ -165,7 +205,7 @@ __global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, 
         }
     }
     if (i < n) {
-        keys[i] = (uint64_t)bi;
+        keys[i] = ((uint64_t)super_of[bi] << 8) | (uint64_t)bi;
         vals[i] = (int32_t)i;
     }
 }
@@ -186,6 +226,189 @@ __global__ void k_gather(const Tin *__restrict__ X, const int32_t *__restrict__ 
         } else {
             xw64[idx] = 0.0;
             if (xw32) xw32[idx] = 0.0f;
+        }
+    }
+}
+
+// Exact culling of dense tile pairs.  For kCullPivots pivots p (the centroids
+// of tiles spread evenly over the working order: each tile holds points of one
+// nearest-pivot cell, so its centroid sits near a cluster's centre, where
+// distances to the cluster's points vary least) every tile keeps the range
+// [lo_p, hi_p] of its real
+// points' Euclidean distances to p; by the triangle inequality no point of R
+// is within r of a point of C when lo_p(C) - hi_p(R) > r or lo_p(R) - hi_p(C) > r
+// for some p.  Distances are FP64 (relative error below (d + 3) * 2^-53) and
+// stored in FP32 rounded outward; the test adds a relative margin of 1e-6,
+// which covers both the rounding of the stored bounds and of the FP32
+// subtraction, and the reference's own rounding of the squared distance, so a
+// culled pair never holds a pair with s_ij <= r^2.  A tile with a real point
+// whose distance to p is not finite (non-finite or huge coordinates) gets the
+// range [0, inf] for p, which culls nothing through p.
+constexpr int kCullPivots = 64;
+constexpr int kCullDims = 32;  // dimensions staged per step
+
+// Pivot p = FP64 mean of the real rows of tile p * T / kCullPivots (one
+// workgroup per pivot, lanes over dimensions).  Any point serves as a pivot:
+// the bounds below hold for every p.
+__global__ __launch_bounds__(64) void k_cull_pivots(const double *__restrict__ xw64, int64_t n, int d, int64_t T,
+                                                    double *__restrict__ piv) {
+    const int64_t t = (int64_t)blockIdx.x * T / kCullPivots;
+    const int rows = (int)min<int64_t>(64, n - t * 64);
+    for (int k = threadIdx.x; k < d; k += 64) {
+        double sum = 0.0;
+        for (int r = 0; r < rows; ++r) sum += xw64[(t * 64 + r) * d + k];
+        piv[(int64_t)blockIdx.x * d + k] = sum / (double)rows;
+    }
+}
+
+// home[R] = the pivot with the smallest hi for tile R: the one nearest to all of
+// its points, tried first by the pair test.
+__global__ __launch_bounds__(256) void k_tile_pivot_bounds(const double *__restrict__ xw64, int64_t n, int d,
+                                                           int64_t T, const double *__restrict__ piv,
+                                                           float2 *__restrict__ bnd, int32_t *__restrict__ home) {
+    __shared__ float s_best[4];
+    __shared__ int s_bp[4];
+    static_assert(kCullPivots % 4 == 0, "pivots per wave");
+    constexpr int kPerWave = kCullPivots / 4;
+    __shared__ double sX[kCullDims][65];            // the tile's rows, dimension-major (padded)
+    __shared__ double sP[kCullPivots][kCullDims];  // the pivots
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t R = blockIdx.x;
+    double acc[kPerWave];
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) acc[j] = 0.0;
+    for (int k0 = 0; k0 < d; k0 += kCullDims) {
+        const int kc = min(kCullDims, d - k0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 64 * kc; q += 256) {
+            const int r = q / kc, kk = q - r * kc;
+            sX[kk][r] = xw64[(R * 64 + r) * d + k0 + kk];
+        }
+        for (int q = threadIdx.x; q < kCullPivots * kc; q += 256) {
+            const int p = q / kc, kk = q - p * kc;
+            sP[p][kk] = piv[(int64_t)p * d + k0 + kk];
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; ++kk) {
+            const double x = sX[kk][lane];
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j) {
+                const double t = x - sP[wave + 4 * j][kk];
+                acc[j] = __builtin_fma(t, t, acc[j]);
+            }
+        }
+    }
+    const bool real = R * 64 + lane < n;
+    float best = INFINITY;
+    int bp = wave;
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) {
+        const double dist = sqrt(acc[j]);
+        const bool fin = dist < INFINITY;  // false for +inf and NaN
+        double lo = (real && fin) ? dist : INFINITY, hi = (real && fin) ? dist : 0.0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, off, 64));
+            hi = fmax(hi, __shfl_xor(hi, off, 64));
+        }
+        const bool bad = __ballot(real && !fin) != 0ull;
+        const float2 b = bad ? make_float2(0.0f, INFINITY) : make_float2(__double2float_rd(lo), __double2float_ru(hi));
+        if (b.y < best) {
+            best = b.y;
+            bp = wave + 4 * j;
+        }
+        if (lane == 0) bnd[(int64_t)(wave + 4 * j) * T + R] = b;
+    }
+    if (lane == 0) {
+        s_best[wave] = best;
+        s_bp[wave] = bp;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int h = s_bp[0];
+        float bb = s_best[0];
+        for (int w = 1; w < 4; ++w)
+            if (s_best[w] < bb) {
+                bb = s_best[w];
+                h = s_bp[w];
+            }
+        home[R] = h;
+    }
+}
+
+// Live bits of the dense tile pairs: one wave per (row tile R, 64 column tiles);
+// counts[0] = live pairs, counts[1] = live pairs with C >= R.
+__device__ __forceinline__ bool pivot_cuts(const float2 *__restrict__ bnd, int64_t T, int p, int64_t R, int64_t C,
+                                           float rc) {
+    const float2 br = bnd[(int64_t)p * T + R], bc = bnd[(int64_t)p * T + C];
+    const float lb = fmaxf(bc.x - br.y, br.x - bc.y);
+    return lb > rc + 1e-6f * (br.y + bc.y);
+}
+
+// The home pivots of R and C are tried first (they cut most pairs of
+// different clusters), then every pivot.
+__global__ __launch_bounds__(256) void k_dense_live(const float2 *__restrict__ bnd, const int32_t *__restrict__ home,
+                                                    int64_t T, int64_t W, float rc, uint64_t *__restrict__ live,
+                                                    unsigned long long *__restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long n_all = 0, n_up = 0;  // this wave's counts: one atomic each at the end
+    for (int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gw < T * W; gw += (int64_t)gridDim.x * 4) {
+        const int64_t R = gw / W, C = (gw - R * W) * 64 + lane;
+        bool cut = false;
+        if (C < T && C != R) {
+            cut = pivot_cuts(bnd, T, home[R], R, C, rc) || pivot_cuts(bnd, T, home[C], R, C, rc);
+            for (int p = 0; p < kCullPivots && !cut; ++p) cut = pivot_cuts(bnd, T, p, R, C, rc);
+        }
+        const bool lv = C < T && !cut;
+        const uint64_t w = __ballot(lv);
+        n_all += __popcll(w);
+        n_up += __popcll(__ballot(lv && C >= R));
+        if (lane == 0) live[gw] = w;
+    }
+    if (lane == 0) {
+        if (n_all) atomicAdd(&counts[0], n_all);
+        if (n_up) atomicAdd(&counts[1], n_up);
+    }
+}
+
+// Per row tile (one wave): live pairs in all, and per word the live bits of
+// the row's earlier words.
+__global__ __launch_bounds__(256) void k_live_rows(const uint64_t *__restrict__ live, int64_t T, int64_t W,
+                                                   int32_t *__restrict__ pre, int32_t *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= T) return;
+    int run = 0;
+    for (int64_t w0 = 0; w0 < W; w0 += 64) {
+        const int64_t w = w0 + lane;
+        const int c = w < W ? __popcll(live[R * W + w]) : 0;
+        const int incl = wave_incl_scan(c);
+        if (w < W) pre[R * W + w] = run + incl - c;
+        run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) cnt[R] = run;
+}
+
+// Explicit lists of the live pairs (column tiles ascending) and the mirror of
+// every pair: the index of (C, R) in C's list.
+__global__ __launch_bounds__(256) void k_live_lists(const uint64_t *__restrict__ live, int64_t T, int64_t W,
+                                                    const int32_t *__restrict__ pre, const int64_t *__restrict__ tp_ptr,
+                                                    int32_t *__restrict__ tp_col, int64_t *__restrict__ mirror) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= T) return;
+    const int64_t rw = R >> 6;
+    const uint64_t rbelow = (1ull << (R & 63)) - 1ull;
+    for (int64_t w = lane; w < W; w += 64) {
+        uint64_t bits = live[R * W + w];
+        int64_t k = tp_ptr[R] + pre[R * W + w];
+        while (bits) {
+            const int b = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int64_t C = w * 64 + b;
+            tp_col[k] = (int32_t)C;
+            mirror[k] = tp_ptr[C] + pre[C * W + rw] + __popcll(live[C * W + rw] & rbelow);
+            ++k;
         }
     }
 }
@@ -351,7 +574,28 @@ struct DistArgs {
     uint8_t *pair_nz;           // per pair of the batch: 1 if its words hold any bit (nullable)
     const GroupMask *pair_groups; // per pair (global index): row groups of R in reach of C
     const double *bounds;       // periodic metric: period per dimension (<= 0: not periodic); nullptr: none
+    // dense layout with pivot culling: bit C % 64 of live[R * live_words + C / 64]
+    // is set when tile pair (R, C) may hold a neighbour (nullptr: every pair)
+    // (the live pairs are also the explicit lists tp_ptr / tp_col; live_pre
+    // holds, per row tile and word, the live bits of the row's earlier words)
+    const uint64_t *live;
+    const int32_t *live_pre;
+    int64_t live_words;
+    int deal_rr;  // dense MFMA kernel: work items dealt round-robin over the XCDs
 };
+
+__device__ __forceinline__ bool pair_live(const DistArgs &a, int64_t R, int64_t C) {
+    return !a.live || ((a.live[R * a.live_words + (C >> 6)] >> (C & 63)) & 1ull);
+}
+
+// Mask index of tile pair (R, C) for kernels that walk column ranges: all pairs
+// (R * T + C), or the position of C in R's list of live pairs.
+__device__ __forceinline__ int64_t pair_index(const DistArgs &a, int64_t R, int64_t C) {
+    if (!a.live) return R * a.n_tiles + C;
+    const int64_t w = R * a.live_words + (C >> 6);
+    const uint64_t below = a.live[w] & ((1ull << (C & 63)) - 1ull);
+    return a.tp_ptr[R] + a.live_pre[w] + __popcll(below);
+}
 
 __device__ __forceinline__ double r2_of(const DistArgs &a, int64_t R) { return a.tile_r2 ? a.tile_r2[R] : a.bp.r2; }
 
@@ -501,6 +745,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
     int mydeg = 0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
+        if (!pair_live(a, R, C)) continue;
         const double *__restrict__ xc = a.xw64 + C * 64 * d;
         double acc[64];
 #pragma unroll
@@ -816,6 +1061,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
     double maxerr = 0.0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
+        if (!pair_live(a, R, C)) continue;
         const float *__restrict__ xc = a.xw32 + C * 64 * d;
         float acc[64];
 #pragma unroll
@@ -1178,6 +1424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     int mydeg = 0;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
+        if (!pair_live(a, R, C)) continue;
         const float e_col = a.emarg[C * 64 + lane];
         const MfmaTile t = mfma_tile(a, R, C);
         mydeg += mfma_settle_store(a, t, k, R, C, i, pair_margin(e_row, wave_max(e_col)));
@@ -1205,6 +1452,7 @@ __host__ __device__ inline int64_t mfma_chunk_blocks(int sym, int64_t cc, int64_
 // contiguous ranges of equal length (workgroup b runs on XCD b % 8), so every
 // XCD gets the same work and workgroups that share an L2 read the same column
 // tiles.  Degrees are added atomically (several chunks contribute to a row).
+constexpr int kMfmaMaxChunk = 1024;  // largest column chunk of the tuning table
 template <int G>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_dist_mfma_lds(
     DistArgs a, int64_t n_rt, int64_t n_rb, int64_t n_cc, int64_t per_xcd, int64_t n_items) {
@@ -1212,9 +1460,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     constexpr int kPer = (kTileF4 + 255) / 256;
     __shared__ float4 sB[2][kTileF4];
     __shared__ float sE[2];  // largest margin of the staged column tile
+    __shared__ int32_t sCol[kMfmaMaxChunk];  // live column tiles of the item (culled layout)
+    __shared__ int sNcol;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 31, h = lane >> 5;
-    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    // items in contiguous ranges per XCD, or dealt round-robin (deal_rr)
+    const int64_t item = a.deal_rr ? (int64_t)blockIdx.x : (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (item >= n_items) return;
     int64_t cc = 0, rb = item;
     for (; cc < n_cc; ++cc) {
@@ -1231,6 +1482,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int64_t c_begin = a.sym ? max(c_lo, R_lo) : c_lo;
     if (c_begin >= c_hi) return;  // the whole workgroup
     const int tail = a.mfma_k - 4 * (G - 1);
+    // the column tiles to visit: all of [c_begin, c_hi), or (culled layout)
+    // those with a live pair for any of the 4 row tiles
+    int n_col = (int)(c_hi - c_begin);
+    if (a.live) {
+        if (wave == 0) {
+            const int64_t r_end = a.R0 + n_rt;
+            int cnt = 0;
+            for (int64_t cw = c_begin; cw < c_hi; cw += 64) {
+                const int64_t C = cw + lane;
+                bool any = false;
+                if (C < c_hi) {
+                    for (int q = 0; q < 4; ++q) {
+                        const int64_t Rq = R_lo + q;
+                        if (Rq < r_end && (!a.sym || C >= Rq) && pair_live(a, Rq, C)) any = true;
+                    }
+                }
+                const uint64_t m = __ballot(any);
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (any) sCol[cnt + rank] = (int32_t)C;
+                cnt += __popcll(m);
+            }
+            if (lane == 0) sNcol = cnt;
+        }
+        __syncthreads();
+        n_col = sNcol;
+        if (n_col == 0) return;  // the whole workgroup
+    }
+    auto col_at = [&](int idx) -> int64_t { return a.live ? (int64_t)sCol[idx] : c_begin + idx; };
+    const int64_t C0 = col_at(0);
 
     float4 A0[G], A1[G];
     float e_row = 0.0f;
@@ -1250,21 +1531,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int q = threadIdx.x + 256 * u;
-        if (q < kTileF4) sB[0][q] = xb[c_begin * kTileF4 + q];
+        if (q < kTileF4) sB[0][q] = xb[C0 * kTileF4 + q];
     }
     if (wave == 0) {
-        const float e = wave_max(a.emarg[c_begin * 64 + lane]);
+        const float e = wave_max(a.emarg[C0 * 64 + lane]);
         if (lane == 0) sE[0] = e;
     }
     __syncthreads();
     const int64_t i = R * 64 + lane;
     int mydeg = 0;
-    for (int64_t C = c_begin; C < c_hi; ++C) {
-        const int buf = (int)((C - c_begin) & 1);
-        const bool more = C + 1 < c_hi;
+    for (int idx = 0; idx < n_col; ++idx) {
+        const int64_t C = col_at(idx);
+        const int buf = idx & 1;
+        const bool more = idx + 1 < n_col;
         // the next tile's operands and margins (the last tile reloads itself, in
         // bounds, unused): loads in flight during this tile's MFMAs, no branch
-        const int64_t Cn = more ? C + 1 : C;
+        const int64_t Cn = more ? col_at(idx + 1) : C;
         // named registers, not an array: a private array here is placed in LDS or
         // scratch memory by the compiler, which waits for the loads right away
         static_assert(kPer <= 5, "staging registers");
@@ -1276,7 +1558,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if constexpr (kPer > 3) n3 = src[min((int)threadIdx.x + 768, kTileF4 - 1)];
         if constexpr (kPer > 4) n4 = src[min((int)threadIdx.x + 1024, kTileF4 - 1)];
         const float e_nx = a.emarg[Cn * 64 + lane];
-        const bool work = rvalid && (!a.sym || C >= R);
+        const bool work = rvalid && (!a.sym || C >= R) && pair_live(a, R, C);
         MfmaTile t;
         mfma_zero(t);
         if (work) {
@@ -1314,7 +1596,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
         asm volatile("" ::: "memory");
-        if (work) mydeg += mfma_settle_store(a, t, R * a.n_tiles + C, R, C, i, pair_margin(e_row, e_col));
+        if (work) mydeg += mfma_settle_store(a, t, pair_index(a, R, C), R, C, i, pair_margin(e_row, e_col));
         __syncthreads();
     }
     if (rvalid && mydeg && i < a.n) atomicAdd(&a.deg[i], mydeg);
@@ -1343,25 +1625,21 @@ __global__ __launch_bounds__(64) void k_dist_mfma_band(DistArgs a, int64_t n_red
     const uint64_t cm = valid_cols(a.n, C);
     const bool col_a = (cm >> r) & 1ull, col_b = (cm >> (32 + r)) & 1ull;
     uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
-    // element (row 4h + rr + rb, column cb + r) of lane (r, h) with value x
+    // element (row 4h + rr + rb, column cb + r) of lane (r, h) with value x,
+    // settled by its own lane: the exact FP64 distance in reference order
     auto settle = [&](float x, int rr, int rb, int cb, bool col_ok, uint32_t &part) {
         const bool band = !(x >= E) && !(x < nE) && col_ok && (R * 64 + 4 * h + rr + rb) < a.n;
-        uint64_t todo = __ballot(band);
-        while (todo) {
-            const int src = __ffsll((unsigned long long)todo) - 1;
-            todo &= todo - 1;
-            const int64_t gi = R * 64 + 4 * (src >> 5) + rr + rb;
-            const int64_t gj = C * 64 + cb + (src & 31);
-            const double e = exact_d2_wave(a.xw64, a.d, gi, gj);
+        if (band) {
+            const int64_t gi = R * 64 + 4 * h + rr + rb;
+            const int64_t gj = C * 64 + cb + r;
+            const double e = exact_d2(a.xw64, a.d, gi, gj);
             const bool in = e <= a.bp.r2;
-            if (lane == src) {
-                if (in) part |= 1u << rr;
-                ++hits;
-                const double d2 = a.bp.r2 - 2.0 * (double)x;
-                flips += (in != (d2 <= a.bp.r2));
-                const double err = fabs(d2 - e);
-                if (err == err) maxerr = fmax(maxerr, err);
-            }
+            if (in) part |= 1u << rr;
+            ++hits;
+            const double d2 = a.bp.r2 - 2.0 * (double)x;
+            flips += (in != (d2 <= a.bp.r2));
+            const double err = fabs(d2 - e);
+            if (err == err) maxerr = fmax(maxerr, err);
         }
     };
 #pragma unroll
@@ -1413,6 +1691,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double 
     const int64_t c_begin = a.sym ? max(c_lo, R) : c_lo;
     int mydeg = 0;
     for (int64_t C = c_begin; C < c_hi; ++C) {
+        if (!pair_live(a, R, C)) continue;
         const double *__restrict__ xc = xp + uniform64(C) * 64 * KMAX;
         uint64_t m = 0;
 #pragma unroll 2
@@ -1430,7 +1709,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double 
             m |= (uint64_t)(s0 <= a.bp.r2) << jj;
             m |= (uint64_t)(s1 <= a.bp.r2) << (jj + 1);
         }
-        mydeg += store_pair(a, R * T + C, R, i, C, m, false, 0);
+        mydeg += store_pair(a, pair_index(a, R, C), R, i, C, m, false, 0);
     }
     if (i < a.n && mydeg) atomicAdd(&a.deg[i], mydeg);
 }
@@ -1590,23 +1869,47 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit_rows(DistArgs a, const
     for (;;) {
         __syncthreads();
         if (wave == 0) {
+            // windows of 512 pairs, 8 consecutive flags per lane (all loads of a
+            // window in flight together)
             int cnt = 0;
             while (cnt < 64 && next < ke) {
-                const int64_t kk = next + lane;
-                const bool nz = kk < ke && (a.pair_nz == nullptr || a.pair_nz[kk - a.k_base] != 0);
-                const uint64_t m = __ballot(nz);
+                const int64_t b0 = next + (int64_t)lane * 8;
+                uint32_t nzb = 0;  // bit q: pair b0 + q holds a word with a bit
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t kk = b0 + q;
+                    const bool nz = kk < ke && (a.pair_nz == nullptr || a.pair_nz[kk - a.k_base] != 0);
+                    nzb |= (uint32_t)nz << q;
+                }
+                const int c = __popc(nzb);
+                const int incl = wave_incl_scan(c);
+                const int excl = incl - c;
+                const int total = __shfl(incl, 63, 64);
                 const int room = 64 - cnt;
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                if (nz && rank < room) s_k[cnt + rank] = kk;
-                if (__popcll(m) > room) {
-                    // the window holds more than fits: resume after the last pair taken
-                    const uint64_t last = __ballot(nz && rank == room - 1);
-                    next += __ffsll((unsigned long long)last);
+                uint32_t m = nzb;
+                for (int rank = excl; m && rank < room; ++rank) {
+                    const int q = __ffs(m) - 1;
+                    m &= m - 1;
+                    s_k[cnt + rank] = b0 + q;
+                }
+                if (total > room) {
+                    // the window holds more than fits: resume after the last pair
+                    // taken (in the lane whose ranks hold room - 1)
+                    int64_t last = 0;
+                    const bool here = excl < room && incl >= room;
+                    if (here) {
+                        uint32_t mm = nzb;
+                        for (int t = 0; t < room - 1 - excl; ++t) mm &= mm - 1;
+                        last = b0 + (__ffs(mm) - 1);
+                    }
+                    const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
+                    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)last, src);
+                    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)last >> 32), src);
+                    next = (int64_t)(((uint64_t)hi << 32) | lo) + 1;
                     cnt = 64;
                 } else {
-                    cnt += __popcll(m);
-                    next = min(next + 64, ke);
+                    cnt += total;
+                    next = min(next + 512, ke);
                 }
             }
             if (lane == 0) s_nk = cnt;
@@ -1788,7 +2091,7 @@ static void make_band_gram(int d, int dp, bool rounded_coords, double r2, float 
 
 // Padded width of the wide FP64 kernel (dense layout), 0 if unused.
 static int wide_kmax(const DistArgs &a) {
-    if (a.tp_ptr || a.d <= 8 || a.d > 64) return 0;
+    if ((a.tp_ptr && !a.live) || a.d <= 8 || a.d > 64) return 0;
     return a.d <= 8 ? 8 : a.d <= 16 ? 16 : a.d <= 32 ? 32 : 64;
 }
 
@@ -1820,9 +2123,13 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         b.redo_cnt = (unsigned long long *)(cnt.p + 1);
         b.redo = redo.p;
         const int G = a.dp / 8;
-        if (!a.tp_ptr && !a.tile_valid && G >= 3 && G <= 10) {
+        if ((!a.tp_ptr || a.live) && !a.tile_valid && G >= 3 && G <= 10) {
             // dense pairs, row operands in registers, column tiles shared through LDS
             b.col_chunk = mfma_col_chunk;
+            // culled items range from no work to the full chunk: round-robin
+            // dealing balances them (measured 5.5 ms against 9.2 ms for
+            // contiguous ranges at n = 2e5, d = 64)
+            b.deal_rr = a.live ? 1 : 0;
             const int64_t n_rb = ceil_div(n_rt, 4), n_cc = ceil_div(a.n_tiles, b.col_chunk);
             int64_t n_items = 0;
             for (int64_t cc = 0; cc < n_cc; ++cc)
@@ -2082,13 +2389,55 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipMemcpyAsync(rows_evaluated, rows_eval.p, sizeof(rows_evaluated), hipMemcpyDeviceToHost, s));
         CNN_HIP(hipStreamSynchronize(s));
     }
+    // dense layout in nearest-pivot order: tile pairs out of reach by the pivot
+    // bounds are culled (never evaluated, stored as empty pairs)
+    std::unique_ptr<Scratch<uint64_t>> live;
+    std::unique_ptr<Scratch<int32_t>> live_pre;
+    int64_t live_words = 0;
+    unsigned long long live_count[2] = {0, 0};
+    if (g->layout == 0 && g->perm && !g->bounds && !pk && T >= 2) {
+        Scratch<float2> bnd(ctx, (int64_t)kCullPivots * T);
+        Scratch<double> piv(ctx, (int64_t)kCullPivots * d);
+        Scratch<int32_t> home(ctx, T);
+        k_cull_pivots<<<kCullPivots, 64, 0, s>>>(g->xw64, n, d, T, piv.p);
+        k_tile_pivot_bounds<<<(unsigned)T, 256, 0, s>>>(g->xw64, n, d, T, piv.p, bnd.p, home.p);
+        CNN_HIP(hipGetLastError());
+        live_words = ceil_div(T, 64);
+        live.reset(new Scratch<uint64_t>(ctx, T * live_words));
+        Scratch<unsigned long long> lcnt(ctx, 2);
+        fill_u64(ctx, lcnt.p, 0ull, 2);
+        const double rr = std::sqrt(g->r2) * (1.0 + 1e-6);
+        float rc = (float)rr;
+        if ((double)rc < rr) rc = std::nextafter(rc, INFINITY);
+        k_dense_live<<<(unsigned)std::min<int64_t>(ceil_div(T * live_words, 4), 2048), 256, 0, s>>>(
+            bnd.p, home.p, T, live_words, rc, live->p, lcnt.p);
+        CNN_HIP(hipGetLastError());
+        CNN_HIP(hipMemcpyAsync(live_count, lcnt.p, sizeof(live_count), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+        // the live pairs become the graph's explicit tile-pair lists: the
+        // words of culled pairs are never stored, and the label pass can test
+        // similarities on the words as for the spatial layout
+        live_pre.reset(new Scratch<int32_t>(ctx, T * live_words));
+        Scratch<int32_t> cnt(ctx, T);
+        k_live_rows<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(live->p, T, live_words, live_pre->p, cnt.p);
+        CNN_HIP(hipGetLastError());
+        tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
+        exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
+        total_pairs = (int64_t)live_count[0];
+        CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        tp_col = g->tp_col = dev_alloc<int32_t>(g, std::max<int64_t>(total_pairs, 1));
+        mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
+        k_live_lists<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(live->p, T, live_words, live_pre->p, tp_ptr, tp_col,
+                                                             mirror->p);
+        CNN_HIP(hipGetLastError());
+    }
     std::vector<int64_t> h_tp_ptr;
-    if (g->layout == 1) {
+    if (tp_ptr) {
         h_tp_ptr.resize(T + 1);
         CNN_HIP(hipMemcpyAsync(h_tp_ptr.data(), tp_ptr, (T + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         CNN_HIP(hipStreamSynchronize(s));
     }
-    auto pair_begin = [&](int64_t R) { return g->layout == 1 ? h_tp_ptr[R] : R * T; };
+    auto pair_begin = [&](int64_t R) { return tp_ptr ? h_tp_ptr[R] : R * T; };
     CNN_HIP(hipEventRecord(ctx->ev[1], s));
 
     BandParams bp;
@@ -2120,13 +2469,14 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     for (auto &b : batches) max_batch_pairs = std::max(max_batch_pairs, pair_begin(b.second) - pair_begin(b.first));
     // symmetric evaluation needs every mask of the graph in one buffer
     const int sym = batches.size() == 1 ? 1 : 0;
-    // a single-batch spatial build keeps its words: the label pass walks a row's
-    // lists tile by tile from them
+    // a single-batch build with explicit tile-pair lists (spatial, or dense
+    // after culling) keeps its words: the label pass walks a row's lists tile
+    // by tile from them
     uint64_t *mask_p = nullptr;
     uint8_t *pair_nz_p = nullptr;
     std::unique_ptr<Scratch<uint64_t>> mask_s;
     std::unique_ptr<Scratch<uint8_t>> pair_nz_s;
-    if (sym && g->layout == 1) {
+    if (sym && tp_ptr) {
         mask_p = g->mask = dev_alloc<uint64_t>(g, std::max<int64_t>(max_batch_pairs, 1) * 64);
         pair_nz_p = g->pair_nz = dev_alloc<uint8_t>(g, std::max<int64_t>(max_batch_pairs, 1));
     } else {
@@ -2139,8 +2489,9 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
-    // A single-batch spatial graph whose words take no more memory than its
-    // lists would (512 bytes per candidate tile pair against 4 per entry) keeps
+    // A single-batch graph with explicit tile-pair lists whose words take no
+    // more memory than its lists would (512 bytes per tile pair against 4 per
+    // entry) keeps
     // only the words: similarities are tested on them and the lists are
     // emitted when a caller needs them (graph_lists).
     bool words_only = false;
@@ -2170,6 +2521,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.pair_nz = pair_nz_p;
         a.pair_groups = gmask ? gmask->p : nullptr;
         a.bounds = g->bounds;
+        if (live) {
+            a.live = live->p;
+            a.live_pre = live_pre->p;
+            a.live_words = live_words;
+        }
         if (pk) {
             a.tile_r2 = pk->tile_r2;
             a.tile_lo = pk->tile_lo;
@@ -2198,7 +2554,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
             CNN_HIP(hipStreamSynchronize(s));
         }
         const int64_t need = nnz_total + batch_total;
-        words_only = sym && g->layout == 1 && !pk && g->mask && (int64_t)512 * total_pairs <= (int64_t)4 * need;
+        words_only = sym && tp_ptr && !pk && g->mask && (int64_t)512 * total_pairs <= (int64_t)4 * need;
         if (words_only) {
             CNN_HIP(hipEventRecord(ctx->ev[5], s));
             CNN_HIP(hipEventSynchronize(ctx->ev[5]));
@@ -2322,10 +2678,6 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         g->stats = cnn_graph_stats{};
         return std::exchange(guard.g, nullptr);
     }
-    std::vector<double> lo, hi;
-    double max_norm2 = 0.0;
-    bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
-
     // ordering: Morton order over the (up to) three widest dims for small d,
     // nearest-pivot cells for larger d; the caller's order (all tile pairs)
     // for the periodic metric, whose neighbours wrap around the box
@@ -2335,6 +2687,11 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         CNN_HIP(hipStreamSynchronize(s));
     }
     g->layout = (!bounds_host && d >= 1 && d <= kSpatialMaxD && n > 128) ? 1 : 0;
+    // the box feeds the Morton keys and the direct FP32 form's band (largest
+    // norm); the plain FP64 and MFMA forms of the dense layout need neither
+    std::vector<double> lo, hi;
+    double max_norm2 = 0.0;
+    if (g->layout == 1 || (band && d < kMfmaMinD)) bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
     if (g->layout == 1) {
         const KeySpec ks = make_keyspec(d, lo, hi, 64);
         g->perm = dev_alloc<int32_t>(g, n);
@@ -2348,15 +2705,20 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
     } else if (!bounds_host && d > kSpatialMaxD && n >= kPivotMinN) {
         const int L = (int)std::min<int64_t>(256, n / 64);
+        const int S = std::min(kSuperPivots, L);
         g->perm = dev_alloc<int32_t>(g, n);
         Scratch<uint64_t> keys(ctx, n);
+        Scratch<int32_t> super_of(ctx, L);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
-        if (dtype == CNN_F32)
-            k_pivot_cells<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, keys.p, g->perm);
-        else
-            k_pivot_cells<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, keys.p, g->perm);
+        if (dtype == CNN_F32) {
+            k_pivot_super<float><<<1, 256, 0, s>>>((const float *)X, n, d, L, S, super_of.p);
+            k_pivot_cells<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, super_of.p, keys.p, g->perm);
+        } else {
+            k_pivot_super<double><<<1, 256, 0, s>>>((const double *)X, n, d, L, S, super_of.p);
+            k_pivot_cells<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, super_of.p, keys.p, g->perm);
+        }
         CNN_HIP(hipGetLastError());
-        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, 8);
+        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, 12);
     }
     g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
     if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
@@ -2575,7 +2937,7 @@ void graph_lists(cnn_ctx *ctx, const cnn_graph *gc) {
     a.mask = g->mask;
     a.pair_nz = g->pair_nz;
     CNN_HIP(hipEventRecord(ctx->ev[4], s));
-    launch_emit(s, a, T, false, g->tune.emit_waves, g->row_ptr, col, selfpos);
+    launch_emit(s, a, T, g->layout != 1, g->tune.emit_waves, g->row_ptr, col, selfpos);
     CNN_HIP(hipEventRecord(ctx->ev[5], s));
     // complete before any stream (of any context) reads them
     CNN_HIP(hipEventSynchronize(ctx->ev[5]));

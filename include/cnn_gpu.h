@@ -75,7 +75,8 @@ enum {
  * 64-bit words over the column tiles both rows reach.  CNN_SIM_AUTO takes the
  * words when the graph keeps them (spatial graphs whose words take no more
  * memory than their lists: at least two neighbours per word on average) or,
- * for lists, when a sample of their rows averages at least 8 entries per word
+ * for lists, when a sample of their rows (every stride-th row, the stride at
+ * least 16 and about nnz / 2^21) averages at least 8 entries per word
  * (dense rows; the threshold is tunable, cnn_tune_params); otherwise the
  * merge. */
 enum { CNN_SIM_AUTO = 0, CNN_SIM_MERGE = 1, CNN_SIM_BITSET = 2 };
@@ -96,7 +97,8 @@ typedef struct {
     float ms_emit;             /* degree scan + CSR emit */
     int32_t distance_launches; /* launches of the distance kernel timed in ms_distance */
     int32_t layout;            /* 0 identity order, 1 spatial (Morton) order */
-    int64_t tile_pairs_computed; /* 64x64 tile pairs the distance kernel evaluated (symmetric: C >= R) */
+    int64_t tile_pairs_computed; /* 64x64 tile pairs the distance kernel evaluated (symmetric: C >= R;
+                                    dense layout: those the pivot bounds left) */
     float ms_distance_kernel;  /* the main distance kernel alone (ms_distance also holds guard-band settling) */
 } cnn_graph_stats;
 

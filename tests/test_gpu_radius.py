@@ -180,3 +180,52 @@ def test_gram_huge_and_nonfinite(cuda):
         rp, col, _ = gpu_csr(Xc, 2.5, "f32_band")
         rp0, col0 = O.neighbourhoods_csr(Xc.astype(np.float64), 2.5)
         assert (rp == rp0).all() and (col == col0).all()
+
+
+def _tiles_upper(n):
+    T = (n + 63) // 64
+    return T * (T + 1) // 2
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+@pytest.mark.parametrize("d", [12, 24, 64, 100])
+def test_dense_pivot_culling(cuda, precision, d):
+    """Dense layout (d > 8, nearest-pivot order) on separated clusters: the
+    pivot bounds cull most tile pairs and the lists stay exact."""
+    X, r = _clustered(6000 + d, d, 300 + d)
+    st = assert_same(X.astype(np.float32), r, precision)
+    assert st["tile_pairs_computed"] < _tiles_upper(len(X)) // 2, st
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+@pytest.mark.parametrize("d", [16, 40])
+def test_dense_culling_tight_bound(cuda, precision, d):
+    """Two groups of points on one axis whose closest pair lies at exactly r:
+    every pivot lies on the axis, so the pivot bound of the tile pair holding
+    that pair equals r, and the pair (a neighbour: s = r^2) must not be culled.
+    Pairs one step past r across the gap are culled or rejected."""
+    m = 1100
+    step = 1.0 / 1024
+    a = np.arange(m) * step                      # 0 .. (m-1)/1024
+    r = 1.5                                      # r^2 = 2.25 exactly
+    b = a[-1] + r + np.arange(m) * step          # first point exactly r past the last of a
+    X = np.zeros((2 * m, d))
+    X[:m, 0] = a
+    X[m:, 0] = b
+    st = assert_same(X.astype(np.float32), r, precision)
+    assert st["tile_pairs_computed"] < _tiles_upper(2 * m), st
+
+
+def test_dense_culling_nonfinite(cuda):
+    """Dense layout with NaN, inf and huge rows (their tiles' bounds cull
+    nothing) and exact duplicates: lists equal the oracle's."""
+    X, r = _clustered(4000 + 5, 20, 77)
+    X[100] = np.nan
+    X[200, 3] = np.inf
+    X[300] = 1e19
+    X[301] = 1e19
+    X[2500] = -1e19
+    X[900] = X[901]
+    for precision in ("f64", "f32_band"):
+        for dtype in (np.float64, np.float32):
+            assert_same(X.astype(dtype), r, precision)

@@ -52,7 +52,8 @@ def test_graph_labels_every_mode(cuda, seed, n, d, r, c, precision):
     want, _ = C.labels_from_csr(rp, col, c + 2, c + 1, consider=consider, current_start=start, labels=init.copy())
     g = engine.build_graph(to_dev(X), float(r) * float(r), precision)
     try:
-        kept_words = g.stats["ms_emit"] == 0.0 and g.stats["layout"] == 1
+        # words kept (spatial graphs, dense graphs after culling): no lists emitted
+        kept_words = g.stats["ms_emit"] == 0.0
         forms = {}
         for mode in (AUTO, BITSET, MERGE, AUTO):
             lab = to_dev(init.copy())
@@ -77,8 +78,10 @@ def test_caller_lists_every_mode(cuda, seed, n, d, r, c):
     X = _data(seed, n, d, dtype=np.float64)
     rp, col = C.neighbours_csr(X, r, use_grid=d <= 3)
     want, _ = C.labels_from_csr(rp, col, c + 2, c + 1)
-    # the automatic choice estimates the fill on every 16th row
-    rows = range(0, n, 16)
+    # the automatic choice estimates the fill on every stride-th row (at least
+    # every 16th, about 2^21 entries in all)
+    stride = max(16, -(-int(rp[-1]) // (1 << 21)))
+    rows = range(0, n, stride)
     fill = sum(rp[i + 1] - rp[i] for i in rows) / max(1, sum(len(np.unique(col[rp[i]:rp[i + 1]] >> 6))
                                                          for i in rows))
     for mode in (AUTO, MERGE, BITSET):
