@@ -691,25 +691,32 @@ __device__ __forceinline__ int64_t first_pair(const DistArgs &a, int64_t R, int6
 // the words of a pair without any bit are not stored: readers skip flagged-
 // empty pairs (the mirror of an empty pair is empty too).  `force` stores them
 // anyway, for pairs whose guard-band bits a later kernel ors in.
+// Mask index of the mirror pair (C, R) of pair k = (R, C).
+__device__ __forceinline__ int64_t mirror_of(const DistArgs &a, int64_t k, int64_t R, int64_t C) {
+    return a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
+}
+
+// km: the mirror's index when the caller has it (loaded ahead), else -1.
 __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t R, int64_t i, int64_t C, uint64_t m,
-                                          bool have_col, uint64_t colword, bool force = false) {
+                                          bool have_col, uint64_t colword, bool force = false, int64_t km = -1) {
     m &= tile_mask(a, C);
     if (!row_real(a, i)) m = 0;
     const int lane = threadIdx.x & 63;
     const bool any = __ballot(m != 0ull) != 0ull;
+    const bool mirrored = a.sym && C != R;
+    if (mirrored && km < 0) km = mirror_of(a, k, R, C);
     if (a.pair_nz && lane == 0) {
         a.pair_nz[k - a.k_base] = any ? 1 : 0;
-        if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = any ? 1 : 0;
+        if (mirrored) a.pair_nz[km - a.k_base] = any ? 1 : 0;
     }
     if (a.pair_nz && !any && !force) return 0;
     a.mask[(k - a.k_base) * 64 + lane] = m;
-    if (a.sym && C != R) {
+    if (mirrored) {
         if (!have_col) colword = transpose_bits(m);
         else {
             colword &= tile_mask(a, R);                   // padding rows of R
             if (!row_real(a, C * 64 + lane)) colword = 0;  // padding columns of C
         }
-        const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
         a.mask[(km - a.k_base) * 64 + lane] = colword;
         const int cd = __popcll(colword);
         if (cd) atomicAdd(&a.deg[C * 64 + lane], cd);  // column side of the symmetric pair
@@ -862,16 +869,35 @@ __device__ __forceinline__ uint64_t band_resolve(const DistArgs &a, double r2, f
     return m;
 }
 
+// Band statistics of the workgroup, added with one set of atomics per
+// workgroup (the counters are single addresses: per-wave atomics there
+// serialize).  Every thread of the workgroup must call it.
 __device__ __forceinline__ void flush_band(const DistArgs &a, unsigned hits, unsigned flips, double maxerr) {
+    __shared__ unsigned long long s_h[16], s_f[16];
+    __shared__ double s_m[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = (blockDim.x + 63) >> 6;
     unsigned long long h = (unsigned long long)wave_sum((int)hits);
     unsigned long long f = (unsigned long long)wave_sum((int)flips);
     double mx = maxerr;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0 && h) {
-        atomicAdd(&a.band_stats[0], h);
-        atomicAdd(&a.band_stats[1], f);
-        atomic_max_double(&a.band_stats[2], mx);
+    if (lane == 0) {
+        s_h[wave] = h;
+        s_f[wave] = f;
+        s_m[wave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < n_waves; ++w) {
+            h += s_h[w];
+            f += s_f[w];
+            mx = fmax(mx, s_m[w]);
+        }
+        if (h) {
+            atomicAdd(&a.band_stats[0], h);
+            atomicAdd(&a.band_stats[1], f);
+            atomic_max_double(&a.band_stats[2], mx);
+        }
     }
 }
 
@@ -1152,7 +1178,7 @@ __global__ void k_augment_pack(const float *__restrict__ xw32, int64_t n, int64_
 
 __device__ __forceinline__ void mark_pair_nz(const DistArgs &a, int64_t k, int64_t R, int64_t C) {
     a.pair_nz[k - a.k_base] = 1;
-    if (a.sym && C != R) a.pair_nz[(a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R) - a.k_base] = 1;
+    if (a.sym && C != R) a.pair_nz[mirror_of(a, k, R, C) - a.k_base] = 1;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -1229,90 +1255,112 @@ __device__ __forceinline__ uint64_t merge_col_words(uint32_t a_lo, uint32_t a_hi
     return h == 0 ? fa : fb;
 }
 
-// Lane l's 64 elements of a tile pair are numbered 4v + b: v the accumulator
-// slot, b the block (0: c00, 1: c10, 2: c01, 3: c11); element 4v + b is row
-// 8(v/4) + 4(l/32) + v%4 (+32 for b odd) and column l%32 (+32 for b >= 2).
+// Band flags of a lane use the layout of its column words: bits 0-31 column
+// l%32 (blocks c00, c10), bits 32-63 column 32 + l%32 (c01, c11); within a
+// half, bit q is accumulator slot v = 4(q/8) + q%4 of the first row block
+// (q/4 even) or of the second (q/4 odd, rows + 32), i.e. row
+// 8(q/8) + 4(l/32) + q%4 (+32).
 __device__ __forceinline__ void band_element(int lane, int e, int &row, int &col) {
-    const int v = e >> 2, b = e & 3;
-    row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3) + ((b & 1) ? 32 : 0);
-    col = (lane & 31) + ((b & 2) ? 32 : 0);
+    const int q = e & 31;
+    row = 8 * (q >> 3) + 4 * (lane >> 5) + (q & 3) + (((q >> 2) & 1) ? 32 : 0);
+    col = (lane & 31) + (e >= 32 ? 32 : 0);
 }
 
-// Settles the guard-band elements of listed tile pairs (one wave per record):
-// the flagged elements are taken one at a time and decided by the exact FP64
-// distance in reference order, evaluated by the whole wave; neighbours are
-// or-ed into the stored words of the pair (and of its mirror for a symmetric
-// evaluation) and counted in both degrees.  The FP32 statistics use a wave
-// sum of the augmented vectors' products.
-__device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hits, unsigned &flips,
-                                   double &maxerr);
+// The column-word halves (a_lo | a_hi << 4, b_lo | b_hi << 4) as one flag word.
+__device__ __forceinline__ uint64_t pack_halves(uint32_t a_lo, uint32_t a_hi, uint32_t b_lo, uint32_t b_hi) {
+    return (uint64_t)(a_lo | (a_hi << 4)) | ((uint64_t)(b_lo | (b_hi << 4)) << 32);
+}
 
-__global__ __launch_bounds__(256) void k_dist_band_flags(DistArgs a, int64_t n_rec) {
+// Guard-band elements of the listed tile pairs, settled one per thread.  The
+// records (k, R, 64 per-lane flag words) are flattened into a list of
+// elements (record << 12 | lane << 6 | bit): k_band_count counts each
+// record's elements, an exclusive scan places them, k_band_list writes them,
+// and k_band_settle decides each in FP64 in reference order and ors its bit
+// into the stored words of the pair and of its mirror (and counts it in both
+// degrees).  The FP32 statistics recompute the Gram value from the augmented
+// vectors (sequential FMAs; statistics only).
+__global__ __launch_bounds__(256) void k_band_count(const int64_t *__restrict__ band_list, int64_t n_rec,
+                                                    int32_t *__restrict__ cnt) {
     const int lane = threadIdx.x & 63;
+    const int64_t rec = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (rec >= n_rec) return;
+    if (band_list[rec * 66] < 0) {  // a reserved slot left empty
+        if (lane == 0) cnt[rec] = 0;
+        return;
+    }
+    const int c = wave_sum(__popcll((uint64_t)band_list[rec * 66 + 2 + lane]));
+    if (lane == 0) cnt[rec] = c;
+}
+
+__global__ __launch_bounds__(256) void k_band_list(const int64_t *__restrict__ band_list, int64_t n_rec,
+                                                   const int64_t *__restrict__ off, int64_t *__restrict__ elem) {
+    const int lane = threadIdx.x & 63;
+    const int64_t rec = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (rec >= n_rec || band_list[rec * 66] < 0) return;
+    uint64_t f = (uint64_t)band_list[rec * 66 + 2 + lane];
+    const int c = __popcll(f);
+    int64_t o = off[rec] + wave_incl_scan(c) - c;
+    while (f) {
+        const int b = __ffsll((unsigned long long)f) - 1;
+        f &= f - 1;
+        elem[o++] = (rec << 12) | ((int64_t)lane << 6) | b;
+    }
+}
+
+// FP32 Gram value of points gi, gj from the packed augmented vectors (four
+// dimensions per float4; statistics only).
+__device__ __forceinline__ float gram_value(const DistArgs &a, int64_t gi, int64_t gj) {
+    const int G = a.dp >> 3;
+    const float4 *pa = (const float4 *)a.xra + (gi >> 6) * G * 128 + (gi & 63);
+    const float4 *pb = (const float4 *)a.xca + (gj >> 6) * G * 128 + (gj & 63);
+    float prod = 0.0f;
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float4 x = pa[(g * 2 + h) * 64], y = pb[(g * 2 + h) * 64];
+            prod = __builtin_fmaf(x.x, y.x, prod);
+            prod = __builtin_fmaf(x.y, y.y, prod);
+            prod = __builtin_fmaf(x.z, y.z, prod);
+            prod = __builtin_fmaf(x.w, y.w, prod);
+        }
+    }
+    return prod;
+}
+
+__global__ __launch_bounds__(256) void k_band_settle(DistArgs a, const int64_t *__restrict__ elem, int64_t n_el) {
     unsigned hits = 0, flips = 0;
     double maxerr = 0.0;
-    const int64_t n_waves = (int64_t)gridDim.x * 4;
-    for (int64_t rec = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); rec < n_rec; rec += n_waves)
-        settle_band_record(a, rec, hits, flips, maxerr);
-    flush_band(a, hits, flips, maxerr);
-}
-
-__device__ void settle_band_record(const DistArgs &a, int64_t rec, unsigned &hits, unsigned &flips,
-                                   double &maxerr) {
-    const int lane = threadIdx.x & 63;
-    const int64_t *p = a.band_list + rec * 66;
-    const int64_t k = p[0], R = p[1];
-    uint64_t flags = (uint64_t)p[2 + lane];
-    int64_t kb, ke;
-    tile_range(a, R, kb, ke);
-    const int64_t C = uniform64(tile_col(a, k, kb));
-    uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
-    for (uint64_t todo = __ballot(flags != 0ull); todo; todo = __ballot(flags != 0ull)) {
-        const int src = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t flo = __builtin_amdgcn_readlane((uint32_t)flags, src);
-        const uint32_t fhi = __builtin_amdgcn_readlane((uint32_t)(flags >> 32), src);
-        const uint64_t fs = ((uint64_t)fhi << 32) | flo;
-        const int e = __ffsll((unsigned long long)fs) - 1;
-        if (lane == src) flags &= flags - 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_el; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = elem[t];
+        const int64_t rec = v >> 12;
+        const int le = (int)((v >> 6) & 63), e = (int)(v & 63);
+        const int64_t k = a.band_list[rec * 66], R = a.band_list[rec * 66 + 1];
+        int64_t kb, ke;
+        tile_range(a, R, kb, ke);
+        const int64_t C = tile_col(a, k, kb);
         int row, col;
-        band_element(src, e, row, col);
+        band_element(le, e, row, col);
         const int64_t gi = R * 64 + row, gj = C * 64 + col;
-        const double ex = exact_d2_wave(a.xw64, a.d, gi, gj);
+        const double ex = exact_d2(a.xw64, a.d, gi, gj);
         const bool in = ex <= a.bp.r2;
-        float prod = 0.0f;
-        for (int q = lane; q < a.dp; q += 64)
-            prod = __builtin_fmaf(a.xra[packed_at(gi, q, a.dp >> 3)], a.xca[packed_at(gj, q, a.dp >> 3)], prod);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) prod += __shfl_xor(prod, off, 64);
-        if (lane == src) {
-            const double d2 = a.bp.r2 - 2.0 * (double)prod;
-            ++hits;
-            flips += (in != (d2 <= a.bp.r2));
-            const double err = fabs(d2 - ex);
-            if (err == err) maxerr = fmax(maxerr, err);
-            if (in) {
-                const int v = e >> 2, b = e & 3;
-                const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
-                if (b == 0) a_lo |= bit;
-                else if (b == 1) a_hi |= bit;
-                else if (b == 2) b_lo |= bit;
-                else b_hi |= bit;
+        const double d2 = a.bp.r2 - 2.0 * (double)gram_value(a, gi, gj);
+        ++hits;
+        flips += in != (d2 <= a.bp.r2);
+        const double err = fabs(d2 - ex);
+        if (err == err) maxerr = fmax(maxerr, err);
+        if (in) {
+            atomicOr(&a.mask[(k - a.k_base) * 64 + row], 1ull << col);
+            atomicAdd(&a.deg[gi], 1);
+            if (a.pair_nz) a.pair_nz[k - a.k_base] = 1;
+            if (a.sym && C != R) {
+                const int64_t km = mirror_of(a, k, R, C);
+                atomicOr(&a.mask[(km - a.k_base) * 64 + col], 1ull << row);
+                atomicAdd(&a.deg[gj], 1);
+                if (a.pair_nz) a.pair_nz[km - a.k_base] = 1;
             }
         }
     }
-    const uint64_t colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
-    const uint64_t word = transpose_bits(colword);
-    const bool any_new = __ballot(word != 0ull) != 0ull;
-    if (a.pair_nz && lane == 0 && any_new) mark_pair_nz(a, k, R, C);
-    if (word) {
-        atomicOr(&a.mask[(k - a.k_base) * 64 + lane], (unsigned long long)word);
-        atomicAdd(&a.deg[R * 64 + lane], __popcll(word));
-    }
-    if (a.sym && C != R && colword) {
-        const int64_t km = a.tp_ptr ? a.mirror[k] : C * a.n_tiles + R;
-        atomicOr(&a.mask[(km - a.k_base) * 64 + lane], (unsigned long long)colword);
-        atomicAdd(&a.deg[C * 64 + lane], __popcll(colword));
-    }
+    flush_band(a, hits, flips, maxerr);
 }
 
 // Decision margin of a tile pair: NaN (every element fails both the
@@ -1331,8 +1379,22 @@ __device__ __forceinline__ float pair_margin(float e_row, float e_col) {
 // No LDS: each lane sets its rows' bits in its column words, one butterfly
 // transpose gives the row words, and the column words are the mirror pair's
 // row words.  Stores both and returns the row word's popcount.
+// Band record slots of a wave, reserved kBandSlotChunk at a time with one
+// atomic (the counter is a single address); slots left at the wave's end are
+// marked empty (k = -1) by band_slots_release.
+constexpr int kBandSlotChunk = 8;
+struct BandSlots {
+    unsigned long long next = 0, end = 0;
+};
+
+__device__ __forceinline__ void band_slots_release(const DistArgs &a, BandSlots &bs) {
+    if ((threadIdx.x & 63) == 0)
+        for (unsigned long long q = bs.next; q < bs.end && (int64_t)q < a.band_cap; ++q) a.band_list[q * 66] = -1;
+    bs.next = bs.end;
+}
+
 __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTile &t, int64_t k, int64_t R,
-                                                 int64_t C, int64_t i, float E) {
+                                                 int64_t C, int64_t i, float E, BandSlots &bs, int64_t km = -1) {
     const int lane = threadIdx.x & 63;
     const float nE = -E;
     // Fast path: the largest element is below -E.  fmaxf skips NaN elements, but
@@ -1347,8 +1409,10 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
     uint64_t word = 0, colword = 0;
     bool queued = false;  // guard-band elements settled later: the words must be stored
     if (__ballot(!(mx < nE))) {
+        // per element: in = (T >= E), not-out = !(T < -E) (true for NaN); the
+        // band is not-out and not in
         uint32_t a_lo = 0, a_hi = 0, b_lo = 0, b_hi = 0;
-        float mn = INFINITY;  // smallest |T|: no element is in the band when it exceeds E
+        uint32_t na_lo = 0, na_hi = 0, nb_lo = 0, nb_hi = 0;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const uint32_t bit = 1u << (8 * (v >> 2) + (v & 3));
@@ -1356,28 +1420,17 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
             a_hi = t.c10[v] >= E ? (a_hi | bit) : a_hi;
             b_lo = t.c01[v] >= E ? (b_lo | bit) : b_lo;
             b_hi = t.c11[v] >= E ? (b_hi | bit) : b_hi;
-            mn = fminf(fminf(mn, fabsf(t.c00[v])), fabsf(t.c01[v]));
-            mn = fminf(fminf(mn, fabsf(t.c10[v])), fabsf(t.c11[v]));
+            na_lo = t.c00[v] < nE ? na_lo : (na_lo | bit);
+            na_hi = t.c10[v] < nE ? na_hi : (na_hi | bit);
+            nb_lo = t.c01[v] < nE ? nb_lo : (nb_lo | bit);
+            nb_hi = t.c11[v] < nE ? nb_hi : (nb_hi | bit);
         }
-        uint64_t flags = 0;  // band flags of elements 4v + b (see band_element)
-        if (__ballot(!(mn > E))) {
-            uint32_t f_lo = 0, f_hi = 0;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                uint32_t &f = v < 8 ? f_lo : f_hi;
-                const int sh = 4 * (v & 7);
-                f |= (uint32_t)(!(t.c00[v] >= E) && !(t.c00[v] < nE)) << sh;
-                f |= (uint32_t)(!(t.c10[v] >= E) && !(t.c10[v] < nE)) << (sh + 1);
-                f |= (uint32_t)(!(t.c01[v] >= E) && !(t.c01[v] < nE)) << (sh + 2);
-                f |= (uint32_t)(!(t.c11[v] >= E) && !(t.c11[v] < nE)) << (sh + 3);
-            }
-            flags = ((uint64_t)f_hi << 32) | f_lo;
-        }
-        {  // only real rows and columns
+        uint64_t flags = pack_halves(na_lo, na_hi, nb_lo, nb_hi) & ~pack_halves(a_lo, a_hi, b_lo, b_hi);
+        if (__ballot(flags != 0ull)) {
+            // only real rows and columns
             const uint64_t cm = valid_cols(a.n, C);
             const bool ca = (cm >> (lane & 31)) & 1ull, cb = (cm >> (32 + (lane & 31))) & 1ull;
-            const uint64_t colmask = (ca ? 0x3333333333333333ull : 0ull) | (cb ? 0xccccccccccccccccull : 0ull);
-            flags &= colmask;
+            flags &= (ca ? 0x00000000ffffffffull : 0ull) | (cb ? 0xffffffff00000000ull : 0ull);
             if (R * 64 + 64 > a.n) {
                 for (int e = 0; e < 64; ++e) {
                     int row, col;
@@ -1388,9 +1441,14 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
         }
         if (__ballot(flags != 0ull)) {
             queued = true;
-            unsigned long long slot = 0;
-            if (lane == 0) slot = atomicAdd(a.band_cnt, 1ull);
-            slot = __shfl(slot, 0, 64);
+            if (bs.next == bs.end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.band_cnt, (unsigned long long)kBandSlotChunk);
+                base = __shfl(base, 0, 64);
+                bs.next = base;
+                bs.end = base + kBandSlotChunk;
+            }
+            const unsigned long long slot = bs.next++;
             if ((int64_t)slot < a.band_cap) {
                 int64_t *p = a.band_list + (int64_t)slot * 66;
                 if (lane == 0) {
@@ -1407,7 +1465,7 @@ __device__ __forceinline__ int mfma_settle_store(const DistArgs &a, const MfmaTi
         colword = merge_col_words(a_lo, a_hi, b_lo, b_hi);
         word = transpose_bits(colword);
     }
-    return store_pair(a, k, R, i, C, word, true, colword, queued);
+    return store_pair(a, k, R, i, C, word, true, colword, queued, km);
 }
 
 // General MFMA form (any d, explicit or dense pair lists): one wave per 64x64
@@ -1422,13 +1480,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     tile_range(a, R, kb, ke);
     const int64_t k0 = uniform64(first_pair(a, R, kb, ke));
     int mydeg = 0;
+    BandSlots bs;
     for (int64_t k = k0 + wave; k < ke; k += 4) {
         const int64_t C = uniform64(tile_col(a, k, kb));
         if (!pair_live(a, R, C)) continue;
         const float e_col = a.emarg[C * 64 + lane];
         const MfmaTile t = mfma_tile(a, R, C);
-        mydeg += mfma_settle_store(a, t, k, R, C, i, pair_margin(e_row, wave_max(e_col)));
+        mydeg += mfma_settle_store(a, t, k, R, C, i, pair_margin(e_row, wave_max(e_col)), bs);
     }
+    band_slots_release(a, bs);
     finish_deg(a, i, mydeg);
 }
 
@@ -1540,6 +1600,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __syncthreads();
     const int64_t i = R * 64 + lane;
     int mydeg = 0;
+    BandSlots bs;
     for (int idx = 0; idx < n_col; ++idx) {
         const int64_t C = col_at(idx);
         const int buf = idx & 1;
@@ -1559,6 +1620,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if constexpr (kPer > 4) n4 = src[min((int)threadIdx.x + 1024, kTileF4 - 1)];
         const float e_nx = a.emarg[Cn * 64 + lane];
         const bool work = rvalid && (!a.sym || C >= R) && pair_live(a, R, C);
+        // the pair's mask index and its mirror's, loaded during the MFMAs
+        int64_t kp = 0, km = -1;
+        if (work) {
+            kp = pair_index(a, R, C);
+            if (a.sym && C != R) km = mirror_of(a, kp, R, C);
+        }
         MfmaTile t;
         mfma_zero(t);
         if (work) {
@@ -1596,9 +1663,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
         asm volatile("" ::: "memory");
-        if (work) mydeg += mfma_settle_store(a, t, pair_index(a, R, C), R, C, i, pair_margin(e_row, e_col));
+        if (work) mydeg += mfma_settle_store(a, t, kp, R, C, i, pair_margin(e_row, e_col), bs, km);
         __syncthreads();
     }
+    band_slots_release(a, bs);
     if (rvalid && mydeg && i < a.n) atomicAdd(&a.deg[i], mydeg);
 }
 
@@ -2160,8 +2228,20 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         CNN_HIP(hipStreamSynchronize(s));
         const int64_t n_elems = std::min(counts[0], cap), n_redo = counts[1];
         if (n_redo > batch_pairs) throw HipFailure{hipErrorUnknown, "band pair queue overflow"};
-        if (n_elems > 0)
-            k_dist_band_flags<<<(unsigned)std::min<int64_t>(ceil_div(n_elems, 4), 4096), 256, 0, s>>>(b, n_elems);
+        if (n_elems > 0) {
+            Scratch<int32_t> rcnt(ctx, n_elems);
+            Scratch<int64_t> roff(ctx, n_elems);
+            k_band_count<<<(unsigned)ceil_div(n_elems, 4), 256, 0, s>>>(list.p, n_elems, rcnt.p);
+            CNN_HIP(hipGetLastError());
+            exclusive_scan_i32_to_i64(ctx, rcnt.p, roff.p, n_elems);
+            const int64_t n_el = read_scalar(ctx, roff.p + (n_elems - 1)) + read_scalar(ctx, rcnt.p + (n_elems - 1));
+            if (n_el > 0) {
+                Scratch<int64_t> elem(ctx, n_el);
+                k_band_list<<<(unsigned)ceil_div(n_elems, 4), 256, 0, s>>>(list.p, n_elems, roff.p, elem.p);
+                k_band_settle<<<(unsigned)std::min<int64_t>(ceil_div(n_el, 256), 4096), 256, 0, s>>>(b, elem.p, n_el);
+                CNN_HIP(hipGetLastError());
+            }
+        }
         if (n_redo > 0) k_dist_mfma_band<<<(unsigned)n_redo, 64, 0, s>>>(b, n_redo);
     } else if (!band && wide_kmax(a) > 0) {
         const int kmax = wide_kmax(a);
