@@ -152,7 +152,8 @@ int cnn_graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t d
 int cnn_graph_destroy(cnn_graph *g);
 int cnn_graph_nnz(const cnn_graph *g, int64_t *nnz);
 int cnn_graph_stats_get(const cnn_graph *g, cnn_graph_stats *out);
-/* Device bytes held by the graph. */
+/* Device bytes held by the graph (working coordinates, tile-pair lists, row
+ * words and/or neighbour lists; lists emitted on demand count once emitted). */
 int cnn_graph_bytes(const cnn_graph *g, int64_t *bytes);
 /* Write the neighbour lists in the caller's point order, each row ascending:
  * row_ptr (device int64[n+1]), col (device int32[nnz]).  col_cap < nnz returns
@@ -249,8 +250,9 @@ int cnn_fit_scan(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t dtyp
  * (ri * n_c + ci) * n of labels (device int64).  Per radius the graph's row
  * words are re-thresholded exactly in FP64; each labelling kernel runs once
  * per radius for all cutoff pairs.  Needs a graph that keeps its row words
- * (spatial graphs of dense enough lists: CNN_ERR_CAPACITY otherwise; use
- * cnn_fit_scan, which falls back to lists).  The scan holds its buffers until
+ * (graphs whose words take no more memory than their lists would, in either
+ * layout: CNN_ERR_CAPACITY otherwise; use cnn_fit_scan, which falls back to
+ * lists).  The scan holds its buffers until
  * cnn_scan_destroy; the graph and labels must outlive it.  lstats (nullable):
  * HOST array of n_r * n_c entries.  Replaces, like cnn_fit_scan, a loop of
  * Clustering.fit calls or HierarchicalFitterRepeat._fit steps

@@ -120,3 +120,21 @@ def test_full_scan_workload(cuda):
     # two pairs against the oracle as well
     for ri, ci in ((7, 2), (3, 5)):
         assert (lab[ri, ci].cpu().numpy() == C.fit(X, radii[ri], cs[ci], member_cutoff=2)).all()
+
+
+@pytest.mark.parametrize("precision", ["f32_band", "f64"])
+def test_scan_captured_on_dense_words(cuda, precision):
+    """A dense-layout graph (d = 24, nearest-pivot order, culled tile pairs)
+    keeps its words as well: its scan runs captured, each radius's words cut
+    exactly from the largest, and every pair equals the oracle's fit."""
+    rng = np.random.default_rng(17)
+    d, n = 24, 5000
+    centres = rng.normal(0.0, 4.0, size=(6, d))
+    X = (centres[rng.integers(0, 6, size=n)] + rng.normal(size=(n, d))).astype(np.float32)
+    radii, cs = [6.0, 5.2, 4.5], [3, 15, 60]
+    lab, gs, ls = engine.fit_scan(torch.from_numpy(X).cuda(), radii, cs, precision=precision, member_cutoff=4)
+    assert gs["layout"] == 0
+    assert all(s["sim_form"] == 2 for s in ls)
+    for ri, r in enumerate(radii):
+        for ci, c in enumerate(cs):
+            assert (lab[ri, ci].cpu().numpy() == C.fit(X, r, c, member_cutoff=4)).all(), (r, c)
