@@ -51,13 +51,11 @@ FP64_PEAK_TFLOPS = 78.6      # FMA-counted; separately rounded sub/mul/add reach
 # scaled by a copy kernel of known size (profiles/*_traffic.json).
 TRAFFIC_FILES = {
     ("k_dist_f32_cols", "md_basins_3d"): "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
-    ("k_emit_rows", "md_basins_3d"): "profiles/k_emit_rows_md_basins_3d_traffic.json",
     ("k_dist_mfma_lds", "features_64d"): "profiles/k_dist_mfma_lds_features_64d_traffic.json",
-    ("k_emit_rows", "features_64d"): "profiles/k_emit_rows_features_64d_traffic.json",
     ("k_round_b", "md_basins_3d"): "profiles/k_round_b_md_basins_3d_traffic.json",
-    ("k_hook", "md_basins_3d"): "profiles/k_hook_md_basins_3d_traffic.json",
+    ("hook", "md_basins_3d"): "profiles/hook_md_basins_3d_traffic.json",
     ("k_round_b", "features_64d"): "profiles/k_round_b_features_64d_traffic.json",
-    ("k_hook", "features_64d"): "profiles/k_hook_features_64d_traffic.json",
+    ("hook", "features_64d"): "profiles/hook_features_64d_traffic.json",
 }
 
 
@@ -273,7 +271,8 @@ def su_rooflines(ls, workload_name):
     union-find tests only a small share of the candidate edges, so B over the
     phase time can exceed the HBM peak -- that effective rate is not a frac."""
     out = {}
-    for kernel, key in (("k_round_b", "ms_round"), ("k_hook", "ms_hook")):
+    # the hook phase: k_hook_tile + k_hook_queue on row words, k_hook on lists
+    for kernel, key in (("k_round_b", "ms_round"), ("hook", "ms_hook")):
         ms = mean(ls, key)
         traffic = None
         path = TRAFFIC_FILES.get((kernel, workload_name))

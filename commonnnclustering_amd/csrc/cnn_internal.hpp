@@ -104,10 +104,10 @@ struct cnn_graph {
     float *xw32 = nullptr;       // working-order coordinates, FP32 (band precision)
     int64_t *row_ptr = nullptr;  // working CSR
     int32_t *col = nullptr;
-    int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout only)
+    int64_t *tp_ptr = nullptr;   // candidate column tiles per row tile (spatial layout, culled dense layout)
     int32_t *tp_col = nullptr;
     int32_t *selfpos = nullptr;  // per working row: position of the row's own index in its list
-    // single-batch spatial builds: per candidate tile pair k (rows of tile R = the
+    // single-batch builds with tile-pair lists: per candidate tile pair k (rows of tile R = the
     // pairs tp_ptr[R] .. tp_ptr[R+1]), mask[64k + r] holds row 64R+r's bits over
     // the 64 points of column tile tp_col[k], valid where pair_nz[k] != 0
     uint64_t *mask = nullptr;

@@ -1307,6 +1307,36 @@ __global__ __launch_bounds__(256) void k_band_list(const int64_t *__restrict__ b
     }
 }
 
+// exact_d2 with the row reads grouped: eight dimensions of both rows loaded
+// together as 16-byte pairs (rows start 16-byte aligned when d is even), so a
+// thread's accesses to a row's cache lines come close together; the sum is
+// the same sequence of separately rounded steps in ascending k.
+__device__ __forceinline__ double exact_d2_grouped(const double *__restrict__ xw64, int d, int64_t i, int64_t j) {
+    const double *a = xw64 + i * d;
+    const double *b = xw64 + j * d;
+    double s = 0.0;
+    int k = 0;
+    if ((d & 1) == 0) {
+        for (; k + 8 <= d; k += 8) {
+            const double2 *pa = (const double2 *)(a + k), *pb = (const double2 *)(b + k);
+            const double2 a0 = pa[0], a1 = pa[1], a2 = pa[2], a3 = pa[3];
+            const double2 b0 = pb[0], b1 = pb[1], b2 = pb[2], b3 = pb[3];
+            const double av[8] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y, a3.x, a3.y};
+            const double bv[8] = {b0.x, b0.y, b1.x, b1.y, b2.x, b2.y, b3.x, b3.y};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double t = av[q] - bv[q];
+                s = s + t * t;
+            }
+        }
+    }
+    for (; k < d; ++k) {
+        const double t = a[k] - b[k];
+        s = s + t * t;
+    }
+    return s;
+}
+
 // FP32 Gram value of points gi, gj from the packed augmented vectors (four
 // dimensions per float4; statistics only).
 __device__ __forceinline__ float gram_value(const DistArgs &a, int64_t gi, int64_t gj) {
@@ -1341,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_band_settle(DistArgs a, const int64_t *
         int row, col;
         band_element(le, e, row, col);
         const int64_t gi = R * 64 + row, gj = C * 64 + col;
-        const double ex = exact_d2(a.xw64, a.d, gi, gj);
+        const double ex = exact_d2_grouped(a.xw64, a.d, gi, gj);
         const bool in = ex <= a.bp.r2;
         const double d2 = a.bp.r2 - 2.0 * (double)gram_value(a, gi, gj);
         ++hits;
@@ -3085,10 +3115,73 @@ __global__ __launch_bounds__(256) void k_rethreshold(const int64_t *__restrict__
     if (!FILL && lane == 0) deg[w] = (int32_t)out;
 }
 
+__global__ void k_f64_to_f32(const double *__restrict__ x, int64_t count, float *__restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+// The graph at r2 built afresh from src's working order and coordinates (no
+// ordering step): tile pairs, words and lists as a new build at r2 makes them,
+// so the lists are exactly those of a fresh graph.  Used for graphs that hold
+// no lists (emitting them only to filter them would cost more than the build)
+// and for d > 8, where filtering costs d FP64 steps per list entry while the
+// build evaluates tile pairs in FP32 / MFMA.  Working FP32 coordinates are the
+// FP64 ones rounded, which equals rounding the caller's values (the FP64
+// copies are exact).
+static cnn_graph *graph_rebuild(cnn_ctx *ctx, const cnn_graph *src, double r2, int32_t precision) {
+    hipStream_t s = ctx->stream;
+    GraphGuard guard{new cnn_graph()};
+    cnn_graph *g = guard.g;
+    g->device = src->device;
+    g->n = src->n;
+    g->n_points = src->n_points;
+    g->d = src->d;
+    g->dtype = src->dtype;
+    g->precision = precision;
+    g->r2 = r2;
+    g->radius = std::sqrt(r2);
+    g->layout = src->layout;
+    g->n_tiles = src->n_tiles;
+    g->tune = tune_lookup(src->d, src->n, precision);
+    const int64_t n = g->n, n_pad = g->n_tiles * 64;
+    const int d = g->d;
+    const bool band = precision == CNN_F32_BAND;
+    CNN_HIP(hipEventRecord(ctx->ev[0], s));
+    g->row_ptr = dev_alloc<int64_t>(g, n + 1);
+    if (src->perm) {
+        g->perm = dev_alloc<int32_t>(g, n);
+        CNN_HIP(hipMemcpyAsync(g->perm, src->perm, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    if (src->bounds) {
+        g->bounds = dev_alloc<double>(g, d);
+        CNN_HIP(hipMemcpyAsync(g->bounds, src->bounds, d * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    g->xw64 = dev_alloc<double>(g, n_pad * d);
+    CNN_HIP(hipMemcpyAsync(g->xw64, src->xw64, n_pad * d * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (band) {
+        g->xw32 = dev_alloc<float>(g, n_pad * d);
+        if (src->xw32 && src->dp == d) {
+            CNN_HIP(hipMemcpyAsync(g->xw32, src->xw32, n_pad * d * sizeof(float), hipMemcpyDeviceToDevice, s));
+        } else {
+            k_f64_to_f32<<<(unsigned)std::min<int64_t>(ceil_div(n_pad * d, 256), 16384), 256, 0, s>>>(
+                g->xw64, n_pad * d, g->xw32);
+            CNN_HIP(hipGetLastError());
+        }
+    }
+    g->dp = d;
+    std::vector<double> lo, hi;
+    double max_norm2 = 0.0;
+    if (band && d < kMfmaMinD) bbox(ctx, g->xw64, n, d, CNN_F64, lo, hi, max_norm2);
+    build_pairs_and_csr(ctx, g, band, g->dtype, max_norm2, nullptr, nullptr);
+    return std::exchange(guard.g, nullptr);
+}
+
 cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int32_t precision) {
     if (precision != CNN_PLAIN_F64 && precision != CNN_F32_BAND) throw InvalidArg{"unknown precision"};
     if (!(r2 <= src->r2)) throw InvalidArg{"re-threshold radius must not exceed the graph radius"};
     CNN_HIP(hipSetDevice(ctx->device));
+    if (src->n > 0 && src->d > 0 && !src->bounds && (!src->col || src->d > kSpatialMaxD))
+        return graph_rebuild(ctx, src, r2, precision);
     if (src->n > 0) graph_lists(ctx, src);
     hipStream_t s = ctx->stream;
     GraphGuard guard{new cnn_graph()};

@@ -229,3 +229,34 @@ def test_dense_culling_nonfinite(cuda):
     for precision in ("f64", "f32_band"):
         for dtype in (np.float64, np.float32):
             assert_same(X.astype(dtype), r, precision)
+
+
+@pytest.mark.parametrize("src_precision,precision", [("f32_band", "f32_band"), ("f64", "f32_band"),
+                                                     ("f32_band", "f64"), ("f64", "f64")])
+@pytest.mark.parametrize("d,n,sparse", [(24, 5000, False), (3, 6000, False), (2, 3000, True)])
+def test_rethreshold_lists_exact(cuda, src_precision, precision, d, n, sparse):
+    """A graph re-thresholded to smaller radii holds exactly the oracle's lists
+    there: rebuilt from its working order (graphs that keep only their words,
+    and d > 8) or filtered in FP64 (graphs that hold lists)."""
+    import torch
+
+    from commonnnclustering_amd import engine
+
+    X, r = _clustered(n, d, 500 + d)
+    if sparse:
+        r *= 0.3  # few neighbours per word: the graph holds lists
+    X = X.astype(np.float32)
+    g = engine.build_graph(torch.from_numpy(X).cuda(), r * r, src_precision)
+    try:
+        for f in (0.9, 0.6):
+            r2 = (f * r) ** 2
+            g2 = g.rethreshold(r2, precision=precision)
+            try:
+                rp, col = g2.to_csr()
+                rp0, col0 = oracle_csr(X, f * r)
+                assert (rp.cpu().numpy() == rp0).all() and (col.cpu().numpy() == col0).all(), f
+                assert g2.stats["nnz"] == len(col0)
+            finally:
+                g2.release()
+    finally:
+        g.release()
