@@ -2678,7 +2678,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         }
         // every row with a finite point holds itself; rows without one (degree 0) are never core
         if (!g->selfpos) g->selfpos = dev_alloc<int32_t>(g, std::max<int64_t>(n, 1));
-        if (need > col_cap) {
+        if (need > col_cap || !col) {  // (a graph without any entry still holds its empty lists)
             const int64_t new_cap = batches.size() == 1 ? need : std::max<int64_t>(need, col_cap + col_cap / 2);
             int32_t *nc = nullptr;
             try {

@@ -260,3 +260,18 @@ def test_rethreshold_lists_exact(cuda, src_precision, precision, d, n, sparse):
                 g2.release()
     finally:
         g.release()
+
+
+@pytest.mark.parametrize("d", [3, 20])
+def test_graph_without_entries(cuda, d):
+    """Only non-finite points: no point is a neighbour of any (not even of
+    itself), the lists are empty and every label is noise."""
+    from gpu_util import gpu_fit
+
+    X = np.full((2500, d), np.nan)
+    X[::7, 0] = np.inf
+    for precision in ("f64", "f32_band"):
+        rp, col, st = gpu_csr(X, 0.5, precision)
+        assert (rp == 0).all() and col.size == 0 and st["nnz"] == 0
+        lab, gs, ls = gpu_fit(X, 0.5, 2, precision=precision)
+        assert (lab == 0).all() and ls["n_clusters"] == 0
