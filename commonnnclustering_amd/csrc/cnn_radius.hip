@@ -497,7 +497,7 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
                              int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
                              const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
                              int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
-                             int32_t *__restrict__ tp_col) {
+                             int32_t *__restrict__ tp_col, int32_t *__restrict__ diag) {
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
@@ -515,7 +515,11 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
             const int64_t C = S * 64 + lane;
             const bool ok = (C >= clo) && (C < chi) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
             const uint64_t b = __ballot(ok);
-            if (FILL && ok) tp_col[out + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)C;
+            if (FILL && ok) {
+                const int64_t pos = out + __popcll(b & ((1ull << lane) - 1ull));
+                tp_col[pos] = (int32_t)C;
+                if (diag && C == R) diag[R] = (int32_t)(pos - tp_ptr[R]);  // R's place in its own list
+            }
             out += __popcll(b);
         }
     }
@@ -1831,7 +1835,8 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
                                                    const double *__restrict__ bhi, int d, double thr_all,
                                                    const double *__restrict__ tile_thr,
                                                    GroupMask *__restrict__ gmask,
-                                                   unsigned long long *__restrict__ rows_evaluated) {
+                                                   unsigned long long *__restrict__ rows_evaluated,
+                                                   const int32_t *__restrict__ diag) {
     // rows_evaluated[2R]: rows of R evaluated against C over the pairs with
     // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs
     // the group boxes of R, shared by all of R's pairs: staged once per wave
@@ -1847,15 +1852,37 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     if (R >= n_tiles) return;
     const double thr = tile_thr ? tile_thr[R] : thr_all;
     unsigned long long rows = 0, rows_all = 0;
-    for (int64_t k = tp_ptr[R] + lane; k < tp_ptr[R + 1]; k += 64) {
+    const int64_t kR = tp_ptr[R];
+    const int32_t dR = diag ? diag[R] : -1;
+    for (int64_t k = kR + lane; k < tp_ptr[R + 1]; k += 64) {
         const int64_t C = tp_col[k];
-        int64_t lo = tp_ptr[C], hi = tp_ptr[C + 1];
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (tp_col[mid] < R) lo = mid + 1;
-            else hi = mid;
+        const int64_t cb = tp_ptr[C], ce = tp_ptr[C + 1];
+        // neighbouring tiles have similar lists: R sits in C's list about as far
+        // from C's own place as C sits from R's in R's list.  Look there first,
+        // 16 entries at once; search the whole list when R is not among them.
+        int64_t m = -1;
+        const int32_t dC = diag ? diag[C] : -1;
+        if (dR >= 0 && dC >= 0 && ce - cb >= 16) {
+            const int64_t est = dC - ((k - kR) - dR);
+            const int64_t w0 = cb + std::min<int64_t>(std::max<int64_t>(est - 8, 0), ce - cb - 16);
+            int32_t v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = tp_col[w0 + q];
+            int below = 0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) below += v[q] < (int32_t)R;
+            if (below < 16 && v[below] == (int32_t)R) m = w0 + below;
         }
-        mirror[k] = lo;
+        if (m < 0) {
+            int64_t lo = cb, hi = ce;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (tp_col[mid] < R) lo = mid + 1;
+                else hi = mid;
+            }
+            m = lo;
+        }
+        mirror[k] = m;
         double cl[kSpatialMaxD], ch[kSpatialMaxD];  // box of C, in registers
 #pragma unroll
         for (int q = 0; q < kSpatialMaxD; ++q) {
@@ -2126,6 +2153,9 @@ __global__ __launch_bounds__(256) void k_scatter_rows(const int64_t *__restrict_
 __global__ void k_set_last(int64_t *p, int64_t idx, const int64_t *prev_excl, const int64_t *prev_val) {
     p[idx] = prev_excl[idx - 1] + prev_val[idx - 1];
 }
+
+// p[idx] = p[idx - 1] + cnt[idx - 1]: the total after an exclusive scan of counts.
+__global__ void k_scan_total(int64_t *p, int64_t idx, const int32_t *cnt) { p[idx] = p[idx - 1] + cnt[idx - 1]; }
 
 // ------------------------------------------------------------------ host side
 
@@ -2461,6 +2491,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     std::unique_ptr<Scratch<double>> glo, ghi;
     std::unique_ptr<Scratch<GroupMask>> gmask;
     unsigned long long rows_evaluated[2] = {0, 0};  // spatial layout: rows evaluated (C >= R, all pairs)
+    std::unique_ptr<Scratch<unsigned long long>> rows_eval;  // (read with the final statistics)
     std::unique_ptr<Scratch<int64_t>> mirror;
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
@@ -2476,28 +2507,27 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const double *tthr = pk ? pk->tile_thr : nullptr;
         const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
         k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                     sl, sh, cnt.p, nullptr, nullptr);
+                                                                     sl, sh, cnt.p, nullptr, nullptr, nullptr);
         CNN_HIP(hipGetLastError());
         tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
-        const int64_t last_excl = read_scalar(ctx, tp_ptr + (T - 1));
-        const int32_t last_cnt = read_scalar(ctx, cnt.p + (T - 1));
-        total_pairs = last_excl + last_cnt;
-        CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        k_scan_total<<<1, 1, 0, s>>>(tp_ptr, T, cnt.p);
+        total_pairs = read_scalar(ctx, tp_ptr + T);
         tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
+        Scratch<int32_t> diag(ctx, T);
+        fill_i32(ctx, diag.p, -1, T);
         k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                    sl, sh, nullptr, tp_ptr, tp_col);
+                                                                    sl, sh, nullptr, tp_ptr, tp_col, diag.p);
         CNN_HIP(hipGetLastError());
         gmask.reset(new Scratch<GroupMask>(ctx, std::max<int64_t>(total_pairs, 1)));
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
-        Scratch<unsigned long long> rows_part(ctx, 2 * T), rows_eval(ctx, 2);
+        Scratch<unsigned long long> rows_part(ctx, 2 * T);
+        rows_eval.reset(new Scratch<unsigned long long>(ctx, 2));
         k_pair_info<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
-                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p);
+                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag.p);
         CNN_HIP(hipGetLastError());
-        k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval.p);
+        k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
-        CNN_HIP(hipMemcpyAsync(rows_evaluated, rows_eval.p, sizeof(rows_evaluated), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipStreamSynchronize(s));
     }
     // dense layout in nearest-pivot order: tile pairs out of reach by the pivot
     // bounds are culled (never evaluated, stored as empty pairs)
@@ -2533,21 +2563,32 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipGetLastError());
         tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
         exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
+        k_scan_total<<<1, 1, 0, s>>>(tp_ptr, T, cnt.p);
         total_pairs = (int64_t)live_count[0];
-        CNN_HIP(hipMemcpyAsync(tp_ptr + T, &total_pairs, sizeof(int64_t), hipMemcpyHostToDevice, s));
         tp_col = g->tp_col = dev_alloc<int32_t>(g, std::max<int64_t>(total_pairs, 1));
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         k_live_lists<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(live->p, T, live_words, live_pre->p, tp_ptr, tp_col,
                                                              mirror->p);
         CNN_HIP(hipGetLastError());
     }
+    // row-tile batches bounded by the mask budget (512 B per tile pair); the
+    // row offsets of the pair lists are read back only when several batches
+    // are needed
+    int64_t mask_budget = (int64_t)8 << 30;
+    if (const char *ev = std::getenv("CNN_HIP_MASK_BUDGET")) mask_budget = std::max<int64_t>(512, std::atoll(ev));
+    const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
+    const bool one_batch = total_pairs <= max_pairs_per_batch;
     std::vector<int64_t> h_tp_ptr;
-    if (tp_ptr) {
+    if (tp_ptr && !one_batch) {
         h_tp_ptr.resize(T + 1);
         CNN_HIP(hipMemcpyAsync(h_tp_ptr.data(), tp_ptr, (T + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         CNN_HIP(hipStreamSynchronize(s));
     }
-    auto pair_begin = [&](int64_t R) { return tp_ptr ? h_tp_ptr[R] : R * T; };
+    auto pair_begin = [&](int64_t R) -> int64_t {
+        if (R <= 0) return 0;
+        if (R >= T) return total_pairs;
+        return tp_ptr ? h_tp_ptr[R] : R * T;
+    };
     CNN_HIP(hipEventRecord(ctx->ev[1], s));
 
     BandParams bp;
@@ -2564,12 +2605,9 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     CNN_HIP(hipMemsetAsync(bstats.p, 0, 4 * sizeof(unsigned long long), s));
     Scratch<int32_t> deg(ctx, n);
 
-    // row-tile batches bounded by the mask budget (512 B per tile pair)
-    int64_t mask_budget = (int64_t)8 << 30;
-    if (const char *ev = std::getenv("CNN_HIP_MASK_BUDGET")) mask_budget = std::max<int64_t>(512, std::atoll(ev));
-    const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
     std::vector<std::pair<int64_t, int64_t>> batches;
-    for (int64_t R = 0; R < T;) {
+    if (one_batch) batches.push_back({0, T});
+    for (int64_t R = one_batch ? T : 0; R < T;) {
         int64_t R1 = R + 1;
         while (R1 < T && pair_begin(R1 + 1) - pair_begin(R) <= max_pairs_per_batch) ++R1;
         batches.push_back({R, R1});
@@ -2596,6 +2634,16 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         pair_nz_p = pair_nz_s->p;
     }
     float ms_dist = 0.0f, ms_emit = 0.0f, ms_kernel = 0.0f;
+    bool emitted = false;  // the last batch emitted lists (events 4, 5 hold its emission)
+    auto add_batch_times = [&]() {
+        float t1 = 0, t2 = 0, t3 = 0;
+        CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
+        CNN_HIP(hipEventElapsedTime(&t3, ctx->ev[6], ctx->ev[7]));
+        if (emitted) CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[4], ctx->ev[5]));
+        ms_dist += t1;
+        ms_emit += t2;
+        ms_kernel += t3;
+    };
     int64_t nnz_total = 0;
     int64_t col_cap = 0;
     int32_t *col = nullptr;
@@ -2651,8 +2699,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
         const int64_t rows = row_hi - row_lo;
         exclusive_scan_i32_to_i64(ctx, deg.p + row_lo, g->row_ptr + row_lo, rows);
-        const int64_t last_excl = read_scalar(ctx, g->row_ptr + row_hi - 1);
-        const int32_t last_deg = read_scalar(ctx, deg.p + row_hi - 1);
+        int64_t last_excl = 0;
+        int32_t last_deg = 0;
+        CNN_HIP(hipMemcpyAsync(&last_excl, g->row_ptr + row_hi - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipMemcpyAsync(&last_deg, deg.p + row_hi - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
         const int64_t batch_total = last_excl + last_deg;
         if (nnz_total) {
             // shift by the entries of earlier batches
@@ -2665,14 +2716,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         }
         const int64_t need = nnz_total + batch_total;
         words_only = sym && tp_ptr && !pk && g->mask && (int64_t)512 * total_pairs <= (int64_t)4 * need;
-        if (words_only) {
-            CNN_HIP(hipEventRecord(ctx->ev[5], s));
-            CNN_HIP(hipEventSynchronize(ctx->ev[5]));
-            float t1 = 0, t3 = 0;
-            CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
-            CNN_HIP(hipEventElapsedTime(&t3, ctx->ev[6], ctx->ev[7]));
-            ms_dist += t1;
-            ms_kernel += t3;
+        if (words_only) {  // (one batch: its times are read after the final synchronisation)
+            emitted = false;
             nnz_total = need;
             continue;
         }
@@ -2703,14 +2748,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         if (rows > 0)
             launch_emit(s, a, b.second - b.first, g->layout != 1, g->tune.emit_waves, g->row_ptr, col, g->selfpos);
         CNN_HIP(hipEventRecord(ctx->ev[5], s));
-        CNN_HIP(hipEventSynchronize(ctx->ev[5]));
-        float t1 = 0, t2 = 0, t3 = 0;
-        CNN_HIP(hipEventElapsedTime(&t1, ctx->ev[2], ctx->ev[3]));
-        CNN_HIP(hipEventElapsedTime(&t2, ctx->ev[4], ctx->ev[5]));
-        CNN_HIP(hipEventElapsedTime(&t3, ctx->ev[6], ctx->ev[7]));
-        ms_dist += t1;
-        ms_emit += t2;
-        ms_kernel += t3;
+        emitted = true;
+        if (batches.size() > 1) {  // the events are recorded again by the next batch
+            CNN_HIP(hipEventSynchronize(ctx->ev[5]));
+            add_batch_times();
+        }
         nnz_total = need;
     }
     if (!words_only && g->mask) {
@@ -2725,7 +2767,10 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &nnz_total, sizeof(int64_t), hipMemcpyHostToDevice, s));
     unsigned long long hb[4];
     CNN_HIP(hipMemcpyAsync(hb, bstats.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+    if (rows_eval)
+        CNN_HIP(hipMemcpyAsync(rows_evaluated, rows_eval->p, sizeof(rows_evaluated), hipMemcpyDeviceToHost, s));
     CNN_HIP(hipStreamSynchronize(s));
+    if (batches.size() == 1) add_batch_times();
     g->nnz = nnz_total;
     float ms_prep = 0.0f;
     CNN_HIP(hipEventElapsedTime(&ms_prep, ctx->ev[0], ctx->ev[1]));
