@@ -81,6 +81,11 @@ struct cnn_ctx {
     // when set, every Scratch buffer comes from this fixed buffer instead of
     // the arena (no device allocation)
     cnn::FixedWorkspace *fixed = nullptr;
+    // page-locked host staging for small device-to-host reads: copies into it
+    // are asynchronous, so several reads share one synchronisation (a copy to
+    // pageable memory is a round trip of its own)
+    static constexpr size_t kPinnedBytes = (size_t)1 << 16;
+    uint8_t *pinned = nullptr;
 };
 
 struct cnn_graph {
@@ -223,11 +228,34 @@ void fill_i64(cnn_ctx *ctx, int64_t *p, int64_t v, int64_t n);
 
 template <typename T>
 T read_scalar(cnn_ctx *ctx, const T *dev) {
-    T h{};
-    CNN_HIP(hipMemcpyAsync(&h, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    T *h = (T *)ctx->pinned;
+    CNN_HIP(hipMemcpyAsync(h, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
     CNN_HIP(hipStreamSynchronize(ctx->stream));
-    return h;
+    return *h;
 }
+
+// Small device-to-host reads gathered in the context's page-locked staging
+// buffer: get() enqueues a copy and returns where its value will be, wait()
+// synchronises once; the values are valid after wait() until the next use of
+// the staging buffer (read_scalar included).
+struct PinnedReads {
+    cnn_ctx *ctx;
+    size_t off = 0;
+    explicit PinnedReads(cnn_ctx *c) : ctx(c) {}
+    template <typename T>
+    T *get(const T *dev, size_t count = 1) {
+        off = (off + 15) & ~(size_t)15;
+        if (off + count * sizeof(T) > cnn_ctx::kPinnedBytes) throw HipFailure{hipErrorInvalidValue, "staging buffer"};
+        T *h = (T *)(ctx->pinned + off);
+        CNN_HIP(hipMemcpyAsync(h, dev, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+        off += count * sizeof(T);
+        return h;
+    }
+    void wait() { CNN_HIP(hipStreamSynchronize(ctx->stream)); }
+};
+
+// dev[0] = v, enqueued as a kernel (the value travels as an argument).
+void set_i64(cnn_ctx *ctx, int64_t *dev, int64_t v);
 
 // --------------------------------------------------------------- graph / labels
 // transient: the graph is freed before the calling entry point returns, on the

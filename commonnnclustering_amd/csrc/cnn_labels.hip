@@ -1057,9 +1057,9 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             const int64_t rows = ceil_div(n, stride);
             k_row_word_sample<<<blocks_for(rows, 4), 256, 0, s>>>(req.row_ptr, req.col, n, stride, sums.p);
             CNN_HIP(hipGetLastError());
-            unsigned long long h[2] = {0, 0};
-            CNN_HIP(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, s));
-            CNN_HIP(hipStreamSynchronize(s));
+            PinnedReads rd(ctx);
+            const unsigned long long *h = rd.get(sums.p, 2);
+            rd.wait();
             build = h[1] > 0 && h[0] >= (unsigned long long)req.words_min_fill * h[1];
         }
         if (build) {
@@ -1068,16 +1068,16 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             k_row_word_count<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wcnt.p);
             CNN_HIP(hipGetLastError());
             exclusive_scan_i32_to_i64(ctx, wcnt.p, wptr->p, n);
-            const int64_t last_excl = read_scalar(ctx, wptr->p + (n - 1));
-            const int32_t last_cnt = read_scalar(ctx, wcnt.p + (n - 1));
-            const int64_t n_words = last_excl + last_cnt;
-            CNN_HIP(hipMemcpyAsync(wptr->p + n, &n_words, sizeof(int64_t), hipMemcpyHostToDevice, s));
+            PinnedReads rd(ctx);
+            const int64_t *last_excl = rd.get(wptr->p + (n - 1));
+            const int32_t *last_cnt = rd.get(wcnt.p + (n - 1));
+            rd.wait();
+            const int64_t n_words = *last_excl + *last_cnt;
+            set_i64(ctx, wptr->p + n, n_words);
             wtile.reset(new Scratch<int32_t>(ctx, n_words));
             wword.reset(new Scratch<uint64_t>(ctx, n_words));
             k_row_word_fill<<<blocks_for(n, 4), 256, 0, s>>>(req.row_ptr, req.col, n, wptr->p, wtile->p, wword->p);
             CNN_HIP(hipGetLastError());
-            // the host value copied above must outlive the copy
-            CNN_HIP(hipStreamSynchronize(s));
             words = true;
             a.wptr = wptr->p;
             a.wtile = wtile->p;
@@ -1200,9 +1200,14 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     CNN_HIP(hipEventRecord(ctx->ev[2], s));
     unsigned long long h[8];
     unsigned long long ht[4 * kStatSlots];
-    CNN_HIP(hipMemcpyAsync(h, counters.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipMemcpyAsync(ht, tstats.p, sizeof(ht), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipStreamSynchronize(s));
+    {
+        PinnedReads rd(ctx);
+        const unsigned long long *hp = rd.get(counters.p, 8);
+        const unsigned long long *htp = rd.get(tstats.p, 4 * kStatSlots);
+        rd.wait();
+        std::copy(hp, hp + 8, h);
+        std::copy(htp, htp + 4 * kStatSlots, ht);
+    }
     h[0] = h[1] = 0;
     unsigned long long rows = 0;
     for (int i = 0; i < kStatSlots; ++i) {

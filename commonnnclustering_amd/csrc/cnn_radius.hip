@@ -90,6 +90,7 @@ __device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every t
 
 struct KeySpec {
     int nk;         // dims in the key (1..3)
+    int bits;       // bits per dimension (the key holds nk * bits)
     int dim[3];
     double lo[3];
     double scale[3];
@@ -2283,9 +2284,9 @@ static void launch_dist(cnn_ctx *ctx, const DistArgs &a, int64_t n_rt, bool band
         }
         CNN_HIP(hipGetLastError());
         CNN_HIP(hipEventRecord(ctx->ev[7], s));
-        int64_t counts[2];
-        CNN_HIP(hipMemcpyAsync(counts, cnt.p, sizeof(counts), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipStreamSynchronize(s));
+        PinnedReads rd(ctx);
+        const int64_t *counts = rd.get(cnt.p, 2);
+        rd.wait();
         const int64_t n_elems = std::min(counts[0], cap), n_redo = counts[1];
         if (n_redo > batch_pairs) throw HipFailure{hipErrorUnknown, "band pair queue overflow"};
         if (n_elems > 0) {
@@ -2421,9 +2422,18 @@ static void bbox(cnn_ctx *ctx, const void *X, int64_t n, int d, int dtype, std::
     else
         k_bbox_partial<double><<<nb, 256, 0, s>>>((const double *)X, n, d, dpart.p, dnorm.p);
     CNN_HIP(hipGetLastError());
-    CNN_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipMemcpyAsync(pnorm.data(), dnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipStreamSynchronize(s));
+    if ((part.size() + nb) * sizeof(double) + 32 <= cnn_ctx::kPinnedBytes) {
+        PinnedReads rd(ctx);
+        const double *hp = rd.get(dpart.p, part.size());
+        const double *hn = rd.get(dnorm.p, (size_t)nb);
+        rd.wait();
+        std::copy(hp, hp + part.size(), part.begin());
+        std::copy(hn, hn + nb, pnorm.begin());
+    } else {
+        CNN_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipMemcpyAsync(pnorm.data(), dnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        CNN_HIP(hipStreamSynchronize(s));
+    }
     lo.assign(d, INFINITY);
     hi.assign(d, -INFINITY);
     max_norm2 = 0.0;
@@ -2436,15 +2446,21 @@ static void bbox(cnn_ctx *ctx, const void *X, int64_t n, int d, int dtype, std::
     }
 }
 
-// Morton key over the (up to) three widest dims, `bits` bits per dim.
-static KeySpec make_keyspec(int d, const std::vector<double> &lo, const std::vector<double> &hi, int max_bits) {
+// Morton key over the (up to) three widest dims.  Bits per dimension: about 2^(3 nk) cells per point (ceil(log2 n) / nk + 3),
+// so points rarely share a cell and the order is that of a full-resolution
+// key, while the radix sort runs over fewer key bits.
+static KeySpec make_keyspec(int d, const std::vector<double> &lo, const std::vector<double> &hi, int max_bits,
+                            int64_t n) {
     KeySpec ks{};
     std::vector<int> dims(d);
     for (int k = 0; k < d; ++k) dims[k] = k;
     auto ext = [&](int k) { return (hi[k] >= lo[k]) ? hi[k] - lo[k] : 0.0; };
     std::stable_sort(dims.begin(), dims.end(), [&](int x, int y) { return ext(x) > ext(y); });
     ks.nk = std::min(d, 3);
-    const int bits = std::min(max_bits / ks.nk, ks.nk == 1 ? 62 : (ks.nk == 2 ? 32 : 21));
+    int lg = 1;
+    while (lg < 62 && ((int64_t)1 << lg) < n) ++lg;
+    const int bits = std::min({max_bits / ks.nk, ks.nk == 1 ? 62 : (ks.nk == 2 ? 32 : 21), (lg + ks.nk - 1) / ks.nk + 3});
+    ks.bits = bits;
     ks.qmax = std::ldexp(1.0, bits) - 1.0;
     for (int t = 0; t < ks.nk; ++t) {
         int k = dims[t];
@@ -2552,8 +2568,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         k_dense_live<<<(unsigned)std::min<int64_t>(ceil_div(T * live_words, 4), 2048), 256, 0, s>>>(
             bnd.p, home.p, T, live_words, rc, live->p, lcnt.p);
         CNN_HIP(hipGetLastError());
-        CNN_HIP(hipMemcpyAsync(live_count, lcnt.p, sizeof(live_count), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipStreamSynchronize(s));
+        {
+            PinnedReads rd(ctx);
+            const unsigned long long *lc = rd.get(lcnt.p, 2);
+            rd.wait();
+            live_count[0] = lc[0];
+            live_count[1] = lc[1];
+        }
         // the live pairs become the graph's explicit tile-pair lists: the
         // words of culled pairs are never stored, and the label pass can test
         // similarities on the words as for the spatial layout
@@ -2699,12 +2720,11 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
         const int64_t rows = row_hi - row_lo;
         exclusive_scan_i32_to_i64(ctx, deg.p + row_lo, g->row_ptr + row_lo, rows);
-        int64_t last_excl = 0;
-        int32_t last_deg = 0;
-        CNN_HIP(hipMemcpyAsync(&last_excl, g->row_ptr + row_hi - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipMemcpyAsync(&last_deg, deg.p + row_hi - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        CNN_HIP(hipStreamSynchronize(s));
-        const int64_t batch_total = last_excl + last_deg;
+        PinnedReads rd(ctx);
+        const int64_t *last_excl = rd.get(g->row_ptr + row_hi - 1);
+        const int32_t *last_deg = rd.get(deg.p + row_hi - 1);
+        rd.wait();
+        const int64_t batch_total = *last_excl + *last_deg;
         if (nnz_total) {
             // shift by the entries of earlier batches
             std::vector<int64_t> hbuf(rows);
@@ -2764,12 +2784,15 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         g->mask = nullptr;
         g->pair_nz = nullptr;
     }
-    CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &nnz_total, sizeof(int64_t), hipMemcpyHostToDevice, s));
-    unsigned long long hb[4];
-    CNN_HIP(hipMemcpyAsync(hb, bstats.p, sizeof(hb), hipMemcpyDeviceToHost, s));
-    if (rows_eval)
-        CNN_HIP(hipMemcpyAsync(rows_evaluated, rows_eval->p, sizeof(rows_evaluated), hipMemcpyDeviceToHost, s));
-    CNN_HIP(hipStreamSynchronize(s));
+    set_i64(ctx, g->row_ptr + n, nnz_total);
+    PinnedReads rd(ctx);
+    const unsigned long long *hb = rd.get(bstats.p, 4);
+    const unsigned long long *hre = rows_eval ? rd.get(rows_eval->p, 2) : nullptr;
+    rd.wait();
+    if (hre) {
+        rows_evaluated[0] = hre[0];
+        rows_evaluated[1] = hre[1];
+    }
     if (batches.size() == 1) add_batch_times();
     g->nnz = nnz_total;
     float ms_prep = 0.0f;
@@ -2848,7 +2871,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     double max_norm2 = 0.0;
     if (g->layout == 1 || (band && d < kMfmaMinD)) bbox(ctx, X, n, d, dtype, lo, hi, max_norm2);
     if (g->layout == 1) {
-        const KeySpec ks = make_keyspec(d, lo, hi, 64);
+        const KeySpec ks = make_keyspec(d, lo, hi, 64, n);
         g->perm = dev_alloc<int32_t>(g, n);
         Scratch<uint64_t> keys(ctx, n);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
@@ -2857,7 +2880,7 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         else
             k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
         CNN_HIP(hipGetLastError());
-        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk == 1 ? 62 : (ks.nk == 2 ? 64 : 63));
+        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk * ks.bits);
     } else if (!bounds_host && d > kSpatialMaxD && n >= kPivotMinN) {
         const int L = (int)std::min<int64_t>(256, n / 64);
         const int S = std::min(kSuperPivots, L);
@@ -2995,7 +3018,9 @@ cnn_graph *graph_build_packed(cnn_ctx *ctx, const void *X, const int64_t *offset
     bbox(ctx, X, N, d, dtype, lo, hi, max_norm2);
     int ds_bits = 1;
     while ((1 << ds_bits) < n_sets) ++ds_bits;
-    const KeySpec ks = make_keyspec(d, lo, hi, 63 - ds_bits);
+    int64_t n_max = 0;
+    for (int32_t b = 0; b < n_sets; ++b) n_max = std::max(n_max, offsets[b + 1] - offsets[b]);
+    const KeySpec ks = make_keyspec(d, lo, hi, 63 - ds_bits, n_max);
     const int key_bits = (int)std::log2(ks.qmax + 1.0) * ks.nk;
     Scratch<int64_t> doffs(ctx, n_sets + 1), dtstart(ctx, n_sets + 1);
     CNN_HIP(hipMemcpyAsync(doffs.p, offsets, (n_sets + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -3281,7 +3306,7 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
     const int64_t last_excl = read_scalar(ctx, g->row_ptr + (n - 1));
     const int32_t last_deg = read_scalar(ctx, deg.p + (n - 1));
     g->nnz = last_excl + last_deg;
-    CNN_HIP(hipMemcpyAsync(g->row_ptr + n, &g->nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    set_i64(ctx, g->row_ptr + n, g->nnz);
     g->col = dev_alloc<int32_t>(g, g->nnz);
     k_rethreshold<true><<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(src->row_ptr, src->col, src->xw64, n, g->d, r2, nullptr,
                                                                  g->row_ptr, g->col);
@@ -3378,7 +3403,7 @@ int64_t csr_filter(cnn_ctx *ctx, const double *xw64, int64_t n, int d, const int
     const int64_t last_excl = read_scalar(ctx, out_rp + (n - 1));
     const int32_t last_deg = read_scalar(ctx, deg.p + (n - 1));
     const int64_t nnz = last_excl + last_deg;
-    CNN_HIP(hipMemcpyAsync(out_rp + n, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    set_i64(ctx, out_rp + n, nnz);
     hipError_t e = hipMalloc(out_col, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t));
     if (e != hipSuccess) {
         (void)hipGetLastError();

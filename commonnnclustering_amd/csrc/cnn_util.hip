@@ -84,6 +84,12 @@ static void fill(cnn_ctx *ctx, T *p, T v, int64_t n) {
 }
 
 void fill_i32(cnn_ctx *ctx, int32_t *p, int32_t v, int64_t n) { fill(ctx, p, v, n); }
+
+__global__ void k_set_i64(int64_t *p, int64_t v) { *p = v; }
+void set_i64(cnn_ctx *ctx, int64_t *dev, int64_t v) {
+    k_set_i64<<<1, 1, 0, ctx->stream>>>(dev, v);
+    CNN_HIP(hipGetLastError());
+}
 void fill_u8(cnn_ctx *ctx, uint8_t *p, uint8_t v, int64_t n) { fill(ctx, p, v, n); }
 void fill_i64(cnn_ctx *ctx, int64_t *p, int64_t v, int64_t n) { fill(ctx, p, v, n); }
 void fill_u64(cnn_ctx *ctx, unsigned long long *p, unsigned long long v, int64_t n) { fill(ctx, p, v, n); }
@@ -202,6 +208,7 @@ int cnn_ctx_create(int device, cnn_ctx **out) {
         CNN_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
         ctx->arena = new Arena(device);
+        CNN_HIP(hipHostMalloc((void **)&ctx->pinned, cnn_ctx::kPinnedBytes, hipHostMallocDefault));
         noalloc_setup(device);
         return (int)CNN_OK;
     });
@@ -222,6 +229,7 @@ int cnn_ctx_destroy(cnn_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     delete ctx->arena;
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
