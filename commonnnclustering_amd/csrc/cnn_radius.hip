@@ -492,6 +492,35 @@ __device__ __forceinline__ bool boxes_close(const double *__restrict__ alo, cons
 // Count (FILL == false) or list (FILL == true) the candidate column tiles of
 // each row tile: super-tiles are culled first (lane per super-tile), then the
 // 64 tiles of every surviving super-tile (lane per tile), in ascending order.
+// Box test against R's box held in registers (d <= kSpatialMaxD): all of the
+// other box's coordinates are loaded before any is used.
+struct RowBox {
+    double lo[kSpatialMaxD], hi[kSpatialMaxD];
+    int d;
+    double thr;
+    __device__ __forceinline__ bool close(const double *__restrict__ olo, const double *__restrict__ ohi) const {
+        double l[kSpatialMaxD], h[kSpatialMaxD];
+#pragma unroll
+        for (int k = 0; k < kSpatialMaxD; ++k) {
+            l[k] = k < d ? olo[k] : 0.0;
+            h[k] = k < d ? ohi[k] : 0.0;
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSpatialMaxD; ++k) {
+            double g = fmax(l[k] - hi[k], lo[k] - h[k]);
+            g = fmax(g, 0.0);
+            s += k < d ? g * g : 0.0;
+        }
+        return s <= thr;
+    }
+};
+
+// Count (FILL == false) or list (FILL == true) the candidate column tiles of
+// each row tile: super-tiles are culled first (lane per super-tile), then the
+// 64 tiles of every surviving super-tile (lane per tile), in ascending order,
+// four surviving super-tiles per step (their loads in flight together).
+// FILL also records diag[R], R's place in its own list.
 template <bool FILL>
 __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi,
                              const double *__restrict__ slo, const double *__restrict__ shi, int d, int64_t n_tiles,
@@ -502,26 +531,45 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
     const int lane = threadIdx.x & 63;
     const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (R >= n_tiles) return;
-    const double *rlo = blo + R * d, *rhi = bhi + R * d;
-    const double thr = tile_thr ? tile_thr[R] : thr_all;
+    RowBox rb;
+    rb.d = d;
+    rb.thr = tile_thr ? tile_thr[R] : thr_all;
+#pragma unroll
+    for (int k = 0; k < kSpatialMaxD; ++k) {
+        rb.lo[k] = k < d ? blo[R * d + k] : 0.0;
+        rb.hi[k] = k < d ? bhi[R * d + k] : 0.0;
+    }
     const int64_t clo = seg_lo ? seg_lo[R] : 0, chi = seg_hi ? seg_hi[R] : n_tiles;
     int64_t out = FILL ? tp_ptr[R] : 0;
     for (int64_t S0 = clo / 64; S0 * 64 < chi; S0 += 64) {
         const int64_t Sl = S0 + lane;
-        const bool sok = (Sl < n_super) && (Sl * 64 < chi) && boxes_close(rlo, rhi, slo + Sl * d, shi + Sl * d, d, thr);
+        const bool sok = (Sl < n_super) && (Sl * 64 < chi) && rb.close(slo + Sl * d, shi + Sl * d);
         uint64_t sbits = __ballot(sok);
         while (sbits) {
-            const int64_t S = S0 + __ffsll((unsigned long long)sbits) - 1;
-            sbits &= sbits - 1;
-            const int64_t C = S * 64 + lane;
-            const bool ok = (C >= clo) && (C < chi) && boxes_close(rlo, rhi, blo + C * d, bhi + C * d, d, thr);
-            const uint64_t b = __ballot(ok);
-            if (FILL && ok) {
-                const int64_t pos = out + __popcll(b & ((1ull << lane) - 1ull));
-                tp_col[pos] = (int32_t)C;
-                if (diag && C == R) diag[R] = (int32_t)(pos - tp_ptr[R]);  // R's place in its own list
+            int64_t Sq[4];
+            int m = 0;
+            while (sbits && m < 4) {
+                Sq[m++] = S0 + __ffsll((unsigned long long)sbits) - 1;
+                sbits &= sbits - 1;
             }
-            out += __popcll(b);
+            bool ok[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t C = (q < m ? Sq[q] : 0) * 64 + lane;
+                ok[q] = q < m && C >= clo && C < chi && rb.close(blo + C * d, bhi + C * d);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= m) break;
+                const int64_t C = Sq[q] * 64 + lane;
+                const uint64_t b = __ballot(ok[q]);
+                if (FILL && ok[q]) {
+                    const int64_t pos = out + __popcll(b & ((1ull << lane) - 1ull));
+                    tp_col[pos] = (int32_t)C;
+                    if (diag && C == R) diag[R] = (int32_t)(pos - tp_ptr[R]);  // R's place in its own list
+                }
+                out += __popcll(b);
+            }
         }
     }
     if (!FILL && lane == 0) cnt[R] = (int32_t)out;
@@ -1841,9 +1889,13 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     // rows_evaluated[2R]: rows of R evaluated against C over the pairs with
     // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs
     // the group boxes of R, shared by all of R's pairs: staged once per wave
+    // two waves per row tile (pairs 64h + lane, step 128): twice the loads in
+    // flight of one wave's dependent chain
     __shared__ double sgl[4][kRowGroups * kSpatialMaxD], sgh[4][kRowGroups * kSpatialMaxD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t R = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t gw = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t R = gw >> 1;
+    const int half = (int)(gw & 1);
     if (R < n_tiles)
         for (int t = lane; t < kRowGroups * d; t += 64) {
             sgl[wave][t] = glo[R * kRowGroups * d + t];
@@ -1855,7 +1907,7 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     unsigned long long rows = 0, rows_all = 0;
     const int64_t kR = tp_ptr[R];
     const int32_t dR = diag ? diag[R] : -1;
-    for (int64_t k = kR + lane; k < tp_ptr[R + 1]; k += 64) {
+    for (int64_t k = kR + 64 * half + lane; k < tp_ptr[R + 1]; k += 128) {
         const int64_t C = tp_col[k];
         const int64_t cb = tp_ptr[C], ce = tp_ptr[C + 1];
         // neighbouring tiles have similar lists: R sits in C's list about as far
@@ -1911,9 +1963,9 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     }
     rows = (unsigned long long)wave_sum64((long long)rows);
     rows_all = (unsigned long long)wave_sum64((long long)rows_all);
-    if (lane == 0) {
-        rows_evaluated[2 * R] = rows;
-        rows_evaluated[2 * R + 1] = rows_all;
+    if (lane == 0) {  // (zeroed beforehand; the two waves of R add)
+        if (rows) atomicAdd(&rows_evaluated[2 * R], rows);
+        if (rows_all) atomicAdd(&rows_evaluated[2 * R + 1], rows_all);
     }
 }
 
@@ -2539,7 +2591,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         Scratch<unsigned long long> rows_part(ctx, 2 * T);
         rows_eval.reset(new Scratch<unsigned long long>(ctx, 2));
-        k_pair_info<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
+        fill_u64(ctx, rows_part.p, 0ull, 2 * T);
+        k_pair_info<<<(unsigned)ceil_div(2 * T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
                                                              bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag.p);
         CNN_HIP(hipGetLastError());
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
