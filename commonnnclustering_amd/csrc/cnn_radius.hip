@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
             }
         });
         const uint64_t colword = ((uint64_t)chi << 32) | clo;
-        mydeg += store_pair(a, k, R, i_lane, C, transpose_bits(colword), true, colword);
+        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_bits(colword) : 0ull, true, colword);
     }
     finish_deg(a, i_lane, mydeg);
 }
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
                 }
             }
         }
-        mydeg += store_pair(a, k, R, i_lane, C, transpose_bits(colword), true, colword);
+        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_bits(colword) : 0ull, true, colword);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i_lane, mydeg);
