@@ -447,21 +447,58 @@ __device__ __forceinline__ int atomic_load_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bit transpose across the wave: lane j receives the word whose bit i is bit j
-// of lane i's input.  Six butterfly stages swap the off-diagonal blocks of
-// width 32, 16, ..., 1 between partner lanes (lane ^ s).
-__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int sft = 32 >> t;
-        const uint64_t m = masks[t];
-        const uint64_t y = __shfl_xor(x, sft, 64);
-        x = (lane & sft) ? ((x & ~m) | ((y >> sft) & m)) : ((x & m) | ((y & m) << sft));
+// Value of lane ^ S (S = 1..16) without LDS address traffic: DPP quad
+// permutes for 1 and 2, half-row mirror (^7) then quad reverse (^3) for 4,
+// row rotate by 8 for 8, and gfx950's v_permlane16_swap for 16 (it swaps the
+// odd 16-lane rows of its first operand with the even rows of its second).
+template <int S>
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v) {
+    if constexpr (S == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (S == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (S == 4) {
+        const int m7 = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp(m7, 0x1B, 0xF, 0xF, false);
+    } else if constexpr (S == 8) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
+    } else {
+        static_assert(S == 16, "lane_xor32 covers 1, 2, 4, 8 and 16");
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (threadIdx.x & 16) ? r[0] : r[1];
     }
-    return x;
+}
+
+// One butterfly stage of transpose_bits for S < 32 on the two 32-bit halves
+// of the lane's word.  The half of each 32-bit half that the partner lane
+// (lane ^ S) needs is packed into one 32-bit payload (the low half's share in
+// the M positions, the high half's in ~M), exchanged once, and inserted.
+template <int S, uint32_t M>
+__device__ __forceinline__ void transpose_stage(uint32_t &lo, uint32_t &hi) {
+    const bool up = (threadIdx.x & S) != 0;
+    const uint32_t sa = up ? 0u : (uint32_t)S, sb = (uint32_t)S - sa;
+    const uint32_t keep = up ? ~M : M;
+    const uint32_t p = ((lo >> sa) & M) | ((hi << sb) & ~M);
+    const uint32_t q = lane_xor32<S>(p);
+    lo = (lo & keep) | ((q << sa) & ~keep);
+    hi = (hi & keep) | ((q >> sb) & ~keep);
+}
+
+// Bit transpose across the wave: lane j receives the word whose bit i is bit j
+// of lane i's input.  Each stage swaps the off-diagonal blocks of width
+// 32, 16, ..., 1 between partner lanes (lane ^ s); the stages act on
+// independent (lane bit, word bit) pairs, so their order is free.  Width 32 is
+// one v_permlane32_swap of the low halves' upper row with the high halves'
+// lower row; the others are transpose_stage.
+__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
+    const auto r = __builtin_amdgcn_permlane32_swap((uint32_t)x, (uint32_t)(x >> 32), false, false);
+    uint32_t lo = r[0], hi = r[1];
+    transpose_stage<16, 0x0000FFFFu>(lo, hi);
+    transpose_stage<8, 0x00FF00FFu>(lo, hi);
+    transpose_stage<4, 0x0F0F0F0Fu>(lo, hi);
+    transpose_stage<2, 0x33333333u>(lo, hi);
+    transpose_stage<1, 0x55555555u>(lo, hi);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // |A & B| >= c for sorted, duplicate-free lists, evaluated by the whole wave:
