@@ -310,7 +310,7 @@ cnn_scan *scan_create(cnn_ctx *ctx, const cnn_graph *g, const double *r2, int32_
                 cur_r2 = r2[ri];
                 next ^= 1;
             }
-            MultiLabelSource src{cur_mask, cur_nz, g->tp_ptr, g->tp_col, cur_deg, g->perm};
+            MultiLabelSource src{cur_mask, cur_nz, g->tp_ptr, g->tp_col, cur_deg, g->perm, g->n_tile_pairs};
             labels_enqueue_multi(ctx, b, src, g->tune.root_passes, member_cutoff, max_clusters,
                                  labels + (int64_t)ri * n_c * n, sc->counters + (int64_t)ri * n_c * 8,
                                  sc->tstats + (int64_t)ri * n_c * kScanStatSlots);

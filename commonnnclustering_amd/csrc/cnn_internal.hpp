@@ -313,6 +313,7 @@ struct LabelRequest {
     const int64_t *tp_ptr = nullptr;  // candidate column tiles per row tile
     const int32_t *tp_col = nullptr;
     int64_t n_tiles = 0;              // 0: no tile structure (caller CSR)
+    int64_t n_tile_pairs = 0;         // tp_ptr[n_tiles], known on the host
     const int32_t *selfpos = nullptr; // per row: position of the row's own index in its list (nullable)
     const uint64_t *mask = nullptr;   // the lists as row words per candidate tile pair (nullable)
     const uint8_t *pair_nz = nullptr;
@@ -383,6 +384,7 @@ struct MultiLabelSource {
     const int32_t *tp_col;
     const int32_t *deg;
     const int32_t *perm;
+    int64_t n_tile_pairs;  // tp_ptr[n_tiles], known on the host
 };
 void multi_labels_bytes(int64_t n, int64_t n_tiles, int n_c, MultiLabelSizes &sz);
 // labels: n_c blocks of n (block y = pair y), sorted by size; counters: 8 per

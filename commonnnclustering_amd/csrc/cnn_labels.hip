@@ -95,6 +95,12 @@ struct LabelArgs {
     const int32_t *csim_y;
     const int32_t *mcore_y;
     int64_t tiles;
+    // rows k_round_tile hands to k_round_queue (nullable: it tests them itself):
+    // {row, first pair, end pair (both relative to the row tile's first pair),
+    // component the row has joined or -1}
+    int4 *rq;
+    unsigned long long *rq_len;
+    int64_t rq_cap;
 };
 
 // The cutoff pair of this launch's blockIdx.y and its slices of the
@@ -500,6 +506,57 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
     add_stats(a, tests, passed);
 }
 
+// Row w's part of a full round on row words, over its candidate tiles
+// [t_from, t_to) (clamped to the row's tiles): 64 tiles a step (ascending
+// column tile, so entries come in list order), tiles whose points share a
+// rank that is not above w's, or that w has joined, are skipped without
+// reading their bits; tests are bitset intersections of the two rows' words.
+// resolved: the snapshot component w is known to have joined (-1: none).
+__device__ void round_row_words(const LabelArgs &a, int w, uint64_t rw, int c, int64_t t_from, int64_t t_to,
+                                int resolved, unsigned &tests, unsigned &passed) {
+    const int lane = threadIdx.x & 63;
+    const WordRow A = word_row(a, w);
+    const int64_t dw = deg_of(a, w);
+    t_to = min(t_to, A.len);
+    for (int64_t t0 = t_from; t0 < t_to; t0 += 64) {
+        const int64_t t = t0 + lane;
+        uint64_t v = 0ull, tr = 0ull;
+        int C = 0;
+        if (t < t_to) v = word_at(A, t);
+        if (v) {
+            C = A.tile[t];
+            tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
+        }
+        uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
+        while (tiles) {
+            const int src = __ffsll((unsigned long long)tiles) - 1;
+            tiles &= tiles - 1;
+            const uint32_t tlo = __builtin_amdgcn_readlane((uint32_t)tr, src);
+            const uint32_t thi = __builtin_amdgcn_readlane((uint32_t)(tr >> 32), src);
+            const uint64_t trs = ((uint64_t)thi << 32) | tlo;
+            if (trs != ~0ull && (int)tlo == resolved) continue;
+            const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
+            const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+            const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
+            const int j = __builtin_amdgcn_readlane(C, src) * 64 + lane;
+            uint64_t rj = 0ull;
+            if (bit) rj = trs == ~0ull ? a.rank[j] : trs;
+            const int kj = (int)(uint32_t)rj;
+            uint64_t bits = __ballot(bit && rj > rw && kj != resolved);
+            while (bits) {
+                const int jsrc = __ffsll((unsigned long long)bits) - 1;
+                bits &= bits - 1;
+                const int jj = __builtin_amdgcn_readlane(j, jsrc);
+                const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
+                if (test_and_link_words(a, w, jj, A, dw, c, tests, passed)) {
+                    resolved = kjj;
+                    bits &= ~__ballot(kj == kjj);
+                }
+            }
+        }
+    }
+}
+
 // Full round.  Every edge between two different snapshot components is owned
 // by the endpoint whose component ranks lower (rank = size << 32 | root, all
 // distinct), and only the owner examines it.  A row whose rank is at least the
@@ -520,48 +577,7 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
     unsigned tests = 0, passed = 0;
     int resolved = -1;  // last snapshot component found joined to w
     if constexpr (WORDS) {
-        // The row's words, 64 tiles a step (ascending column tile, so entries
-        // come in list order): tiles whose points share a rank that is not
-        // above w's, or that w has joined, are skipped without reading their
-        // bits; tests are bitset intersections of the two rows' words.
-        const WordRow A = word_row(a, w);
-        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
-            const int64_t t = t0 + lane;
-            uint64_t v = 0ull, tr = 0ull;
-            int C = 0;
-            if (t < A.len) v = word_at(A, t);
-            if (v) {
-                C = A.tile[t];
-                tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
-            }
-            uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
-            while (tiles) {
-                const int src = __ffsll((unsigned long long)tiles) - 1;
-                tiles &= tiles - 1;
-                const uint32_t tlo = __builtin_amdgcn_readlane((uint32_t)tr, src);
-                const uint32_t thi = __builtin_amdgcn_readlane((uint32_t)(tr >> 32), src);
-                const uint64_t trs = ((uint64_t)thi << 32) | tlo;
-                if (trs != ~0ull && (int)tlo == resolved) continue;
-                const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, src);
-                const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
-                const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
-                const int j = __builtin_amdgcn_readlane(C, src) * 64 + lane;
-                uint64_t rj = 0ull;
-                if (bit) rj = trs == ~0ull ? a.rank[j] : trs;
-                const int kj = (int)(uint32_t)rj;
-                uint64_t bits = __ballot(bit && rj > rw && kj != resolved);
-                while (bits) {
-                    const int jsrc = __ffsll((unsigned long long)bits) - 1;
-                    bits &= bits - 1;
-                    const int jj = __builtin_amdgcn_readlane(j, jsrc);
-                    const int kjj = __builtin_amdgcn_readlane(kj, jsrc);
-                    if (test_and_link_words(a, (int)w, jj, A, deg_of(a, w), c, tests, passed)) {
-                        resolved = kjj;
-                        bits &= ~__ballot(kj == kjj);
-                    }
-                }
-            }
-        }
+        round_row_words(a, (int)w, rw, c, 0, INT64_MAX, resolved, tests, passed);
         add_stats(a, tests, passed, 1);
         return;
     }
@@ -605,6 +621,188 @@ __global__ __launch_bounds__(256) void k_round_b(LabelArgs a) {
         }
     }
     add_stats(a, tests, passed, 1);
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Columns (lanes) of a column tile whose rank, held by lane b for column b,
+// lies above rw and belongs to no component numbered `resolved`: one pass per
+// distinct rank of the tile (a ballot each), the per-row test folded in.
+__device__ __forceinline__ uint64_t columns_above(uint64_t rk, uint64_t rw, int resolved) {
+    uint64_t rem = ~0ull, bits = 0ull;
+    while (rem) {
+        const uint64_t val = readlane_u64(rk, __ffsll((unsigned long long)rem) - 1);
+        const uint64_t m = __ballot(rk == val);
+        rem &= ~m;
+        if (val > rw && (int)(uint32_t)val != resolved) bits |= m;
+    }
+    return bits;
+}
+
+// Full round on a spatial graph's tile-pair words, `split` waves per row tile
+// (lane r owns row w = 64R + r; wave q of the tile takes the q-th of `split`
+// equal parts of its candidate pairs), with k_round_b's ownership and skip
+// rules: an edge is examined by the endpoint of lower snapshot rank, rows at
+// or above the tile's candidate maximum own none, and a row drops every column
+// of a component it has joined (per wave: the skip is only a saving, every
+// test re-checks with two finds).  The pairs are walked once for all 64 rows
+// -- pair k's 64 row words are one contiguous 512-byte load, 8 pairs in
+// flight -- and the rank filter and the already-joined check run for all rows
+// in parallel.  An edge whose endpoints are still apart needs a wave-wide bitset
+// test (test_and_link_words): the wave runs kInlineTests of them itself, then
+// hands each row that still has one, with the rest of the wave's pairs, to
+// k_round_queue (one wave per row there, as k_round_b), so no wave runs the
+// tests of many rows one after another; when the queue is full (or absent)
+// the wave tests every edge itself.
+__global__ __launch_bounds__(256) void k_round_tile(LabelArgs a, int64_t n_tiles, int split) {
+    a = slice(a);
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= n_tiles * split) return;
+    const int64_t R = g / split;
+    const int q = (int)(g - R * split);
+    const int64_t w = R * 64 + lane;
+    const uint64_t rw = w < a.n ? a.rank[w] : 0ull;
+    const uint64_t cmax = a.cand_max ? a.cand_max[R] : ~0ull;
+    const bool active = rw != 0ull && rw < cmax;
+    const uint64_t act = __ballot(active);
+    if (act == 0ull) return;
+    // a column tile whose points share a rank not above every active row's has no candidate
+    uint64_t rmin = active ? rw : ~0ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = __shfl_xor(rmin, off, 64);
+        rmin = o < rmin ? o : rmin;
+    }
+    const int c = csim_of(a, R * 64);
+    unsigned tests = 0, passed = 0;
+    int resolved = -1;  // last snapshot component this row was found joined to
+    bool live = active;  // false once the row is queued
+    bool rq_full = false;
+    // wave-wide tests a wave runs itself before it queues open rows: a passing
+    // test usually joins the components the other rows' open edges connect
+    constexpr unsigned kInlineTests = 2;
+    const int64_t kb0 = a.tp_ptr[R], ke0 = a.tp_ptr[R + 1];
+    const int64_t per = (ke0 - kb0 + split - 1) / split;
+    const int64_t kb = kb0 + q * per, ke = min(ke0, kb + per);
+    constexpr int kInFlight = 8;
+    for (int64_t k0 = kb; k0 < ke; k0 += 64) {
+        const int64_t kk = k0 + lane;
+        int C = 0;
+        uint64_t tr = 0ull;
+        bool want = false;
+        if (kk < ke && a.pair_nz[kk] != 0) {
+            C = a.tp_col[kk];
+            tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
+            want = tr == ~0ull || tr > rmin;
+        }
+        uint64_t wm = __ballot(want);
+        while (wm) {
+            int idx[kInFlight];
+#pragma unroll
+            for (int u = 0; u < kInFlight; ++u) {
+                idx[u] = wm ? __ffsll((unsigned long long)wm) - 1 : -1;
+                wm &= wm - 1;
+            }
+            uint64_t v[kInFlight], rk[kInFlight];
+#pragma unroll
+            for (int u = 0; u < kInFlight; ++u) {
+                v[u] = 0ull;
+                rk[u] = 0ull;
+                if (idx[u] >= 0) {
+                    const int64_t Cu = __builtin_amdgcn_readlane(C, idx[u]);
+                    if (live) v[u] = a.mask[(k0 + idx[u]) * 64 + lane];
+                    if (readlane_u64(tr, idx[u]) == ~0ull && Cu * 64 + lane < a.n) rk[u] = a.rank[Cu * 64 + lane];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kInFlight; ++u) {
+                if (!live) v[u] = 0ull;  // queued by an earlier pair of this batch
+                if (idx[u] < 0 || __ballot(v[u] != 0ull) == 0ull) continue;
+                const int Cu = __builtin_amdgcn_readlane(C, idx[u]);
+                const uint64_t tru = readlane_u64(tr, idx[u]);
+                const bool common = tru != ~0ull;
+                uint64_t bits;
+                if (common) bits = (tru > rw && (int)(uint32_t)tru != resolved) ? v[u] : 0ull;
+                else bits = columns_above(rk[u], rw, resolved) & v[u];
+                while (__ballot(bits != 0ull)) {
+                    const int col = bits ? __ffsll((unsigned long long)bits) - 1 : 0;
+                    const uint64_t rb = common ? tru : __shfl(rk[u], col, 64);
+                    int j = -1;
+                    bool joined = false;
+                    if (bits) {
+                        bits &= bits - 1;
+                        j = Cu * 64 + col;
+                        joined = uf_find(a.parent, (int)w) == uf_find(a.parent, j);
+                    }
+                    bool changed = joined;
+                    uint64_t need = __ballot(j >= 0 && !joined);
+                    while (need) {
+                        if (tests >= kInlineTests && a.rq && !rq_full) {
+                            // queue the rows still open, from this pair on
+                            unsigned long long base = 0;
+                            if (lane == 0) base = atomicAdd(a.rq_len, (unsigned long long)__popcll(need));
+                            base = __shfl(base, 0, 64);
+                            const bool mine = ((need >> lane) & 1ull) != 0ull;
+                            const unsigned long long slot = base + __popcll(need & ((1ull << lane) - 1ull));
+                            const bool queued = mine && slot < (unsigned long long)a.rq_cap;
+                            if (queued) {
+                                a.rq[slot] = make_int4((int)w, (int)(k0 + idx[u] - kb0), (int)(ke - kb0), resolved);
+                                live = false;
+                                bits = 0ull;
+                            }
+                            rq_full = __ballot(mine && !queued) != 0ull;
+                            need &= ~__ballot(queued);
+                            if (!need) break;
+                        }
+                        const int src = __ffsll((unsigned long long)need) - 1;
+                        need &= need - 1;
+                        const int ws = (int)(R * 64 + src);
+                        const int js = __builtin_amdgcn_readlane(j, src);
+                        if (test_and_link_words(a, ws, js, word_row(a, ws), deg_of(a, ws), c, tests, passed) &&
+                            lane == src)
+                            changed = true;
+                    }
+                    // a joined component: its remaining columns are joined as well
+                    if (changed) resolved = (int)(uint32_t)rb;
+                    if (__ballot(changed)) {
+                        if (common) {
+                            if (changed) bits = 0ull;
+                        } else {
+                            bits &= columns_above(rk[u], rw, resolved);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    add_stats(a, tests, passed, q == 0 ? (unsigned)__popcll(act) : 0u);
+}
+
+// The rows k_round_tile queued, one wave each (grid-stride): the row's
+// candidate pairs from the queued one to the end of its wave's part, with the
+// per-row walk of k_round_b.
+__global__ __launch_bounds__(256) void k_round_queue(LabelArgs a) {
+    a = slice(a);
+    const int64_t nq = (int64_t)min<unsigned long long>(*a.rq_len, (unsigned long long)a.rq_cap);
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); q < nq; q += waves) {
+        const int4 e = a.rq[q];
+        unsigned tests = 0, passed = 0;
+        round_row_words(a, e.x, a.rank[e.x], csim_of(a, e.x), e.y, e.z, e.w, tests, passed);
+        add_stats(a, tests, passed);
+    }
+}
+
+// Waves per row tile for k_round_tile: about 32 candidate pairs each.
+static int round_split(int64_t n_tile_pairs, int64_t n_tiles) {
+    if (n_tiles <= 0) return 1;
+    const int64_t per_tile = (n_tile_pairs + n_tiles - 1) / n_tiles;
+    return (int)std::min<int64_t>(32, std::max<int64_t>(1, (per_tile + 31) / 32));
 }
 
 // Largest snapshot rank among the core points of each tile (0: none), and the
@@ -1136,7 +1334,24 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.tile_rank = tile_common->p;
     }
     CNN_HIP(hipEventRecord(ctx->ev[10], s));
-    if (words) k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
+    if (words && a.mask && req.n_tiles > 0) {
+        // rows with tests to run go to a queue of at most kRoundQueueMax
+        // entries (beyond it the tile kernel tests them itself)
+        constexpr int64_t kRoundQueueMax = int64_t(1) << 22;
+        const int split = round_split(req.n_tile_pairs, req.n_tiles);
+        const int64_t cap = std::min<int64_t>(std::max<int64_t>(n, 64), kRoundQueueMax);
+        Scratch<int4> rq(ctx, cap);
+        Scratch<unsigned long long> rq_len(ctx, 1);
+        CNN_HIP(hipMemsetAsync(rq_len.p, 0, sizeof(unsigned long long), s));
+        a.rq = rq.p;
+        a.rq_len = rq_len.p;
+        a.rq_cap = cap;
+        k_round_tile<<<blocks_for(req.n_tiles * split, 4), 256, 0, s>>>(a, req.n_tiles, split);
+        k_round_queue<<<(unsigned)std::min<int64_t>(blocks_for(n, 4), 4096), 256, 0, s>>>(a);
+        a.rq = nullptr;
+        a.rq_len = nullptr;
+        a.rq_cap = 0;
+    } else if (words) k_round_b<true><<<blocks_for(n, 4), 256, 0, s>>>(a);
     else k_round_b<false><<<blocks_for(n, 4), 256, 0, s>>>(a);
     CNN_HIP(hipEventRecord(ctx->ev[11], s));
     CNN_HIP(hipGetLastError());
@@ -1250,6 +1465,7 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
     r.selfpos = g->selfpos;
     r.mask = g->mask;
     r.pair_nz = g->pair_nz;
+    r.n_tile_pairs = g->n_tile_pairs;
     r.root_passes = g->tune.root_passes;
     r.words_min_fill = g->tune.words_min_fill;
     return r;
