@@ -52,9 +52,9 @@ FP64_PEAK_TFLOPS = 78.6      # FMA-counted; separately rounded sub/mul/add reach
 TRAFFIC_FILES = {
     ("k_dist_f32_cols", "md_basins_3d"): "profiles/k_dist_f32_cols_md_basins_3d_traffic.json",
     ("k_dist_mfma_lds", "features_64d"): "profiles/k_dist_mfma_lds_features_64d_traffic.json",
-    ("k_round_b", "md_basins_3d"): "profiles/k_round_b_md_basins_3d_traffic.json",
+    ("round", "md_basins_3d"): "profiles/round_md_basins_3d_traffic.json",
     ("hook", "md_basins_3d"): "profiles/hook_md_basins_3d_traffic.json",
-    ("k_round_b", "features_64d"): "profiles/k_round_b_features_64d_traffic.json",
+    ("round", "features_64d"): "profiles/round_features_64d_traffic.json",
     ("hook", "features_64d"): "profiles/hook_features_64d_traffic.json",
 }
 
@@ -271,8 +271,10 @@ def su_rooflines(ls, workload_name):
     union-find tests only a small share of the candidate edges, so B over the
     phase time can exceed the HBM peak -- that effective rate is not a frac."""
     out = {}
-    # the hook phase: k_hook_tile + k_hook_queue on row words, k_hook on lists
-    for kernel, key in (("k_round_b", "ms_round"), ("hook", "ms_hook")):
+    # the full round: k_round_tile + k_round_queue on tile-pair words (k_round_b
+    # on other graphs); the hook phase: k_hook_tile + k_hook_queue on row words,
+    # k_hook on lists
+    for kernel, key in (("round", "ms_round"), ("hook", "ms_hook")):
         ms = mean(ls, key)
         traffic = None
         path = TRAFFIC_FILES.get((kernel, workload_name))
@@ -753,7 +755,7 @@ def main():
     dominant = dist_rf
     if emit_rf and emit_rf["ms_per_launch"] > dominant["ms_per_launch"]:
         dominant = emit_rf
-    rb = su_rf["k_round_b"]
+    rb = su_rf["round"]
     if rb["ms_per_launch"] > dominant["ms_per_launch"] and rb["achieved"] is not None:
         dominant = rb
     rows_bad, entries = neighbour_mismatches(Xd, r2)

@@ -390,7 +390,10 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
             }
         }
         hooked = open && nbr && cnt >= c;
-        if (hooked) a.parent[w] = (int)(w - 1);
+        // a run of hooked rows hangs from the unhooked row before it (lane 0
+        // is never hooked): depth one instead of a chain through w - 1
+        const uint64_t below = ~__ballot(hooked) & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        if (hooked) a.parent[w] = (int)(R * 64 + (63 - __clzll((long long)below)));
         tests = (unsigned)__popcll(__ballot(open && nbr));
         passed = (unsigned)__popcll(__ballot(hooked));
     }
@@ -426,11 +429,8 @@ __global__ __launch_bounds__(256) void k_hook_queue(LabelArgs a, const int32_t *
 // re-points itself at that root on the first pass.  Only roots write their own
 // entry and only to smaller values, so the parents stay a forest.
 template <bool WORDS>
-__global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) {
-    a = slice(a);
+__device__ void hook_root(const LabelArgs &a, int64_t w, int max_tests) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= a.n || !a.core[w] || a.parent[w] != (int)w) return;
     unsigned tests = 0, passed = 0;
     int attempts = 0;
     if constexpr (WORDS) {
@@ -504,6 +504,27 @@ __global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) 
         if (done) break;
     }
     add_stats(a, tests, passed);
+}
+
+template <bool WORDS>
+__global__ __launch_bounds__(256) void k_hook_roots(LabelArgs a, int max_tests) {
+    a = slice(a);
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= a.n || !a.core[w] || a.parent[w] != (int)w) return;
+    hook_root<WORDS>(a, w, max_tests);
+}
+
+// The same for the roots k_compress_roots listed, one wave each (grid-stride):
+// no wave is started for the points that are not roots.
+template <bool WORDS>
+__global__ __launch_bounds__(256) void k_hook_roots_list(LabelArgs a, const int32_t *__restrict__ roots,
+                                                         const unsigned int *__restrict__ n_roots, int max_tests) {
+    const int64_t nr = *n_roots;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); q < nr; q += waves) {
+        const int64_t w = roots[q];
+        if (a.parent[w] == (int)w) hook_root<WORDS>(a, w, max_tests);
+    }
 }
 
 // Row w's part of a full round on row words, over its candidate tiles
@@ -798,11 +819,11 @@ __global__ __launch_bounds__(256) void k_round_queue(LabelArgs a) {
     }
 }
 
-// Waves per row tile for k_round_tile: about 32 candidate pairs each.
+// Waves per row tile for k_round_tile: about 64 candidate pairs each.
 static int round_split(int64_t n_tile_pairs, int64_t n_tiles) {
     if (n_tiles <= 0) return 1;
     const int64_t per_tile = (n_tile_pairs + n_tiles - 1) / n_tiles;
-    return (int)std::min<int64_t>(32, std::max<int64_t>(1, (per_tile + 31) / 32));
+    return (int)std::min<int64_t>(32, std::max<int64_t>(1, (per_tile + 63) / 64));
 }
 
 // Largest snapshot rank among the core points of each tile (0: none), and the
@@ -882,6 +903,32 @@ __global__ void k_compress(int *parent, int64_t n) {
         p = atomic_load_relaxed(&parent[x]);
     }
     __hip_atomic_store(&parent[w], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// k_compress that also lists the core points that are roots afterwards
+// (one atomic per wave), for k_hook_roots_list.
+__global__ void k_compress_roots(int *parent, const uint8_t *__restrict__ core, int64_t n, int32_t *__restrict__ roots,
+                                 unsigned int *__restrict__ n_roots) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool root = false;
+    if (w < n) {
+        int x = (int)w;
+        int p = atomic_load_relaxed(&parent[x]);
+        while (p != x) {
+            x = p;
+            p = atomic_load_relaxed(&parent[x]);
+        }
+        __hip_atomic_store(&parent[w], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        root = x == (int)w && core[w] != 0;
+    }
+    const uint64_t m = __ballot(root);
+    if (m) {
+        const int lane = threadIdx.x & 63;
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(n_roots, (unsigned)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (root) roots[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)w;
+    }
 }
 
 // Lanes sharing a key are combined before one atomic per distinct key (points
@@ -1304,11 +1351,20 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     CNN_HIP(hipGetLastError());
     CNN_HIP(hipEventRecord(ctx->ev[9], s));
     // flatten (every parent becomes its root), then let roots re-point to smaller
-    // neighbouring roots twice, flattening after each pass
-    k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
-    for (int pass = 0; pass < req.root_passes; ++pass) {
-        if (words) k_hook_roots<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
-        else k_hook_roots<false><<<blocks_for(n, 4), 256, 0, s>>>(a, 4);
+    // neighbouring roots root_passes times, flattening after each pass; each
+    // flattening before a pass lists the core roots, and only they get a wave
+    if (req.root_passes <= 0) {
+        k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
+    } else {
+        Scratch<int32_t> roots(ctx, n);
+        Scratch<unsigned int> n_roots(ctx, 1);
+        const unsigned rblocks = (unsigned)std::min<int64_t>(blocks_for(n, 4), 4096);
+        for (int pass = 0; pass < req.root_passes; ++pass) {
+            CNN_HIP(hipMemsetAsync(n_roots.p, 0, sizeof(unsigned int), s));
+            k_compress_roots<<<blocks_for(n), 256, 0, s>>>(parent.p, core.p, n, roots.p, n_roots.p);
+            if (words) k_hook_roots_list<true><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots.p, 4);
+            else k_hook_roots_list<false><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots.p, 4);
+        }
         k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     }
     CNN_HIP(hipGetLastError());
