@@ -219,6 +219,9 @@ void exclusive_scan_i32_to_i64(cnn_ctx *ctx, const int32_t *in, int64_t *out, in
 void exclusive_scan_i64(cnn_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
 void inclusive_scan_u8_to_i32(cnn_ctx *ctx, const uint8_t *in, int32_t *out, int64_t n);
 void sort_pairs_u64_i32(cnn_ctx *ctx, uint64_t *keys, int32_t *vals, int64_t n, int end_bit);
+// Stable sort of vals by keys[0, end_bit); only vals come back in key order
+// (keys are left in an unspecified order).
+void sort_pairs_u32_i32(cnn_ctx *ctx, uint32_t *keys, int32_t *vals, int64_t n, int end_bit);
 void sort_pairs_i64_i32(cnn_ctx *ctx, int64_t *keys, int32_t *vals, int64_t n);
 void segmented_sort_i32(cnn_ctx *ctx, int32_t *keys, int64_t nnz, const int64_t *offsets,
                         int64_t n_segments, int end_bit);

@@ -335,9 +335,10 @@ __global__ __launch_bounds__(256) void k_hook(LabelArgs a, int max_tests) {
 // r's word), so one pass over the tile's non-empty pairs counts
 // |N(w) & N(w - 1)| for all 63 adjacent row pairs at once (w - 1's word comes
 // from the lane below).  A core row whose predecessor is a core neighbour with
-// at least c_sim common neighbours hooks to it.  The pass stops once every
-// row still in question has reached c_sim.  Writes only parent[w] of hooked
-// rows, to w - 1: a forest, as the per-row hooking.
+// at least c_sim common neighbours hooks to it (to the first row of its run of
+// hooked rows, which is connected to it through them).  The pass stops once
+// every row still in question has reached c_sim.  Writes only parent[w] of
+// hooked rows, to a smaller index: a forest, as the per-row hooking.
 __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles, int32_t *__restrict__ queue,
                                                    unsigned int *__restrict__ queue_len) {
     a = slice(a);
@@ -356,14 +357,29 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
     unsigned tests = 0, passed = 0;
     if (__ballot(open) != 0ull) {
         const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
+        // the row tile's own pair first (its column tiles ascend: a ballot per
+        // 64 of them finds it): it tells every row whether w - 1 is a
+        // neighbour, so the walk below can stop as soon as the open rows
+        // with that neighbour have c common neighbours
+        int64_t kd = -1;
+        for (int64_t k0 = kb; k0 < ke && kd < 0; k0 += 64) {
+            const int64_t kk = k0 + lane;
+            const uint64_t hit = __ballot(kk < ke && a.tp_col[kk] == (int)R);
+            if (hit) kd = k0 + __ffsll((unsigned long long)hit) - 1;
+        }
         int cnt = 0;
         bool nbr = false;  // w - 1 in N(w): bit lane-1 of the row's word in its own tile
-        bool done = false;
+        if (kd >= 0 && a.pair_nz[kd] != 0) {
+            const uint64_t v = a.mask[kd * 64 + lane];
+            const uint64_t pv = __shfl_up(v, 1, 64);
+            cnt = __popcll(v & pv);
+            nbr = lane > 0 && ((v >> (lane - 1)) & 1ull) != 0ull;
+        }
+        bool done = __ballot(open && nbr && cnt < c) == 0ull;
         for (int64_t k0 = kb; k0 < ke && !done; k0 += 64) {
-            // the window's non-empty pairs, then their words 8 at a time (loads in flight together)
+            // the window's other non-empty pairs, then their words 8 at a time (loads in flight together)
             const int64_t kk = k0 + lane;
-            const bool nzl = kk < ke && a.pair_nz[kk] != 0;
-            const int coll = kk < ke ? a.tp_col[kk] : INT_MAX;
+            const bool nzl = kk < ke && kk != kd && a.pair_nz[kk] != 0;
             uint64_t nzm = __ballot(nzl);
             while (nzm && !done) {
                 int idx[8];
@@ -375,18 +391,12 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
                 uint64_t v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) v[u] = idx[u] >= 0 ? a.mask[(k0 + idx[u]) * 64 + lane] : 0ull;
-                int cmax = INT_MIN;
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    if (idx[u] < 0) continue;
                     const uint64_t pv = __shfl_up(v[u], 1, 64);
                     cnt += __popcll(v[u] & pv);
-                    const int C = __shfl(coll, idx[u], 64);
-                    if (C == (int)R) nbr = lane > 0 && ((v[u] >> (lane - 1)) & 1ull) != 0ull;
-                    cmax = C;
                 }
-                // past the row tile's own tile every row knows whether w - 1 is a neighbour
-                if (cmax >= (int)R && __ballot(open && nbr && cnt < c) == 0ull) done = true;
+                if (__ballot(open && nbr && cnt < c) == 0ull) done = true;
             }
         }
         hooked = open && nbr && cnt >= c;

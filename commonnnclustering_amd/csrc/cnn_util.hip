@@ -137,6 +137,19 @@ void sort_pairs_u64_i32(cnn_ctx *ctx, uint64_t *keys, int32_t *vals, int64_t n, 
     CNN_HIP(hipMemcpyAsync(vals, v2.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
 }
 
+void sort_pairs_u32_i32(cnn_ctx *ctx, uint32_t *keys, int32_t *vals, int64_t n, int end_bit) {
+    if (n <= 1) return;
+    Scratch<uint32_t> k2(ctx, n);
+    Scratch<int32_t> v2(ctx, n);
+    size_t tmp = 0;
+    CNN_HIP(rocprim::radix_sort_pairs(nullptr, tmp, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit,
+                                      ctx->stream));
+    Scratch<char> t(ctx, tmp);
+    CNN_HIP(rocprim::radix_sort_pairs(t.p, tmp, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit,
+                                      ctx->stream));
+    CNN_HIP(hipMemcpyAsync(vals, v2.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
+}
+
 void sort_pairs_i64_i32(cnn_ctx *ctx, int64_t *keys, int32_t *vals, int64_t n) {
     if (n <= 1) return;
     Scratch<int64_t> k2(ctx, n);

@@ -87,6 +87,40 @@ def test_band_hits_forced(cuda):
     assert st["band_hits"] > 100
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_band_offset_low_d(cuda, d, dtype):
+    """Low-d pairs within ~1e-6 relative of r, far from the origin (f64 inputs
+    lose bits to FP32 rounding, which widens the guard band): every band pair
+    is settled in FP64 and the lists match the oracle.  (In float32 inputs at
+    this offset the perturbation is below the coordinates' spacing, so few
+    pairs fall in the band.)"""
+    rng = np.random.default_rng(40 + d)
+    m = 2048
+    A = rng.normal(size=(m, d)) * 0.3 + 1000.0
+    u = rng.normal(size=(m, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = 0.05
+    B = A + u * (r * (1.0 + rng.uniform(-1e-6, 1e-6, size=(m, 1))))
+    X = np.concatenate([A, B]).astype(dtype)
+    st = assert_same(X, r, "f32_band")
+    if dtype == np.float64:
+        assert st["band_hits"] >= m // 4
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_wide_tiles_low_d(cuda, d):
+    """Tiles whose points span ~1e6 radii (a sparse far field around a dense
+    core, float64 and float32 inputs): lists match the oracle."""
+    rng = np.random.default_rng(70 + d)
+    core = rng.normal(size=(300, d)) * 2e-3
+    far = rng.uniform(-1e3, 1e3, size=(90, d))
+    X = np.concatenate([core, far, far[:5] + 4e-4])
+    rng.shuffle(X)
+    assert_same(X, 1e-3, "f32_band")
+    assert_same(X.astype(np.float32), 1e-3, "f32_band")
+
+
 def test_duplicates_and_nonfinite(cuda):
     X = np.array([[0, 0], [0, 0], [np.nan, 0], [np.inf, 1], [0.1, 0.1], [-np.inf, 5], [0.2, -0.1]],
                  dtype=np.float64)
