@@ -97,8 +97,8 @@ struct KeySpec {
     double qmax;
 };
 
-template <typename Tin>
-__global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks, uint64_t *__restrict__ keys,
+template <typename Tin, typename K>
+__global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks, K *__restrict__ keys,
                          int32_t *__restrict__ vals) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -114,7 +114,7 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
     if (ks.nk == 1) key = q[0];
     else if (ks.nk == 2) key = spread2(q[0]) | (spread2(q[1]) << 1);
     else key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
-    keys[i] = key;
+    keys[i] = (K)key;  // K = uint32_t only when the key has at most 32 bits
     vals[i] = (int32_t)i;
 }
 
@@ -166,10 +166,10 @@ __global__ __launch_bounds__(256) void k_pivot_super(const Tin *__restrict__ X, 
     super_of[p] = bs;
 }
 
-template <typename Tin>
+template <typename Tin, typename K>
 __global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, int64_t n, int d, int L,
                                                      const int32_t *__restrict__ super_of,
-                                                     uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+                                                     K *__restrict__ keys, int32_t *__restrict__ vals) {
     __shared__ float4 sp[kPivotChunk * kPivotDims / 4];
     const int dd = min(d, kPivotDims);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, 
         }
     }
     if (i < n) {
-        keys[i] = ((uint64_t)super_of[bi] << 8) | (uint64_t)bi;
+        keys[i] = (K)(((uint32_t)super_of[bi] << 8) | (uint32_t)bi);
         vals[i] = (int32_t)i;
     }
 }
@@ -2926,30 +2926,42 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
     if (g->layout == 1) {
         const KeySpec ks = make_keyspec(d, lo, hi, 64, n);
         g->perm = dev_alloc<int32_t>(g, n);
-        Scratch<uint64_t> keys(ctx, n);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
-        if (dtype == CNN_F32)
-            k_morton<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
-        else
-            k_morton<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
-        CNN_HIP(hipGetLastError());
-        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, ks.nk * ks.bits);
+        const int key_bits = ks.nk * ks.bits;
+        if (key_bits <= 32) {
+            // 32-bit keys: half the key traffic, and the sort's 32-bit configuration
+            Scratch<uint32_t> keys(ctx, n);
+            if (dtype == CNN_F32)
+                k_morton<float, uint32_t><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
+            else
+                k_morton<double, uint32_t><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
+            CNN_HIP(hipGetLastError());
+            sort_pairs_u32_i32(ctx, keys.p, g->perm, n, key_bits);
+        } else {
+            Scratch<uint64_t> keys(ctx, n);
+            if (dtype == CNN_F32)
+                k_morton<float, uint64_t><<<blocks, 256, 0, s>>>((const float *)X, n, d, ks, keys.p, g->perm);
+            else
+                k_morton<double, uint64_t><<<blocks, 256, 0, s>>>((const double *)X, n, d, ks, keys.p, g->perm);
+            CNN_HIP(hipGetLastError());
+            sort_pairs_u64_i32(ctx, keys.p, g->perm, n, key_bits);
+        }
     } else if (!bounds_host && d > kSpatialMaxD && n >= kPivotMinN) {
         const int L = (int)std::min<int64_t>(256, n / 64);
         const int S = std::min(kSuperPivots, L);
         g->perm = dev_alloc<int32_t>(g, n);
-        Scratch<uint64_t> keys(ctx, n);
+        Scratch<uint32_t> keys(ctx, n);
         Scratch<int32_t> super_of(ctx, L);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
         if (dtype == CNN_F32) {
             k_pivot_super<float><<<1, 256, 0, s>>>((const float *)X, n, d, L, S, super_of.p);
-            k_pivot_cells<float><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, super_of.p, keys.p, g->perm);
+            k_pivot_cells<float, uint32_t><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, super_of.p, keys.p, g->perm);
         } else {
             k_pivot_super<double><<<1, 256, 0, s>>>((const double *)X, n, d, L, S, super_of.p);
-            k_pivot_cells<double><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, super_of.p, keys.p, g->perm);
+            k_pivot_cells<double, uint32_t><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, super_of.p, keys.p, g->perm);
         }
         CNN_HIP(hipGetLastError());
-        sort_pairs_u64_i32(ctx, keys.p, g->perm, n, 12);
+        sort_pairs_u32_i32(ctx, keys.p, g->perm, n, 12);
     }
     g->xw64 = dev_alloc<double>(g, n_pad * std::max(d, 1));
     if (band) g->xw32 = dev_alloc<float>(g, n_pad * std::max(d, 1));
