@@ -489,6 +489,23 @@ __device__ __forceinline__ void transpose_stage(uint32_t &lo, uint32_t &hi) {
     hi = (hi & keep) | ((q >> sb) & ~keep);
 }
 
+// The same transpose as transpose_bits, by six 64-bit lane permutes through
+// the LDS crossbar: slower, but it keeps no lane-dependent constants live, for
+// kernels whose occupancy is set by their register count.
+__device__ __forceinline__ uint64_t transpose_bits_shfl(uint64_t x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int sft = 32 >> t;
+        const uint64_t m = masks[t];
+        const uint64_t y = __shfl_xor(x, sft, 64);
+        x = (lane & sft) ? ((x & ~m) | ((y >> sft) & m)) : ((x & m) | ((y & m) << sft));
+    }
+    return x;
+}
+
 // Bit transpose across the wave: lane j receives the word whose bit i is bit j
 // of lane i's input.  Each stage swaps the off-diagonal blocks of width
 // 32, 16, ..., 1 between partner lanes (lane ^ s); the stages act on

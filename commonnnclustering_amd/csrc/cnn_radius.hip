@@ -750,6 +750,9 @@ __device__ __forceinline__ int64_t mirror_of(const DistArgs &a, int64_t k, int64
 }
 
 // km: the mirror's index when the caller has it (loaded ahead), else -1.
+// LOW_REGS: transpose with transpose_bits_shfl (callers at the edge of an
+// occupancy step).
+template <bool LOW_REGS = false>
 __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t R, int64_t i, int64_t C, uint64_t m,
                                           bool have_col, uint64_t colword, bool force = false, int64_t km = -1) {
     m &= tile_mask(a, C);
@@ -765,7 +768,7 @@ __device__ __forceinline__ int store_pair(const DistArgs &a, int64_t k, int64_t 
     if (a.pair_nz && !any && !force) return 0;
     a.mask[(k - a.k_base) * 64 + lane] = m;
     if (mirrored) {
-        if (!have_col) colword = transpose_bits(m);
+        if (!have_col) colword = LOW_REGS ? transpose_bits_shfl(m) : transpose_bits(m);
         else {
             colword &= tile_mask(a, R);                   // padding rows of R
             if (!row_real(a, C * 64 + lane)) colword = 0;  // padding columns of C
@@ -832,7 +835,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_generic(DistArgs a) {
         uint64_t m = 0;
 #pragma unroll
         for (int jj = 0; jj < 64; ++jj) m |= (uint64_t)(acc[jj] <= r2) << jj;
-        mydeg += store_pair(a, k, R, i, C, m, false, 0);
+        mydeg += store_pair<true>(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
 }
@@ -872,7 +875,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_periodic(DistArgs a) {
             }
             m |= (uint64_t)(total <= r2) << jj;
         }
-        mydeg += store_pair(a, k, R, i, C, m, false, 0);
+        mydeg += store_pair<true>(a, k, R, i, C, m, false, 0);
     }
     finish_deg(a, i, mydeg);
 }
@@ -1043,9 +1046,17 @@ __global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
             }
         });
         const uint64_t colword = ((uint64_t)chi << 32) | clo;
-        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_bits(colword) : 0ull, true, colword);
+        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_bits_shfl(colword) : 0ull, true, colword);
     }
     finish_deg(a, i_lane, mydeg);
+}
+
+// Transpose for k_dist_f32_cols<D>: the DPP / permlane form where it costs
+// no occupancy step (D <= 4), the register-light permute form above that.
+template <int D>
+__device__ __forceinline__ uint64_t transpose_cols(uint64_t x) {
+    if constexpr (D <= 4) return transpose_bits(x);
+    else return transpose_bits_shfl(x);
 }
 
 template <int D>
@@ -1118,7 +1129,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
                 }
             }
         }
-        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_bits(colword) : 0ull, true, colword);
+        mydeg += store_pair(a, k, R, i_lane, C, __ballot(colword != 0ull) ? transpose_cols<D>(colword) : 0ull, true, colword);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i_lane, mydeg);
@@ -1165,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
             }
         }
         uint64_t m = band_resolve(a, r2, lo, hi, i, C, acc, hits, flips, maxerr, xc);
-        mydeg += store_pair(a, k, R, i, C, m, false, 0);
+        mydeg += store_pair<true>(a, k, R, i, C, m, false, 0);
     }
     flush_band(a, hits, flips, maxerr);
     finish_deg(a, i, mydeg);
@@ -1860,7 +1871,7 @@ __global__ __launch_bounds__(256) void k_dist_f64_wide(DistArgs a, const double 
             m |= (uint64_t)(s0 <= a.bp.r2) << jj;
             m |= (uint64_t)(s1 <= a.bp.r2) << (jj + 1);
         }
-        mydeg += store_pair(a, pair_index(a, R, C), R, i, C, m, false, 0);
+        mydeg += store_pair<true>(a, pair_index(a, R, C), R, i, C, m, false, 0);
     }
     if (i < a.n && mydeg) atomicAdd(&a.deg[i], mydeg);
 }
