@@ -113,8 +113,8 @@ typedef struct {
     float ms_similarity;       /* similarity + union-find kernels */
     float ms_labels;           /* compress, label assignment, sort_by_size */
     int32_t sim_form;          /* how similarities were tested: CNN_SIM_FORM_* */
-    float ms_hook;             /* the hooking kernel (k_hook) alone */
-    float ms_round;            /* the full-round kernel (k_round_b) alone */
+    float ms_hook;             /* the hooking kernels (k_hook_tile + k_hook_queue, or k_hook) */
+    float ms_round;            /* the full-round kernels (k_round_tile + k_round_queue, or k_round_b) */
 } cnn_label_stats;
 
 /* cnn_label_stats.sim_form */
