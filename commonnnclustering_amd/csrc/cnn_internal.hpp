@@ -640,12 +640,13 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
         }
         return false;
     }
+    // windows of 64 tiles of each list; the next windows' tile ids are loaded
+    // together with this window's words
     int64_t ia = 0, ib = 0;
+    int va = (int)min<int64_t>(64, A.len), vb = (int)min<int64_t>(64, B.len);
+    int ta = lane < va ? A.tile[lane] : INT_MAX;
+    int tb = lane < vb ? B.tile[lane] : INT_MAX;
     while (ia < A.len && ib < B.len) {
-        const int va = (int)min<int64_t>(64, A.len - ia);
-        const int vb = (int)min<int64_t>(64, B.len - ib);
-        const int ta = lane < va ? A.tile[ia + lane] : INT_MAX;
-        const int tb = lane < vb ? B.tile[ib + lane] : INT_MAX;
         const int alast = __shfl(ta, va - 1, 64);
         const int blast = __shfl(tb, vb - 1, 64);
         const int m = min(alast, blast);
@@ -660,13 +661,24 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
             }
         }
         const int found = __shfl(ta, min(lo, 63), 64);
+        const bool hit = lane < vb && tb <= m && lo < va && found == tb;
+        const int64_t pa = ia + lo, pb = ib + lane;
+        const int64_t ia2 = ia + __popcll(__ballot(lane < va && ta <= m));
+        const int64_t ib2 = ib + __popcll(__ballot(lane < vb && tb <= m));
+        const int va2 = (int)max<int64_t>(0, min<int64_t>(64, A.len - ia2));
+        const int vb2 = (int)max<int64_t>(0, min<int64_t>(64, B.len - ib2));
+        const int ta2 = lane < va2 ? A.tile[ia2 + lane] : INT_MAX;
+        const int tb2 = lane < vb2 ? B.tile[ib2 + lane] : INT_MAX;
         int cnt = 0;
-        if (lane < vb && tb <= m && lo < va && found == tb)
-            cnt = __popcll(word_at(A, ia + lo) & word_at(B, ib + lane));
+        if (hit) cnt = __popcll(word_at(A, pa) & word_at(B, pb));
         count += wave_sum(cnt);
         if (count >= c) return true;
-        ia += __popcll(__ballot(lane < va && ta <= m));
-        ib += __popcll(__ballot(lane < vb && tb <= m));
+        ia = ia2;
+        ib = ib2;
+        va = va2;
+        vb = vb2;
+        ta = ta2;
+        tb = tb2;
     }
     return false;
 }
