@@ -489,9 +489,6 @@ __device__ __forceinline__ bool boxes_close(const double *__restrict__ alo, cons
     return s <= thr;
 }
 
-// Count (FILL == false) or list (FILL == true) the candidate column tiles of
-// each row tile: super-tiles are culled first (lane per super-tile), then the
-// 64 tiles of every surviving super-tile (lane per tile), in ascending order.
 // Box test against R's box held in registers (d <= kSpatialMaxD): all of the
 // other box's coordinates are loaded before any is used.
 struct RowBox {
@@ -516,21 +513,19 @@ struct RowBox {
     }
 };
 
-// Count (FILL == false) or list (FILL == true) the candidate column tiles of
-// each row tile: super-tiles are culled first (lane per super-tile), then the
-// 64 tiles of every surviving super-tile (lane per tile), in ascending order,
-// four surviving super-tiles per step (their loads in flight together).
-// FILL also records diag[R], R's place in its own list.
-template <bool FILL>
-__global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi,
-                             const double *__restrict__ slo, const double *__restrict__ shi, int d, int64_t n_tiles,
-                             int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
-                             const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
-                             int32_t *__restrict__ cnt, const int64_t *__restrict__ tp_ptr,
-                             int32_t *__restrict__ tp_col, int32_t *__restrict__ diag) {
+// Candidate column tiles of row tile R, by one wave: super-tiles are culled
+// first (lane per super-tile), then the 64 tiles of every surviving super-tile
+// (lane per tile), in ascending order, four surviving super-tiles per step
+// (their loads in flight together).  Each candidate C at list position pos
+// (0-based) is handed to emit(pos, C); returns the count.
+template <typename Emit>
+__device__ __forceinline__ int64_t tile_pair_search(const double *__restrict__ blo, const double *__restrict__ bhi,
+                                                    const double *__restrict__ slo, const double *__restrict__ shi,
+                                                    int d, int64_t n_tiles, int64_t n_super, double thr_all,
+                                                    const double *__restrict__ tile_thr,
+                                                    const int32_t *__restrict__ seg_lo,
+                                                    const int32_t *__restrict__ seg_hi, int64_t R, Emit emit) {
     const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (R >= n_tiles) return;
     RowBox rb;
     rb.d = d;
     rb.thr = tile_thr ? tile_thr[R] : thr_all;
@@ -540,7 +535,7 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
         rb.hi[k] = k < d ? bhi[R * d + k] : 0.0;
     }
     const int64_t clo = seg_lo ? seg_lo[R] : 0, chi = seg_hi ? seg_hi[R] : n_tiles;
-    int64_t out = FILL ? tp_ptr[R] : 0;
+    int64_t out = 0;
     for (int64_t S0 = clo / 64; S0 * 64 < chi; S0 += 64) {
         const int64_t Sl = S0 + lane;
         const bool sok = (Sl < n_super) && (Sl * 64 < chi) && rb.close(slo + Sl * d, shi + Sl * d);
@@ -563,16 +558,60 @@ __global__ void k_tile_pairs(const double *__restrict__ blo, const double *__res
                 if (q >= m) break;
                 const int64_t C = Sq[q] * 64 + lane;
                 const uint64_t b = __ballot(ok[q]);
-                if (FILL && ok[q]) {
-                    const int64_t pos = out + __popcll(b & ((1ull << lane) - 1ull));
-                    tp_col[pos] = (int32_t)C;
-                    if (diag && C == R) diag[R] = (int32_t)(pos - tp_ptr[R]);  // R's place in its own list
-                }
+                if (ok[q]) emit(out + __popcll(b & ((1ull << lane) - 1ull)), C);
                 out += __popcll(b);
             }
         }
     }
-    if (!FILL && lane == 0) cnt[R] = (int32_t)out;
+    return out;
+}
+
+// List entries a row tile keeps in the first search pass: at most
+// kPairBucketMax, fewer when T * kPairBucketMax * 4 bytes would pass
+// kPairBucketBytes (row tiles whose lists do not fit search again).
+constexpr int kPairBucketMax = 512;
+constexpr int64_t kPairBucketBytes = int64_t(512) << 20;
+
+// First search pass: the count of every row tile, and its list in its bucket
+// of B entries (complete when the count fits) with diag[R], R's place in its
+// own list.
+__global__ void k_tile_pairs(const double *__restrict__ blo, const double *__restrict__ bhi,
+                             const double *__restrict__ slo, const double *__restrict__ shi, int d, int64_t n_tiles,
+                             int64_t n_super, double thr_all, const double *__restrict__ tile_thr,
+                             const int32_t *__restrict__ seg_lo, const int32_t *__restrict__ seg_hi,
+                             int32_t *__restrict__ cnt, int32_t *__restrict__ bucket, int B,
+                             int32_t *__restrict__ diag) {
+    const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    int32_t *mine = bucket + R * B;
+    const int64_t total = tile_pair_search(blo, bhi, slo, shi, d, n_tiles, n_super, thr_all, tile_thr, seg_lo, seg_hi,
+                                           R, [&](int64_t pos, int64_t C) {
+                                               if (pos < B) mine[pos] = (int32_t)C;
+                                               if (C == R) diag[R] = (int32_t)pos;
+                                           });
+    if ((threadIdx.x & 63) == 0) cnt[R] = (int32_t)total;
+}
+
+// Second pass: each row tile's list moves from its bucket to its place in
+// tp_col; a row tile whose list overflowed its bucket searches again, writing
+// in place.
+__global__ void k_tile_pairs_place(const double *__restrict__ blo, const double *__restrict__ bhi,
+                                   const double *__restrict__ slo, const double *__restrict__ shi, int d,
+                                   int64_t n_tiles, int64_t n_super, double thr_all,
+                                   const double *__restrict__ tile_thr, const int32_t *__restrict__ seg_lo,
+                                   const int32_t *__restrict__ seg_hi, const int32_t *__restrict__ cnt,
+                                   const int32_t *__restrict__ bucket, int B, const int64_t *__restrict__ tp_ptr,
+                                   int32_t *__restrict__ tp_col) {
+    const int lane = threadIdx.x & 63;
+    const int64_t R = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (R >= n_tiles) return;
+    const int64_t c = cnt[R], base = tp_ptr[R];
+    if (c <= B) {
+        for (int64_t t = lane; t < c; t += 64) tp_col[base + t] = bucket[R * B + t];
+        return;
+    }
+    (void)tile_pair_search(blo, bhi, slo, shi, d, n_tiles, n_super, thr_all, tile_thr, seg_lo, seg_hi, R,
+                           [&](int64_t pos, int64_t C) { tp_col[base + pos] = (int32_t)C; });
 }
 
 // ------------------------------------------------------------------ distance kernels
@@ -2585,19 +2624,28 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         Scratch<int32_t> cnt(ctx, T);
         const double *tthr = pk ? pk->tile_thr : nullptr;
         const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
-        k_tile_pairs<false><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                     sl, sh, cnt.p, nullptr, nullptr, nullptr);
-        CNN_HIP(hipGetLastError());
-        tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
-        exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
-        k_scan_total<<<1, 1, 0, s>>>(tp_ptr, T, cnt.p);
-        total_pairs = read_scalar(ctx, tp_ptr + T);
-        tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
+        // one search: counts and lists (in per-tile buckets), then the lists
+        // move to their places once the counts are scanned
         Scratch<int32_t> diag(ctx, T);
         fill_i32(ctx, diag.p, -1, T);
-        k_tile_pairs<true><<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                    sl, sh, nullptr, tp_ptr, tp_col, diag.p);
-        CNN_HIP(hipGetLastError());
+        {
+            int B = (int)std::max<int64_t>(
+                16, std::min<int64_t>(kPairBucketMax, kPairBucketBytes / (4 * std::max<int64_t>(T, 1))));
+            if (const char *ev = std::getenv("CNN_HIP_PAIR_BUCKET")) B = std::max(1, std::atoi(ev));  // tests
+            Scratch<int32_t> bucket(ctx, T * B);
+            k_tile_pairs<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
+                                                                  sl, sh, cnt.p, bucket.p, B, diag.p);
+            CNN_HIP(hipGetLastError());
+            tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
+            exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
+            k_scan_total<<<1, 1, 0, s>>>(tp_ptr, T, cnt.p);
+            total_pairs = read_scalar(ctx, tp_ptr + T);
+            tp_col = g->tp_col = dev_alloc<int32_t>(g, total_pairs);
+            k_tile_pairs_place<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr,
+                                                                        tthr, sl, sh, cnt.p, bucket.p, B, tp_ptr,
+                                                                        tp_col);
+            CNN_HIP(hipGetLastError());
+        }
         gmask.reset(new Scratch<GroupMask>(ctx, std::max<int64_t>(total_pairs, 1)));
         mirror.reset(new Scratch<int64_t>(ctx, std::max<int64_t>(total_pairs, 1)));
         Scratch<unsigned long long> rows_part(ctx, 2 * T);

@@ -180,6 +180,16 @@ def test_row_tile_batches(cuda, monkeypatch, precision, d):
     assert st["distance_launches"] > 1
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32_band"])
+def test_tile_pair_lists_searched_again(cuda, monkeypatch, precision):
+    """Row tiles whose candidate lists overflow the first search pass's
+    bucket (forced to 4 entries) are searched again in place; lists are
+    unchanged."""
+    X, r = _clustered(6000 + 5, 3, 7)
+    monkeypatch.setenv("CNN_HIP_PAIR_BUCKET", "4")
+    assert_same(X.astype(np.float32), r, precision)
+
+
 @pytest.mark.parametrize("d", [16, 64])
 def test_gram_band_redo_path(cuda, monkeypatch, d):
     """With room for a single band record, the other guard-band tile pairs are
