@@ -640,12 +640,16 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
         }
         return false;
     }
-    // windows of 64 tiles of each list; the next windows' tile ids are loaded
-    // together with this window's words
+    // windows of 64 tiles of each list; the next windows' tile ids and pair
+    // flags are loaded together with this window's words, and a matched A
+    // position's flag comes from the lane that holds it: one memory round trip
+    // per window, words only for pairs whose flags are both set
     int64_t ia = 0, ib = 0;
     int va = (int)min<int64_t>(64, A.len), vb = (int)min<int64_t>(64, B.len);
     int ta = lane < va ? A.tile[lane] : INT_MAX;
     int tb = lane < vb ? B.tile[lane] : INT_MAX;
+    int za = lane < va ? (A.nz ? (int)A.nz[lane] : 1) : 0;
+    int zb = lane < vb ? (B.nz ? (int)B.nz[lane] : 1) : 0;
     while (ia < A.len && ib < B.len) {
         const int alast = __shfl(ta, va - 1, 64);
         const int blast = __shfl(tb, vb - 1, 64);
@@ -661,7 +665,8 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
             }
         }
         const int found = __shfl(ta, min(lo, 63), 64);
-        const bool hit = lane < vb && tb <= m && lo < va && found == tb;
+        const int zfound = __shfl(za, min(lo, 63), 64);
+        const bool hit = lane < vb && tb <= m && lo < va && found == tb && zfound != 0 && zb != 0;
         const int64_t pa = ia + lo, pb = ib + lane;
         const int64_t ia2 = ia + __popcll(__ballot(lane < va && ta <= m));
         const int64_t ib2 = ib + __popcll(__ballot(lane < vb && tb <= m));
@@ -669,8 +674,10 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
         const int vb2 = (int)max<int64_t>(0, min<int64_t>(64, B.len - ib2));
         const int ta2 = lane < va2 ? A.tile[ia2 + lane] : INT_MAX;
         const int tb2 = lane < vb2 ? B.tile[ib2 + lane] : INT_MAX;
+        const int za2 = lane < va2 ? (A.nz ? (int)A.nz[ia2 + lane] : 1) : 0;
+        const int zb2 = lane < vb2 ? (B.nz ? (int)B.nz[ib2 + lane] : 1) : 0;
         int cnt = 0;
-        if (hit) cnt = __popcll(word_at(A, pa) & word_at(B, pb));
+        if (hit) cnt = __popcll(A.word[pa * (int64_t)A.stride] & B.word[pb * (int64_t)B.stride]);
         count += wave_sum(cnt);
         if (count >= c) return true;
         ia = ia2;
@@ -679,6 +686,8 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
         vb = vb2;
         ta = ta2;
         tb = tb2;
+        za = za2;
+        zb = zb2;
     }
     return false;
 }
