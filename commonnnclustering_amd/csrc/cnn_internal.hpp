@@ -630,13 +630,16 @@ __device__ inline bool wave_sim_words(const WordRow &A, const WordRow &B, int c)
     const int lane = __lane_id();
     int count = 0;
     if (A.tile == B.tile && A.len == B.len) {
+        // one list: the next window's pair flags load with this window's words
+        int z = lane < A.len ? (A.nz ? (int)A.nz[lane] : 1) : 0;
         for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
-            const int64_t t = t0 + lane;
+            const int64_t t = t0 + lane, t2 = t + 64;
+            const int z2 = t2 < A.len ? (A.nz ? (int)A.nz[t2] : 1) : 0;
             int cnt = 0;
-            if (t < A.len && (!A.nz || A.nz[t] != 0))
-                cnt = __popcll(A.word[t * (int64_t)A.stride] & B.word[t * (int64_t)B.stride]);
+            if (z) cnt = __popcll(A.word[t * (int64_t)A.stride] & B.word[t * (int64_t)B.stride]);
             count += wave_sum(cnt);
             if (count >= c) return true;
+            z = z2;
         }
         return false;
     }
