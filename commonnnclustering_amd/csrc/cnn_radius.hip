@@ -347,14 +347,18 @@ __device__ __forceinline__ bool pivot_cuts(const float2 *__restrict__ bnd, int64
 }
 
 // The home pivots of R and C are tried first (they cut most pairs of
-// different clusters), then every pivot.
+// different clusters), then every pivot.  The test is symmetric in R and C,
+// so only the words of the 64 x 64 blocks on or right of the diagonal block
+// are evaluated here (k_live_mirror fills the others); counts[1] = live pairs
+// with C >= R.
 __global__ __launch_bounds__(256) void k_dense_live(const float2 *__restrict__ bnd, const int32_t *__restrict__ home,
                                                     int64_t T, int64_t W, float rc, uint64_t *__restrict__ live,
                                                     unsigned long long *__restrict__ counts) {
     const int lane = threadIdx.x & 63;
-    unsigned long long n_all = 0, n_up = 0;  // this wave's counts: one atomic each at the end
+    unsigned long long n_up = 0;  // this wave's count: one atomic at the end
     for (int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gw < T * W; gw += (int64_t)gridDim.x * 4) {
-        const int64_t R = gw / W, C = (gw - R * W) * 64 + lane;
+        const int64_t R = gw / W, cw = gw - R * W, C = cw * 64 + lane;
+        if (cw < (R >> 6)) continue;  // below the diagonal block (wave-uniform)
         bool cut = false;
         if (C < T && C != R) {
             cut = pivot_cuts(bnd, T, home[R], R, C, rc) || pivot_cuts(bnd, T, home[C], R, C, rc);
@@ -362,14 +366,25 @@ __global__ __launch_bounds__(256) void k_dense_live(const float2 *__restrict__ b
         }
         const bool lv = C < T && !cut;
         const uint64_t w = __ballot(lv);
-        n_all += __popcll(w);
         n_up += __popcll(__ballot(lv && C >= R));
         if (lane == 0) live[gw] = w;
     }
-    if (lane == 0) {
-        if (n_all) atomicAdd(&counts[0], n_all);
-        if (n_up) atomicAdd(&counts[1], n_up);
-    }
+    if (lane == 0 && n_up) atomicAdd(&counts[1], n_up);
+}
+
+// The words below the diagonal blocks: block (a, b), b < a (rows 64a.., word
+// b) is the bit transpose of block (b, a).  One wave per block.
+__global__ __launch_bounds__(256) void k_live_mirror(uint64_t *__restrict__ live, int64_t T, int64_t W) {
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blk >= W * W) return;
+    const int64_t a = blk / W, b = blk - a * W;
+    if (b >= a) return;
+    const int64_t C = b * 64 + lane;  // row of block (b, a)
+    const uint64_t src = C < T ? live[C * W + a] : 0ull;
+    const uint64_t t = transpose_bits_shfl(src);  // lane l: row 64a + l over columns 64b..
+    const int64_t R = a * 64 + lane;
+    if (R < T) live[R * W + b] = t;
 }
 
 // Per row tile (one wave): live pairs in all, and per word the live bits of
@@ -2679,13 +2694,14 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         if ((double)rc < rr) rc = std::nextafter(rc, INFINITY);
         k_dense_live<<<(unsigned)std::min<int64_t>(ceil_div(T * live_words, 4), 2048), 256, 0, s>>>(
             bnd.p, home.p, T, live_words, rc, live->p, lcnt.p);
+        k_live_mirror<<<(unsigned)ceil_div(live_words * live_words, 4), 256, 0, s>>>(live->p, T, live_words);
         CNN_HIP(hipGetLastError());
         {
             PinnedReads rd(ctx);
             const unsigned long long *lc = rd.get(lcnt.p, 2);
             rd.wait();
-            live_count[0] = lc[0];
             live_count[1] = lc[1];
+            live_count[0] = 2 * lc[1] - (unsigned long long)T;  // the diagonal pairs are live and counted once
         }
         // the live pairs become the graph's explicit tile-pair lists: the
         // words of culled pairs are never stored, and the label pass can test
