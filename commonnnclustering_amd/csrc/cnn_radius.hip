@@ -128,8 +128,9 @@ __global__ void k_morton(const Tin *__restrict__ X, int64_t n, int d, KeySpec ks
 // (k_dense_live) removes them before the distance kernels run.  Key = (super
 // cell, pivot index); the sort is stable (caller order within a cell).
 constexpr int kPivotDims = 64;
-constexpr int kPivotChunk = 128;
 constexpr int kSuperPivots = 16;
+
+typedef float v16f __attribute__((ext_vector_type(16)));  // a 32x32 FP32 MFMA accumulator
 
 // super_of[p] = nearest super-pivot of pivot p (one thread per pivot, L <= 256).
 template <typename Tin>
@@ -166,48 +167,95 @@ __global__ __launch_bounds__(256) void k_pivot_super(const Tin *__restrict__ X, 
     super_of[p] = bs;
 }
 
-template <typename Tin, typename K>
-__global__ __launch_bounds__(256) void k_pivot_cells(const Tin *__restrict__ X, int64_t n, int d, int L,
-                                                     const int32_t *__restrict__ super_of,
-                                                     K *__restrict__ keys, int32_t *__restrict__ vals) {
-    __shared__ float4 sp[kPivotChunk * kPivotDims / 4];
+// Pivot table of the nearest-pivot order: Lp >= L rows of 64 floats (the
+// pivot's first min(d, 64) coordinates, zero-padded) and |p|^2 / 2 per row
+// (+inf for rows past L, which are never nearest).  One wave per row.
+template <typename Tin>
+__global__ __launch_bounds__(64) void k_pivot_table(const Tin *__restrict__ X, int64_t n, int d, int L,
+                                                    float *__restrict__ piv, float *__restrict__ phalf) {
+    const int lane = threadIdx.x & 63;
+    const int64_t j = blockIdx.x;
     const int dd = min(d, kPivotDims);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float x[kPivotDims];
+    const float v = (j < L && lane < dd) ? (float)X[(j * n / L) * d + lane] : 0.0f;
+    piv[j * kPivotDims + lane] = v;
+    float s = v * v;
 #pragma unroll
-    for (int k = 0; k < kPivotDims; ++k) x[k] = (i < n && k < dd) ? (float)X[i * d + k] : 0.0f;
-    float best = INFINITY;
-    int bi = 0;
-    for (int p0 = 0; p0 < L; p0 += kPivotChunk) {
-        __syncthreads();
-        float *spf = (float *)sp;
-        for (int q = threadIdx.x; q < kPivotChunk * kPivotDims; q += 256) {
-            const int pj = p0 + q / kPivotDims, k = q % kPivotDims;
-            spf[q] = (pj < L && k < dd) ? (float)X[((int64_t)pj * n / L) * d + k] : 0.0f;
-        }
-        __syncthreads();
-        const int m = min(kPivotChunk, L - p0);
-        for (int j = 0; j < m; ++j) {
-            float acc = 0.0f;
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) phalf[j] = j < L ? 0.5f * s : INFINITY;
+}
+
+// Nearest-pivot cells on MFMA: per wave 32 points against the pivots in blocks
+// of 32, x.p accumulated in FP32 by v_mfma_f32_32x32x2f32 over the first
+// min(d, 64) dimensions (lane l supplies point l%32 and pivot l%32 at
+// dimension 2s + l/32 of step s), and the cell is argmin_p |p|^2/2 - x.p,
+// ties to the smaller pivot.  Only the working order depends on the cells.
+template <typename Tin, typename K>
+__global__ __launch_bounds__(256) void k_pivot_cells_mfma(const Tin *__restrict__ X, int64_t n, int d, int L,
+                                                          const float *__restrict__ piv,
+                                                          const float *__restrict__ phalf,
+                                                          const int32_t *__restrict__ super_of,
+                                                          K *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    if (base >= n) return;
+    const int dd = min(d, kPivotDims);
+    const int64_t i = base + r;
+    float a[kPivotDims / 2];
 #pragma unroll
-            for (int k4 = 0; k4 < kPivotDims / 4; ++k4) {
-                const float4 v = sp[j * (kPivotDims / 4) + k4];
-                const float t0 = x[4 * k4] - v.x, t1 = x[4 * k4 + 1] - v.y;
-                const float t2 = x[4 * k4 + 2] - v.z, t3 = x[4 * k4 + 3] - v.w;
-                acc = __builtin_fmaf(t0, t0, acc);
-                acc = __builtin_fmaf(t1, t1, acc);
-                acc = __builtin_fmaf(t2, t2, acc);
-                acc = __builtin_fmaf(t3, t3, acc);
-            }
-            if (acc < best) {
-                best = acc;
-                bi = p0 + j;
+    for (int s = 0; s < kPivotDims / 2; ++s) {
+        const int k = 2 * s + h;
+        a[s] = (i < n && k < dd) ? (float)X[i * d + k] : 0.0f;
+    }
+    float best[16];
+    int bidx[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        best[v] = INFINITY;
+        bidx[v] = 0;
+    }
+    const int nb = (L + 31) / 32;
+    for (int blk = 0; blk < nb; ++blk) {
+        const int j = blk * 32 + r;
+        const float *__restrict__ pj = piv + (int64_t)j * kPivotDims;
+        float b[kPivotDims / 2];
+#pragma unroll
+        for (int s = 0; s < kPivotDims / 2; ++s) b[s] = pj[2 * s + h];
+        const float hn = phalf[j];
+        v16f c = v16f{0};
+#pragma unroll
+        for (int s = 0; s < kPivotDims / 2; ++s) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], c, 0, 0, 0);
+        // element v of lane l: point 8(v/4) + 4h + v%4, pivot blk*32 + l%32
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const float sc = hn - c[v];
+            if (sc < best[v]) {
+                best[v] = sc;
+                bidx[v] = j;
             }
         }
     }
-    if (i < n) {
-        keys[i] = (K)(((uint32_t)super_of[bi] << 8) | (uint32_t)bi);
-        vals[i] = (int32_t)i;
+    // the nearest over the 32 lanes that hold the same points
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) {
+            const float ob = __shfl_xor(best[v], off, 64);
+            const int oi = __shfl_xor(bidx[v], off, 64);
+            if (ob < best[v] || (ob == best[v] && oi < bidx[v])) {
+                best[v] = ob;
+                bidx[v] = oi;
+            }
+        }
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int64_t ii = base + 8 * (v >> 2) + 4 * h + (v & 3);
+            if (ii < n) {
+                keys[ii] = (K)(((uint32_t)super_of[bidx[v]] << 8) | (uint32_t)bidx[v]);
+                vals[ii] = (int32_t)ii;
+            }
+        }
     }
 }
 
@@ -1238,7 +1286,6 @@ __global__ __launch_bounds__(256) void k_dist_f32_generic(DistArgs a) {
 
 // ------------------------------------------------------------------ FP32 Gram form on MFMA (d >= 16)
 
-typedef float v16f __attribute__((ext_vector_type(16)));
 
 // Augmented FP32 rows for the Gram form.  With w = |x|^2 (sequential FMA),
 // margin e = ka * w + kb and c = r2/4 - w/2:
@@ -3028,12 +3075,19 @@ cnn_graph *graph_build(cnn_ctx *ctx, const void *X, int64_t n, int32_t d, int32_
         Scratch<uint32_t> keys(ctx, n);
         Scratch<int32_t> super_of(ctx, L);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
+        const int Lp = (L + 31) / 32 * 32;
+        Scratch<float> piv(ctx, (int64_t)Lp * kPivotDims), phalf(ctx, Lp);
+        const unsigned mblocks = (unsigned)ceil_div(n, 128);  // 32 points per wave
         if (dtype == CNN_F32) {
             k_pivot_super<float><<<1, 256, 0, s>>>((const float *)X, n, d, L, S, super_of.p);
-            k_pivot_cells<float, uint32_t><<<blocks, 256, 0, s>>>((const float *)X, n, d, L, super_of.p, keys.p, g->perm);
+            k_pivot_table<float><<<(unsigned)Lp, 64, 0, s>>>((const float *)X, n, d, L, piv.p, phalf.p);
+            k_pivot_cells_mfma<float, uint32_t><<<mblocks, 256, 0, s>>>((const float *)X, n, d, L, piv.p, phalf.p,
+                                                                       super_of.p, keys.p, g->perm);
         } else {
             k_pivot_super<double><<<1, 256, 0, s>>>((const double *)X, n, d, L, S, super_of.p);
-            k_pivot_cells<double, uint32_t><<<blocks, 256, 0, s>>>((const double *)X, n, d, L, super_of.p, keys.p, g->perm);
+            k_pivot_table<double><<<(unsigned)Lp, 64, 0, s>>>((const double *)X, n, d, L, piv.p, phalf.p);
+            k_pivot_cells_mfma<double, uint32_t><<<mblocks, 256, 0, s>>>((const double *)X, n, d, L, piv.p,
+                                                                        phalf.p, super_of.p, keys.p, g->perm);
         }
         CNN_HIP(hipGetLastError());
         sort_pairs_u32_i32(ctx, keys.p, g->perm, n, 12);
