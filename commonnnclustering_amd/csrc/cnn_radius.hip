@@ -2845,10 +2845,10 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     int32_t *col = nullptr;
     // A single-batch graph with explicit tile-pair lists whose words take no
     // more memory than its lists would (512 bytes per tile pair against 4 per
-    // entry) keeps
-    // only the words: similarities are tested on them and the lists are
-    // emitted when a caller needs them (graph_lists).
+    // entry) keeps only the words: similarities are tested on them and the
+    // lists are emitted when a caller needs them (graph_lists).
     bool words_only = false;
+    const bool words_deferred = sym && tp_ptr && !pk && g->mask && batches.size() == 1;
     for (auto &b : batches) {
         DistArgs a{};
         a.xw64 = g->xw64;
@@ -2895,6 +2895,17 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const int64_t row_lo = b.first * 64, row_hi = std::min<int64_t>(b.second * 64, n);
         const int64_t rows = row_hi - row_lo;
         exclusive_scan_i32_to_i64(ctx, deg.p + row_lo, g->row_ptr + row_lo, rows);
+        if (words_deferred) {
+            // one batch that may keep its words: row_ptr[n] is completed on the
+            // device and the entry count read with the final statistics (no
+            // synchronisation here); the lists are emitted after it only if
+            // they are the smaller form
+            k_scan_total<<<1, 1, 0, s>>>(g->row_ptr, n, deg.p);
+            CNN_HIP(hipGetLastError());
+            words_only = true;
+            emitted = false;
+            continue;
+        }
         PinnedReads rd(ctx);
         const int64_t *last_excl = rd.get(g->row_ptr + row_hi - 1);
         const int32_t *last_deg = rd.get(deg.p + row_hi - 1);
@@ -2959,11 +2970,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         g->mask = nullptr;
         g->pair_nz = nullptr;
     }
-    set_i64(ctx, g->row_ptr + n, nnz_total);
+    if (!words_deferred) set_i64(ctx, g->row_ptr + n, nnz_total);
     PinnedReads rd(ctx);
     const unsigned long long *hb = rd.get(bstats.p, 4);
     const unsigned long long *hre = rows_eval ? rd.get(rows_eval->p, 2) : nullptr;
+    const int64_t *hnnz = words_deferred ? rd.get(g->row_ptr + n) : nullptr;
     rd.wait();
+    if (hnnz) nnz_total = *hnnz;
     if (hre) {
         rows_evaluated[0] = hre[0];
         rows_evaluated[1] = hre[1];
@@ -2990,6 +3003,16 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     st.ms_distance_kernel = ms_kernel;
     st.distance_launches = (int32_t)batches.size();
     st.layout = g->layout;
+    if (words_deferred && (int64_t)512 * total_pairs > (int64_t)4 * nnz_total) {
+        // the lists are the smaller form: emit them from the words, then drop the words
+        graph_lists(ctx, g);
+        graph_mem_free(g, g->mask);
+        graph_mem_free(g, g->pair_nz);
+        g->bytes -= std::max<int64_t>(max_batch_pairs, 1) * 64 * (int64_t)sizeof(uint64_t) +
+                    std::max<int64_t>(max_batch_pairs, 1);
+        g->mask = nullptr;
+        g->pair_nz = nullptr;
+    }
 }
 
 static void check_build_args(const void *X, int64_t n, int32_t d, int32_t dtype, int32_t precision) {
