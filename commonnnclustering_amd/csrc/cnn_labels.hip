@@ -1083,6 +1083,51 @@ __global__ void k_rank_clusters(const int32_t *vals, int64_t K, int64_t max_clus
     newlab[vals[t]] = (max_clusters < 0 || t < max_clusters) ? (int32_t)(t + 1) : 0;
 }
 
+// sort_by_size without the host: the K cluster keys (K read on the device;
+// they are distinct) are sorted by one workgroup in LDS (bitonic, over the
+// next power of two at or above K) and ranked as k_rank_clusters does.  More
+// than kSmallClusters clusters: *big = 1 and nothing is written (the host
+// sorts them after its next read of the counters).
+constexpr int kSmallClusters = 1024;
+__global__ __launch_bounds__(kSmallClusters) void k_rank_clusters_small(const uint64_t *__restrict__ keys,
+                                                                        const int32_t *__restrict__ vals,
+                                                                        const unsigned long long *__restrict__ Kp,
+                                                                        int64_t max_clusters,
+                                                                        int32_t *__restrict__ newlab,
+                                                                        unsigned long long *__restrict__ big) {
+    __shared__ uint64_t sk[kSmallClusters];
+    __shared__ int32_t sv[kSmallClusters];
+    const int64_t K = (int64_t)*Kp;
+    const int t = threadIdx.x;
+    if (K > kSmallClusters) {
+        if (t == 0) *big = 1ull;
+        return;
+    }
+    int n2 = 1;
+    while (n2 < K) n2 <<= 1;
+    sk[t] = t < K ? keys[t] : ~0ull;
+    sv[t] = t < K ? vals[t] : -1;
+    __syncthreads();
+    for (int size = 2; size <= n2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int q = t ^ stride;
+            if (q > t && q < n2) {
+                const bool up = (t & size) == 0;
+                const uint64_t ka = sk[t], kb = sk[q];
+                if ((ka > kb) == up) {
+                    sk[t] = kb;
+                    sk[q] = ka;
+                    const int32_t v = sv[t];
+                    sv[t] = sv[q];
+                    sv[q] = v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t < K) newlab[sv[t]] = (max_clusters < 0 || t < max_clusters) ? (int32_t)(t + 1) : 0;
+}
+
 // Packed batches: clusters of all datasets sorted together by
 // (dataset, size descending, smallest original index); ranks restart at 1 in
 // every dataset.
@@ -1430,9 +1475,17 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     k_comp_stats<<<blocks_for(n, 256 * kRunSteps), 256, 0, s>>>(core.p, parent.p, req.perm, n, cmin.p, csize.p,
                                                                 counters.p + 2);
     CNN_HIP(hipGetLastError());
+    // the cluster keys outlive the sort block: a fit with more clusters than
+    // k_rank_clusters_small takes sorts them again after the final read
+    std::unique_ptr<Scratch<uint64_t>> skeys;
+    std::unique_ptr<Scratch<int32_t>> svals, snewlab;
+    bool big_sort = false;
     if (req.sort) {
-        Scratch<uint64_t> keys(ctx, n);
-        Scratch<int32_t> vals(ctx, n), newlab(ctx, n);
+        skeys.reset(new Scratch<uint64_t>(ctx, n));
+        svals.reset(new Scratch<int32_t>(ctx, n));
+        snewlab.reset(new Scratch<int32_t>(ctx, n));
+        Scratch<uint64_t> &keys = *skeys;
+        Scratch<int32_t> &vals = *svals, &newlab = *snewlab;
         fill_i32(ctx, newlab.p, 0, n);
         if (req.packed) {
             // (only the packed keys: their top bits are the dataset, which
@@ -1458,13 +1511,13 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         } else {
             k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
                                                              req.member_cutoff, counters.p + 5, keys.p, vals.p);
-            CNN_HIP(hipGetLastError());
-            const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
-            sort_pairs_u64_i32(ctx, keys.p, vals.p, K, 63);
-            if (K > 0) k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(vals.p, K, req.max_clusters, newlab.p);
+            // up to kSmallClusters clusters sorted on the device; counters[4]
+            // flags more, which the host sorts after the final read below
+            k_rank_clusters_small<<<1, kSmallClusters, 0, s>>>(keys.p, vals.p, counters.p + 5, req.max_clusters,
+                                                               newlab.p, counters.p + 4);
             k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
             CNN_HIP(hipGetLastError());
-            local.n_clusters = (req.max_clusters >= 0) ? std::min<int64_t>(K, req.max_clusters) : K;
+            big_sort = true;
         }
     } else {
         Scratch<uint8_t> isfirst(ctx, n);
@@ -1488,6 +1541,19 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         rd.wait();
         std::copy(hp, hp + 8, h);
         std::copy(htp, htp + 4 * kStatSlots, ht);
+    }
+    if (big_sort) {
+        const int64_t K = (int64_t)h[5];
+        if (h[4]) {
+            // more clusters than one workgroup sorts: sort them here and write the labels again
+            fill_i32(ctx, snewlab->p, 0, n);
+            sort_pairs_u64_i32(ctx, skeys->p, svals->p, K, 63);
+            k_rank_clusters<<<blocks_for(K), 256, 0, s>>>(svals->p, K, req.max_clusters, snewlab->p);
+            k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, snewlab->p, req.perm, n, req.labels);
+            CNN_HIP(hipGetLastError());
+            CNN_HIP(hipStreamSynchronize(s));
+        }
+        local.n_clusters = (req.max_clusters >= 0) ? std::min<int64_t>(K, req.max_clusters) : K;
     }
     h[0] = h[1] = 0;
     unsigned long long rows = 0;

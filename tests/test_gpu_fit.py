@@ -24,6 +24,24 @@ def test_blobs_2d_labels(cuda, precision):
     assert (got == want).all()
 
 
+@pytest.mark.parametrize("n_blobs", [1000, 1500])
+def test_many_clusters_sort(cuda, n_blobs):
+    """Up to 1024 clusters are sorted by one workgroup on the device; more
+    are sorted afterwards by the host path.  Labels equal the oracle's on
+    either side of the limit (blob sizes vary, so ranks follow size first and
+    the smallest original index second)."""
+    rng = np.random.default_rng(n_blobs)
+    centres = np.stack(np.meshgrid(np.arange(40), np.arange(40)), -1).reshape(-1, 2)[:n_blobs] * 10.0
+    sizes = rng.integers(6, 12, size=n_blobs)
+    X = np.concatenate([c + rng.normal(0, 0.2, size=(k, 2)) for c, k in zip(centres, sizes)])
+    X = X[rng.permutation(len(X))]
+    want = C.fit(X, 1.5, 2, member_cutoff=5)
+    for precision in ("f64", "f32_band"):
+        got, gs, ls = gpu_fit(X, 1.5, 2, member_cutoff=5, precision=precision)
+        assert (got == want).all()
+        assert ls["n_clusters"] == int(want.max())
+
+
 def test_quickstart_via_clustering(cuda):
     from commonnnclustering_amd import Clustering
 
