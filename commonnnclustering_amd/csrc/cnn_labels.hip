@@ -553,9 +553,10 @@ __device__ void round_row_words(const LabelArgs &a, int w, uint64_t rw, int c, i
         const int64_t t = t0 + lane;
         uint64_t v = 0ull, tr = 0ull;
         int C = 0;
-        if (t < t_to) v = word_at(A, t);
-        if (v) {
+        if (t < t_to) {
+            // the tile id with the pair's flag, the tile's rank with the word
             C = A.tile[t];
+            v = word_at(A, t);
             tr = a.tile_rank ? a.tile_rank[C] : ~0ull;
         }
         uint64_t tiles = __ballot(v != 0ull && (tr == ~0ull || tr > rw));
