@@ -187,8 +187,9 @@ __global__ __launch_bounds__(64) void k_pivot_table(const Tin *__restrict__ X, i
 // Nearest-pivot cells on MFMA: per wave 32 points against the pivots in blocks
 // of 32, x.p accumulated in FP32 by v_mfma_f32_32x32x2f32 over the first
 // min(d, 64) dimensions (lane l supplies point l%32 and pivot l%32 at
-// dimension 2s + l/32 of step s), and the cell is argmin_p |p|^2/2 - x.p,
-// ties to the smaller pivot.  Only the working order depends on the cells.
+// dimension 32(l/32) + s of step s: each lane reads 32 contiguous floats,
+// float4 at a time), and the cell is argmin_p |p|^2/2 - x.p, ties to the
+// smaller pivot.  Only the working order depends on the cells.
 template <typename Tin, typename K>
 __global__ __launch_bounds__(256) void k_pivot_cells_mfma(const Tin *__restrict__ X, int64_t n, int d, int L,
                                                           const float *__restrict__ piv,
@@ -200,11 +201,26 @@ __global__ __launch_bounds__(256) void k_pivot_cells_mfma(const Tin *__restrict_
     if (base >= n) return;
     const int dd = min(d, kPivotDims);
     const int64_t i = base + r;
+    const int kb = (kPivotDims / 2) * h;  // this lane's dimensions: kb .. kb + 31
     float a[kPivotDims / 2];
+    if (std::is_same<Tin, float>::value && (d & 3) == 0 && ((uintptr_t)X & 15) == 0) {
+        // rows start 16-byte aligned: float4 loads (dd is a multiple of 4 as well)
+        const float4 *xr = (const float4 *)((const float *)X + i * d + kb);
 #pragma unroll
-    for (int s = 0; s < kPivotDims / 2; ++s) {
-        const int k = 2 * s + h;
-        a[s] = (i < n && k < dd) ? (float)X[i * d + k] : 0.0f;
+        for (int q = 0; q < kPivotDims / 8; ++q) {
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i < n && kb + 4 * q < dd) v = xr[q];
+            a[4 * q] = v.x;
+            a[4 * q + 1] = v.y;
+            a[4 * q + 2] = v.z;
+            a[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < kPivotDims / 2; ++s) {
+            const int k = kb + s;
+            a[s] = (i < n && k < dd) ? (float)X[i * d + k] : 0.0f;
+        }
     }
     float best[16];
     int bidx[16];
@@ -216,10 +232,16 @@ __global__ __launch_bounds__(256) void k_pivot_cells_mfma(const Tin *__restrict_
     const int nb = (L + 31) / 32;
     for (int blk = 0; blk < nb; ++blk) {
         const int j = blk * 32 + r;
-        const float *__restrict__ pj = piv + (int64_t)j * kPivotDims;
+        const float4 *__restrict__ pj = (const float4 *)(piv + (int64_t)j * kPivotDims + kb);
         float b[kPivotDims / 2];
 #pragma unroll
-        for (int s = 0; s < kPivotDims / 2; ++s) b[s] = pj[2 * s + h];
+        for (int q = 0; q < kPivotDims / 8; ++q) {
+            const float4 v = pj[q];
+            b[4 * q] = v.x;
+            b[4 * q + 1] = v.y;
+            b[4 * q + 2] = v.z;
+            b[4 * q + 3] = v.w;
+        }
         const float hn = phalf[j];
         v16f c = v16f{0};
 #pragma unroll
