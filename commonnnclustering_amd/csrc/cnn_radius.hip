@@ -692,7 +692,20 @@ __global__ void k_tile_pairs_place(const double *__restrict__ blo, const double 
     if (R >= n_tiles) return;
     const int64_t c = cnt[R], base = tp_ptr[R];
     if (c <= B) {
-        for (int64_t t = lane; t < c; t += 64) tp_col[base + t] = bucket[R * B + t];
+        // 512 entries a step, their loads in flight together
+        for (int64_t t0 = 0; t0 < c; t0 += 512) {
+            int32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t t = t0 + 64 * u + lane;
+                v[u] = t < c ? bucket[R * B + t] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t t = t0 + 64 * u + lane;
+                if (t < c) tp_col[base + t] = v[u];
+            }
+        }
         return;
     }
     (void)tile_pair_search(blo, bhi, slo, shi, d, n_tiles, n_super, thr_all, tile_thr, seg_lo, seg_hi, R,
