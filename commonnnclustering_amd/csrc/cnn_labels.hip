@@ -1325,10 +1325,15 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     Scratch<uint8_t> core(ctx, n);
     Scratch<uint64_t> rank(ctx, n);
     Scratch<int> parent(ctx, n);
-    Scratch<unsigned long long> counters(ctx, 8), tstats(ctx, 4 * kStatSlots);
-    CNN_HIP(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), s));
-    CNN_HIP(hipMemsetAsync(tstats.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
-    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats.p, rank.p, nullptr,
+    // the pass's small counters in one buffer, zeroed by one fill kernel:
+    // counters (8), striped test statistics (4 per slot), then one 8-byte slot
+    // each for the hook queue length, the round queue length and the root count
+    constexpr int64_t kSmallSlots = 8 + 4 * kStatSlots + 3;
+    Scratch<unsigned long long> small(ctx, kSmallSlots);
+    fill_u64(ctx, small.p, 0ull, kSmallSlots);
+    unsigned long long *const counters_p = small.p, *const tstats_p = small.p + 8;
+    unsigned long long *const slot_p = small.p + 8 + 4 * kStatSlots;
+    LabelArgs a{req.row_ptr, req.col, n, req.c_sim, req.m_core, core.p, parent.p, tstats_p, rank.p, nullptr,
                 nullptr, req.selfpos, req.tile_csim, req.tile_mcore};
 
     CNN_HIP(hipEventRecord(ctx->ev[0], s));
@@ -1393,12 +1398,11 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     if (words && a.mask) {
         // adjacent rows per row tile, then the rows left, one wave each
         Scratch<int32_t> hq(ctx, n);
-        Scratch<unsigned int> hq_len(ctx, 1);
-        CNN_HIP(hipMemsetAsync(hq_len.p, 0, sizeof(unsigned int), s));
-        k_hook_tile<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(a, req.n_tiles, hq.p, hq_len.p);
+        unsigned int *const hq_len = (unsigned int *)(slot_p + 0);  // zeroed with the counters
+        k_hook_tile<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(a, req.n_tiles, hq.p, hq_len);
         CNN_HIP(hipGetLastError());
         const unsigned qblocks = (unsigned)std::min<int64_t>(blocks_for(n, 4), 4096);
-        k_hook_queue<<<qblocks, 256, 0, s>>>(a, hq.p, hq_len.p, 2);
+        k_hook_queue<<<qblocks, 256, 0, s>>>(a, hq.p, hq_len, 2);
     } else if (words) {
         k_hook<true><<<blocks_for(n, 4), 256, 0, s>>>(a, 2);
     } else {
@@ -1413,13 +1417,13 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     } else {
         Scratch<int32_t> roots(ctx, n);
-        Scratch<unsigned int> n_roots(ctx, 1);
+        unsigned int *const n_roots = (unsigned int *)(slot_p + 2);  // zeroed with the counters
         const unsigned rblocks = (unsigned)std::min<int64_t>(blocks_for(n, 4), 4096);
         for (int pass = 0; pass < req.root_passes; ++pass) {
-            CNN_HIP(hipMemsetAsync(n_roots.p, 0, sizeof(unsigned int), s));
-            k_compress_roots<<<blocks_for(n), 256, 0, s>>>(parent.p, core.p, n, roots.p, n_roots.p);
-            if (words) k_hook_roots_list<true><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots.p, 4);
-            else k_hook_roots_list<false><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots.p, 4);
+            if (pass > 0) fill_u64(ctx, slot_p + 2, 0ull, 1);
+            k_compress_roots<<<blocks_for(n), 256, 0, s>>>(parent.p, core.p, n, roots.p, n_roots);
+            if (words) k_hook_roots_list<true><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots, 4);
+            else k_hook_roots_list<false><<<rblocks, 256, 0, s>>>(a, roots.p, n_roots, 4);
         }
         k_compress<<<blocks_for(n), 256, 0, s>>>(parent.p, n);
     }
@@ -1428,7 +1432,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         Scratch<int32_t> cnt(ctx, n);
         fill_i32(ctx, cnt.p, 0, n);
         k_root_count<<<blocks_for(n, 256 * kRunSteps), 256, 0, s>>>(core.p, parent.p, n, cnt.p);
-        k_count_roots<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters.p + 6);
+        k_count_roots<<<blocks_for(n), 256, 0, s>>>(cnt.p, n, counters_p + 6);
         k_snapshot<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cnt.p, n, rank.p);
         CNN_HIP(hipGetLastError());
     }
@@ -1453,10 +1457,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         const int split = round_split(req.n_tile_pairs, req.n_tiles);
         const int64_t cap = std::min<int64_t>(std::max<int64_t>(n, 64), kRoundQueueMax);
         Scratch<int4> rq(ctx, cap);
-        Scratch<unsigned long long> rq_len(ctx, 1);
-        CNN_HIP(hipMemsetAsync(rq_len.p, 0, sizeof(unsigned long long), s));
         a.rq = rq.p;
-        a.rq_len = rq_len.p;
+        a.rq_len = slot_p + 1;  // zeroed with the counters
         a.rq_cap = cap;
         k_round_tile<<<blocks_for(req.n_tiles * split, 4), 256, 0, s>>>(a, req.n_tiles, split);
         k_round_queue<<<(unsigned)std::min<int64_t>(blocks_for(n, 4), 4096), 256, 0, s>>>(a);
@@ -1474,7 +1476,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     fill_i32(ctx, cmin.p, INT_MAX, n);
     fill_i32(ctx, csize.p, 0, n);
     k_comp_stats<<<blocks_for(n, 256 * kRunSteps), 256, 0, s>>>(core.p, parent.p, req.perm, n, cmin.p, csize.p,
-                                                                counters.p + 2);
+                                                                counters_p + 2);
     CNN_HIP(hipGetLastError());
     // the cluster keys outlive the sort block: a fit with more clusters than
     // k_rank_clusters_small takes sorts them again after the final read
@@ -1494,28 +1496,28 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
             const PackedSort &ps = *req.packed;
             PackedKey pk{ps.ds_shift, ps.size_shift, ps.size_max, ps.tile_ds, ps.ds_offset};
             k_collect_clusters_packed<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
-                                                                    req.member_cutoff, pk, counters.p + 5, keys.p,
+                                                                    req.member_cutoff, pk, counters_p + 5, keys.p,
                                                                     vals.p);
             CNN_HIP(hipGetLastError());
-            const int64_t K = (int64_t)read_scalar(ctx, counters.p + 5);
+            const int64_t K = (int64_t)read_scalar(ctx, counters_p + 5);
             sort_pairs_u64_i32(ctx, keys.p, vals.p, K, ps.key_bits);
             if (K > 0) {
                 Scratch<int32_t> first(ctx, ps.n_sets);
                 k_first_of_set<<<blocks_for(K), 256, 0, s>>>(keys.p, K, ps.ds_shift, first.p);
                 k_rank_clusters_packed<<<blocks_for(K), 256, 0, s>>>(keys.p, vals.p, K, ps.ds_shift, first.p,
-                                                                     req.max_clusters, newlab.p, counters.p + 7);
+                                                                     req.max_clusters, newlab.p, counters_p + 7);
                 CNN_HIP(hipGetLastError());
             }
             k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
             CNN_HIP(hipGetLastError());
-            local.n_clusters = (int64_t)read_scalar(ctx, counters.p + 7);
+            local.n_clusters = (int64_t)read_scalar(ctx, counters_p + 7);
         } else {
             k_collect_clusters<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, cmin.p, csize.p, n,
-                                                             req.member_cutoff, counters.p + 5, keys.p, vals.p);
+                                                             req.member_cutoff, counters_p + 5, keys.p, vals.p);
             // up to kSmallClusters clusters sorted on the device; counters[4]
             // flags more, which the host sorts after the final read below
-            k_rank_clusters_small<<<1, kSmallClusters, 0, s>>>(keys.p, vals.p, counters.p + 5, req.max_clusters,
-                                                               newlab.p, counters.p + 4);
+            k_rank_clusters_small<<<1, kSmallClusters, 0, s>>>(keys.p, vals.p, counters_p + 5, req.max_clusters,
+                                                               newlab.p, counters_p + 4);
             k_write_sorted<<<blocks_for(n), 256, 0, s>>>(core.p, parent.p, newlab.p, req.perm, n, req.labels);
             CNN_HIP(hipGetLastError());
             big_sort = true;
@@ -1537,8 +1539,8 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
     unsigned long long ht[4 * kStatSlots];
     {
         PinnedReads rd(ctx);
-        const unsigned long long *hp = rd.get(counters.p, 8);
-        const unsigned long long *htp = rd.get(tstats.p, 4 * kStatSlots);
+        const unsigned long long *hp = rd.get(counters_p, 8);
+        const unsigned long long *htp = rd.get(tstats_p, 4 * kStatSlots);
         rd.wait();
         std::copy(hp, hp + 8, h);
         std::copy(htp, htp + 4 * kStatSlots, ht);

@@ -2833,7 +2833,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         bp.hi = 0.0f;
     }
     Scratch<unsigned long long> bstats(ctx, 4);
-    CNN_HIP(hipMemsetAsync(bstats.p, 0, 4 * sizeof(unsigned long long), s));
+    fill_u64(ctx, bstats.p, 0ull, 4);
     Scratch<int32_t> deg(ctx, n);
 
     std::vector<std::pair<int64_t, int64_t>> batches;
