@@ -118,6 +118,7 @@ struct cnn_graph {
     uint64_t *mask = nullptr;
     uint8_t *pair_nz = nullptr;
     int64_t n_tile_pairs = 0;
+    int32_t *diag = nullptr;     // spatial layout: per row tile, its own pair's place in its list (nullable)
     int64_t bytes = 0;
     cnn_graph_stats stats{};
     cnn::TuneParams tune{};      // launch shapes of the graph's class
@@ -320,6 +321,7 @@ struct LabelRequest {
     const int32_t *selfpos = nullptr; // per row: position of the row's own index in its list (nullable)
     const uint64_t *mask = nullptr;   // the lists as row words per candidate tile pair (nullable)
     const uint8_t *pair_nz = nullptr;
+    const int32_t *diag = nullptr;    // per row tile: its own pair's place in its list (nullable)
     // lists as per-row word lists (nullable): row w's words are wword[wptr[w] ..
     // wptr[w+1]), word t holding the row's neighbours among the 64 points of
     // column tile wtile[t], tiles ascending

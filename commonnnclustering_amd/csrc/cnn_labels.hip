@@ -86,6 +86,7 @@ struct LabelArgs {
     const int64_t *wptr;
     const int32_t *wtile;
     const uint64_t *wword;
+    const int32_t *diag;  // per row tile: its own pair's place in its list (nullable)
     // per-row degrees (nullable: row_ptr differences); word-only passes need no row_ptr
     const int32_t *deg;
     // several cutoff pairs in one launch, one per blockIdx.y (nullable: one
@@ -357,11 +358,12 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
     unsigned tests = 0, passed = 0;
     if (__ballot(open) != 0ull) {
         const int64_t kb = a.tp_ptr[R], ke = a.tp_ptr[R + 1];
-        // the row tile's own pair first (its column tiles ascend: a ballot per
-        // 64 of them finds it): it tells every row whether w - 1 is a
-        // neighbour, so the walk below can stop as soon as the open rows
-        // with that neighbour have c common neighbours
+        // the row tile's own pair first (recorded when the lists were built,
+        // or found by a ballot per 64 of its ascending column tiles): it tells
+        // every row whether w - 1 is a neighbour, so the walk below can stop
+        // as soon as the open rows with that neighbour have c common neighbours
         int64_t kd = -1;
+        if (a.diag && a.diag[R] >= 0) kd = kb + a.diag[R];
         for (int64_t k0 = kb; k0 < ke && kd < 0; k0 += 64) {
             const int64_t kk = k0 + lane;
             const uint64_t hit = __ballot(kk < ke && a.tp_col[kk] == (int)R);
@@ -1351,6 +1353,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.pair_nz = req.pair_nz;
         a.tp_ptr = req.tp_ptr;
         a.tp_col = req.tp_col;
+        a.diag = req.diag;
     } else if (req.sim_mode != CNN_SIM_MERGE && req.col) {
         bool build = req.sim_mode == CNN_SIM_BITSET;
         if (!build && req.words_min_fill > 0 && req.words_min_fill <= (int64_t)1 << 24) {
@@ -1600,6 +1603,7 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
     r.selfpos = g->selfpos;
     r.mask = g->mask;
     r.pair_nz = g->pair_nz;
+    r.diag = g->diag;
     r.n_tile_pairs = g->n_tile_pairs;
     r.root_passes = g->tune.root_passes;
     r.words_min_fill = g->tune.words_min_fill;

@@ -2586,6 +2586,7 @@ void graph_free(cnn_graph *g) {
     graph_mem_free(g, g->col);
     graph_mem_free(g, g->tp_ptr);
     graph_mem_free(g, g->tp_col);
+    graph_mem_free(g, g->diag);
     graph_mem_free(g, g->selfpos);
     graph_mem_free(g, g->mask);
     graph_mem_free(g, g->pair_nz);
@@ -2723,15 +2724,16 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         const int32_t *sl = pk ? pk->seg_lo : nullptr, *sh = pk ? pk->seg_hi : nullptr;
         // one search: counts and lists (in per-tile buckets), then the lists
         // move to their places once the counts are scanned
-        Scratch<int32_t> diag(ctx, T);
-        fill_i32(ctx, diag.p, -1, T);
+        // kept with the graph: the label pass's tile hooking starts at it
+        int32_t *diag = g->diag = dev_alloc<int32_t>(g, T);
+        fill_i32(ctx, diag, -1, T);
         {
             int B = (int)std::max<int64_t>(
                 16, std::min<int64_t>(kPairBucketMax, kPairBucketBytes / (4 * std::max<int64_t>(T, 1))));
             if (const char *ev = std::getenv("CNN_HIP_PAIR_BUCKET")) B = std::max(1, std::atoi(ev));  // tests
             Scratch<int32_t> bucket(ctx, T * B);
             k_tile_pairs<<<(unsigned)ceil_div(T, 4), 256, 0, s>>>(blo.p, bhi.p, slo.p, shi.p, d, T, NS, thr, tthr,
-                                                                  sl, sh, cnt.p, bucket.p, B, diag.p);
+                                                                  sl, sh, cnt.p, bucket.p, B, diag);
             CNN_HIP(hipGetLastError());
             tp_ptr = g->tp_ptr = dev_alloc<int64_t>(g, T + 1);
             exclusive_scan_i32_to_i64(ctx, cnt.p, tp_ptr, T);
@@ -2749,7 +2751,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         rows_eval.reset(new Scratch<unsigned long long>(ctx, 2));
         fill_u64(ctx, rows_part.p, 0ull, 2 * T);
         k_pair_info<<<(unsigned)ceil_div(2 * T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
-                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag.p);
+                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag);
         CNN_HIP(hipGetLastError());
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
@@ -3542,6 +3544,10 @@ cnn_graph *graph_rethreshold(cnn_ctx *ctx, const cnn_graph *src, double r2, int3
         g->tp_col = dev_alloc<int32_t>(g, pairs);
         CNN_HIP(hipMemcpyAsync(g->tp_ptr, src->tp_ptr, (T + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
         CNN_HIP(hipMemcpyAsync(g->tp_col, src->tp_col, pairs * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        if (src->diag) {
+            g->diag = dev_alloc<int32_t>(g, T);
+            CNN_HIP(hipMemcpyAsync(g->diag, src->diag, T * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        }
     }
     if (n == 0) {
         fill_i64(ctx, g->row_ptr, 0, 1);
