@@ -229,11 +229,15 @@ __device__ __forceinline__ void hook_row_words(const LabelArgs &a, int64_t w, in
     const WordRow A = word_row(a, w);
     const int Rw = (int)(w >> 6);
     int64_t top = 0;  // tiles up to and including the row's own tile
-    for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
-        const int64_t t = t0 + lane;
-        const uint64_t le = __ballot(t < A.len && A.tile[t] <= Rw);
-        top += __popcll(le);
-        if (__popcll(le) < 64) break;
+    if (a.mask && a.diag && a.diag[Rw] >= 0) {
+        top = a.diag[Rw] + 1;  // recorded when the lists were built
+    } else {
+        for (int64_t t0 = 0; t0 < A.len; t0 += 64) {
+            const int64_t t = t0 + lane;
+            const uint64_t le = __ballot(t < A.len && A.tile[t] <= Rw);
+            top += __popcll(le);
+            if (__popcll(le) < 64) break;
+        }
     }
     const int c = csim_of(a, w);
     unsigned tests = 0, passed = 0;
