@@ -471,7 +471,9 @@ __device__ void hook_root(const LabelArgs &a, int64_t w, int max_tests) {
                 const int Cs = __builtin_amdgcn_readlane(C, src);
                 const bool bit = ((lane < 32 ? vlo >> lane : vhi >> (lane - 32)) & 1u) != 0u;
                 const int j = Cs * 64 + lane;
-                const int rj = (bit && a.core[j]) ? a.parent[j] : INT_MAX;  // parents are flattened
+                // the parent loads with the core flag (parents are flattened)
+                const int pj = bit ? a.parent[j] : INT_MAX;
+                const int rj = (bit && a.core[j]) ? pj : INT_MAX;
                 uint64_t bits = __ballot(rj < (int)w);
                 while (bits && attempts < max_tests) {
                     const int src2 = __ffsll((unsigned long long)bits) - 1;
@@ -498,7 +500,8 @@ __device__ void hook_root(const LabelArgs &a, int64_t w, int max_tests) {
     for (int64_t t0 = b; t0 < e && attempts < max_tests; t0 += 64) {
         const int64_t t = t0 + lane;
         const int j = t < e ? a.col[t] : -1;
-        const int rj = (t < e && a.core[j]) ? a.parent[j] : INT_MAX;  // parents are flattened
+        const int pj = t < e ? a.parent[j] : INT_MAX;  // with the core flag (parents are flattened)
+        const int rj = (t < e && a.core[j]) ? pj : INT_MAX;
         uint64_t bits = __ballot(rj < (int)w);
         bool done = false;
         while (bits && attempts < max_tests) {
