@@ -2057,12 +2057,18 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     for (int64_t k = kR + 64 * half + lane; k < tp_ptr[R + 1]; k += 128) {
         const int64_t C = tp_col[k];
         const int64_t cb = tp_ptr[C], ce = tp_ptr[C + 1];
-        // neighbouring tiles have similar lists: R sits in C's list about as far
+        // Only pairs with C > R read their mirror: the one-sided evaluation
+        // visits C >= R and writes the words of (C, R) too.  The others keep
+        // their own index.
+        // Neighbouring tiles have similar lists: R sits in C's list about as far
         // from C's own place as C sits from R's in R's list.  Look there first,
-        // 16 entries at once; search the whole list when R is not among them.
-        int64_t m = -1;
+        // 16 entries at once; on a miss, search the side of the window that R
+        // is on, below C's own place (R < C).
+        int64_t m = C > R ? -1 : k;
         const int32_t dC = diag ? diag[C] : -1;
-        if (dR >= 0 && dC >= 0 && ce - cb >= 16) {
+        int64_t lo = cb, hi = ce;  // R's place in C's list is in [lo, hi)
+        if (dC >= 0) hi = cb + dC;
+        if (m < 0 && dR >= 0 && dC >= 0 && ce - cb >= 16) {
             const int64_t est = dC - ((k - kR) - dR);
             const int64_t w0 = cb + std::min<int64_t>(std::max<int64_t>(est - 8, 0), ce - cb - 16);
             int32_t v[16];
@@ -2072,9 +2078,10 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
 #pragma unroll
             for (int q = 0; q < 16; ++q) below += v[q] < (int32_t)R;
             if (below < 16 && v[below] == (int32_t)R) m = w0 + below;
+            else if (below == 0) hi = std::min(hi, w0);
+            else if (below == 16) lo = std::max(lo, w0 + 16);
         }
         if (m < 0) {
-            int64_t lo = cb, hi = ce;
             while (lo < hi) {
                 const int64_t mid = (lo + hi) >> 1;
                 if (tp_col[mid] < R) lo = mid + 1;
