@@ -2032,9 +2032,12 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
                                                    const double *__restrict__ tile_thr,
                                                    GroupMask *__restrict__ gmask,
                                                    unsigned long long *__restrict__ rows_evaluated,
-                                                   const int32_t *__restrict__ diag) {
+                                                   const int32_t *__restrict__ diag, int upper_only) {
     // rows_evaluated[2R]: rows of R evaluated against C over the pairs with
-    // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs
+    // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs.
+    // upper_only (the graph is evaluated one-sided): pairs with C < R are
+    // never read, so they get neither mirror nor groups, and rows_evaluated
+    // [2R + 1] is not filled.
     // the group boxes of R, shared by all of R's pairs: staged once per wave
     // two waves per row tile (pairs 64h + lane, step 128): twice the loads in
     // flight of one wave's dependent chain
@@ -2054,8 +2057,10 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     unsigned long long rows = 0, rows_all = 0;
     const int64_t kR = tp_ptr[R];
     const int32_t dR = diag ? diag[R] : -1;
-    for (int64_t k = kR + 64 * half + lane; k < tp_ptr[R + 1]; k += 128) {
+    const int64_t kS = upper_only && dR >= 0 ? kR + dR : kR;  // the pairs with C >= R start at the diagonal
+    for (int64_t k = kS + 64 * half + lane; k < tp_ptr[R + 1]; k += 128) {
         const int64_t C = tp_col[k];
+        if (upper_only && C < R) continue;
         const int64_t cb = tp_ptr[C], ce = tp_ptr[C + 1];
         // Only pairs with C > R read their mirror: the one-sided evaluation
         // visits C >= R and writes the words of (C, R) too.  The others keep
@@ -2706,6 +2711,12 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipGetLastError());
     }
 
+    // row-tile batches are bounded by the mask budget (512 B per tile pair);
+    // one batch is evaluated one-sided (C >= R)
+    int64_t mask_budget = (int64_t)8 << 30;
+    if (const char *ev = std::getenv("CNN_HIP_MASK_BUDGET")) mask_budget = std::max<int64_t>(512, std::atoll(ev));
+    const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
+
     // candidate tile pairs
     int64_t *tp_ptr = nullptr;
     int32_t *tp_col = nullptr;
@@ -2758,7 +2769,8 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         rows_eval.reset(new Scratch<unsigned long long>(ctx, 2));
         fill_u64(ctx, rows_part.p, 0ull, 2 * T);
         k_pair_info<<<(unsigned)ceil_div(2 * T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
-                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag);
+                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag,
+                                                             total_pairs <= max_pairs_per_batch ? 1 : 0);
         CNN_HIP(hipGetLastError());
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
@@ -2811,12 +2823,9 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
                                                              mirror->p);
         CNN_HIP(hipGetLastError());
     }
-    // row-tile batches bounded by the mask budget (512 B per tile pair); the
+    // row-tile batches bounded by the mask budget (max_pairs_per_batch); the
     // row offsets of the pair lists are read back only when several batches
     // are needed
-    int64_t mask_budget = (int64_t)8 << 30;
-    if (const char *ev = std::getenv("CNN_HIP_MASK_BUDGET")) mask_budget = std::max<int64_t>(512, std::atoll(ev));
-    const int64_t max_pairs_per_batch = std::max<int64_t>(mask_budget / 512, T);
     const bool one_batch = total_pairs <= max_pairs_per_batch;
     std::vector<int64_t> h_tp_ptr;
     if (tp_ptr && !one_batch) {
