@@ -772,7 +772,15 @@ struct DistArgs {
     const int32_t *live_pre;
     int64_t live_words;
     int deal_rr;  // dense MFMA kernel: work items dealt round-robin over the XCDs
+    // spatial layout, one batch: the row tile of workgroup b is row_order[b]
+    // (heaviest first), nullptr: R0 + b
+    const int32_t *row_order;
 };
+
+// Row tile of a workgroup of the row-tile kernels.
+__device__ __forceinline__ int64_t block_row_tile(const DistArgs &a) {
+    return a.row_order ? (int64_t)a.row_order[blockIdx.x] : a.R0 + blockIdx.x;
+}
 
 __device__ __forceinline__ bool pair_live(const DistArgs &a, int64_t R, int64_t C) {
     return !a.live || ((a.live[R * a.live_words + (C >> 6)] >> (C & 63)) & 1ull);
@@ -1149,7 +1157,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_dist_f64_cols(DistArgs a) {
     constexpr int DP = lds_row_stride<double, D>();
     __shared__ double srow[64 * DP];
-    const int64_t R = a.R0 + blockIdx.x;
+    const int64_t R = block_row_tile(a);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i_lane = R * 64 + lane;
     load_row_tile<double, D>(srow, a.xw64, R);
@@ -1212,7 +1220,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_dist_f32_cols(DistArgs a) {
     constexpr int DP = lds_row_stride<float, D>();
     __shared__ float srow[64 * DP];
-    const int64_t R = a.R0 + blockIdx.x;
+    const int64_t R = block_row_tile(a);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i_lane = R * 64 + lane;
     load_row_tile<float, D>(srow, a.xw32, R);
@@ -2128,6 +2136,18 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     }
 }
 
+// Sort keys that put the row tiles with the most rows to evaluate first (the
+// one-sided count rows[2R]): a workgroup that starts late on a heavy tile
+// would run on alone after the others.
+__global__ void k_row_order_keys(const unsigned long long *__restrict__ rows, int64_t T, uint32_t *__restrict__ key,
+                                 int32_t *__restrict__ order) {
+    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= T) return;
+    const unsigned long long v = rows[2 * R];
+    key[R] = ~(uint32_t)(v < 0xffffffffull ? v : 0xffffffffull);
+    order[R] = (int32_t)R;
+}
+
 // One workgroup: out[0] = sum of v[2i], out[1] = sum of v[2i + 1], i < n.
 __global__ __launch_bounds__(256) void k_sum_pairs_u64(const unsigned long long *__restrict__ v, int64_t n,
                                                        unsigned long long *__restrict__ out) {
@@ -2727,6 +2747,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     unsigned long long rows_evaluated[2] = {0, 0};  // spatial layout: rows evaluated (C >= R, all pairs)
     std::unique_ptr<Scratch<unsigned long long>> rows_eval;  // (read with the final statistics)
     std::unique_ptr<Scratch<int64_t>> mirror;
+    std::unique_ptr<Scratch<int32_t>> row_order;  // one batch: row tiles heaviest first
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
         glo.reset(new Scratch<double>(ctx, T * kRowGroups * d));
@@ -2774,6 +2795,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         CNN_HIP(hipGetLastError());
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
+        if (total_pairs <= max_pairs_per_batch) {
+            row_order.reset(new Scratch<int32_t>(ctx, T));
+            Scratch<uint32_t> okey(ctx, T);
+            k_row_order_keys<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(rows_part.p, T, okey.p, row_order->p);
+            CNN_HIP(hipGetLastError());
+            sort_pairs_u32_i32(ctx, okey.p, row_order->p, T, 32);
+        }
     }
     // dense layout in nearest-pivot order: tile pairs out of reach by the pivot
     // bounds are culled (never evaluated, stored as empty pairs)
@@ -2927,6 +2955,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.col_chunk = g->tune.wide_col_chunk;
         a.pair_nz = pair_nz_p;
         a.pair_groups = gmask ? gmask->p : nullptr;
+        a.row_order = row_order && b.first == 0 && b.second == T ? row_order->p : nullptr;
         a.bounds = g->bounds;
         if (live) {
             a.live = live->p;
