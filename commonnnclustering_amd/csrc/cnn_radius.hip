@@ -600,9 +600,10 @@ struct RowBox {
 
 // Candidate column tiles of row tile R, by one wave: super-tiles are culled
 // first (lane per super-tile), then the 64 tiles of every surviving super-tile
-// (lane per tile), in ascending order, four surviving super-tiles per step
-// (their loads in flight together).  Each candidate C at list position pos
-// (0-based) is handed to emit(pos, C); returns the count.
+// (lane per tile), in ascending order, kSuperPerStep surviving super-tiles
+// per step (their loads in flight together).  Each candidate C at list
+// position pos (0-based) is handed to emit(pos, C); returns the count.
+constexpr int kSuperPerStep = 8;
 template <typename Emit>
 __device__ __forceinline__ int64_t tile_pair_search(const double *__restrict__ blo, const double *__restrict__ bhi,
                                                     const double *__restrict__ slo, const double *__restrict__ shi,
@@ -626,20 +627,20 @@ __device__ __forceinline__ int64_t tile_pair_search(const double *__restrict__ b
         const bool sok = (Sl < n_super) && (Sl * 64 < chi) && rb.close(slo + Sl * d, shi + Sl * d);
         uint64_t sbits = __ballot(sok);
         while (sbits) {
-            int64_t Sq[4];
+            int64_t Sq[kSuperPerStep];
             int m = 0;
-            while (sbits && m < 4) {
+            while (sbits && m < kSuperPerStep) {
                 Sq[m++] = S0 + __ffsll((unsigned long long)sbits) - 1;
                 sbits &= sbits - 1;
             }
-            bool ok[4];
+            bool ok[kSuperPerStep];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < kSuperPerStep; ++q) {
                 const int64_t C = (q < m ? Sq[q] : 0) * 64 + lane;
                 ok[q] = q < m && C >= clo && C < chi && rb.close(blo + C * d, bhi + C * d);
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < kSuperPerStep; ++q) {
                 if (q >= m) break;
                 const int64_t C = Sq[q] * 64 + lane;
                 const uint64_t b = __ballot(ok[q]);
