@@ -774,7 +774,7 @@ struct DistArgs {
     int64_t live_words;
     int deal_rr;  // dense MFMA kernel: work items dealt round-robin over the XCDs
     // spatial layout, one batch: the row tile of workgroup b is row_order[b]
-    // (heaviest first), nullptr: R0 + b
+    // (heavy tiles first), nullptr: R0 + b
     const int32_t *row_order;
 };
 
@@ -2137,15 +2137,20 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     }
 }
 
-// Sort keys that put the row tiles with the most rows to evaluate first (the
-// one-sided count rows[2R]): a workgroup that starts late on a heavy tile
-// would run on alone after the others.
+// Sort keys that put the heavy row tiles first, heaviest first (rows to
+// evaluate, the one-sided count rows[2R], above kHeavyRowFactor times the
+// mean of total[0] / T): a workgroup that starts late on a heavy tile would
+// run on alone after the others.  The other tiles share the largest key and
+// keep their order in the stable sort, so that neighbouring workgroups still
+// read the same column tiles through L2.
+constexpr unsigned long long kHeavyRowFactor = 2;
 __global__ void k_row_order_keys(const unsigned long long *__restrict__ rows, int64_t T, uint32_t *__restrict__ key,
-                                 int32_t *__restrict__ order) {
+                                 int32_t *__restrict__ order, const unsigned long long *__restrict__ total) {
     const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (R >= T) return;
     const unsigned long long v = rows[2 * R];
-    key[R] = ~(uint32_t)(v < 0xffffffffull ? v : 0xffffffffull);
+    const bool heavy = v * (unsigned long long)T > kHeavyRowFactor * total[0];
+    key[R] = heavy ? ~(uint32_t)(v < 0xffffffffull ? v : 0xffffffffull) : 0xffffffffu;
     order[R] = (int32_t)R;
 }
 
@@ -2748,7 +2753,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     unsigned long long rows_evaluated[2] = {0, 0};  // spatial layout: rows evaluated (C >= R, all pairs)
     std::unique_ptr<Scratch<unsigned long long>> rows_eval;  // (read with the final statistics)
     std::unique_ptr<Scratch<int64_t>> mirror;
-    std::unique_ptr<Scratch<int32_t>> row_order;  // one batch: row tiles heaviest first
+    std::unique_ptr<Scratch<int32_t>> row_order;  // one batch: heavy row tiles first
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
         glo.reset(new Scratch<double>(ctx, T * kRowGroups * d));
@@ -2799,7 +2804,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         if (total_pairs <= max_pairs_per_batch) {
             row_order.reset(new Scratch<int32_t>(ctx, T));
             Scratch<uint32_t> okey(ctx, T);
-            k_row_order_keys<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(rows_part.p, T, okey.p, row_order->p);
+            k_row_order_keys<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(rows_part.p, T, okey.p, row_order->p, rows_eval->p);
             CNN_HIP(hipGetLastError());
             sort_pairs_u32_i32(ctx, okey.p, row_order->p, T, 32);
         }
