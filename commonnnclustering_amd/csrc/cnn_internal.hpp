@@ -119,6 +119,8 @@ struct cnn_graph {
     uint8_t *pair_nz = nullptr;
     int64_t n_tile_pairs = 0;
     int32_t *diag = nullptr;     // spatial layout: per row tile, its own pair's place in its list (nullable)
+    // spatial one-batch builds: the row tiles, heavy ones first (nullable)
+    int32_t *tile_order = nullptr;
     int64_t bytes = 0;
     cnn_graph_stats stats{};
     cnn::TuneParams tune{};      // launch shapes of the graph's class
@@ -322,6 +324,7 @@ struct LabelRequest {
     const uint64_t *mask = nullptr;   // the lists as row words per candidate tile pair (nullable)
     const uint8_t *pair_nz = nullptr;
     const int32_t *diag = nullptr;    // per row tile: its own pair's place in its list (nullable)
+    const int32_t *tile_order = nullptr;  // the row tiles, heavy ones first (nullable: ascending)
     // lists as per-row word lists (nullable): row w's words are wword[wptr[w] ..
     // wptr[w+1]), word t holding the row's neighbours among the 64 points of
     // column tile wtile[t], tiles ascending

@@ -102,6 +102,9 @@ struct LabelArgs {
     int4 *rq;
     unsigned long long *rq_len;
     int64_t rq_cap;
+    // the row tiles in the order the per-tile kernels take them: heavy ones
+    // first (nullable: ascending)
+    const int32_t *tile_order;
 };
 
 // The cutoff pair of this launch's blockIdx.y and its slices of the
@@ -350,8 +353,9 @@ __global__ __launch_bounds__(256) void k_hook_tile(LabelArgs a, int64_t n_tiles,
     queue += (int64_t)blockIdx.y * a.n;
     queue_len += blockIdx.y;
     const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (R >= n_tiles) return;
+    const int64_t gR = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gR >= n_tiles) return;
+    const int64_t R = a.tile_order ? (int64_t)a.tile_order[gR] : gR;
     const int64_t w = R * 64 + lane;
     const bool core_w = w < a.n && a.core[w] != 0;
     const bool core_p = __shfl_up((int)core_w, 1, 64) != 0;
@@ -704,8 +708,9 @@ __global__ __launch_bounds__(256) void k_round_tile(LabelArgs a, int64_t n_tiles
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= n_tiles * split) return;
-    const int64_t R = g / split;
-    const int q = (int)(g - R * split);
+    const int64_t gR = g / split;
+    const int q = (int)(g - gR * split);
+    const int64_t R = a.tile_order ? (int64_t)a.tile_order[gR] : gR;
     const int64_t w = R * 64 + lane;
     const uint64_t rw = w < a.n ? a.rank[w] : 0ull;
     const uint64_t cmax = a.cand_max ? a.cand_max[R] : ~0ull;
@@ -1361,6 +1366,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         a.tp_ptr = req.tp_ptr;
         a.tp_col = req.tp_col;
         a.diag = req.diag;
+        a.tile_order = req.tile_order;
     } else if (req.sim_mode != CNN_SIM_MERGE && req.col) {
         bool build = req.sim_mode == CNN_SIM_BITSET;
         if (!build && req.words_min_fill > 0 && req.words_min_fill <= (int64_t)1 << 24) {
@@ -1611,6 +1617,7 @@ LabelRequest graph_label_request(cnn_ctx *ctx, const cnn_graph *g, int32_t sim_m
     r.mask = g->mask;
     r.pair_nz = g->pair_nz;
     r.diag = g->diag;
+    r.tile_order = g->tile_order;
     r.n_tile_pairs = g->n_tile_pairs;
     r.root_passes = g->tune.root_passes;
     r.words_min_fill = g->tune.words_min_fill;

@@ -2625,6 +2625,7 @@ void graph_free(cnn_graph *g) {
     graph_mem_free(g, g->tp_ptr);
     graph_mem_free(g, g->tp_col);
     graph_mem_free(g, g->diag);
+    graph_mem_free(g, g->tile_order);
     graph_mem_free(g, g->selfpos);
     graph_mem_free(g, g->mask);
     graph_mem_free(g, g->pair_nz);
@@ -2753,7 +2754,6 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
     unsigned long long rows_evaluated[2] = {0, 0};  // spatial layout: rows evaluated (C >= R, all pairs)
     std::unique_ptr<Scratch<unsigned long long>> rows_eval;  // (read with the final statistics)
     std::unique_ptr<Scratch<int64_t>> mirror;
-    std::unique_ptr<Scratch<int32_t>> row_order;  // one batch: heavy row tiles first
     if (g->layout == 1) {
         Scratch<double> blo(ctx, T * d), bhi(ctx, T * d);
         glo.reset(new Scratch<double>(ctx, T * kRowGroups * d));
@@ -2802,11 +2802,13 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
         if (total_pairs <= max_pairs_per_batch) {
-            row_order.reset(new Scratch<int32_t>(ctx, T));
+            // kept with the graph: the label pass's per-tile kernels take the same order
+            g->tile_order = dev_alloc<int32_t>(g, T);
             Scratch<uint32_t> okey(ctx, T);
-            k_row_order_keys<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(rows_part.p, T, okey.p, row_order->p, rows_eval->p);
+            k_row_order_keys<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(rows_part.p, T, okey.p, g->tile_order,
+                                                                       rows_eval->p);
             CNN_HIP(hipGetLastError());
-            sort_pairs_u32_i32(ctx, okey.p, row_order->p, T, 32);
+            sort_pairs_u32_i32(ctx, okey.p, g->tile_order, T, 32);
         }
     }
     // dense layout in nearest-pivot order: tile pairs out of reach by the pivot
@@ -2961,7 +2963,7 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         a.col_chunk = g->tune.wide_col_chunk;
         a.pair_nz = pair_nz_p;
         a.pair_groups = gmask ? gmask->p : nullptr;
-        a.row_order = row_order && b.first == 0 && b.second == T ? row_order->p : nullptr;
+        a.row_order = g->tile_order && b.first == 0 && b.second == T ? g->tile_order : nullptr;
         a.bounds = g->bounds;
         if (live) {
             a.live = live->p;
