@@ -2029,6 +2029,17 @@ __global__ void k_pad_rows(const double *__restrict__ xw64, int64_t n_pad, int d
     xp[idx] = k < d ? xw64[w * d + k] : 0.0;
 }
 
+// Pairs per wave of k_pair_info in one-sided builds, and the count of such
+// chunks of each row tile (its pairs from its own on).
+constexpr int64_t kPairChunk = 128;
+__global__ void k_pair_chunks(const int64_t *__restrict__ tp_ptr, const int32_t *__restrict__ diag, int64_t T,
+                              int32_t *__restrict__ cnt) {
+    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= T) return;
+    const int64_t kS = tp_ptr[R] + (diag[R] >= 0 ? diag[R] : 0);
+    cnt[R] = (int32_t)((tp_ptr[R + 1] - kS + kPairChunk - 1) / kPairChunk);
+}
+
 // Mirror index of every candidate pair (R, C): position of R in C's list.
 // Per candidate pair k = (R, C), lane-parallel over the pairs of a row tile:
 // mirror[k] = index of (C, R) in C's list, and gmask[k] = the row groups of R
@@ -2041,20 +2052,38 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
                                                    const double *__restrict__ tile_thr,
                                                    GroupMask *__restrict__ gmask,
                                                    unsigned long long *__restrict__ rows_evaluated,
-                                                   const int32_t *__restrict__ diag, int upper_only) {
+                                                   const int32_t *__restrict__ diag, int upper_only,
+                                                   const int64_t *__restrict__ chunk_off,
+                                                   const int32_t *__restrict__ chunk_cnt) {
     // rows_evaluated[2R]: rows of R evaluated against C over the pairs with
     // C >= R (one-sided evaluation), rows_evaluated[2R + 1]: over all pairs.
     // upper_only (the graph is evaluated one-sided): pairs with C < R are
     // never read, so they get neither mirror nor groups, and rows_evaluated
     // [2R + 1] is not filled.
-    // the group boxes of R, shared by all of R's pairs: staged once per wave
-    // two waves per row tile (pairs 64h + lane, step 128): twice the loads in
-    // flight of one wave's dependent chain
+    // the group boxes of R, shared by all of R's pairs: staged once per wave.
+    // Without chunks two waves per row tile (pairs 64h + lane, step 128):
+    // twice the loads in flight of one wave's dependent chain.  With chunks
+    // (one-sided), wave gw takes chunk gw of kPairChunk pairs from C = R on,
+    // numbered row by row (chunk_off: exclusive scan of chunk_cnt), so a row
+    // tile with thousands of candidates is spread over many waves.
     __shared__ double sgl[4][kRowGroups * kSpatialMaxD], sgh[4][kRowGroups * kSpatialMaxD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t gw = (int64_t)blockIdx.x * 4 + wave;
-    const int64_t R = gw >> 1;
-    const int half = (int)(gw & 1);
+    int64_t R, part;
+    if (chunk_off) {
+        int64_t lo = 0, hi = n_tiles;  // the last row tile whose chunks start at or before gw
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (chunk_off[mid] <= gw) lo = mid;
+            else hi = mid;
+        }
+        R = lo;
+        part = gw - chunk_off[R];
+        if (part >= chunk_cnt[R]) R = n_tiles;  // past the last chunk
+    } else {
+        R = gw >> 1;
+        part = gw & 1;
+    }
     if (R < n_tiles)
         for (int t = lane; t < kRowGroups * d; t += 64) {
             sgl[wave][t] = glo[R * kRowGroups * d + t];
@@ -2067,7 +2096,11 @@ __global__ __launch_bounds__(256) void k_pair_info(const int64_t *__restrict__ t
     const int64_t kR = tp_ptr[R];
     const int32_t dR = diag ? diag[R] : -1;
     const int64_t kS = upper_only && dR >= 0 ? kR + dR : kR;  // the pairs with C >= R start at the diagonal
-    for (int64_t k = kS + 64 * half + lane; k < tp_ptr[R + 1]; k += 128) {
+    const int64_t kE = tp_ptr[R + 1];
+    const int64_t k_lo = chunk_off ? kS + kPairChunk * part : kS + 64 * part;
+    const int64_t k_hi = chunk_off ? std::min<int64_t>(kE, kS + kPairChunk * (part + 1)) : kE;
+    const int64_t k_step = chunk_off ? 64 : 128;
+    for (int64_t k = k_lo + lane; k < k_hi; k += k_step) {
         const int64_t C = tp_col[k];
         if (upper_only && C < R) continue;
         const int64_t cb = tp_ptr[C], ce = tp_ptr[C + 1];
@@ -2795,9 +2828,22 @@ static void build_pairs_and_csr(cnn_ctx *ctx, cnn_graph *g, bool band, int dtype
         Scratch<unsigned long long> rows_part(ctx, 2 * T);
         rows_eval.reset(new Scratch<unsigned long long>(ctx, 2));
         fill_u64(ctx, rows_part.p, 0ull, 2 * T);
-        k_pair_info<<<(unsigned)ceil_div(2 * T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p, blo.p,
-                                                             bhi.p, d, thr, tthr, gmask->p, rows_part.p, diag,
-                                                             total_pairs <= max_pairs_per_batch ? 1 : 0);
+        if (total_pairs <= max_pairs_per_batch) {
+            // one-sided: chunks of kPairChunk pairs from each row tile's own pair on
+            Scratch<int32_t> ccnt(ctx, T);
+            Scratch<int64_t> coff(ctx, T);
+            k_pair_chunks<<<(unsigned)ceil_div(T, 256), 256, 0, s>>>(tp_ptr, diag, T, ccnt.p);
+            CNN_HIP(hipGetLastError());
+            exclusive_scan_i32_to_i64(ctx, ccnt.p, coff.p, T);
+            const int64_t waves = T + total_pairs / kPairChunk + 1;  // bounds the chunks
+            k_pair_info<<<(unsigned)ceil_div(waves, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p,
+                                                                   blo.p, bhi.p, d, thr, tthr, gmask->p, rows_part.p,
+                                                                   diag, 1, coff.p, ccnt.p);
+        } else {
+            k_pair_info<<<(unsigned)ceil_div(2 * T, 4), 256, 0, s>>>(tp_ptr, tp_col, T, mirror->p, glo->p, ghi->p,
+                                                                   blo.p, bhi.p, d, thr, tthr, gmask->p, rows_part.p,
+                                                                   diag, 0, nullptr, nullptr);
+        }
         CNN_HIP(hipGetLastError());
         k_sum_pairs_u64<<<1, 256, 0, s>>>(rows_part.p, T, rows_eval->p);
         CNN_HIP(hipGetLastError());
