@@ -878,17 +878,21 @@ __global__ void k_tile_rank_max(const uint64_t *__restrict__ rank, int64_t n, in
     }
 }
 
-// Largest rank over the candidate column tiles of each row tile.
-__global__ void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__restrict__ tp_ptr,
-                           const int32_t *__restrict__ tp_col, int64_t n_tiles, uint64_t *__restrict__ cmax) {
+// Largest rank over the candidate column tiles of each row tile, one
+// workgroup per row tile: a tile with thousands of candidates takes a few
+// steps of 256 instead of a long serial walk by one wave.
+__global__ __launch_bounds__(256) void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__restrict__ tp_ptr,
+                                                  const int32_t *__restrict__ tp_col, int64_t n_tiles,
+                                                  uint64_t *__restrict__ cmax) {
     tmax += (int64_t)blockIdx.y * n_tiles;
     cmax += (int64_t)blockIdx.y * n_tiles;
-    const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (R >= n_tiles) return;
+    __shared__ uint64_t sv[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t R = blockIdx.x;
+    if (R >= n_tiles) return;  // the whole workgroup
     const int64_t kb = tp_ptr ? tp_ptr[R] : 0, ke = tp_ptr ? tp_ptr[R + 1] : n_tiles;
     uint64_t v = 0;
-    for (int64_t k = kb + lane; k < ke; k += 64) {
+    for (int64_t k = kb + threadIdx.x; k < ke; k += 256) {
         const int64_t C = tp_ptr ? (int64_t)tp_col[k] : k;
         const uint64_t x = tmax[C];
         v = x > v ? x : v;
@@ -898,7 +902,13 @@ __global__ void k_cand_max(const uint64_t *__restrict__ tmax, const int64_t *__r
         const uint64_t o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    if (lane == 0) cmax[R] = v;
+    if (lane == 0) sv[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t m = sv[0];
+        for (int q = 1; q < 4; ++q) m = sv[q] > m ? sv[q] : m;
+        cmax[R] = m;
+    }
 }
 
 __global__ void k_snapshot(const uint8_t *core, const int *parent, const int32_t *cnt, int64_t n, uint64_t *rank) {
@@ -1459,7 +1469,7 @@ void labels_run(cnn_ctx *ctx, const LabelRequest &req, cnn_label_stats *st) {
         tile_common.reset(new Scratch<uint64_t>(ctx, req.n_tiles));
         k_tile_rank_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(rank.p, n, req.n_tiles, tmax.p,
                                                                    tile_common->p);
-        k_cand_max<<<blocks_for(req.n_tiles, 4), 256, 0, s>>>(tmax.p, req.tp_ptr, req.tp_col, req.n_tiles,
+        k_cand_max<<<blocks_for(req.n_tiles, 1), 256, 0, s>>>(tmax.p, req.tp_ptr, req.tp_col, req.n_tiles,
                                                               cand_max->p);
         CNN_HIP(hipGetLastError());
         a.cand_max = cand_max->p;
@@ -1724,7 +1734,7 @@ void labels_enqueue_multi(cnn_ctx *ctx, const MultiLabelBufs &b, const MultiLabe
     k_root_count<<<g2(n, 256 * kRunSteps), 256, 0, s>>>(b.core, b.parent, n, b.cnt);
     k_snapshot<<<g2(n, 256), 256, 0, s>>>(b.core, b.parent, b.cnt, n, b.rank);
     k_tile_rank_max<<<g2(T, 4), 256, 0, s>>>(b.rank, n, T, b.tmax, b.tile_common);
-    k_cand_max<<<g2(T, 4), 256, 0, s>>>(b.tmax, src.tp_ptr, src.tp_col, T, b.cand_max);
+    k_cand_max<<<g2(T, 1), 256, 0, s>>>(b.tmax, src.tp_ptr, src.tp_col, T, b.cand_max);
     CNN_HIP(hipGetLastError());
     a.cand_max = b.cand_max;
     a.tile_rank = b.tile_common;
